@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""bench.py -- throughput of the QuIDS rule-application step on one MI355X.
+
+Workload: plain QCGD, 12-node cycle graph with seeded particles (srand(seed),
+rand() & 1 in left[0], right[0], left[1], ... order), one iteration =
+step -> split_merge(theta=pi/4, phi=0) -> erase_create(theta=pi/3, phi=pi/5),
+max_num_object = 2^22, FP64 magnitudes, tolerance 1e-30.  The first
+`--warmup` iterations grow the state untimed; the next `--steps` iterations
+are timed.  A bench step is one iteration (two simulate() calls and the
+step modifier); the value is candidates (sy_it.num_object summed over the
+timed simulate() calls) per second of timed wall time.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import math
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+METRIC = "children/s per simulate() step; batched+adjoint vs plain GPU at 1e-10 rel err"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def iteration_rules():
+    from quids_amd.rules import qcgd
+    return [qcgd.step, qcgd.split_merge(math.pi / 4, 0.0), qcgd.erase_create(math.pi / 3, math.pi / 5)]
+
+
+def alg_bytes(st):
+    """Algorithmic HBM bytes of one step: parents read once, per candidate a
+    24-byte {hash, mag} write + read + table update, per unique 32 bytes, per
+    survivor its bytes + 28 (magnitude, offset, size)."""
+    P, C, U, K = st["num_parent"], st["num_candidates"], st["num_unique"], st["num_survivors"]
+    gen = st["in_bytes"] + 24 * P + 24 * C
+    dedup = 48 * C + 32 * U
+    fin = st["output_bytes"] + 28 * K
+    return {"generate": gen, "dedup": dedup, "finalize": fin, "total": gen + dedup + fin}
+
+
+def cpu_baseline(nodes, seed, cap, budget_s):
+    """The compiled reference (oracle/_ref/ref_driver) on the same workload's
+    first iteration, on this host's cores."""
+    from oracle import oracle as orc
+    from quids_amd.rules import qcgd
+    from quids_amd.state import HostState
+    if not os.path.exists(orc.REF_DRIVER):
+        return None
+    threads = min(16, os.cpu_count() or 1)
+    hs = HostState.from_objects([qcgd.seeded_graph(nodes, seed)])
+    rules = iteration_rules()
+    ops = [orc.mod_op(rules[0]), orc.rule_op(rules[1], cap, dump=False), orc.rule_op(rules[2], cap, dump=False)]
+    try:
+        _, out = orc.run_reference(1, hs, ops, threads=threads, timeout=budget_s)
+    except Exception as e:  # noqa: BLE001
+        log(f"cpu baseline failed: {e}")
+        return None
+    C = secs = 0.0
+    for line in out.splitlines():
+        f = dict(kv.split("=") for kv in line.split()[2:])
+        C += float(f["C"])
+        secs += float(f["seconds"])
+    return {"value": C / secs if secs else None, "unit": "children/s", "cores": threads, "kind": "reference",
+            "sample": f"reference simulate() on iteration 1 of the same workload ({nodes} nodes, cap 2^{cap.bit_length() - 1}), "
+                      f"{int(C)} candidates in {secs:.2f} s, OMP_NUM_THREADS={threads}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=6)
+    ap.add_argument("--nodes", type=int, default=12)
+    ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--log2-cap", type=int, default=22)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=120.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import quids_amd as q
+    from quids_amd.rules import qcgd
+    from quids_amd.state import HostState
+
+    cap = 1 << args.log2_cap
+    rules = iteration_rules()
+    ctx = q.default_context()
+    a, b = q.it_t(ctx=ctx), q.it_t(ctx=ctx)
+    sy = q.sy_it_t(ctx=ctx)
+    a.upload(HostState.from_objects([qcgd.seeded_graph(args.nodes, args.seed)]))
+
+    def run_iteration(timed_stats=None):
+        nonlocal a, b
+        q.simulate(a, rules[0])
+        for r in rules[1:]:
+            in_bytes = a.info()[1] if timed_stats is not None else 0
+            st = q.simulate(a, r, b, sy, cap)
+            if timed_stats is not None:
+                st["in_bytes"] = in_bytes
+                timed_stats.append(st)
+            a, b = b, a
+
+    for i in range(args.warmup):
+        t = time.time()
+        run_iteration()
+        ctx.synchronize()
+        log(f"warmup iteration {i + 1}: {a.num_object} objects, last C={sy.num_object} ({time.time() - t:.2f} s)")
+
+    ctx.set_profiling(True)
+    ctx.reset_phase_times()
+    stats = []
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        run_iteration(stats)
+        log(f"timed iteration {args.warmup + i + 1}: {a.num_object} objects, C={stats[-1]['num_candidates']}")
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    phases = ctx.phase_times()
+    ctx.set_profiling(False)
+
+    C = sum(s["num_candidates"] for s in stats)
+    value = C / elapsed
+    ab = {k: sum(alg_bytes(s)[k] for s in stats) for k in ("generate", "dedup", "finalize", "total")}
+    kernel_ms = {k: phases[k][0] for k in ("generate", "dedup", "finalize")}
+    dom = max(kernel_ms, key=kernel_ms.get)
+    dom_ms = kernel_ms[dom]
+    achieved = ab[dom] / (dom_ms / 1e3) if dom_ms > 0 else None
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9 if achieved else None,
+                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK if achieved else None,
+                "traffic": None}
+    step_frac = ab["total"] / elapsed / HBM_PEAK
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args.nodes, args.seed, cap, args.cpu_budget)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "children/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"plain QCGD, {args.nodes}-node cycle graph, seed {args.seed}, "
+                               f"step+split_merge(pi/4,0)+erase_create(pi/3,pi/5), max_num_object=2^{args.log2_cap}, "
+                               f"iterations {args.warmup + 1}..{args.warmup + args.steps} timed",
+                   "candidates": C, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "step_hbm_frac": step_frac,
+        "phase_ms": {k: round(v[0], 3) for k, v in phases.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,199 @@
+/*
+ * quids_hip.h -- C ABI of the MI355X (gfx950) QuIDS rule-application step.
+ *
+ * Plain C types only: opaque handles, pointers and sizes.  Every entry point
+ * returns a qd_status; 0 is success.  The C++ header quids_amd/quids.hpp
+ * maps non-zero codes to std::runtime_error, as the reference throws on
+ * allocation failure (src/utils/vector.hpp:124-127).
+ *
+ * Handles are owned by the caller and freed explicitly.  A context owns one
+ * HIP stream on one device; it is not thread-safe (the reference is driven
+ * from one host thread too, src/quids.hpp:448).  All device work of a call is
+ * ordered on the context's stream; calls that return counters synchronise.
+ */
+#ifndef QUIDS_HIP_H
+#define QUIDS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QD_ABI_VERSION 1
+
+typedef enum qd_status {
+    QD_OK = 0,
+    QD_ERR_INVALID = 1,     /* bad argument or malformed object */
+    QD_ERR_OOM = 2,         /* device allocation failed */
+    QD_ERR_HIP = 3,         /* a HIP runtime call failed */
+    QD_ERR_CAPACITY = 4,    /* a size limit of this build was exceeded */
+    QD_ERR_UNSUPPORTED = 5  /* rule / option without a device implementation */
+} qd_status;
+
+/* Rules with a device implementation (reference: subclasses of quids::rule,
+ * src/quids.hpp:105-146). */
+typedef enum qd_rule_kind {
+    QD_RULE_HADAMARD = 1,          /* src/rules/quantum_computer.hpp:31-50 */
+    QD_RULE_QCGD_ERASE_CREATE = 2, /* src/rules/qcgd.hpp:459-532 */
+    QD_RULE_QCGD_COIN = 3,         /* src/rules/qcgd.hpp:534-605 */
+    QD_RULE_QCGD_SPLIT_MERGE = 4   /* src/rules/qcgd.hpp:607-1036 */
+} qd_rule_kind;
+
+/* Modifiers with a device implementation (reference: quids::modifier_t,
+ * src/quids.hpp:86, applied by simulate(it, modifier), 436-438). */
+typedef enum qd_modifier_kind {
+    QD_MOD_CNOT = 1,               /* src/rules/quantum_computer.hpp:25-29 */
+    QD_MOD_XGATE = 2,              /* src/rules/quantum_computer.hpp:52-56 */
+    QD_MOD_YGATE = 3,              /* src/rules/quantum_computer.hpp:58-66 */
+    QD_MOD_ZGATE = 4,              /* src/rules/quantum_computer.hpp:68-75 */
+    QD_MOD_QCGD_STEP = 5,          /* src/rules/qcgd.hpp:443-449 */
+    QD_MOD_QCGD_REVERSED_STEP = 6  /* src/rules/qcgd.hpp:451-457 */
+} qd_modifier_kind;
+
+/* Rule descriptor.  Coefficients are computed on the host exactly as the
+ * reference constructors do: hadamard uses coef = 1/std::sqrt(2)
+ * (quantum_computer.hpp:41); QCGD rules use do_ = polar(sin(theta), phi),
+ * do_not = polar(cos(theta), xi) and their conjugates (qcgd.hpp:466-471,
+ * 541-546, 614-619). */
+typedef struct qd_rule {
+    int32_t kind;        /* qd_rule_kind */
+    uint32_t bit;        /* hadamard: qubit (byte) index */
+    double coef;         /* hadamard: 1/sqrt(2) */
+    double do_re, do_im;
+    double do_not_re, do_not_im;
+    double do_conj_re, do_conj_im;
+    double do_not_conj_re, do_not_conj_im;
+} qd_rule;
+
+typedef struct qd_modifier {
+    int32_t kind;         /* qd_modifier_kind */
+    uint32_t control_bit; /* cnot: control byte index */
+    uint32_t bit;         /* target byte index */
+    uint32_t reserved;
+} qd_modifier;
+
+/* Globals of the reference that change results (src/quids.hpp:59-75). */
+typedef struct qd_params {
+    double tolerance;           /* quids::tolerance, default 1e-30 */
+    uint32_t align_byte_length; /* quids::align_byte_length, default 8 */
+    uint32_t simple_truncation; /* quids::simple_truncation; must be 1 */
+} qd_params;
+
+/* Counters of one simulate() call (reference: sy_it.num_object and
+ * sy_it.num_object_after_interferences, src/quids.hpp:343-346;
+ * it.num_object and it.total_proba, 151-154). */
+typedef struct qd_step_stats {
+    uint64_t num_parent;        /* objects of the input state */
+    uint64_t num_parent_kept;   /* parents left after pre-truncation */
+    uint64_t num_candidates;    /* C = sy_it.num_object */
+    uint64_t num_unique;        /* U = sy_it.num_object_after_interferences */
+    uint64_t num_survivors;     /* K = next.num_object */
+    uint64_t output_bytes;      /* padded object bytes of the next state */
+    double total_proba;         /* next.total_proba (before normalisation) */
+    uint32_t dedup_levels;      /* radix partition passes used by the dedup */
+    uint32_t reserved;
+} qd_step_stats;
+
+typedef struct qd_ctx qd_ctx;
+typedef struct qd_state qd_state;
+typedef struct qd_symbolic qd_symbolic;
+
+/* ---- library ---------------------------------------------------------- */
+int qd_abi_version(void);
+const char *qd_status_string(int status);
+/* Last error message recorded on this thread (never NULL). */
+const char *qd_last_error(void);
+
+/* ---- context (no reference counterpart: owns device + stream) --------- */
+int qd_ctx_create(qd_ctx **ctx, int device);
+int qd_ctx_destroy(qd_ctx *ctx);
+int qd_ctx_synchronize(qd_ctx *ctx);
+/* Device-side bytes currently held by the context's scratch pool. */
+int qd_ctx_scratch_bytes(qd_ctx *ctx, uint64_t *bytes);
+/* Release the context's cached scratch buffers. */
+int qd_ctx_trim(qd_ctx *ctx);
+
+/* Phase timing of simulate() with HIP events on the context's stream (the
+ * reference's only instrumentation is the mid_step_function hook called at
+ * phase boundaries, src/quids.hpp:90, 454-990).  Times accumulate over calls
+ * until reset; a phase's time spans its kernels only (no host waits). */
+typedef enum qd_phase {
+    QD_PHASE_PREP = 0,      /* num_child + per-parent tables */
+    QD_PHASE_SCAN = 1,      /* child_begin prefix */
+    QD_PHASE_GENERATE = 2,  /* child hash + magnitude */
+    QD_PHASE_PARTITION = 3, /* radix passes on the hash */
+    QD_PHASE_DEDUP = 4,     /* interference + tolerance */
+    QD_PHASE_SELECT = 5,    /* top-K truncation */
+    QD_PHASE_ORDER = 6,     /* survivor ranks */
+    QD_PHASE_FINALIZE = 7,  /* sizes, offsets, survivor bytes */
+    QD_PHASE_NORMALIZE = 8, /* sum |m|^2 and scale */
+    QD_NUM_PHASES = 9
+} qd_phase;
+int qd_ctx_set_profiling(qd_ctx *ctx, int enable);
+int qd_ctx_reset_phase_times(qd_ctx *ctx);
+/* ms[QD_NUM_PHASES] accumulated GPU milliseconds, calls[QD_NUM_PHASES]. */
+int qd_ctx_phase_times(qd_ctx *ctx, double *ms, uint64_t *calls);
+
+/* ---- state: quids::it_t (src/quids.hpp:149-335) ------------------------ */
+int qd_state_create(qd_ctx *ctx, qd_state **state);
+int qd_state_destroy(qd_state *state);
+/* Replace the state's contents with N host objects.  object_begin has N+1
+ * entries (byte offsets into objects, object i spans object_size[i] bytes
+ * from object_begin[i]); magnitude holds N complex values as (re, im)
+ * pairs.  Mirrors iteration::append (src/quids.hpp:174-188) applied N times
+ * when offsets are padded to align_byte_length. */
+int qd_state_upload(qd_state *state, uint64_t num_object, const void *objects,
+                    const uint64_t *object_begin, const uint32_t *object_size,
+                    const double *magnitude, double total_proba);
+/* num_object, padded object byte length (object_begin[N]) and total_proba. */
+int qd_state_info(qd_state *state, uint64_t *num_object, uint64_t *object_bytes,
+                  double *total_proba);
+/* Copy the state to host buffers sized from qd_state_info; any pointer may be
+ * NULL to skip that array. */
+int qd_state_download(qd_state *state, void *objects, uint64_t *object_begin,
+                      uint32_t *object_size, double *magnitude);
+/* Device pointers of the state arrays (valid until the state is modified). */
+int qd_state_device_ptrs(qd_state *state, void **objects, uint64_t **object_begin,
+                         uint32_t **object_size, double **magnitude);
+
+/* ---- symbolic iteration: quids::sy_it_t (src/quids.hpp:338-429) -------- */
+int qd_symbolic_create(qd_ctx *ctx, qd_symbolic **sy);
+int qd_symbolic_destroy(qd_symbolic *sy);
+/* num_object (C) and num_object_after_interferences (U) of the last step. */
+int qd_symbolic_info(qd_symbolic *sy, uint64_t *num_object,
+                     uint64_t *num_object_after_interferences);
+
+/* ---- the step: quids::simulate(it, rule, next, sy_it, max_num_object)
+ *      (src/quids.hpp:448-543).  max_num_object = UINT64_MAX means no
+ *      limit; 0 means memory-driven truncation from free device memory
+ *      (src/quids.hpp:459-486, 510-536).  `in` and `out` must differ. */
+int qd_simulate(qd_ctx *ctx, qd_state *in, const qd_rule *rule, qd_state *out,
+                qd_symbolic *sy, uint64_t max_num_object, const qd_params *params,
+                qd_step_stats *stats);
+
+/* ---- modifier: quids::simulate(it, modifier) (src/quids.hpp:436-438,
+ *      973-980) ----------------------------------------------------------- */
+int qd_apply_modifier(qd_ctx *ctx, qd_state *state, const qd_modifier *mod);
+
+/* ---- normalisation: iteration::normalize (src/quids.hpp:985-1017) ------- */
+int qd_normalize(qd_ctx *ctx, qd_state *state, double *total_proba);
+
+/* ---- single-object rule calls on the host (the rule_t virtual methods,
+ *      src/quids.hpp:115-145), run by the same functor code as the device
+ *      kernels.  Used by the C++ header's rule classes. ------------------- */
+int qd_rule_num_child(const qd_rule *rule, const void *parent, uint32_t parent_size,
+                      uint32_t *num_child, uint32_t *max_child_size);
+int qd_rule_populate_child(const qd_rule *rule, const void *parent, uint32_t parent_size,
+                           void *child, uint32_t child_capacity, uint32_t child_id,
+                           uint32_t *child_size, double *mag /* in: parent, out: child */);
+int qd_rule_hash(const qd_rule *rule, const void *object, uint32_t size, uint64_t *hash);
+int qd_modifier_apply_host(const qd_modifier *mod, void *object, uint32_t size,
+                           double *mag);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QUIDS_HIP_H */

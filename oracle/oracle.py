@@ -1,0 +1,233 @@
+"""Python access to the CPU checker.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product (quids_amd/).
+
+* ``liboracle.so`` -- the C++ restatement of the reference step
+  (oracle/oracle.cpp), loaded with ctypes.
+* ``ref_driver`` -- the reference headers compiled in place
+  (oracle/_ref/ref_driver, built by oracle/Makefile), driven through
+  program files (format in oracle/ref_driver.cpp).
+"""
+import ctypes
+import os
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library {LIB_PATH} missing: run `make -C oracle`")
+        from quids_amd._abi import QdModifier, QdParams, QdRule, QdStepStats
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u64, u32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32
+        pu64, pu32, pd = ctypes.POINTER(u64), ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_double)
+        L.or_state_new.argtypes = [u64, vp, pu64, pu32, pd, ctypes.c_double]
+        L.or_state_new.restype = vp
+        L.or_state_free.argtypes = [vp]
+        L.or_state_info.argtypes = [vp, pu64, pu64, pd]
+        L.or_state_get.argtypes = [vp, vp, pu64, pu32, pd]
+        L.or_simulate.argtypes = [vp, ctypes.POINTER(QdRule), ctypes.POINTER(vp), u64, ctypes.POINTER(QdParams),
+                                  ctypes.POINTER(QdStepStats)]
+        L.or_modifier.argtypes = [vp, ctypes.POINTER(QdModifier)]
+        L.or_num_child.argtypes = [ctypes.POINTER(QdRule), vp, u32, pu32, pu32]
+        L.or_populate.argtypes = [ctypes.POINTER(QdRule), vp, u32, u32, vp, pu32, pd]
+        L.or_hash.argtypes = [ctypes.POINTER(QdRule), vp, u32]
+        L.or_hash.restype = u64
+        L.or_std_hash.argtypes = [vp, u64]
+        L.or_std_hash.restype = u64
+        L.or_hash_combine.argtypes = [u64, u64]
+        L.or_hash_combine.restype = u64
+        L.or_canonical_sum.argtypes = [pd, u64]
+        L.or_canonical_sum.restype = ctypes.c_double
+        L.or_average_value.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.or_average_value.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _to_or(hs):
+    L = _load()
+    mag = hs.mag_pairs()
+    return L.or_state_new(hs.num_object, hs.objects.ctypes.data_as(ctypes.c_void_p),
+                          hs.object_begin.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                          hs.object_size.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                          mag.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), hs.total_proba)
+
+
+def _from_or(h):
+    from quids_amd.state import HostState
+    L = _load()
+    n, b, t = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_double()
+    L.or_state_info(h, ctypes.byref(n), ctypes.byref(b), ctypes.byref(t))
+    objects = np.zeros(b.value, np.uint8)
+    begin = np.zeros(n.value + 1, np.uint64)
+    size = np.zeros(n.value, np.uint32)
+    mag = np.zeros((n.value, 2), np.float64)
+    L.or_state_get(h, objects.ctypes.data_as(ctypes.c_void_p), begin.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                   size.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                   mag.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return HostState(objects, begin, size, mag.view(np.complex128).reshape(-1), t.value)
+
+
+def simulate(hs, rule, max_num_object=(1 << 64) - 1, tolerance=1e-30, align=8):
+    """One oracle step on a HostState -> (HostState, stats dict)."""
+    from quids_amd._abi import QdParams, QdStepStats
+    L = _load()
+    h = _to_or(hs)
+    out = ctypes.c_void_p()
+    st = QdStepStats()
+    prm = QdParams(tolerance, align, 1)
+    d = rule.desc()
+    rc = L.or_simulate(h, ctypes.byref(d), ctypes.byref(out), int(max_num_object), ctypes.byref(prm),
+                       ctypes.byref(st))
+    L.or_state_free(h)
+    if rc:
+        raise RuntimeError(f"oracle simulate failed: {rc}")
+    res = _from_or(out)
+    L.or_state_free(out)
+    return res, st.as_dict()
+
+
+def apply_modifier(hs, mod):
+    L = _load()
+    h = _to_or(hs)
+    d = mod.desc()
+    rc = L.or_modifier(h, ctypes.byref(d))
+    res = _from_or(h)
+    L.or_state_free(h)
+    if rc:
+        raise RuntimeError(f"oracle modifier failed: {rc}")
+    return res
+
+
+def num_child(rule, obj):
+    L = _load()
+    nc, ms = ctypes.c_uint32(), ctypes.c_uint32()
+    d = rule.desc()
+    o = bytes(obj)
+    L.or_num_child(ctypes.byref(d), o, len(o), ctypes.byref(nc), ctypes.byref(ms))
+    return nc.value, ms.value
+
+
+def populate(rule, obj, child_id, mag=1.0):
+    L = _load()
+    o = bytes(obj)
+    _, ms = num_child(rule, o)
+    buf = ctypes.create_string_buffer(ms + 64)
+    sz = ctypes.c_uint32()
+    m = (ctypes.c_double * 2)(complex(mag).real, complex(mag).imag)
+    d = rule.desc()
+    L.or_populate(ctypes.byref(d), o, len(o), int(child_id), buf, ctypes.byref(sz), m)
+    return buf.raw[:sz.value], complex(m[0], m[1])
+
+
+def rule_hash(rule, obj):
+    d = rule.desc()
+    o = bytes(obj)
+    return _load().or_hash(ctypes.byref(d), o, len(o))
+
+
+def std_hash(data):
+    """std::hash<std::string_view> of the host C++ library."""
+    b = bytes(data)
+    return _load().or_std_hash(b, len(b))
+
+
+def hash_combine(seed, v):
+    return _load().or_hash_combine(seed & (2**64 - 1), v & (2**64 - 1))
+
+
+def canonical_sum(values):
+    a = np.ascontiguousarray(values, np.float64)
+    return _load().or_canonical_sum(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.shape[0])
+
+
+def average_value(hs, observable, arg=0):
+    """0 = num_nodes, 1 = particle count, 2 = [num_nodes == arg]."""
+    L = _load()
+    h = _to_or(hs)
+    v = L.or_average_value(h, int(observable), int(arg))
+    L.or_state_free(h)
+    return v
+
+
+# ---------------------------------------------------------------------------
+# reference driver
+# ---------------------------------------------------------------------------
+OP_RULE, OP_MOD = 1, 2
+
+
+def rule_op(rule, max_num_object=(1 << 64) - 1, dump=True):
+    """Program op for a rule object of quids_amd.rules."""
+    from quids_amd._abi import QD_RULE_HADAMARD
+    d = rule.desc()
+    if d.kind == QD_RULE_HADAMARD:
+        return (OP_RULE, d.kind, rule.bit, 0, 0.0, 0.0, 0.0, max_num_object, int(dump))
+    return (OP_RULE, d.kind, 0, 0, rule.theta, rule.phi, rule.xi, max_num_object, int(dump))
+
+
+def mod_op(mod, dump=False):
+    d = mod.desc()
+    return (OP_MOD, d.kind, d.control_bit, d.bit, 0.0, 0.0, 0.0, 0, int(dump))
+
+
+def write_program(path, family, hs, ops, tolerance=1e-30, align=8):
+    with open(path, "wb") as f:
+        f.write(b"QDP1")
+        f.write(struct.pack("<IIIdII", family, len(ops), align, tolerance, hs.num_object, 0))
+        for i in range(hs.num_object):
+            o, m = hs.get_object(i)
+            f.write(struct.pack("<IIdd", len(o), 0, m.real, m.imag))
+            f.write(o)
+        for op in ops:
+            f.write(struct.pack("<IIIIdddQII", *op, 0))
+
+
+def read_dumps(path):
+    """List of dicts {N, C, U, total_proba, seconds, objects, mags, hashes}."""
+    out = []
+    with open(path, "rb") as f:
+        data = f.read()
+    pos = 0
+    while pos < len(data):
+        N, C, U, total, secs, nbytes = struct.unpack_from("<QQQddQ", data, pos)
+        pos += 48
+        objs, mags, hashes = [], [], []
+        for _ in range(N):
+            sz, _pad, re, im, h = struct.unpack_from("<IIddQ", data, pos)
+            pos += 32
+            objs.append(data[pos:pos + sz])
+            pos += sz
+            mags.append(complex(re, im))
+            hashes.append(h)
+        out.append(dict(N=N, C=C, U=U, total_proba=total, seconds=secs, objects=objs,
+                        mags=np.array(mags, np.complex128), hashes=np.array(hashes, np.uint64)))
+    return out
+
+
+def run_reference(family, hs, ops, tolerance=1e-30, align=8, threads=None, timeout=600):
+    """Runs the compiled reference on a program; returns (dumps, stdout)."""
+    if not os.path.exists(REF_DRIVER):
+        raise RuntimeError(f"reference driver {REF_DRIVER} missing: run `make -C oracle ref`")
+    with tempfile.TemporaryDirectory() as d:
+        prog, out = os.path.join(d, "p.bin"), os.path.join(d, "o.bin")
+        write_program(prog, family, hs, ops, tolerance, align)
+        env = dict(os.environ)
+        if threads:
+            env["OMP_NUM_THREADS"] = str(threads)
+        r = subprocess.run([REF_DRIVER, prog, out], capture_output=True, text=True, env=env, timeout=timeout)
+        if r.returncode != 0:
+            raise RuntimeError(f"ref_driver failed ({r.returncode}): {r.stderr}")
+        return read_dumps(out), r.stdout

@@ -1,0 +1,142 @@
+"""HIP step vs the oracle, bit for bit, on the GPU (through the C ABI)."""
+import math
+
+import numpy as np
+import pytest
+
+import quids_amd as q
+from oracle import oracle as orc
+from quids_amd.rules import qcgd
+from quids_amd.rules import quantum_computer as qc
+from quids_amd.state import HostState
+
+from helpers import assert_states_identical, qcgd_initial, qcgd_iteration_rules, random_circuit
+
+pytestmark = pytest.mark.gpu
+
+U64MAX = (1 << 64) - 1
+
+
+def gpu_step(hs, rule, max_num_object=U64MAX, tolerance=1e-30, align=8):
+    q.tolerance, q.align_byte_length = tolerance, align
+    a, b = q.it_t(), q.it_t()
+    a.upload(hs)
+    sy = q.sy_it_t()
+    st = q.simulate(a, rule, b, sy, max_num_object)
+    out = b.download()
+    return out, st, sy
+
+
+def gpu_modifier(hs, mod):
+    a = q.it_t()
+    a.upload(hs)
+    q.simulate(a, mod)
+    return a.download()
+
+
+def test_hadamard_single_object(ctx):
+    hs = HostState.from_objects([bytes([1, 0, 1, 1, 0, 0, 1, 0, 1, 1, 0, 1, 0, 0, 0, 1])])
+    r = qc.hadamard(3)
+    out, st, sy = gpu_step(hs, r)
+    ref, rst = orc.simulate(hs, r)
+    assert st["num_candidates"] == 2 == sy.num_object
+    assert_states_identical(out, ref, "hadamard")
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_circuit_chain_16_qubits(ctx, seed):
+    """Config-1 style circuit (16 one-byte qubits, 24 steps), GPU and oracle
+    chained independently, compared after every step."""
+    hs_g = HostState.from_objects([bytes(16)])
+    hs_o = hs_g
+    for k, op in enumerate(random_circuit(16, 24, seed)):
+        if op.is_modifier:
+            hs_g = gpu_modifier(hs_g, op)
+            hs_o = orc.apply_modifier(hs_o, op)
+        else:
+            hs_g, st, _ = gpu_step(hs_g, op)
+            hs_o, _ = orc.simulate(hs_o, op)
+        assert_states_identical(hs_g, hs_o, f"seed {seed} op {k}")
+
+
+@pytest.mark.parametrize("nodes,seed", [(3, 1), (4, 2), (6, 7), (8, 11)])
+def test_qcgd_chain(ctx, nodes, seed):
+    """QCGD iterations (step, split_merge, erase_create), chained on each side."""
+    hs_g = hs_o = qcgd_initial(nodes, seed)
+    rules = qcgd_iteration_rules()
+    for it in range(3):
+        for r in rules:
+            if r.is_modifier:
+                hs_g = gpu_modifier(hs_g, r)
+                hs_o = orc.apply_modifier(hs_o, r)
+            else:
+                hs_g, st, _ = gpu_step(hs_g, r)
+                hs_o, ost = orc.simulate(hs_o, r)
+                assert st["num_candidates"] == ost["num_candidates"]
+                assert st["num_unique"] == ost["num_unique"]
+                assert_states_identical(hs_g, hs_o, f"nodes {nodes} it {it} {type(r).__name__}")
+
+
+@pytest.mark.parametrize("cap", [1, 7, 100, 1000])
+def test_qcgd_truncation(ctx, cap):
+    """Top-K truncation with ties, and parent pre-truncation."""
+    hs = qcgd_initial(6, 3)
+    rules = qcgd_iteration_rules()
+    hs_g = hs_o = hs
+    for it in range(3):
+        for r in rules:
+            if r.is_modifier:
+                hs_g = gpu_modifier(hs_g, r)
+                hs_o = orc.apply_modifier(hs_o, r)
+            else:
+                hs_g, st, _ = gpu_step(hs_g, r, max_num_object=cap)
+                hs_o, _ = orc.simulate(hs_o, r, max_num_object=cap)
+                assert_states_identical(hs_g, hs_o, f"cap {cap} it {it}")
+    # parent pre-truncation: a state larger than the cap
+    big = hs_o
+    if big.num_object > 2:
+        k = big.num_object // 2
+        g, _, _ = gpu_step(big, rules[1], max_num_object=k)
+        o, _ = orc.simulate(big, rules[1], max_num_object=k)
+        assert_states_identical(g, o, "pre-truncation")
+
+
+def test_coin_and_tolerance(ctx):
+    hs = qcgd_initial(7, 5)
+    r1 = qcgd.coin(math.pi / 7, 0.3, 0.2)
+    g, _, _ = gpu_step(hs, r1, tolerance=1e-15)
+    o, _ = orc.simulate(hs, r1, tolerance=1e-15)
+    assert_states_identical(g, o, "coin")
+    r2 = qcgd.split_merge(0.25, 0.25, -0.25)
+    g2, _, _ = gpu_step(g, r2, tolerance=1e-15)
+    o2, _ = orc.simulate(o, r2, tolerance=1e-15)
+    assert_states_identical(g2, o2, "split_merge xi")
+
+
+def test_larger_step_two_level_partition(ctx):
+    """A step with > 4M candidates, so the dedup takes two partition passes."""
+    hs = qcgd_initial(10, 4)
+    rules = qcgd_iteration_rules()
+    hs_g = hs_o = hs
+    done = False
+    for it in range(4):
+        for r in rules:
+            if r.is_modifier:
+                hs_g = gpu_modifier(hs_g, r)
+                hs_o = orc.apply_modifier(hs_o, r)
+                continue
+            hs_g, st, _ = gpu_step(hs_g, r, max_num_object=1 << 16)
+            hs_o, _ = orc.simulate(hs_o, r, max_num_object=1 << 16)
+            assert_states_identical(hs_g, hs_o, f"it {it}")
+            if st["dedup_levels"] == 2:
+                done = True
+                break
+        if done:
+            break
+    assert done, "workload never reached two partition levels"
+
+
+def test_empty_state(ctx):
+    hs = HostState()
+    out, st, _ = gpu_step(hs, qcgd.split_merge(1.0))
+    assert out.num_object == 0 and st["num_candidates"] == 0 and out.total_proba == 0
