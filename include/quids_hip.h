@@ -136,6 +136,19 @@ int qd_ctx_reset_phase_times(qd_ctx *ctx);
 /* ms[QD_NUM_PHASES] accumulated GPU milliseconds, calls[QD_NUM_PHASES]. */
 int qd_ctx_phase_times(qd_ctx *ctx, double *ms, uint64_t *calls);
 
+/* Launch / layout parameters of the step (no reference counterpart; the
+ * autotuner in quids_amd.tuning picks them).  Zero fields keep defaults. */
+typedef struct qd_tuning {
+    uint32_t dedup_bucket_target; /* mean candidates per final dedup bucket, 64..dedup_cap/2 (default 768) */
+    uint32_t first_pass_bits;     /* max bits of the first radix pass, 1..12 (default 12) */
+    uint32_t generate_block;      /* threads per generate block: 64, 128, 256, 512 or 1024 (default 256) */
+    uint32_t generate_grid;       /* max generate blocks (default 2048) */
+    uint32_t dedup_cap;           /* LDS bucket capacity of the dedup (2048; larger buckets take the global-memory path) */
+    uint32_t reserved[3];
+} qd_tuning;
+int qd_ctx_get_tuning(qd_ctx *ctx, qd_tuning *tuning);
+int qd_ctx_set_tuning(qd_ctx *ctx, const qd_tuning *tuning);
+
 /* ---- state: quids::it_t (src/quids.hpp:149-335) ------------------------ */
 int qd_state_create(qd_ctx *ctx, qd_state **state);
 int qd_state_destroy(qd_state *state);

@@ -25,9 +25,11 @@
 //   * representative of a hash group = smallest candidate index (the
 //     reference keeps the first one inserted, src/quids.hpp:799-806, whose
 //     identity depends on the bucket/thread layout);
-//   * merged magnitude = sum of the group's magnitudes in ascending candidate
-//     index order, starting from the first (the reference sums in insertion
-//     order);
+//   * merged magnitude = pairwise sum of the group's magnitudes taken in
+//     ascending candidate index order: adjacent pairs are added level by
+//     level ((v0+v1), (v2+v3), ..., an odd last term carried up), which for
+//     one to three members is the plain left-to-right sum (the reference sums
+//     in insertion order);
 //   * top-K truncation orders by |m|^2 descending, then hash ascending (the
 //     reference's nth_element breaks ties arbitrarily, src/quids.hpp:884-889);
 //     parent pre-truncation breaks ties by ascending object index;
@@ -482,6 +484,17 @@ static double canonical_sum(std::vector<double> v) {
     return v[0];
 }
 
+// Pairwise sum in the given order: level by level, adjacent pairs are
+// added and an odd last term is carried to the next level.
+static cplx pairwise_sum(std::vector<cplx> v) {
+    while (v.size() > 1) {
+        std::vector<cplx> w((v.size() + 1) / 2);
+        for (size_t j = 0; j < w.size(); ++j) w[j] = 2 * j + 1 < v.size() ? cadd(v[2 * j], v[2 * j + 1]) : v[2 * j];
+        v.swap(w);
+    }
+    return v.empty() ? cplx{0, 0} : v[0];
+}
+
 struct State {
     std::vector<u8> objects;
     std::vector<u64> begin{0};
@@ -585,14 +598,19 @@ static int simulate(const State &in, const qd_rule &rule, State &out, u64 max_nu
 #pragma omp parallel for schedule(dynamic, 4)
     for (size_t b = 0; b < B; ++b) {
         std::unordered_map<u64, size_t> idx;
+        std::vector<std::vector<cplx>> members;
         auto &v = per_bucket[b];
         idx.reserve(bcount[b + 1] - bcount[b]);
         for (u64 k = bcount[b]; k < bcount[b + 1]; ++k) {
             const u64 c = order[k];
             auto [it, fresh] = idx.try_emplace(H[c], v.size());
-            if (fresh) v.push_back({H[c], c, M[c]});
-            else v[it->second].mag = cadd(v[it->second].mag, M[c]);
+            if (fresh) {
+                v.push_back({H[c], c, M[c]});
+                members.emplace_back();
+            }
+            members[it->second].push_back(M[c]);  // ascending candidate order
         }
+        for (size_t g = 0; g < v.size(); ++g) v[g].mag = pairwise_sum(members[g]);
     }
     std::vector<Uniq> uniq;
     for (auto &v : per_bucket)

@@ -49,6 +49,20 @@ class Context:
     def trim(self):
         check(lib().qd_ctx_trim(self.handle), "qd_ctx_trim")
 
+    def get_tuning(self):
+        t = _abi.QdTuning()
+        check(lib().qd_ctx_get_tuning(self.handle, ctypes.byref(t)), "qd_ctx_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        """Sets launch / layout parameters (fields of qd_tuning); others keep their value."""
+        t = _abi.QdTuning()
+        for k, v in kw.items():
+            if not hasattr(t, k) or k == "reserved":
+                raise KeyError(k)
+            setattr(t, k, int(v))
+        check(lib().qd_ctx_set_tuning(self.handle, ctypes.byref(t)), "qd_ctx_set_tuning")
+
     def set_profiling(self, enable=True):
         """Per-phase GPU timing of simulate() with HIP events."""
         check(lib().qd_ctx_set_profiling(self.handle, 1 if enable else 0), "qd_ctx_set_profiling")

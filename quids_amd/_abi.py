@@ -83,6 +83,20 @@ class QdStepStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
+class QdTuning(ctypes.Structure):
+    _fields_ = [
+        ("dedup_bucket_target", ctypes.c_uint32),
+        ("first_pass_bits", ctypes.c_uint32),
+        ("generate_block", ctypes.c_uint32),
+        ("generate_grid", ctypes.c_uint32),
+        ("dedup_cap", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 3),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
 _vp = ctypes.c_void_p
 _u32 = ctypes.c_uint32
 _u64 = ctypes.c_uint64
@@ -102,6 +116,8 @@ SIGNATURES = {
     "qd_ctx_scratch_bytes": [_vp, _pu64],
     "qd_ctx_trim": [_vp],
     "qd_ctx_set_profiling": [_vp, ctypes.c_int],
+    "qd_ctx_get_tuning": [_vp, ctypes.POINTER(QdTuning)],
+    "qd_ctx_set_tuning": [_vp, ctypes.POINTER(QdTuning)],
     "qd_ctx_reset_phase_times": [_vp],
     "qd_ctx_phase_times": [_vp, _pd, _pu64],
     "qd_state_create": [_vp, _pvp],

@@ -118,6 +118,7 @@ struct qd_ctx {
     hipEvent_t ev[2 * QD_NUM_PHASES] = {};
     bool pending[QD_NUM_PHASES] = {};
     double phase_ms[QD_NUM_PHASES] = {};
+    qd_tuning tune = {768, 12, 256, 2048, 2048, {0, 0, 0}};
     u64 phase_calls[QD_NUM_PHASES] = {};
     u64 *host = nullptr;  // pinned readback area, 8192 u64
     // scratch of the step
@@ -127,9 +128,11 @@ struct qd_ctx {
     DBuf nhist, counters;
     DBuf sel0, sel1;
     DBuf bitmap, wprefix, padded, chunks, scanblk;
+    DBuf biglist, bigsoff, bigkey, bigcand, bigmag, bigseg;
     std::vector<DBuf *> all() {
         return {&nc, &cb, &pre, &tab, &kept, &keys, &ebuf0, &ebuf1, &hist1, &off1, &cur1, &off2,
-                &nhist, &counters, &sel0, &sel1, &bitmap, &wprefix, &padded, &chunks, &scanblk};
+                &nhist, &counters, &sel0, &sel1, &bitmap, &wprefix, &padded, &chunks, &scanblk,
+                &biglist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg};
     }
 };
 
@@ -335,7 +338,7 @@ __device__ inline u64 find_parent(const u64 *cb, u64 n, u64 c) {
 constexpr int kHistBins = 4096;  // up to 12 bits of LDS histogram
 
 template <int KIND>
-__global__ __launch_bounds__(256) void k_generate(Parents P, const u64 *cb, qd_rule rule, const u64 *pre,
+__global__ __launch_bounds__(1024) void k_generate(Parents P, const u64 *cb, qd_rule rule, const u64 *pre,
                                                   const u64 *tab, u64 C, Entry *E, u32 b1, u64 *hist1) {
     __shared__ u32 hist[kHistBins];
     if (b1)
@@ -470,155 +473,336 @@ __global__ __launch_bounds__(1024) void k_level2(const Entry *in, const u64 *off
 // ===========================================================================
 // phase 5: per-bucket dedup in LDS
 // ===========================================================================
-constexpr int kDedupThreads = 512;
-constexpr int kDedupCap = 4096;   // entries per bucket held in LDS
-constexpr int kDedupBins = 4096;  // direct-addressed bins on the next 12 hash bits
-constexpr int kDedupPer = kDedupCap / kDedupThreads;
-constexpr int kDedupBinsPer = kDedupBins / kDedupThreads;
+// A persistent grid walks the buckets.  A bucket's entries are staged in LDS
+// and bitonic-sorted on (hash, candidate index), so each hash group becomes a
+// run in ascending candidate order: its first candidate is the
+// representative and a segmented pairwise tree sums its magnitudes (adjacent
+// pairs level by level, the oracle's canonical order).  Groups with
+// |m|^2 > tolerance are written compactly at the start of the bucket's own
+// slot range in the output; the rest of the range gets the sentinel
+// representative ~0, so no global atomics order the output.  Buckets larger
+// than the LDS capacity are listed for k_dedup_big.
+constexpr int kNormBins = 2048;  // |m|^2 exponent histogram (the sign bit is 0)
+constexpr u64 kNoRep = ~0ull;
 
 __device__ inline u32 norm_top12(double nrm) { return u32(__double_as_longlong(nrm) >> 52) & 0xFFF; }
 
-// Sums the group of entries with key == key[first] in bin [s, e) of perm, in
-// ascending candidate order; returns the representative (smallest cand).
-__device__ inline u64 group_sum(const u64 *key, const u64 *cand, const u16 *perm, u32 a, u32 e, const Entry *E,
-                                u64 beg, cplx &acc) {
-    const u64 ka = key[perm[a]];
-    acc = cplx{-0.0, -0.0};
-    u64 prev = 0, rep = 0;
-    bool have = false;
-    for (;;) {
-        u64 best = ~0ull;
-        u32 bidx = 0;
-        bool found = false;
-        for (u32 q = a; q < e; ++q) {
-            const u32 iq = perm[q];
-            if (key[iq] != ka) continue;
-            const u64 cq = cand[iq];
-            if ((!have || cq > prev) && (!found || cq < best)) {
-                best = cq;
-                bidx = iq;
-                found = true;
-            }
-        }
-        if (!found) break;
-        if (!have) rep = best;
-        const Entry &x = E[beg + bidx];
-        acc.re = acc.re + x.re;
-        acc.im = acc.im + x.im;
-        prev = best;
-        have = true;
+__device__ inline bool key_gt(u64 ka, u64 ca, u64 kb, u64 cb) { return ka > kb || (ka == kb && ca > cb); }
+
+// Exclusive max-scan of one value per thread over the block.
+__device__ inline u32 block_excl_max(u32 v, u64 *tmp) {
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    u32 inc = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const u32 o = __shfl_up(inc, d, kWave);
+        if (lane >= d) inc = max(inc, o);
     }
-    return rep;
+    const u32 exc_w = __shfl_up(inc, 1, kWave);
+    if (lane == kWave - 1) tmp[wid] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u64 m = 0;
+        for (int w = 0; w < nw; ++w) {
+            const u64 t = tmp[w];
+            tmp[w] = m;
+            m = max(m, t);
+        }
+    }
+    __syncthreads();
+    const u32 r = max(u32(tmp[wid]), lane ? exc_w : 0u);
+    __syncthreads();
+    return r;
 }
 
-__global__ __launch_bounds__(kDedupThreads) void k_dedup(const Entry *E, const u64 *boff, u32 L, double tol,
-                                                         u64 *UH, u64 *UR, double2 *UM, u64 *counters, u64 *nhist,
-                                                         u32 *err) {
-    __shared__ u64 key[kDedupCap];
-    __shared__ u64 cand[kDedupCap];
-    __shared__ u16 perm[kDedupCap];
-    __shared__ u32 bcnt[kDedupBins];
-    __shared__ u32 bstart[kDedupBins + 1];
-    __shared__ u32 nh[kHistBins];
-    __shared__ u64 tmp[kDedupThreads / kWave + 1];
-    __shared__ u64 ubase;
-    const u64 b = blockIdx.x;
-    const u64 beg = boff[b];
-    const u32 n = u32(min<u64>(boff[b + 1] - beg, 0xFFFFFFFFull));
-    if (n == 0) return;
-    if (n > kDedupCap) {
-        if (threadIdx.x == 0) atomicOr(err, ERR_OVERFLOW);
-        return;
-    }
+// Per bucket: entries go into an LDS open-addressing table keyed by hash
+// (home slot from the 12 hash bits below the bucket bits, linear probing),
+// which names each group by its slot.  A counting sort on the slot makes each
+// group contiguous; each member then finds its place in ascending candidate
+// order by counting the group members with a smaller candidate index.
+// perm lists the bucket in (group, candidate) order; magnitudes stay in
+// place and are reached through perm.
+template <int T, int CAP, int TS>
+__global__ __launch_bounds__(T) void k_dedup(const Entry *E, const u64 *boff, u64 nbk, u32 L, double tol, u64 *UH,
+                                             u64 *UR, double2 *UM, u64 *counters, u64 *nhist, u64 *biglist) {
+    constexpr int PER = CAP / T;
+    constexpr int SPER = TS / T;
+    __shared__ u64 key[CAP];
+    __shared__ u64 cand[CAP];
+    __shared__ double2 mag[CAP];
+    __shared__ u32 owner[TS];       // entry index + 1 of the slot's first entry, 0 = empty
+    __shared__ u32 gstart[TS + 1];  // group sizes, then their exclusive prefix
+    __shared__ u16 gslot[CAP];      // group slot of each entry
+    __shared__ u16 perm0[CAP];      // entries grouped by slot
+    __shared__ u16 perm[CAP];       // entries in (group, candidate) order
+    __shared__ u32 nh[kNormBins];
+    __shared__ u64 tmp[T / kWave + 1];
+    __shared__ u32 maxg;
     const u32 shift = (L + 12 <= 64) ? 64 - L - 12 : 0;
-    for (int i = threadIdx.x; i < kDedupBins; i += kDedupThreads) bcnt[i] = 0;
-    for (int i = threadIdx.x; i < kHistBins; i += kDedupThreads) nh[i] = 0;
-    __syncthreads();
-    u32 rank[kDedupPer];
+    for (int i = threadIdx.x; i < kNormBins; i += T) nh[i] = 0;
+    for (int i = threadIdx.x; i < TS; i += T) owner[i] = 0;
+    for (int i = threadIdx.x; i <= TS; i += T) gstart[i] = 0;
+    u64 uniques = 0;  // thread 0's running total
+
+    Entry x[PER];
+    u64 nbeg = 0, nnum = 0;
+    auto fetch = [&](u64 bb) {
+        nbeg = boff[bb];
+        nnum = boff[bb + 1] - nbeg;
+        if (nnum <= CAP) {
 #pragma unroll
-    for (int k = 0; k < kDedupPer; ++k) {
-        const u32 i = threadIdx.x + kDedupThreads * k;
-        if (i < n) {
-            const Entry &x = E[beg + i];
-            key[i] = x.hash;
-            cand[i] = x.cand;
-            rank[k] = atomicAdd(&bcnt[u32(x.hash >> shift) & (kDedupBins - 1)], 1u);
-        }
-    }
-    __syncthreads();
-    {
-        u32 v[kDedupBinsPer];
-        u64 s = 0;
-        for (int k = 0; k < kDedupBinsPer; ++k) {
-            v[k] = bcnt[threadIdx.x * kDedupBinsPer + k];
-            s += v[k];
-        }
-        u64 total;
-        u64 run = block_excl_scan(s, tmp, total);
-        for (int k = 0; k < kDedupBinsPer; ++k) {
-            bstart[threadIdx.x * kDedupBinsPer + k] = u32(run);
-            run += v[k];
-        }
-        if (threadIdx.x == 0) bstart[kDedupBins] = u32(total);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kDedupPer; ++k) {
-        const u32 i = threadIdx.x + kDedupThreads * k;
-        if (i < n) perm[bstart[u32(key[i] >> shift) & (kDedupBins - 1)] + rank[k]] = u16(i);
-    }
-    __syncthreads();
-    // pass A: count surviving groups of this thread's bins
-    u32 mine = 0;
-    for (int k = 0; k < kDedupBinsPer; ++k) {
-        const u32 bin = threadIdx.x * kDedupBinsPer + k;
-        const u32 s = bstart[bin], e = bstart[bin + 1];
-        for (u32 a = s; a < e; ++a) {
-            const u64 ka = key[perm[a]];
-            bool seen = false;
-            for (u32 q = s; q < a; ++q)
-                if (key[perm[q]] == ka) {
-                    seen = true;
-                    break;
-                }
-            if (seen) continue;
-            cplx acc;
-            group_sum(key, cand, perm, a, e, E, beg, acc);
-            if (cnorm(acc) > tol) ++mine;
-        }
-    }
-    u64 total;
-    u64 off = block_excl_scan(mine, tmp, total);
-    if (threadIdx.x == 0) ubase = total ? atomicAdd(reinterpret_cast<unsigned long long *>(counters), total) : 0;
-    __syncthreads();
-    u64 w = ubase + off;
-    for (int k = 0; k < kDedupBinsPer; ++k) {
-        const u32 bin = threadIdx.x * kDedupBinsPer + k;
-        const u32 s = bstart[bin], e = bstart[bin + 1];
-        for (u32 a = s; a < e; ++a) {
-            const u64 ka = key[perm[a]];
-            bool seen = false;
-            for (u32 q = s; q < a; ++q)
-                if (key[perm[q]] == ka) {
-                    seen = true;
-                    break;
-                }
-            if (seen) continue;
-            cplx acc;
-            const u64 rep = group_sum(key, cand, perm, a, e, E, beg, acc);
-            const double nrm = cnorm(acc);
-            if (nrm > tol) {
-                UH[w] = ka;
-                UR[w] = rep;
-                UM[w] = make_double2(acc.re, acc.im);
-                atomicAdd(&nh[norm_top12(nrm)], 1u);
-                ++w;
+            for (int k = 0; k < PER; ++k) {
+                const u32 i = threadIdx.x + T * k;
+                if (i < nnum) x[k] = E[nbeg + i];
             }
         }
+    };
+    if (blockIdx.x < nbk) fetch(blockIdx.x);
+    __syncthreads();
+    for (u64 b = blockIdx.x; b < nbk; b += gridDim.x) {
+        const u64 beg = nbeg, nn = nnum;  // uniform over the block
+        if (nn > CAP) {
+            if (threadIdx.x == 0) biglist[atomicAdd(reinterpret_cast<unsigned long long *>(counters + 5), 1ull)] = b;
+            __syncthreads();
+            if (b + gridDim.x < nbk) fetch(b + gridDim.x);
+            continue;
+        }
+        const u32 n = u32(nn);
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const u32 i = threadIdx.x + T * k;
+            if (i < n) {
+                key[i] = x[k].hash;
+                cand[i] = x[k].cand;
+                mag[i] = make_double2(x[k].re, x[k].im);
+            }
+        }
+        if (threadIdx.x == 0) maxg = 1;
+        __syncthreads();
+        if (b + gridDim.x < nbk) fetch(b + gridDim.x);
+        if (n == 0) continue;  // uniform
+        // insert: find or claim the group slot of every entry
+        u32 rank[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const u32 i = threadIdx.x + T * k;
+            if (i < n) {
+                const u64 h = key[i];
+                u32 sl = u32(h >> shift) & (TS - 1);
+                for (;;) {
+                    const u32 o = atomicCAS(&owner[sl], 0u, i + 1);
+                    if (o == 0 || key[o - 1] == h) break;
+                    sl = (sl + 1) & (TS - 1);
+                }
+                gslot[i] = u16(sl);
+                rank[k] = atomicAdd(&gstart[sl], 1u);
+            }
+        }
+        __syncthreads();
+        // group sizes -> offsets (exclusive scan over the slots)
+        {
+            u32 v[SPER];
+            u64 sum = 0;
+#pragma unroll
+            for (int k = 0; k < SPER; ++k) {
+                v[k] = gstart[threadIdx.x * SPER + k];
+                sum += v[k];
+                if (v[k] > 1) atomicMax(&maxg, v[k]);
+            }
+            u64 total;
+            u64 run = block_excl_scan(sum, tmp, total);
+#pragma unroll
+            for (int k = 0; k < SPER; ++k) {
+                gstart[threadIdx.x * SPER + k] = u32(run);
+                run += v[k];
+            }
+            if (threadIdx.x == 0) gstart[TS] = u32(total);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const u32 i = threadIdx.x + T * k;
+            if (i < n) perm0[gstart[gslot[i]] + rank[k]] = u16(i);
+        }
+        __syncthreads();
+        // place each member by the number of smaller candidates in its group
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const u32 i = threadIdx.x + T * k;
+            if (i < n) {
+                const u32 sl = gslot[i];
+                const u32 g0 = gstart[sl], g1 = gstart[sl + 1];
+                const u64 ci = cand[i];
+                u32 r = 0;
+                for (u32 q = g0; q < g1; ++q) r += cand[perm0[q]] < ci;
+                perm[g0 + r] = u16(i);
+            }
+        }
+        __syncthreads();
+        // segmented pairwise tree over sorted positions (values live at perm[])
+        const u32 mg = maxg;
+        for (u32 st = 1; st < mg; st <<= 1) {
+            for (u32 p = threadIdx.x; p < n; p += T) {
+                const u32 ia = perm[p];
+                const u32 g0 = gstart[gslot[ia]];
+                const u32 gsz = gstart[gslot[ia] + 1] - g0;
+                const u32 r = p - g0;
+                if ((r & (2 * st - 1)) == 0 && r + st < gsz) {
+                    const u32 ib = perm[p + st];
+                    const double2 a = mag[ia], c = mag[ib];
+                    mag[ia] = make_double2(a.x + c.x, a.y + c.y);
+                }
+            }
+            __syncthreads();
+        }
+        // compact the kept groups (their first sorted member) to the front of
+        // the bucket's slot range
+        u32 cnt = 0;
+        u32 keepbits = 0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const u32 p = threadIdx.x * PER + k;
+            if (p < n) {
+                const u32 ia = perm[p];
+                if (gstart[gslot[ia]] == p) {
+                    const double2 a = mag[ia];
+                    const double nrm = a.x * a.x + a.y * a.y;
+                    if (nrm > tol) {
+                        keepbits |= 1u << k;
+                        ++cnt;
+                        atomicAdd(&nh[min(norm_top12(nrm), u32(kNormBins - 1))], 1u);
+                    }
+                }
+            }
+        }
+        u64 total;
+        u64 pos = beg + block_excl_scan(cnt, tmp, total);
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            if (keepbits & (1u << k)) {
+                const u32 pi = perm[threadIdx.x * PER + k];
+                UH[pos] = key[pi];
+                UR[pos] = cand[pi];
+                UM[pos] = mag[pi];
+                ++pos;
+            }
+        }
+        for (u64 jx = beg + total + threadIdx.x; jx < beg + n; jx += T) UR[jx] = kNoRep;
+        if (threadIdx.x == 0) uniques += total;
+        __syncthreads();
+        // clear the table: owners of the slots this bucket used, and every
+        // group offset (the scan wrote all of them)
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const u32 i = threadIdx.x + T * k;
+            if (i < n) owner[gslot[i]] = 0;
+        }
+        for (int i = threadIdx.x; i <= TS; i += T) gstart[i] = 0;
+        __syncthreads();
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < kHistBins; i += kDedupThreads)
+    for (int i = threadIdx.x; i < kNormBins; i += T)
         if (nh[i]) atomicAdd(reinterpret_cast<unsigned long long *>(nhist + i), (unsigned long long)nh[i]);
+    if (threadIdx.x == 0 && uniques) atomicAdd(reinterpret_cast<unsigned long long *>(counters), uniques);
+}
+
+// The same computation for buckets beyond the LDS capacity, one workgroup
+// per bucket, on a global scratch region (rare: a bucket this large needs a
+// group of thousands of candidates with one hash).  Scratch of listed bucket
+// q starts at soff[q] elements: keys, cands, mags (npad each) and u32 segs.
+__global__ __launch_bounds__(1024) void k_dedup_big(const Entry *E, const u64 *boff, const u64 *biglist,
+                                                    const u64 *soff, u64 *skeyg, u64 *scandg, double2 *smagg,
+                                                    u32 *segg, double tol, u64 *UH, u64 *UR, double2 *UM,
+                                                    u64 *counters, u64 *nhist) {
+    __shared__ u32 maxg;
+    __shared__ u64 tmp[1024 / kWave + 1];
+    const u64 b = biglist[blockIdx.x];
+    const u64 beg = boff[b], n = boff[b + 1] - beg;
+    u64 *skey = skeyg + soff[blockIdx.x];
+    u64 *scand = scandg + soff[blockIdx.x];
+    double2 *smag = smagg + soff[blockIdx.x];
+    u32 *seg = segg + soff[blockIdx.x];
+    u64 npad = 2;
+    while (npad < n) npad <<= 1;
+    for (u64 i = threadIdx.x; i < npad; i += blockDim.x) {
+        if (i < n) {
+            skey[i] = E[beg + i].hash;
+            scand[i] = E[beg + i].cand;
+        } else {
+            skey[i] = ~0ull;
+            scand[i] = ~0ull;
+        }
+    }
+    if (threadIdx.x == 0) maxg = 1;
+    __syncthreads();
+    for (u64 k = 2; k <= npad; k <<= 1) {
+        for (u64 j = k >> 1; j > 0; j >>= 1) {
+            for (u64 t = threadIdx.x; t < npad / 2; t += blockDim.x) {
+                const u64 i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+                const u64 p = i | j;
+                const u64 ka = skey[i], ca = scand[i], kb = skey[p], cb = scand[p];
+                if (key_gt(ka, ca, kb, cb) == ((i & k) == 0)) {
+                    skey[i] = kb;
+                    scand[i] = cb;
+                    skey[p] = ka;
+                    scand[p] = ca;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // magnitudes: candidate c of this bucket is found by its position in the
+    // input (entries carry their candidate index, not their position), so
+    // re-read them by a search over the sorted (hash, cand) pairs
+    for (u64 i = threadIdx.x; i < n; i += blockDim.x) {
+        const Entry e = E[beg + i];
+        u64 lo = 0, hi = n - 1;
+        while (lo < hi) {
+            const u64 mid = (lo + hi) >> 1;
+            if (key_gt(e.hash, e.cand, skey[mid], scand[mid])) lo = mid + 1;
+            else hi = mid;
+        }
+        smag[lo] = make_double2(e.re, e.im);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u32 run = 0;
+        for (u64 i = 0; i < n; ++i) {
+            if (i == 0 || skey[i] != skey[i - 1]) run = u32(i);
+            seg[i] = run;
+            if (i + 1 - run > maxg) maxg = u32(i + 1 - run);
+        }
+    }
+    __syncthreads();
+    for (u64 st = 1; st < maxg; st <<= 1) {
+        for (u64 i = threadIdx.x; i < n; i += blockDim.x) {
+            const u64 r = i - seg[i];
+            if ((r & (2 * st - 1)) == 0 && i + st < n && seg[i + st] == seg[i]) {
+                const double2 a = smag[i], c = smag[i + st];
+                smag[i] = make_double2(a.x + c.x, a.y + c.y);
+            }
+        }
+        __syncthreads();
+    }
+    // serial compaction by thread 0 (this path is rare)
+    if (threadIdx.x == 0) {
+        u64 pos = beg;
+        for (u64 i = 0; i < n; ++i) {
+            if (seg[i] != i) continue;
+            const double2 a = smag[i];
+            const double nrm = a.x * a.x + a.y * a.y;
+            if (nrm > tol) {
+                UH[pos] = skey[i];
+                UR[pos] = scand[i];
+                UM[pos] = a;
+                ++pos;
+                atomicAdd(reinterpret_cast<unsigned long long *>(nhist + min(norm_top12(nrm), u32(kNormBins - 1))),
+                          1ull);
+            }
+        }
+        atomicAdd(reinterpret_cast<unsigned long long *>(counters), (unsigned long long)(pos - beg));
+        for (u64 j = pos; j < beg + n; ++j) UR[j] = kNoRep;
+    }
 }
 
 // ===========================================================================
@@ -636,21 +820,62 @@ __device__ inline u32 key_digit(u64 hi, u64 lo, u32 pos) {
     return u32(top >> 52);
 }
 
+// Appends for the lanes with pred set: one atomic per wave; returns this
+// lane's slot (meaningful only when pred).
+__device__ inline u64 wave_append(bool pred, u64 *counter) {
+    const u64 mask = __ballot(pred);
+    if (!mask) return 0;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    u64 base = 0;
+    if (lane == leader) base = atomicAdd(reinterpret_cast<unsigned long long *>(counter), (unsigned long long)__popcll(mask));
+    base = __shfl(base, leader, kWave);
+    return base + __popcll(mask & ((1ull << lane) - 1));
+}
+
 __device__ inline void mark(u64 *bitmap, u64 rep) {
     atomicOr(reinterpret_cast<unsigned long long *>(bitmap + (rep >> 6)), 1ull << (rep & 63));
 }
 
-__global__ __launch_bounds__(256) void k_sel_first(const u64 *UH, const u64 *UR, const double2 *UM, u64 U, u32 d1,
-                                                   SelItem *S, u64 *scount, u64 *bitmap) {
+// First select stage over the dedup output slots: histogram of |m|^2 bits
+// 51..40 among the uniques whose bits 63..52 equal d1.
+__global__ __launch_bounds__(256) void k_sel_hist_u(const u64 *UR, const double2 *UM, u64 U, u32 d1, u64 *hist) {
+    __shared__ u32 h[kHistBins];
+    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) h[i] = 0;
+    __syncthreads();
     for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < U; i += u64(gridDim.x) * blockDim.x) {
+        if (UR[i] == kNoRep) continue;
         const double2 m = UM[i];
-        const double nrm = m.x * m.x + m.y * m.y;
-        const u32 t = norm_top12(nrm);
-        if (t > d1) mark(bitmap, UR[i]);
-        else if (t == d1) {
-            const u64 at = atomicAdd(reinterpret_cast<unsigned long long *>(scount), 1ull);
-            S[at] = SelItem{u64(__double_as_longlong(nrm)), ~UH[i], UR[i]};
+        const u64 bits = u64(__double_as_longlong(m.x * m.x + m.y * m.y));
+        if (u32(bits >> 52) == d1) atomicAdd(&h[u32(bits >> 40) & 0xFFF], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x)
+        if (h[i]) atomicAdd(reinterpret_cast<unsigned long long *>(hist + i), (unsigned long long)h[i]);
+}
+
+// Marks the uniques whose top 24 |m|^2 bits exceed prefix24 and collects
+// the ones equal to it.
+__global__ __launch_bounds__(256) void k_sel_first(const u64 *UH, const u64 *UR, const double2 *UM, u64 U,
+                                                   u32 prefix24, SelItem *S, u64 *scount, u64 *bitmap) {
+    const u64 stride = u64(gridDim.x) * blockDim.x;
+    // uniform trip count per wave so that wave_append sees every lane
+    for (u64 base = u64(blockIdx.x) * blockDim.x; base < U; base += stride) {
+        const u64 i = base + threadIdx.x;
+        bool take = false;
+        SelItem it{};
+        if (i < U && UR[i] != kNoRep) {
+            const double2 m = UM[i];
+            const u64 bits = u64(__double_as_longlong(m.x * m.x + m.y * m.y));
+            const u32 t = u32(bits >> 40);
+            if (t > prefix24) mark(bitmap, UR[i]);
+            else if (t == prefix24) {
+                take = true;
+                it = SelItem{bits, ~UH[i], UR[i]};
+            }
         }
+        const u64 at = wave_append(take, scount);
+        if (take) S[at] = it;
     }
 }
 
@@ -667,11 +892,19 @@ __global__ __launch_bounds__(256) void k_sel_hist(const SelItem *S, u64 n, u32 p
 
 __global__ __launch_bounds__(256) void k_sel_split(const SelItem *S, u64 n, u32 pos, u32 d, SelItem *out,
                                                    u64 *ocount, u64 *bitmap) {
-    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x) {
-        const SelItem s = S[i];
-        const u32 t = key_digit(s.hi, s.lo, pos);
-        if (t > d) mark(bitmap, s.rep);
-        else if (t == d) out[atomicAdd(reinterpret_cast<unsigned long long *>(ocount), 1ull)] = s;
+    const u64 stride = u64(gridDim.x) * blockDim.x;
+    for (u64 base = u64(blockIdx.x) * blockDim.x; base < n; base += stride) {
+        const u64 i = base + threadIdx.x;
+        bool take = false;
+        SelItem it{};
+        if (i < n) {
+            it = S[i];
+            const u32 t = key_digit(it.hi, it.lo, pos);
+            if (t > d) mark(bitmap, it.rep);
+            else take = t == d;
+        }
+        const u64 at = wave_append(take, ocount);
+        if (take) out[at] = it;
     }
 }
 
@@ -682,7 +915,7 @@ __global__ __launch_bounds__(256) void k_sel_mark_items(const SelItem *S, u64 n,
 
 __global__ __launch_bounds__(256) void k_mark_all(const u64 *UR, u64 U, u64 *bitmap) {
     for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < U; i += u64(gridDim.x) * blockDim.x)
-        mark(bitmap, UR[i]);
+        if (UR[i] != kNoRep) mark(bitmap, UR[i]);
 }
 
 // ===========================================================================
@@ -692,6 +925,7 @@ __global__ __launch_bounds__(256) void k_place(const u64 *UR, const double2 *UM,
                                                const u64 *wprefix, u64 *orep, double2 *omag) {
     for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < U; i += u64(gridDim.x) * blockDim.x) {
         const u64 rep = UR[i];
+        if (rep == kNoRep) continue;
         const u64 word = bitmap[rep >> 6];
         const u32 bit = u32(rep & 63);
         if ((word >> bit) & 1) {
@@ -964,6 +1198,36 @@ static int select_loop(qd_ctx *ctx, SelItem *S0, SelItem *S1, u64 ns, u64 r, u32
     return QD_OK;
 }
 
+
+// Buckets beyond the LDS capacity: one workgroup each on global scratch.
+static int dedup_big_buckets(qd_ctx *ctx, const Entry *D, const u64 *boff, u64 nbig, double tol, u64 *UH, u64 *UR,
+                             double2 *UM, u64 *U) {
+    hipStream_t s = ctx->stream;
+    std::vector<u64> list(nbig), soff(nbig + 1, 0);
+    QD_TRY(sync_read(ctx, ctx->biglist.p, nbig * sizeof(u64), list.data()));
+    for (u64 q = 0; q < nbig; ++q) {
+        u64 be[2];
+        QD_TRY(sync_read(ctx, boff + list[q], 2 * sizeof(u64), be));
+        u64 npad = 2;
+        while (npad < be[1] - be[0]) npad <<= 1;
+        soff[q + 1] = soff[q] + npad;
+    }
+    const u64 tot = soff[nbig];
+    QD_TRY(ctx->bigsoff.ensure((nbig + 1) * sizeof(u64)));
+    QD_TRY(ctx->bigkey.ensure(tot * sizeof(u64)));
+    QD_TRY(ctx->bigcand.ensure(tot * sizeof(u64)));
+    QD_TRY(ctx->bigmag.ensure(tot * sizeof(double2)));
+    QD_TRY(ctx->bigseg.ensure(tot * sizeof(u32)));
+    QD_HIP(hipMemcpyAsync(ctx->bigsoff.p, soff.data(), (nbig + 1) * sizeof(u64), hipMemcpyHostToDevice, s));
+    u64 *counters = ctx->counters.as<u64>();
+    hipLaunchKernelGGL(k_dedup_big, dim3(nbig), dim3(1024), 0, s, D, boff, ctx->biglist.as<u64>(),
+                       ctx->bigsoff.as<u64>(), ctx->bigkey.as<u64>(), ctx->bigcand.as<u64>(), ctx->bigmag.as<double2>(),
+                       ctx->bigseg.as<u32>(), tol, UH, UR, UM, counters, ctx->nhist.as<u64>());
+    QD_HIP(hipGetLastError());
+    QD_TRY(sync_read(ctx, counters, sizeof(u64), U));
+    return QD_OK;
+}
+
 static int state_reserve(qd_state *s, u64 n, u64 bytes) {
     QD_TRY(s->objects.ensure(std::max<u64>(bytes, 1)));
     QD_TRY(s->begin.ensure((n + 1) * sizeof(u64)));
@@ -973,11 +1237,14 @@ static int state_reserve(qd_state *s, u64 n, u64 bytes) {
 }
 
 // Dedup layout for C candidates: total bucket bits L so that buckets
-// average <= 1024 entries; first pass up to 12 bits, second pass the rest.
-static void dedup_layout(u64 C, u32 &b1, u32 &b2) {
+// average <= 1536 entries (half the LDS capacity, so a bucket overflows only
+// under a pathological hash distribution); first pass up to 12 bits, second
+// pass the rest.
+static void dedup_layout(u64 C, const qd_tuning &t, u32 &b1, u32 &b2) {
     u32 L = 0;
-    while ((C >> L) > 1024 && L < 24) ++L;
-    b1 = std::min<u32>(L, 12);
+    while ((C >> L) > t.dedup_bucket_target && L < 24) ++L;
+    b1 = std::min<u32>(L, t.first_pass_bits);
+    if (L - b1 > 12) b1 = L - 12;
     b2 = L - b1;
 }
 
@@ -1061,7 +1328,7 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
 
     // ---- 3 generate
     u32 b1, b2;
-    dedup_layout(C, b1, b2);
+    dedup_layout(C, ctx->tune, b1, b2);
     st.dedup_levels = (b1 ? 1 : 0) + (b2 ? 1 : 0);
     QD_TRY(ctx->ebuf0.ensure(C * sizeof(Entry)));
     QD_TRY(ctx->ebuf1.ensure(C * sizeof(Entry)));
@@ -1074,8 +1341,12 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
     QD_HIP(hipMemsetAsync(hist1, 0, kHistBins * sizeof(u64), s));
     QD_HIP(hipMemsetAsync(ctx->nhist.p, 0, kHistBins * sizeof(u64), s));
     prof_begin(ctx, QD_PHASE_GENERATE);
-    hipLaunchKernelGGL(k_generate<KIND>, dim3(grid_for(C, 256, 16384)), dim3(256), 0, s, P, ctx->cb.as<u64>(), rule,
-                       ctx->pre.as<u64>(), ctx->tab.as<u64>(), C, E0, b1, hist1);
+    {
+        const unsigned gb = ctx->tune.generate_block;
+        const unsigned gg = grid_for(C, gb, ctx->tune.generate_grid);
+        hipLaunchKernelGGL(k_generate<KIND>, dim3(gg), dim3(gb), 0, s, P, ctx->cb.as<u64>(), rule, ctx->pre.as<u64>(),
+                           ctx->tab.as<u64>(), C, E0, b1, hist1);
+    }
     QD_HIP(hipGetLastError());
 
     prof_end(ctx, QD_PHASE_GENERATE);
@@ -1125,15 +1396,22 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
     u64 *UR = UH + C;
     double2 *UM = reinterpret_cast<double2 *>(UR + C);
     prof_begin(ctx, QD_PHASE_DEDUP);
-    hipLaunchKernelGGL(k_dedup, dim3(nbk), dim3(kDedupThreads), 0, s, D, boff, L, prm.tolerance, UH, UR, UM, counters,
-                       ctx->nhist.as<u64>(), errw);
-    QD_HIP(hipGetLastError());
+    QD_TRY(ctx->biglist.ensure((C / 2048 + 2) * sizeof(u64)));
+    {
+        int ncu = 256;
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device);
+        const u64 grid = std::min<u64>(nbk, u64(ncu));
+        hipLaunchKernelGGL((k_dedup<1024, 2048, 4096>), dim3(grid), dim3(1024), 0, s, D, boff, nbk, L, prm.tolerance, UH,
+                           UR, UM, counters, ctx->nhist.as<u64>(), ctx->biglist.as<u64>());
+        QD_HIP(hipGetLastError());
+    }
     prof_end(ctx, QD_PHASE_DEDUP);
-    QD_HIP(hipMemcpyAsync(hb, counters, 5 * sizeof(u64), hipMemcpyDeviceToHost, s));
+    QD_HIP(hipMemcpyAsync(hb, counters, 6 * sizeof(u64), hipMemcpyDeviceToHost, s));
     QD_HIP(hipStreamSynchronize(s));
     prof_collect(ctx);
     QD_TRY(check_err_word(u32(hb[4])));
-    const u64 U = hb[0];
+    u64 U = hb[0];
+    if (hb[5]) QD_TRY(dedup_big_buckets(ctx, D, boff, hb[5], prm.tolerance, UH, UR, UM, &U));
     st.num_unique = U;
 
     // memory-driven cap (max_num_object == 0, quids.hpp:459-486, 510-536):
@@ -1154,7 +1432,7 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
     prof_begin(ctx, QD_PHASE_SELECT);
     QD_HIP(hipMemsetAsync(bitmap, 0, W * sizeof(u64), s));
     if (U <= max_obj) {
-        if (U) hipLaunchKernelGGL(k_mark_all, dim3(grid_for(U, 256, 16384)), dim3(256), 0, s, UR, U, bitmap);
+        if (U) hipLaunchKernelGGL(k_mark_all, dim3(grid_for(C, 256, 16384)), dim3(256), 0, s, UR, C, bitmap);
         QD_HIP(hipGetLastError());
     } else {
         if (!prm.simple_truncation)
@@ -1168,14 +1446,26 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
             cum += h[d];
         }
         r -= cum;
+        // second 12 bits inside bin d
+        const u32 d1 = u32(d);
+        QD_HIP(hipMemsetAsync(hist1, 0, kHistBins * sizeof(u64), s));
+        hipLaunchKernelGGL(k_sel_hist_u, dim3(grid_for(C, 256, 8192)), dim3(256), 0, s, UR, UM, C, d1, hist1);
+        QD_HIP(hipGetLastError());
+        QD_TRY(sync_read(ctx, hist1, kHistBins * sizeof(u64), h.data()));
+        cum = 0;
+        for (d = kHistBins - 1; d >= 0; --d) {
+            if (cum + h[d] >= r) break;
+            cum += h[d];
+        }
+        r -= cum;
         u64 ns = h[d];
         QD_TRY(ctx->sel0.ensure(std::max<u64>(ns, 1) * sizeof(SelItem)));
         QD_TRY(ctx->sel1.ensure(std::max<u64>(ns, 1) * sizeof(SelItem)));
         SelItem *S0 = ctx->sel0.as<SelItem>(), *S1 = ctx->sel1.as<SelItem>();
-        hipLaunchKernelGGL(k_sel_first, dim3(grid_for(U, 256, 16384)), dim3(256), 0, s, UH, UR, UM, U, u32(d), S0,
-                           counters + 1, bitmap);
+        hipLaunchKernelGGL(k_sel_first, dim3(grid_for(C, 256, 16384)), dim3(256), 0, s, UH, UR, UM, C,
+                           (d1 << 12) | u32(d), S0, counters + 1, bitmap);
         QD_HIP(hipGetLastError());
-        QD_TRY(select_loop(ctx, S0, S1, ns, r, 12, bitmap));
+        QD_TRY(select_loop(ctx, S0, S1, ns, r, 24, bitmap));
     }
     prof_end(ctx, QD_PHASE_SELECT);
     prof_begin(ctx, QD_PHASE_ORDER);
@@ -1194,7 +1484,7 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
     QD_TRY(ctx->keys.ensure(K * sizeof(u64)));
     u64 *orep = ctx->keys.as<u64>();
     prof_begin(ctx, QD_PHASE_ORDER);
-    hipLaunchKernelGGL(k_place, dim3(grid_for(U, 256, 16384)), dim3(256), 0, s, UR, UM, U, bitmap,
+    hipLaunchKernelGGL(k_place, dim3(grid_for(C, 256, 16384)), dim3(256), 0, s, UR, UM, C, bitmap,
                        ctx->wprefix.as<u64>(), orep, out->mag.as<double2>());
     prof_end(ctx, QD_PHASE_ORDER);
     prof_begin(ctx, QD_PHASE_FINALIZE);
@@ -1305,6 +1595,37 @@ int qd_ctx_trim(qd_ctx *c) {
     QD_HIP(hipSetDevice(c->device));
     QD_HIP(hipStreamSynchronize(c->stream));
     for (DBuf *b : c->all()) b->release();
+    return QD_OK;
+}
+
+int qd_ctx_get_tuning(qd_ctx *c, qd_tuning *t) {
+    if (!c || !t) return fail(QD_ERR_INVALID, "null argument");
+    *t = c->tune;
+    return QD_OK;
+}
+
+int qd_ctx_set_tuning(qd_ctx *c, const qd_tuning *t) {
+    if (!c || !t) return fail(QD_ERR_INVALID, "null argument");
+    qd_tuning n = c->tune;
+    if (t->dedup_cap) {
+        if (t->dedup_cap != 2048) return fail(QD_ERR_INVALID, "dedup_cap must be 2048");
+        n.dedup_cap = t->dedup_cap;
+    }
+    if (t->dedup_bucket_target) n.dedup_bucket_target = t->dedup_bucket_target;
+    if (n.dedup_bucket_target < 64 || n.dedup_bucket_target > n.dedup_cap / 2)
+        return fail(QD_ERR_INVALID, "dedup_bucket_target must be in [64, dedup_cap / 2]");
+    if (t->first_pass_bits) {
+        if (t->first_pass_bits > 12) return fail(QD_ERR_INVALID, "first_pass_bits must be in [1, 12]");
+        n.first_pass_bits = t->first_pass_bits;
+    }
+    if (t->generate_block) {
+        const u32 g = t->generate_block;
+        if (g != 64 && g != 128 && g != 256 && g != 512 && g != 1024)
+            return fail(QD_ERR_INVALID, "generate_block must be 64, 128, 256, 512 or 1024");
+        n.generate_block = g;
+    }
+    if (t->generate_grid) n.generate_grid = t->generate_grid;
+    c->tune = n;
     return QD_OK;
 }
 

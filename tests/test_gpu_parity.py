@@ -114,8 +114,18 @@ def test_coin_and_tolerance(ctx):
 
 
 def test_larger_step_two_level_partition(ctx):
-    """A step with > 4M candidates, so the dedup takes two partition passes."""
-    hs = qcgd_initial(10, 4)
+    """Small dedup buckets and a narrow first pass, so that steps of a few
+    hundred thousand candidates take two partition passes."""
+    old = ctx.get_tuning()
+    ctx.set_tuning(dedup_bucket_target=64, first_pass_bits=6)
+    try:
+        _two_level_chain()
+    finally:
+        ctx.set_tuning(dedup_bucket_target=old["dedup_bucket_target"], first_pass_bits=old["first_pass_bits"])
+
+
+def _two_level_chain():
+    hs = qcgd_initial(8, 4)
     rules = qcgd_iteration_rules()
     hs_g = hs_o = hs
     done = False
