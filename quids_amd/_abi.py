@@ -90,7 +90,8 @@ class QdTuning(ctypes.Structure):
         ("generate_block", ctypes.c_uint32),
         ("generate_grid", ctypes.c_uint32),
         ("dedup_cap", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 3),
+        ("generic_path", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 2),
     ]
 
     def as_dict(self):

@@ -10,9 +10,18 @@ namespace qd {
 
 constexpr int kWave = 64;
 
-// One candidate after generation: its hash, its index in the candidate
-// space (parents in order, then child_id: src/quids.hpp:666-694) and its
-// magnitude.  32 bytes, two dwordx4 accesses.
+// Candidates after the first partition pass, as three arrays: hash, index
+// in the candidate space (parents in order, then child_id:
+// src/quids.hpp:666-694) and magnitude.  Generation writes only hash and
+// magnitude, at the candidate's index.
+struct Cands {
+    u64 *h;
+    u64 *c;
+    double2 *m;
+};
+
+// One candidate in the partition passes: hash, candidate index, magnitude
+// (32 bytes, two dwordx4 accesses, one scattered write per candidate).
 struct __align__(16) Entry {
     u64 hash;
     u64 cand;

@@ -1,0 +1,342 @@
+#pragma once
+// Fast path of prep + generate for the QCGD rules (graphs of at most 64
+// nodes): per-parent descriptors with bit masks, and child generation in
+// work items of up to 8 siblings that share the walk over their common
+// leading decisions.
+//
+// Decisions of a parent (src/rules/qcgd.hpp): split_merge consumes one
+// child_id bit for the wrap-around last merge when it is possible
+// (qcgd.hpp:648-668), then one per node holding a split or merge operation,
+// in node order; the first node's split (qcgd.hpp:649, 655-659) is that
+// node's decision.  erase_create / coin consume one bit per eligible node
+// in node order (qcgd.hpp:494-509, 569-583).  Children that agree on their
+// first s bits therefore agree on everything the walk computes before the
+// s-th decision: the magnitude product so far, the folded hashes and the
+// child node count.  A work item fixes the low bits j0 of child_id and
+// enumerates its top t = min(k, 3) bits; the shared walk runs once.
+#include <hip/hip_runtime.h>
+
+#include "qd_device.hpp"
+#include "qd_rules.hpp"
+
+namespace qd {
+
+constexpr u32 kItemBits = 3;  // up to 8 children per work item
+
+struct __align__(16) PDesc {
+    u64 lmask, rmask;  // particle bits
+    u64 dmask;         // nodes that carry a decision
+    u64 nf;            // erase_create / coin: fold of the node root hashes
+    u32 nc;            // num_child, 2^k wrapped to 32 bits like the reference's uint
+    u8 n, k, flags, pad;
+};
+enum : u8 { PD_FS = 1, PD_LM = 2, PD_HMLZ = 4 };
+
+// Reported through the error word: a parent with more than 64 nodes (the
+// step then runs on the generic per-byte kernels).
+constexpr u32 ERR_WIDE = 8u;
+
+// One wavefront per parent.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_prep_fast(Parents P, const u64 *keep, PDesc *D, u32 *nc, u32 *wi, u64 *tab,
+                                                   u32 *maxsz, u32 *err) {
+    const u32 lane = threadIdx.x & (kWave - 1);
+    const u64 nwaves = u64(gridDim.x) * (blockDim.x / kWave);
+    u32 local_max = 0;
+    for (u64 p = u64(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave; p < P.n; p += nwaves) {
+        const u8 *o = P.obj + P.begin[p];
+        const u32 sz = P.size[p];
+        PDesc d{};
+        bool ok = !(keep && !((keep[p >> 6] >> (p & 63)) & 1));
+        // qcgd_valid, one node per lane
+        const u32 n = (ok && sz >= 4) ? ld16(o) : 0;
+        if (ok) {
+            bool valid = n > 0 && 4 + 4 * u64(n) <= sz;
+            if (valid) {
+                const u8 *nb = o + 2 + 2 * n;
+                const u32 m = ld16(nb + 2 * n);
+                valid = ld16(nb) == 0 && 4 + 4 * u64(n) + 16 * u64(m) == sz;
+                if (valid && n <= 64) {
+                    bool bad = false;
+                    if (lane < n) {
+                        const u32 b = ld16(nb + 2 * lane), e = ld16(nb + 2 * lane + 2);
+                        if (e <= b || e > m) bad = true;
+                        else {
+                            const u32 mi = e - b;
+                            const i16 t = ldi16(o + 4 + 4 * n + 16 * size_t(b) + 2);
+                            if (t >= 0) bad = t < 2 || u32(t) >= mi;
+                            else if (t == QCGD_DOT_L || t == QCGD_DOT_R) bad = mi < 2;
+                            else bad = t != QCGD_ELEMENT;
+                        }
+                    }
+                    valid = __ballot(bad) == 0;
+                }
+            }
+            if (!valid) {
+                if (lane == 0) atomicOr(err, ERR_OBJECT);
+                ok = false;
+            }
+        }
+        if (ok && n > 64) {
+            if (lane == 0) atomicOr(err, ERR_WIDE);
+            ok = false;
+        }
+        if (!ok) {
+            if (lane == 0) {
+                nc[p] = 0;
+                wi[p] = 0;
+                D[p] = d;
+            }
+            continue;
+        }
+        Graph g(o);
+        const bool in = lane < n;
+        const bool Li = in && g.L(lane), Ri = in && g.R(lane);
+        const u64 lm = __ballot(Li), rm = __ballot(Ri);
+        d.lmask = lm;
+        d.rmask = rm;
+        d.n = u8(n);
+        u32 k = 0;
+        if (KIND == QD_RULE_QCGD_SPLIT_MERGE) {
+            const bool Ln = lane + 1 < n && ((lm >> (lane + 1)) & 1);
+            const bool Rn = lane + 1 < n && ((rm >> (lane + 1)) & 1);
+            const bool split = Li && Ri;
+            const bool merge = !split && lane + 1 < n && Li && Rn && !Ln;
+            d.dmask = __ballot(in && (split || merge));
+            const bool fs = (lm & 1) && (rm & 1);
+            const bool lmg = !fs && n > 1 && (rm & 1) && ((lm >> (n - 1)) & 1) && !((rm >> (n - 1)) & 1);
+            d.flags = u8((fs ? PD_FS : 0) | (lmg ? PD_LM : 0));
+            if (lane == 0 && sm_first_hmlz(g)) d.flags |= PD_HMLZ;
+            d.flags = u8(__shfl(int(d.flags), 0, kWave));
+            k = u32(__popcll(d.dmask)) + (lmg ? 1 : 0);
+            if (in) {
+                u64 *t = tab + P.begin[p] / 2 + 4 * lane;
+                const u32 b = g.name_begin(lane);
+                t[0] = g.sub_hash(b);
+                t[1] = split_left_hash(g, b);
+                t[2] = split_right_hash(g, b);
+                t[3] = merge_hash(g, b, g.name_begin(lane + 1 < n ? lane + 1 : 0));
+            }
+            local_max = max(local_max, 4 * sz);
+        } else {
+            const bool coin = KIND == QD_RULE_QCGD_COIN;
+            d.dmask = __ballot(in && (coin ? (Li != Ri) : (Li == Ri)));
+            k = u32(__popcll(d.dmask));
+            const u64 hr = in ? g.sub_hash(g.name_begin(lane)) : 0;
+            u64 nf = 0;
+            for (u32 i = 0; i < n; ++i) nf = hash_combine(nf, __shfl(hr, int(i), kWave));
+            d.nf = nf;
+            local_max = max(local_max, sz);
+        }
+        d.k = u8(k);
+        d.nc = k >= 32 ? 0u : (1u << k);
+        if (lane == 0) {
+            const u32 t = min(k, kItemBits);
+            nc[p] = d.nc;
+            wi[p] = d.nc >> t;
+            D[p] = d;
+        }
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) local_max = max(local_max, (u32)__shfl_xor(int(local_max), off, kWave));
+    if (lane == 0 && local_max) atomicMax(maxsz, local_max);
+}
+
+// Walk state of one child (or of the shared prefix of a work item).
+struct Walk {
+    u64 nh, lh, rh;
+    cplx mag;
+    u32 i, end, c, d;
+    bool started, overflow;
+};
+
+// split_merge child walk (populate_child, qcgd.hpp:643-848) over the
+// parent's masks and root-hash table.  Consumes decisions d while d <
+// dlimit, the bit of decision d being bit (d - d0) of `bits` where d0 is
+// w.d on entry; returns when the next decision is not available or the walk
+// is complete.
+__device__ inline void sm_run(Walk &w, const PDesc &pd, const u64 *tab, u32 bits, u32 dlimit, const QcgdCoef &k) {
+    const u32 n = pd.n;
+    const u32 d0 = w.d;
+    auto emit = [&](u64 root, bool l, bool r) {
+        w.nh = hash_combine(w.nh, root);
+        if (l) w.lh = hash_combine_small(w.lh, w.c);
+        if (r) w.rh = hash_combine_small(w.rh, w.c);
+        ++w.c;
+    };
+    if (!w.started) {
+        if (pd.flags & PD_LM) {
+            if (w.d >= dlimit) return;
+            const bool take = (bits >> (w.d++ - d0)) & 1;
+            if (take) {
+                w.mag = cmul(w.mag, k.dc);
+                emit(tab[4 * (n - 1) + 3], true, true);
+                w.i = 1;
+                w.end = n - 1;
+            } else {
+                w.mag = cmul(w.mag, cneg(k.dnc));
+                w.i = 0;
+                w.end = n;
+            }
+        } else if (pd.flags & PD_FS) {
+            if (w.d >= dlimit) return;
+            const bool take = (bits >> (w.d++ - d0)) & 1;
+            if (take) {
+                w.mag = cmul(w.mag, k.d);
+                if (pd.flags & PD_HMLZ) {
+                    emit(tab[1], true, false);
+                    emit(tab[2], false, true);
+                } else {
+                    w.overflow = true;
+                    emit(tab[2], false, true);
+                }
+            } else {
+                w.mag = cmul(w.mag, k.dn);
+                emit(tab[0], true, true);
+            }
+            w.i = 1;
+            w.end = n;
+        } else {
+            w.i = 0;
+            w.end = n;
+        }
+        w.started = true;
+    }
+    while (w.i < w.end) {
+        const u32 i = w.i;
+        const bool l = (pd.lmask >> i) & 1, r = (pd.rmask >> i) & 1;
+        if ((pd.dmask >> i) & 1) {
+            if (w.d >= dlimit) return;
+            const bool take = (bits >> (w.d++ - d0)) & 1;
+            if (l && r) {  // split
+                if (take) {
+                    w.mag = cmul(w.mag, k.d);
+                    emit(tab[4 * i + 1], true, false);
+                    emit(tab[4 * i + 2], false, true);
+                } else {
+                    w.mag = cmul(w.mag, k.dn);
+                    emit(tab[4 * i], l, r);
+                }
+            } else {  // merge with node i+1 (which carries no decision)
+                if (take) {
+                    w.mag = cmul(w.mag, k.dc);
+                    emit(tab[4 * i + 3], true, true);
+                    ++w.i;
+                } else {
+                    w.mag = cmul(w.mag, cneg(k.dnc));
+                    emit(tab[4 * i], l, r);
+                }
+            }
+        } else {
+            emit(tab[4 * i], l, r);
+        }
+        ++w.i;
+    }
+}
+
+__device__ inline u64 sm_finish(Walk &w, const u64 *tab) {
+    if (w.overflow) {  // left half of the first node goes last (qcgd.hpp:838-845)
+        w.nh = hash_combine(w.nh, tab[1]);
+        w.lh = hash_combine_small(w.lh, w.c);
+        ++w.c;
+    }
+    return hash_combine(hash_combine(w.nh, w.lh), w.rh);
+}
+
+// erase_create / coin child walk (qcgd.hpp:487-510, 562-584).
+template <bool COIN>
+__device__ inline void flip_run(Walk &w, const PDesc &pd, u32 bits, u32 dlimit, const QcgdCoef &k) {
+    const u32 n = pd.n;
+    const u32 d0 = w.d;
+    while (w.i < n) {
+        const u32 i = w.i;
+        bool l = (pd.lmask >> i) & 1, r = (pd.rmask >> i) & 1;
+        if ((pd.dmask >> i) & 1) {
+            if (w.d >= dlimit) return;
+            const bool take = (bits >> (w.d++ - d0)) & 1;
+            if (take) {
+                w.mag = cmul(w.mag, l ? k.dc : k.d);
+                l = !l;
+                r = !r;
+            } else {
+                w.mag = cmul(w.mag, l ? cneg(k.dnc) : k.dn);
+            }
+        }
+        if (l) w.lh = hash_combine_small(w.lh, i);
+        if (r) w.rh = hash_combine_small(w.rh, i);
+        ++w.i;
+    }
+}
+
+constexpr int kGenThreads = 256;
+
+// One thread per work item; writes hash and magnitude of its children at
+// their candidate index (coalesced for each of the up-to-8 children) and
+// histograms the top b1 hash bits for the first partition pass.
+template <int KIND>
+__global__ __launch_bounds__(kGenThreads) void k_generate_fast(const PDesc *D, const double2 *pmag, const u64 *cb,
+                                                              const u64 *wb, u64 N, const u64 *tab, const u64 *pbegin,
+                                                              qd_rule rule, u64 W, u64 *H0, double2 *M0, u32 b1,
+                                                              u64 *hist1) {
+    __shared__ u32 hist[kHistBins];
+    __shared__ u64 swb[kGenThreads + 1];
+    __shared__ u64 pfirst;
+    if (b1)
+        for (int i = threadIdx.x; i < (1 << b1); i += blockDim.x) hist[i] = 0;
+    const QcgdCoef k = coef_of(rule);
+    for (u64 w0 = u64(blockIdx.x) * kGenThreads; w0 < W; w0 += u64(gridDim.x) * kGenThreads) {
+        __syncthreads();
+        if (threadIdx.x == 0) pfirst = find_parent(wb, N, w0);
+        __syncthreads();
+        const u64 pf = pfirst;
+        for (u32 q = threadIdx.x; q <= kGenThreads; q += blockDim.x) swb[q] = pf + q <= N ? wb[pf + q] : ~0ull;
+        __syncthreads();
+        const u64 w = w0 + threadIdx.x;
+        if (w >= W) continue;
+        // parent of w: the last p with wb[p] <= w
+        u64 p;
+        if (swb[kGenThreads] > w) {
+            u32 lo = 0, hi = kGenThreads;
+            while (lo + 1 < hi) {
+                const u32 mid = (lo + hi) >> 1;
+                if (swb[mid] <= w) lo = mid;
+                else hi = mid;
+            }
+            p = pf + lo;
+        } else {
+            p = find_parent(wb, N, w);
+        }
+        const PDesc pd = D[p];
+        const u32 j0 = u32(w - wb[p]);
+        const u32 t = min(u32(pd.k), kItemBits);
+        const u32 s = pd.k - t;
+        const double2 m0 = pmag[p];
+        Walk pre{};
+        pre.mag = cplx{m0.x, m0.y};
+        const u64 *ptab = tab + pbegin[p] / 2;
+        if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(pre, pd, ptab, j0, s, k);
+        else flip_run<KIND == QD_RULE_QCGD_COIN>(pre, pd, j0, s, k);
+        const u64 cbase = cb[p] + j0;
+        for (u32 u = 0; u < (1u << t); ++u) {
+            Walk x = pre;
+            u64 h;
+            if (KIND == QD_RULE_QCGD_SPLIT_MERGE) {
+                sm_run(x, pd, ptab, u, pd.k, k);
+                h = sm_finish(x, ptab);
+            } else {
+                flip_run<KIND == QD_RULE_QCGD_COIN>(x, pd, u, pd.k, k);
+                h = hash_combine(hash_combine(pd.nf, x.lh), x.rh);
+            }
+            const u64 c = cbase + (u64(u) << s);
+            H0[c] = h;
+            M0[c] = make_double2(x.mag.re, x.mag.im);
+            if (b1) atomicAdd(&hist[h >> (64 - b1)], 1u);
+        }
+    }
+    if (b1) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < (1 << b1); i += blockDim.x)
+            if (hist[i]) atomicAdd(reinterpret_cast<unsigned long long *>(hist1 + i), (unsigned long long)hist[i]);
+    }
+}
+
+}  // namespace qd

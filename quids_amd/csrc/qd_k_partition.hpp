@@ -1,0 +1,165 @@
+#pragma once
+// MSD radix partition of candidates on their hash
+#include <hip/hip_runtime.h>
+
+#include "qd_device.hpp"
+#include "qd_rules.hpp"
+
+namespace qd {
+
+// ===========================================================================
+// phase 4: radix partition
+// ===========================================================================
+constexpr int kTile1 = 4096;  // entries per workgroup in the first pass
+
+// First pass: from candidate order (index = candidate, hash and magnitude
+// as written by generate) to 2^b1 buckets at exact offsets.
+__global__ __launch_bounds__(256) void k_scatter1(const u64 *H0, const double2 *M0, u64 n, u32 b1, u64 *cursor,
+                                                  Entry *out) {
+    __shared__ u32 cnt[kHistBins];
+    __shared__ u64 gbase[kHistBins];
+    const int nbin = 1 << b1;
+    for (int i = threadIdx.x; i < nbin; i += blockDim.x) cnt[i] = 0;
+    __syncthreads();
+    const u64 base = u64(blockIdx.x) * kTile1;
+    constexpr int kPer = kTile1 / 256;
+    u32 dr[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const u64 i = base + threadIdx.x + 256 * k;
+        if (i < n) {
+            const u32 d = b1 ? u32(H0[i] >> (64 - b1)) : 0u;
+            const u32 r = atomicAdd(&cnt[d], 1u);
+            dr[k] = (d << 16) | r;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nbin; i += blockDim.x)
+        if (cnt[i]) gbase[i] = atomicAdd(reinterpret_cast<unsigned long long *>(cursor + i), (unsigned long long)cnt[i]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const u64 i = base + threadIdx.x + 256 * k;
+        if (i < n) {
+            const double2 m = M0[i];
+            Entry e;
+            e.hash = H0[i];
+            e.cand = i;
+            e.re = m.x;
+            e.im = m.y;
+            out[gbase[dr[k] >> 16] + (dr[k] & 0xFFFF)] = e;
+        }
+    }
+}
+
+// Second pass, one workgroup per first-level bucket, splitting it by the
+// next b2 hash bits in one read: sub-bucket d of bucket b owns cap2 slots at
+// ((b << b2) + d) * cap2; workgroup-local cursors place the entries, and the
+// final counts go to cnt2.  A sub-bucket that would exceed cap2 raises
+// ERR_SPILL (the step then repeats this pass with exact counts).
+constexpr u32 ERR_SPILL = 16u;
+
+__global__ __launch_bounds__(1024) void k_level2_fixed(const Entry *in, const u64 *off1, u32 b1, u32 b2, u64 cap2,
+                                                       Entry *out, u64 *cnt2, u32 *err) {
+    __shared__ u32 ccnt[kHistBins];
+    __shared__ u32 cur[kHistBins];
+    const u64 b = blockIdx.x;
+    const u64 beg = off1[b], end = off1[b + 1];
+    const u32 nb2 = 1u << b2;
+    const u32 shift = 64 - b1 - b2;
+    Entry *region = out + (b << b2) * cap2;
+    for (u32 i = threadIdx.x; i < nb2; i += blockDim.x) cur[i] = 0;
+    bool spill = false;
+    for (u64 cbase = beg; cbase < end; cbase += 4096) {
+        for (u32 i = threadIdx.x; i < nb2; i += blockDim.x) ccnt[i] = 0;
+        __syncthreads();
+        u32 dr[4];
+        Entry e[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const u64 i = cbase + threadIdx.x + 1024 * k;
+            if (i < end) {
+                e[k] = in[i];
+                const u32 d = u32(e[k].hash >> shift) & (nb2 - 1);
+                dr[k] = (d << 16) | atomicAdd(&ccnt[d], 1u);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const u64 i = cbase + threadIdx.x + 1024 * k;
+            if (i < end) {
+                const u32 d = dr[k] >> 16;
+                const u64 slot = u64(cur[d]) + (dr[k] & 0xFFFF);
+                if (slot < cap2) region[u64(d) * cap2 + slot] = e[k];
+                else spill = true;
+            }
+        }
+        __syncthreads();
+        for (u32 i = threadIdx.x; i < nb2; i += blockDim.x) cur[i] += ccnt[i];
+    }
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < nb2; i += blockDim.x) cnt2[(b << b2) + i] = min<u64>(cur[i], cap2);
+    if (spill) atomicOr(err, ERR_SPILL);
+}
+
+// The same split with exact offsets (a counting pass first): used when a
+// sub-bucket spills its fixed capacity.
+__global__ __launch_bounds__(1024) void k_level2_exact(const Entry *in, const u64 *off1, u32 b1, u32 b2, Entry *out,
+                                                       u64 *off2) {
+    __shared__ u32 cnt[kHistBins];
+    __shared__ u32 ccnt[kHistBins];
+    __shared__ u64 cur[kHistBins];
+    __shared__ u64 tmp[1024 / kWave + 1];
+    const u64 b = blockIdx.x;
+    const u64 beg = off1[b], end = off1[b + 1];
+    const u32 nb2 = 1u << b2;
+    const u32 shift = 64 - b1 - b2;
+    for (u32 i = threadIdx.x; i < nb2; i += blockDim.x) cnt[i] = 0;
+    __syncthreads();
+    for (u64 i = beg + threadIdx.x; i < end; i += blockDim.x) atomicAdd(&cnt[u32(in[i].hash >> shift) & (nb2 - 1)], 1u);
+    __syncthreads();
+    {
+        u32 v[4];
+        u64 s = 0;
+        for (int k = 0; k < 4; ++k) {
+            const u32 d = threadIdx.x * 4 + k;
+            v[k] = d < nb2 ? cnt[d] : 0;
+            s += v[k];
+        }
+        u64 total;
+        u64 run = block_excl_scan(s, tmp, total);
+        for (int k = 0; k < 4; ++k) {
+            const u32 d = threadIdx.x * 4 + k;
+            if (d < nb2) {
+                cur[d] = beg + run;
+                off2[(b << b2) + d] = beg + run;
+            }
+            run += v[k];
+        }
+    }
+    __syncthreads();
+    for (u64 cbase = beg; cbase < end; cbase += 4096) {
+        for (u32 i = threadIdx.x; i < nb2; i += blockDim.x) ccnt[i] = 0;
+        __syncthreads();
+        u32 dr[4];
+        for (int k = 0; k < 4; ++k) {
+            const u64 i = cbase + threadIdx.x + 1024 * k;
+            if (i < end) {
+                const u32 d = u32(in[i].hash >> shift) & (nb2 - 1);
+                dr[k] = (d << 16) | atomicAdd(&ccnt[d], 1u);
+            }
+        }
+        __syncthreads();
+        for (int k = 0; k < 4; ++k) {
+            const u64 i = cbase + threadIdx.x + 1024 * k;
+            if (i < end) out[cur[dr[k] >> 16] + (dr[k] & 0xFFFF)] = in[i];
+        }
+        __syncthreads();
+        for (u32 i = threadIdx.x; i < nb2; i += blockDim.x) cur[i] += ccnt[i];
+        __syncthreads();
+    }
+}
+
+
+}  // namespace qd

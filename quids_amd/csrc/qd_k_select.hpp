@@ -1,0 +1,124 @@
+#pragma once
+// top-K selection on (|m|^2, hash) and survivor marking
+#include <hip/hip_runtime.h>
+
+#include "qd_device.hpp"
+#include "qd_rules.hpp"
+
+namespace qd {
+
+// ===========================================================================
+// phase 6: top-K select on the 128-bit key (|m|^2 bits, ~hash)
+// ===========================================================================
+struct SelItem {
+    u64 hi, lo, rep;
+};
+
+__device__ inline u32 key_digit(u64 hi, u64 lo, u32 pos) {
+    u64 top;
+    if (pos == 0) top = hi;
+    else if (pos < 64) top = (hi << pos) | (lo >> (64 - pos));
+    else top = lo << (pos - 64);
+    return u32(top >> 52);
+}
+
+// Appends for the lanes with pred set: one atomic per wave; returns this
+// lane's slot (meaningful only when pred).
+__device__ inline u64 wave_append(bool pred, u64 *counter) {
+    const u64 mask = __ballot(pred);
+    if (!mask) return 0;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    u64 base = 0;
+    if (lane == leader) base = atomicAdd(reinterpret_cast<unsigned long long *>(counter), (unsigned long long)__popcll(mask));
+    base = __shfl(base, leader, kWave);
+    return base + __popcll(mask & ((1ull << lane) - 1));
+}
+
+__device__ inline void mark(u64 *bitmap, u64 rep) {
+    atomicOr(reinterpret_cast<unsigned long long *>(bitmap + (rep >> 6)), 1ull << (rep & 63));
+}
+
+// First select stage over the dedup output slots: histogram of |m|^2 bits
+// 51..40 among the uniques whose bits 63..52 equal d1.
+__global__ __launch_bounds__(256) void k_sel_hist_u(const u64 *UR, const double2 *UM, u64 U, u32 d1, u64 *hist) {
+    __shared__ u32 h[kHistBins];
+    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < U; i += u64(gridDim.x) * blockDim.x) {
+        if (UR[i] == kNoRep) continue;
+        const double2 m = UM[i];
+        const u64 bits = u64(__double_as_longlong(m.x * m.x + m.y * m.y));
+        if (u32(bits >> 52) == d1) atomicAdd(&h[u32(bits >> 40) & 0xFFF], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x)
+        if (h[i]) atomicAdd(reinterpret_cast<unsigned long long *>(hist + i), (unsigned long long)h[i]);
+}
+
+// Marks the uniques whose top 24 |m|^2 bits exceed prefix24 and collects
+// the ones equal to it.
+__global__ __launch_bounds__(256) void k_sel_first(const u64 *UH, const u64 *UR, const double2 *UM, u64 U,
+                                                   u32 prefix24, SelItem *S, u64 *scount, u64 *bitmap) {
+    const u64 stride = u64(gridDim.x) * blockDim.x;
+    // uniform trip count per wave so that wave_append sees every lane
+    for (u64 base = u64(blockIdx.x) * blockDim.x; base < U; base += stride) {
+        const u64 i = base + threadIdx.x;
+        bool take = false;
+        SelItem it{};
+        if (i < U && UR[i] != kNoRep) {
+            const double2 m = UM[i];
+            const u64 bits = u64(__double_as_longlong(m.x * m.x + m.y * m.y));
+            const u32 t = u32(bits >> 40);
+            if (t > prefix24) mark(bitmap, UR[i]);
+            else if (t == prefix24) {
+                take = true;
+                it = SelItem{bits, ~UH[i], UR[i]};
+            }
+        }
+        const u64 at = wave_append(take, scount);
+        if (take) S[at] = it;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sel_hist(const SelItem *S, u64 n, u32 pos, u64 *hist) {
+    __shared__ u32 h[kHistBins];
+    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+        atomicAdd(&h[key_digit(S[i].hi, S[i].lo, pos)], 1u);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x)
+        if (h[i]) atomicAdd(reinterpret_cast<unsigned long long *>(hist + i), (unsigned long long)h[i]);
+}
+
+__global__ __launch_bounds__(256) void k_sel_split(const SelItem *S, u64 n, u32 pos, u32 d, SelItem *out,
+                                                   u64 *ocount, u64 *bitmap) {
+    const u64 stride = u64(gridDim.x) * blockDim.x;
+    for (u64 base = u64(blockIdx.x) * blockDim.x; base < n; base += stride) {
+        const u64 i = base + threadIdx.x;
+        bool take = false;
+        SelItem it{};
+        if (i < n) {
+            it = S[i];
+            const u32 t = key_digit(it.hi, it.lo, pos);
+            if (t > d) mark(bitmap, it.rep);
+            else take = t == d;
+        }
+        const u64 at = wave_append(take, ocount);
+        if (take) out[at] = it;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sel_mark_items(const SelItem *S, u64 n, u64 *bitmap) {
+    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+        mark(bitmap, S[i].rep);
+}
+
+__global__ __launch_bounds__(256) void k_mark_all(const u64 *UR, u64 U, u64 *bitmap) {
+    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < U; i += u64(gridDim.x) * blockDim.x)
+        if (UR[i] != kNoRep) mark(bitmap, UR[i]);
+}
+
+
+}  // namespace qd

@@ -43,6 +43,13 @@
 #include "../../include/quids_hip.h"
 #include "qd_device.hpp"
 #include "qd_rules.hpp"
+#include "qd_k_scan.hpp"
+#include "qd_k_generate.hpp"
+#include "qd_k_fastgen.hpp"
+#include "qd_k_partition.hpp"
+#include "qd_k_dedup.hpp"
+#include "qd_k_select.hpp"
+#include "qd_k_finalize.hpp"
 
 using namespace qd;
 
@@ -118,7 +125,7 @@ struct qd_ctx {
     hipEvent_t ev[2 * QD_NUM_PHASES] = {};
     bool pending[QD_NUM_PHASES] = {};
     double phase_ms[QD_NUM_PHASES] = {};
-    qd_tuning tune = {768, 12, 256, 2048, 2048, {0, 0, 0}};
+    qd_tuning tune = {768, 12, 256, 2048, 2048, 0, {0, 0}};
     u64 phase_calls[QD_NUM_PHASES] = {};
     u64 *host = nullptr;  // pinned readback area, 8192 u64
     // scratch of the step
@@ -129,10 +136,11 @@ struct qd_ctx {
     DBuf sel0, sel1;
     DBuf bitmap, wprefix, padded, chunks, scanblk;
     DBuf biglist, bigsoff, bigkey, bigcand, bigmag, bigseg;
+    DBuf pdesc, wi, wb, ebuf2, cnt2;
     std::vector<DBuf *> all() {
         return {&nc, &cb, &pre, &tab, &kept, &keys, &ebuf0, &ebuf1, &hist1, &off1, &cur1, &off2,
                 &nhist, &counters, &sel0, &sel1, &bitmap, &wprefix, &padded, &chunks, &scanblk,
-                &biglist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg};
+                &biglist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &ebuf2, &cnt2};
     }
 };
 
@@ -150,73 +158,6 @@ struct qd_symbolic {
     u64 num_object_after_interferences = 0;
 };
 
-// ===========================================================================
-// scans: exclusive prefix of n loaded values into out[0..n] (out[n] = total)
-// ===========================================================================
-constexpr int kScanThreads = 256;
-constexpr int kScanItems = 8;
-constexpr int kScanTile = kScanThreads * kScanItems;
-
-struct LoadU32 {
-    const u32 *p;
-    __device__ u64 operator()(u64 i) const { return p[i]; }
-};
-struct LoadU64 {
-    const u64 *p;
-    __device__ u64 operator()(u64 i) const { return p[i]; }
-};
-struct LoadPopc {
-    const u64 *p;
-    __device__ u64 operator()(u64 i) const { return __popcll(p[i]); }
-};
-
-template <class Load>
-__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(Load ld, u64 n, u64 *bsum) {
-    __shared__ u64 tmp[kScanThreads / kWave + 1];
-    const u64 base = u64(blockIdx.x) * kScanTile + u64(threadIdx.x) * kScanItems;
-    u64 s = 0;
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k)
-        if (base + k < n) s += ld(base + k);
-    u64 total;
-    block_excl_scan(s, tmp, total);
-    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
-}
-
-__global__ __launch_bounds__(1024) void k_scan_bsums(u64 *bsum, u64 nb) {
-    __shared__ u64 tmp[1024 / kWave + 1];
-    u64 carry = 0;
-    for (u64 base = 0; base < nb; base += 1024) {
-        const u64 i = base + threadIdx.x;
-        const u64 v = i < nb ? bsum[i] : 0;
-        u64 total;
-        const u64 ex = block_excl_scan(v, tmp, total);
-        if (i < nb) bsum[i] = carry + ex;
-        carry += total;
-    }
-}
-
-template <class Load>
-__global__ __launch_bounds__(kScanThreads) void k_scan_final(Load ld, u64 n, const u64 *boff, u64 *out) {
-    __shared__ u64 tmp[kScanThreads / kWave + 1];
-    const u64 base = u64(blockIdx.x) * kScanTile + u64(threadIdx.x) * kScanItems;
-    u64 v[kScanItems];
-    u64 s = 0;
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        v[k] = base + k < n ? ld(base + k) : 0;
-        s += v[k];
-    }
-    u64 total;
-    u64 run = boff[blockIdx.x] + block_excl_scan(s, tmp, total);
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        if (base + k < n) out[base + k] = run;
-        run += v[k];
-        if (base + k + 1 == n) out[n] = run;
-    }
-}
-
 template <class Load>
 static int run_scan(qd_ctx *ctx, Load ld, u64 n, u64 *out) {
     if (n == 0) {
@@ -231,829 +172,6 @@ static int run_scan(qd_ctx *ctx, Load ld, u64 n, u64 *out) {
     hipLaunchKernelGGL(k_scan_final<Load>, dim3(nb), dim3(kScanThreads), 0, ctx->stream, ld, n, bs, out);
     QD_HIP(hipGetLastError());
     return QD_OK;
-}
-
-// ===========================================================================
-// phase 1: per-parent prep
-// ===========================================================================
-struct Parents {
-    const u8 *obj;
-    const u64 *begin;
-    const u32 *size;
-    const double2 *mag;
-    u64 n;
-};
-
-// Validation bits reported through the error word.
-enum : u32 { ERR_OBJECT = 1u, ERR_OVERFLOW = 2u, ERR_BIT = 4u };
-
-// A QCGD object is well formed when its size matches its header and every
-// node's name tree starts with a root the rules can read: an element, a .l /
-// .r node with its inner node, or a pair with two non-empty halves.  Device
-// code reads only roots, root+1 and root+right_or_type, so this bounds every
-// access and every child size.
-__device__ inline bool qcgd_valid(const u8 *o, u32 size) {
-    if (size < 4) return false;
-    const u32 n = ld16(o);
-    if (n == 0 || 4 + 4 * u64(n) > size) return false;
-    const u8 *nb = o + 2 + 2 * n;
-    if (ld16(nb) != 0) return false;
-    const u32 m = ld16(nb + 2 * n);
-    if (4 + 4 * u64(n) + 16 * u64(m) != size) return false;
-    Graph g(o);
-    for (u32 i = 0; i < n; ++i) {
-        const u32 b = ld16(nb + 2 * i), e = ld16(nb + 2 * i + 2);
-        if (e <= b || e > m) return false;
-        const u32 mi = e - b;
-        const i16 t = g.sub_type(b);
-        if (t >= 0) {
-            if (t < 2 || u32(t) >= mi) return false;
-        } else if (t == QCGD_DOT_L || t == QCGD_DOT_R) {
-            if (mi < 2) return false;
-        } else if (t != QCGD_ELEMENT) {
-            return false;
-        }
-    }
-    return true;
-}
-
-template <int KIND>
-__global__ __launch_bounds__(256) void k_prep(Parents P, const u64 *keep, qd_rule rule, u32 *nc, u64 *pre, u64 *tab,
-                                              u32 *maxsz, u32 *err) {
-    u32 local_max = 0;
-    for (u64 p = u64(blockIdx.x) * blockDim.x + threadIdx.x; p < P.n; p += u64(gridDim.x) * blockDim.x) {
-        const u8 *o = P.obj + P.begin[p];
-        const u32 sz = P.size[p];
-        if (keep && !((keep[p >> 6] >> (p & 63)) & 1)) {
-            nc[p] = 0;
-            continue;
-        }
-        u32 num = 0, ms = 0;
-        if (KIND == QD_RULE_HADAMARD) {
-            if (rule.bit >= sz) {
-                atomicOr(err, ERR_BIT);
-                nc[p] = 0;
-                continue;
-            }
-            num = 2;
-            ms = sz;
-        } else {
-            if (!qcgd_valid(o, sz)) {
-                atomicOr(err, ERR_OBJECT);
-                nc[p] = 0;
-                continue;
-            }
-            Graph g(o);
-            if (KIND == QD_RULE_QCGD_SPLIT_MERGE) {
-                num = sm_num_child(g);
-                ms = 4 * sz;
-                sm_fill_table(g, tab + P.begin[p] / 2);
-                pre[p] = sm_first_hmlz(g) ? 1 : 0;
-            } else {
-                num = flip_num_child(g, KIND == QD_RULE_QCGD_COIN);
-                ms = sz;
-                pre[p] = name_fold(g);
-            }
-        }
-        nc[p] = num;
-        local_max = max(local_max, ms);
-    }
-    if (local_max) atomicMax(maxsz, local_max);
-}
-
-// ===========================================================================
-// phase 3: generate
-// ===========================================================================
-__device__ inline u64 find_parent(const u64 *cb, u64 n, u64 c) {
-    // largest p in [0, n) with cb[p] <= c (cb has n+1 entries, cb[n] = C > c)
-    u64 lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const u64 mid = (lo + hi + 1) >> 1;
-        if (cb[mid] <= c) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
-constexpr int kHistBins = 4096;  // up to 12 bits of LDS histogram
-
-template <int KIND>
-__global__ __launch_bounds__(1024) void k_generate(Parents P, const u64 *cb, qd_rule rule, const u64 *pre,
-                                                  const u64 *tab, u64 C, Entry *E, u32 b1, u64 *hist1) {
-    __shared__ u32 hist[kHistBins];
-    if (b1)
-        for (int i = threadIdx.x; i < (1 << b1); i += blockDim.x) hist[i] = 0;
-    __syncthreads();
-    const QcgdCoef k = coef_of(rule);
-    for (u64 c = u64(blockIdx.x) * blockDim.x + threadIdx.x; c < C; c += u64(gridDim.x) * blockDim.x) {
-        const u64 p = find_parent(cb, P.n, c);
-        const u32 j = u32(c - cb[p]);
-        const double2 m0 = P.mag[p];
-        cplx mag{m0.x, m0.y};
-        const u8 *o = P.obj + P.begin[p];
-        u64 h;
-        if (KIND == QD_RULE_HADAMARD) {
-            h = hadamard_child_hash(o, P.size[p], rule.bit, rule.coef, j, mag);
-        } else if (KIND == QD_RULE_QCGD_SPLIT_MERGE) {
-            Graph g(o);
-            h = sm_child_hash(g, pre[p] != 0, tab + P.begin[p] / 2, j, k, mag);
-        } else {
-            Graph g(o);
-            h = flip_child_hash(g, KIND == QD_RULE_QCGD_COIN, pre[p], j, k, mag);
-        }
-        Entry e;
-        e.hash = h;
-        e.cand = c;
-        e.re = mag.re;
-        e.im = mag.im;
-        E[c] = e;
-        if (b1) atomicAdd(&hist[h >> (64 - b1)], 1u);
-    }
-    if (b1) {
-        __syncthreads();
-        for (int i = threadIdx.x; i < (1 << b1); i += blockDim.x)
-            if (hist[i]) atomicAdd(reinterpret_cast<unsigned long long *>(hist1 + i), (unsigned long long)hist[i]);
-    }
-}
-
-// ===========================================================================
-// phase 4: radix partition
-// ===========================================================================
-constexpr int kTile1 = 4096;  // entries per workgroup in the first pass
-
-__global__ __launch_bounds__(256) void k_scatter1(const Entry *in, u64 n, u32 b1, u64 *cursor, Entry *out) {
-    __shared__ u32 cnt[kHistBins];
-    __shared__ u64 gbase[kHistBins];
-    const int nbin = 1 << b1;
-    for (int i = threadIdx.x; i < nbin; i += blockDim.x) cnt[i] = 0;
-    __syncthreads();
-    const u64 base = u64(blockIdx.x) * kTile1;
-    constexpr int kPer = kTile1 / 256;
-    u32 dr[kPer];
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        const u64 i = base + threadIdx.x + 256 * k;
-        if (i < n) {
-            const u32 d = u32(in[i].hash >> (64 - b1));
-            const u32 r = atomicAdd(&cnt[d], 1u);
-            dr[k] = (d << 16) | r;
-        }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < nbin; i += blockDim.x)
-        if (cnt[i]) gbase[i] = atomicAdd(reinterpret_cast<unsigned long long *>(cursor + i), (unsigned long long)cnt[i]);
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        const u64 i = base + threadIdx.x + 256 * k;
-        if (i < n) out[gbase[dr[k] >> 16] + (dr[k] & 0xFFFF)] = in[i];
-    }
-}
-
-// Second pass: one workgroup per first-level bucket splits it by the next b2
-// bits; counts are workgroup-local, so placement needs no global atomics.
-__global__ __launch_bounds__(1024) void k_level2(const Entry *in, const u64 *off1, u32 b1, u32 b2, Entry *out,
-                                                 u64 *off2) {
-    __shared__ u32 cnt[kHistBins];
-    __shared__ u32 ccnt[kHistBins];
-    __shared__ u64 cur[kHistBins];
-    __shared__ u64 tmp[1024 / kWave + 1];
-    const u64 b = blockIdx.x;
-    const u64 beg = off1[b], end = off1[b + 1];
-    const u32 nb2 = 1u << b2;
-    const u32 shift = 64 - b1 - b2;
-    for (u32 i = threadIdx.x; i < nb2; i += blockDim.x) cnt[i] = 0;
-    __syncthreads();
-    for (u64 i = beg + threadIdx.x; i < end; i += blockDim.x)
-        atomicAdd(&cnt[u32(in[i].hash >> shift) & (nb2 - 1)], 1u);
-    __syncthreads();
-    // exclusive scan of cnt (nb2 <= 4096 entries, 4 per thread)
-    {
-        u32 v[4];
-        u64 s = 0;
-        for (int k = 0; k < 4; ++k) {
-            const u32 d = threadIdx.x * 4 + k;
-            v[k] = d < nb2 ? cnt[d] : 0;
-            s += v[k];
-        }
-        u64 total;
-        u64 run = block_excl_scan(s, tmp, total);
-        for (int k = 0; k < 4; ++k) {
-            const u32 d = threadIdx.x * 4 + k;
-            if (d < nb2) {
-                cur[d] = beg + run;
-                off2[(b << b2) + d] = beg + run;
-            }
-            run += v[k];
-        }
-    }
-    __syncthreads();
-    for (u64 cbase = beg; cbase < end; cbase += 4096) {
-        for (u32 i = threadIdx.x; i < nb2; i += blockDim.x) ccnt[i] = 0;
-        __syncthreads();
-        u32 dr[4];
-        for (int k = 0; k < 4; ++k) {
-            const u64 i = cbase + threadIdx.x + 1024 * k;
-            if (i < end) {
-                const u32 d = u32(in[i].hash >> shift) & (nb2 - 1);
-                dr[k] = (d << 16) | atomicAdd(&ccnt[d], 1u);
-            }
-        }
-        __syncthreads();
-        for (int k = 0; k < 4; ++k) {
-            const u64 i = cbase + threadIdx.x + 1024 * k;
-            if (i < end) out[cur[dr[k] >> 16] + (dr[k] & 0xFFFF)] = in[i];
-        }
-        __syncthreads();
-        for (u32 i = threadIdx.x; i < nb2; i += blockDim.x) cur[i] += ccnt[i];
-        __syncthreads();
-    }
-}
-
-// ===========================================================================
-// phase 5: per-bucket dedup in LDS
-// ===========================================================================
-// A persistent grid walks the buckets.  A bucket's entries are staged in LDS
-// and bitonic-sorted on (hash, candidate index), so each hash group becomes a
-// run in ascending candidate order: its first candidate is the
-// representative and a segmented pairwise tree sums its magnitudes (adjacent
-// pairs level by level, the oracle's canonical order).  Groups with
-// |m|^2 > tolerance are written compactly at the start of the bucket's own
-// slot range in the output; the rest of the range gets the sentinel
-// representative ~0, so no global atomics order the output.  Buckets larger
-// than the LDS capacity are listed for k_dedup_big.
-constexpr int kNormBins = 2048;  // |m|^2 exponent histogram (the sign bit is 0)
-constexpr u64 kNoRep = ~0ull;
-
-__device__ inline u32 norm_top12(double nrm) { return u32(__double_as_longlong(nrm) >> 52) & 0xFFF; }
-
-__device__ inline bool key_gt(u64 ka, u64 ca, u64 kb, u64 cb) { return ka > kb || (ka == kb && ca > cb); }
-
-// Exclusive max-scan of one value per thread over the block.
-__device__ inline u32 block_excl_max(u32 v, u64 *tmp) {
-    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave, nw = blockDim.x / kWave;
-    u32 inc = v;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const u32 o = __shfl_up(inc, d, kWave);
-        if (lane >= d) inc = max(inc, o);
-    }
-    const u32 exc_w = __shfl_up(inc, 1, kWave);
-    if (lane == kWave - 1) tmp[wid] = inc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        u64 m = 0;
-        for (int w = 0; w < nw; ++w) {
-            const u64 t = tmp[w];
-            tmp[w] = m;
-            m = max(m, t);
-        }
-    }
-    __syncthreads();
-    const u32 r = max(u32(tmp[wid]), lane ? exc_w : 0u);
-    __syncthreads();
-    return r;
-}
-
-// Per bucket: entries go into an LDS open-addressing table keyed by hash
-// (home slot from the 12 hash bits below the bucket bits, linear probing),
-// which names each group by its slot.  A counting sort on the slot makes each
-// group contiguous; each member then finds its place in ascending candidate
-// order by counting the group members with a smaller candidate index.
-// perm lists the bucket in (group, candidate) order; magnitudes stay in
-// place and are reached through perm.
-template <int T, int CAP, int TS>
-__global__ __launch_bounds__(T) void k_dedup(const Entry *E, const u64 *boff, u64 nbk, u32 L, double tol, u64 *UH,
-                                             u64 *UR, double2 *UM, u64 *counters, u64 *nhist, u64 *biglist) {
-    constexpr int PER = CAP / T;
-    constexpr int SPER = TS / T;
-    __shared__ u64 key[CAP];
-    __shared__ u64 cand[CAP];
-    __shared__ double2 mag[CAP];
-    __shared__ u32 owner[TS];       // entry index + 1 of the slot's first entry, 0 = empty
-    __shared__ u32 gstart[TS + 1];  // group sizes, then their exclusive prefix
-    __shared__ u16 gslot[CAP];      // group slot of each entry
-    __shared__ u16 perm0[CAP];      // entries grouped by slot
-    __shared__ u16 perm[CAP];       // entries in (group, candidate) order
-    __shared__ u32 nh[kNormBins];
-    __shared__ u64 tmp[T / kWave + 1];
-    __shared__ u32 maxg;
-    const u32 shift = (L + 12 <= 64) ? 64 - L - 12 : 0;
-    for (int i = threadIdx.x; i < kNormBins; i += T) nh[i] = 0;
-    for (int i = threadIdx.x; i < TS; i += T) owner[i] = 0;
-    for (int i = threadIdx.x; i <= TS; i += T) gstart[i] = 0;
-    u64 uniques = 0;  // thread 0's running total
-
-    Entry x[PER];
-    u64 nbeg = 0, nnum = 0;
-    auto fetch = [&](u64 bb) {
-        nbeg = boff[bb];
-        nnum = boff[bb + 1] - nbeg;
-        if (nnum <= CAP) {
-#pragma unroll
-            for (int k = 0; k < PER; ++k) {
-                const u32 i = threadIdx.x + T * k;
-                if (i < nnum) x[k] = E[nbeg + i];
-            }
-        }
-    };
-    if (blockIdx.x < nbk) fetch(blockIdx.x);
-    __syncthreads();
-    for (u64 b = blockIdx.x; b < nbk; b += gridDim.x) {
-        const u64 beg = nbeg, nn = nnum;  // uniform over the block
-        if (nn > CAP) {
-            if (threadIdx.x == 0) biglist[atomicAdd(reinterpret_cast<unsigned long long *>(counters + 5), 1ull)] = b;
-            __syncthreads();
-            if (b + gridDim.x < nbk) fetch(b + gridDim.x);
-            continue;
-        }
-        const u32 n = u32(nn);
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const u32 i = threadIdx.x + T * k;
-            if (i < n) {
-                key[i] = x[k].hash;
-                cand[i] = x[k].cand;
-                mag[i] = make_double2(x[k].re, x[k].im);
-            }
-        }
-        if (threadIdx.x == 0) maxg = 1;
-        __syncthreads();
-        if (b + gridDim.x < nbk) fetch(b + gridDim.x);
-        if (n == 0) continue;  // uniform
-        // insert: find or claim the group slot of every entry
-        u32 rank[PER];
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const u32 i = threadIdx.x + T * k;
-            if (i < n) {
-                const u64 h = key[i];
-                u32 sl = u32(h >> shift) & (TS - 1);
-                for (;;) {
-                    const u32 o = atomicCAS(&owner[sl], 0u, i + 1);
-                    if (o == 0 || key[o - 1] == h) break;
-                    sl = (sl + 1) & (TS - 1);
-                }
-                gslot[i] = u16(sl);
-                rank[k] = atomicAdd(&gstart[sl], 1u);
-            }
-        }
-        __syncthreads();
-        // group sizes -> offsets (exclusive scan over the slots)
-        {
-            u32 v[SPER];
-            u64 sum = 0;
-#pragma unroll
-            for (int k = 0; k < SPER; ++k) {
-                v[k] = gstart[threadIdx.x * SPER + k];
-                sum += v[k];
-                if (v[k] > 1) atomicMax(&maxg, v[k]);
-            }
-            u64 total;
-            u64 run = block_excl_scan(sum, tmp, total);
-#pragma unroll
-            for (int k = 0; k < SPER; ++k) {
-                gstart[threadIdx.x * SPER + k] = u32(run);
-                run += v[k];
-            }
-            if (threadIdx.x == 0) gstart[TS] = u32(total);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const u32 i = threadIdx.x + T * k;
-            if (i < n) perm0[gstart[gslot[i]] + rank[k]] = u16(i);
-        }
-        __syncthreads();
-        // place each member by the number of smaller candidates in its group
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const u32 i = threadIdx.x + T * k;
-            if (i < n) {
-                const u32 sl = gslot[i];
-                const u32 g0 = gstart[sl], g1 = gstart[sl + 1];
-                const u64 ci = cand[i];
-                u32 r = 0;
-                for (u32 q = g0; q < g1; ++q) r += cand[perm0[q]] < ci;
-                perm[g0 + r] = u16(i);
-            }
-        }
-        __syncthreads();
-        // segmented pairwise tree over sorted positions (values live at perm[])
-        const u32 mg = maxg;
-        for (u32 st = 1; st < mg; st <<= 1) {
-            for (u32 p = threadIdx.x; p < n; p += T) {
-                const u32 ia = perm[p];
-                const u32 g0 = gstart[gslot[ia]];
-                const u32 gsz = gstart[gslot[ia] + 1] - g0;
-                const u32 r = p - g0;
-                if ((r & (2 * st - 1)) == 0 && r + st < gsz) {
-                    const u32 ib = perm[p + st];
-                    const double2 a = mag[ia], c = mag[ib];
-                    mag[ia] = make_double2(a.x + c.x, a.y + c.y);
-                }
-            }
-            __syncthreads();
-        }
-        // compact the kept groups (their first sorted member) to the front of
-        // the bucket's slot range
-        u32 cnt = 0;
-        u32 keepbits = 0;
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const u32 p = threadIdx.x * PER + k;
-            if (p < n) {
-                const u32 ia = perm[p];
-                if (gstart[gslot[ia]] == p) {
-                    const double2 a = mag[ia];
-                    const double nrm = a.x * a.x + a.y * a.y;
-                    if (nrm > tol) {
-                        keepbits |= 1u << k;
-                        ++cnt;
-                        atomicAdd(&nh[min(norm_top12(nrm), u32(kNormBins - 1))], 1u);
-                    }
-                }
-            }
-        }
-        u64 total;
-        u64 pos = beg + block_excl_scan(cnt, tmp, total);
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            if (keepbits & (1u << k)) {
-                const u32 pi = perm[threadIdx.x * PER + k];
-                UH[pos] = key[pi];
-                UR[pos] = cand[pi];
-                UM[pos] = mag[pi];
-                ++pos;
-            }
-        }
-        for (u64 jx = beg + total + threadIdx.x; jx < beg + n; jx += T) UR[jx] = kNoRep;
-        if (threadIdx.x == 0) uniques += total;
-        __syncthreads();
-        // clear the table: owners of the slots this bucket used, and every
-        // group offset (the scan wrote all of them)
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const u32 i = threadIdx.x + T * k;
-            if (i < n) owner[gslot[i]] = 0;
-        }
-        for (int i = threadIdx.x; i <= TS; i += T) gstart[i] = 0;
-        __syncthreads();
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < kNormBins; i += T)
-        if (nh[i]) atomicAdd(reinterpret_cast<unsigned long long *>(nhist + i), (unsigned long long)nh[i]);
-    if (threadIdx.x == 0 && uniques) atomicAdd(reinterpret_cast<unsigned long long *>(counters), uniques);
-}
-
-// The same computation for buckets beyond the LDS capacity, one workgroup
-// per bucket, on a global scratch region (rare: a bucket this large needs a
-// group of thousands of candidates with one hash).  Scratch of listed bucket
-// q starts at soff[q] elements: keys, cands, mags (npad each) and u32 segs.
-__global__ __launch_bounds__(1024) void k_dedup_big(const Entry *E, const u64 *boff, const u64 *biglist,
-                                                    const u64 *soff, u64 *skeyg, u64 *scandg, double2 *smagg,
-                                                    u32 *segg, double tol, u64 *UH, u64 *UR, double2 *UM,
-                                                    u64 *counters, u64 *nhist) {
-    __shared__ u32 maxg;
-    __shared__ u64 tmp[1024 / kWave + 1];
-    const u64 b = biglist[blockIdx.x];
-    const u64 beg = boff[b], n = boff[b + 1] - beg;
-    u64 *skey = skeyg + soff[blockIdx.x];
-    u64 *scand = scandg + soff[blockIdx.x];
-    double2 *smag = smagg + soff[blockIdx.x];
-    u32 *seg = segg + soff[blockIdx.x];
-    u64 npad = 2;
-    while (npad < n) npad <<= 1;
-    for (u64 i = threadIdx.x; i < npad; i += blockDim.x) {
-        if (i < n) {
-            skey[i] = E[beg + i].hash;
-            scand[i] = E[beg + i].cand;
-        } else {
-            skey[i] = ~0ull;
-            scand[i] = ~0ull;
-        }
-    }
-    if (threadIdx.x == 0) maxg = 1;
-    __syncthreads();
-    for (u64 k = 2; k <= npad; k <<= 1) {
-        for (u64 j = k >> 1; j > 0; j >>= 1) {
-            for (u64 t = threadIdx.x; t < npad / 2; t += blockDim.x) {
-                const u64 i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-                const u64 p = i | j;
-                const u64 ka = skey[i], ca = scand[i], kb = skey[p], cb = scand[p];
-                if (key_gt(ka, ca, kb, cb) == ((i & k) == 0)) {
-                    skey[i] = kb;
-                    scand[i] = cb;
-                    skey[p] = ka;
-                    scand[p] = ca;
-                }
-            }
-            __syncthreads();
-        }
-    }
-    // magnitudes: candidate c of this bucket is found by its position in the
-    // input (entries carry their candidate index, not their position), so
-    // re-read them by a search over the sorted (hash, cand) pairs
-    for (u64 i = threadIdx.x; i < n; i += blockDim.x) {
-        const Entry e = E[beg + i];
-        u64 lo = 0, hi = n - 1;
-        while (lo < hi) {
-            const u64 mid = (lo + hi) >> 1;
-            if (key_gt(e.hash, e.cand, skey[mid], scand[mid])) lo = mid + 1;
-            else hi = mid;
-        }
-        smag[lo] = make_double2(e.re, e.im);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        u32 run = 0;
-        for (u64 i = 0; i < n; ++i) {
-            if (i == 0 || skey[i] != skey[i - 1]) run = u32(i);
-            seg[i] = run;
-            if (i + 1 - run > maxg) maxg = u32(i + 1 - run);
-        }
-    }
-    __syncthreads();
-    for (u64 st = 1; st < maxg; st <<= 1) {
-        for (u64 i = threadIdx.x; i < n; i += blockDim.x) {
-            const u64 r = i - seg[i];
-            if ((r & (2 * st - 1)) == 0 && i + st < n && seg[i + st] == seg[i]) {
-                const double2 a = smag[i], c = smag[i + st];
-                smag[i] = make_double2(a.x + c.x, a.y + c.y);
-            }
-        }
-        __syncthreads();
-    }
-    // serial compaction by thread 0 (this path is rare)
-    if (threadIdx.x == 0) {
-        u64 pos = beg;
-        for (u64 i = 0; i < n; ++i) {
-            if (seg[i] != i) continue;
-            const double2 a = smag[i];
-            const double nrm = a.x * a.x + a.y * a.y;
-            if (nrm > tol) {
-                UH[pos] = skey[i];
-                UR[pos] = scand[i];
-                UM[pos] = a;
-                ++pos;
-                atomicAdd(reinterpret_cast<unsigned long long *>(nhist + min(norm_top12(nrm), u32(kNormBins - 1))),
-                          1ull);
-            }
-        }
-        atomicAdd(reinterpret_cast<unsigned long long *>(counters), (unsigned long long)(pos - beg));
-        for (u64 j = pos; j < beg + n; ++j) UR[j] = kNoRep;
-    }
-}
-
-// ===========================================================================
-// phase 6: top-K select on the 128-bit key (|m|^2 bits, ~hash)
-// ===========================================================================
-struct SelItem {
-    u64 hi, lo, rep;
-};
-
-__device__ inline u32 key_digit(u64 hi, u64 lo, u32 pos) {
-    u64 top;
-    if (pos == 0) top = hi;
-    else if (pos < 64) top = (hi << pos) | (lo >> (64 - pos));
-    else top = lo << (pos - 64);
-    return u32(top >> 52);
-}
-
-// Appends for the lanes with pred set: one atomic per wave; returns this
-// lane's slot (meaningful only when pred).
-__device__ inline u64 wave_append(bool pred, u64 *counter) {
-    const u64 mask = __ballot(pred);
-    if (!mask) return 0;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int leader = __ffsll((unsigned long long)mask) - 1;
-    u64 base = 0;
-    if (lane == leader) base = atomicAdd(reinterpret_cast<unsigned long long *>(counter), (unsigned long long)__popcll(mask));
-    base = __shfl(base, leader, kWave);
-    return base + __popcll(mask & ((1ull << lane) - 1));
-}
-
-__device__ inline void mark(u64 *bitmap, u64 rep) {
-    atomicOr(reinterpret_cast<unsigned long long *>(bitmap + (rep >> 6)), 1ull << (rep & 63));
-}
-
-// First select stage over the dedup output slots: histogram of |m|^2 bits
-// 51..40 among the uniques whose bits 63..52 equal d1.
-__global__ __launch_bounds__(256) void k_sel_hist_u(const u64 *UR, const double2 *UM, u64 U, u32 d1, u64 *hist) {
-    __shared__ u32 h[kHistBins];
-    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) h[i] = 0;
-    __syncthreads();
-    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < U; i += u64(gridDim.x) * blockDim.x) {
-        if (UR[i] == kNoRep) continue;
-        const double2 m = UM[i];
-        const u64 bits = u64(__double_as_longlong(m.x * m.x + m.y * m.y));
-        if (u32(bits >> 52) == d1) atomicAdd(&h[u32(bits >> 40) & 0xFFF], 1u);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x)
-        if (h[i]) atomicAdd(reinterpret_cast<unsigned long long *>(hist + i), (unsigned long long)h[i]);
-}
-
-// Marks the uniques whose top 24 |m|^2 bits exceed prefix24 and collects
-// the ones equal to it.
-__global__ __launch_bounds__(256) void k_sel_first(const u64 *UH, const u64 *UR, const double2 *UM, u64 U,
-                                                   u32 prefix24, SelItem *S, u64 *scount, u64 *bitmap) {
-    const u64 stride = u64(gridDim.x) * blockDim.x;
-    // uniform trip count per wave so that wave_append sees every lane
-    for (u64 base = u64(blockIdx.x) * blockDim.x; base < U; base += stride) {
-        const u64 i = base + threadIdx.x;
-        bool take = false;
-        SelItem it{};
-        if (i < U && UR[i] != kNoRep) {
-            const double2 m = UM[i];
-            const u64 bits = u64(__double_as_longlong(m.x * m.x + m.y * m.y));
-            const u32 t = u32(bits >> 40);
-            if (t > prefix24) mark(bitmap, UR[i]);
-            else if (t == prefix24) {
-                take = true;
-                it = SelItem{bits, ~UH[i], UR[i]};
-            }
-        }
-        const u64 at = wave_append(take, scount);
-        if (take) S[at] = it;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_sel_hist(const SelItem *S, u64 n, u32 pos, u64 *hist) {
-    __shared__ u32 h[kHistBins];
-    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) h[i] = 0;
-    __syncthreads();
-    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
-        atomicAdd(&h[key_digit(S[i].hi, S[i].lo, pos)], 1u);
-    __syncthreads();
-    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x)
-        if (h[i]) atomicAdd(reinterpret_cast<unsigned long long *>(hist + i), (unsigned long long)h[i]);
-}
-
-__global__ __launch_bounds__(256) void k_sel_split(const SelItem *S, u64 n, u32 pos, u32 d, SelItem *out,
-                                                   u64 *ocount, u64 *bitmap) {
-    const u64 stride = u64(gridDim.x) * blockDim.x;
-    for (u64 base = u64(blockIdx.x) * blockDim.x; base < n; base += stride) {
-        const u64 i = base + threadIdx.x;
-        bool take = false;
-        SelItem it{};
-        if (i < n) {
-            it = S[i];
-            const u32 t = key_digit(it.hi, it.lo, pos);
-            if (t > d) mark(bitmap, it.rep);
-            else take = t == d;
-        }
-        const u64 at = wave_append(take, ocount);
-        if (take) out[at] = it;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_sel_mark_items(const SelItem *S, u64 n, u64 *bitmap) {
-    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
-        mark(bitmap, S[i].rep);
-}
-
-__global__ __launch_bounds__(256) void k_mark_all(const u64 *UR, u64 U, u64 *bitmap) {
-    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < U; i += u64(gridDim.x) * blockDim.x)
-        if (UR[i] != kNoRep) mark(bitmap, UR[i]);
-}
-
-// ===========================================================================
-// phase 7/8: order, sizes, populate
-// ===========================================================================
-__global__ __launch_bounds__(256) void k_place(const u64 *UR, const double2 *UM, u64 U, const u64 *bitmap,
-                                               const u64 *wprefix, u64 *orep, double2 *omag) {
-    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < U; i += u64(gridDim.x) * blockDim.x) {
-        const u64 rep = UR[i];
-        if (rep == kNoRep) continue;
-        const u64 word = bitmap[rep >> 6];
-        const u32 bit = u32(rep & 63);
-        if ((word >> bit) & 1) {
-            const u64 r = wprefix[rep >> 6] + __popcll(word & ((1ull << bit) - 1));
-            orep[r] = rep;
-            omag[r] = UM[i];
-        }
-    }
-}
-
-__device__ inline u32 align_pad(u32 size, u32 align) {
-    if (align <= 1) return 0;
-    const u32 off = align - size % align;
-    return off == align ? 0 : off;
-}
-
-template <int KIND>
-__global__ __launch_bounds__(256) void k_child_size(Parents P, const u64 *cb, qd_rule rule, const u64 *orep, u64 K,
-                                                    u32 align, u32 *osize, u64 *padded) {
-    for (u64 k = u64(blockIdx.x) * blockDim.x + threadIdx.x; k < K; k += u64(gridDim.x) * blockDim.x) {
-        const u64 c = orep[k];
-        const u64 p = find_parent(cb, P.n, c);
-        const u32 j = u32(c - cb[p]);
-        const u8 *o = P.obj + P.begin[p];
-        u32 s;
-        if (KIND == QD_RULE_QCGD_SPLIT_MERGE) {
-            Graph g(o);
-            s = sm_child_size(g, sm_first_hmlz(g), j);
-        } else {
-            s = P.size[p];
-        }
-        osize[k] = s;
-        padded[k] = u64(s) + align_pad(s, align);
-    }
-}
-
-template <int KIND>
-__global__ __launch_bounds__(256) void k_populate(Parents P, const u64 *cb, qd_rule rule, const u64 *orep, u64 K,
-                                                  const u64 *obegin, const u32 *osize, u8 *oobj) {
-    for (u64 k = u64(blockIdx.x) * blockDim.x + threadIdx.x; k < K; k += u64(gridDim.x) * blockDim.x) {
-        const u64 c = orep[k];
-        const u64 p = find_parent(cb, P.n, c);
-        const u32 j = u32(c - cb[p]);
-        const u8 *o = P.obj + P.begin[p];
-        u8 *d = oobj + obegin[k];
-        u32 s;
-        if (KIND == QD_RULE_HADAMARD) {
-            hadamard_child_write(o, P.size[p], rule.bit, j, d);
-            s = P.size[p];
-        } else if (KIND == QD_RULE_QCGD_SPLIT_MERGE) {
-            Graph g(o);
-            s = sm_child_write(g, sm_first_hmlz(g), j, d);
-        } else {
-            flip_child_write(o, P.size[p], KIND == QD_RULE_QCGD_COIN, j, d);
-            s = P.size[p];
-        }
-        const u64 e = obegin[k + 1];
-        for (u64 q = obegin[k] + s; q < e; ++q) oobj[q] = 0;
-        (void)osize;
-    }
-}
-
-// ===========================================================================
-// phase 9: canonical sum of |m|^2 and scaling
-// ===========================================================================
-constexpr int kSumThreads = 256;
-constexpr int kSumChunk = 2048;
-
-// One block per chunk of 2048 values: lane t sums t, t+256, ..., t+1792 in
-// order, then a halving tree s[t] += s[t+stride], stride = 128 .. 1.
-__global__ __launch_bounds__(kSumThreads) void k_sum_chunks(const double2 *mag, const double *vals, u64 n,
-                                                            double *out) {
-    __shared__ double s[kSumThreads];
-    const u64 base = u64(blockIdx.x) * kSumChunk;
-    double a = 0.0;
-    for (int j = 0; j < kSumChunk / kSumThreads; ++j) {
-        const u64 i = base + threadIdx.x + u64(kSumThreads) * j;
-        if (i < n) {
-            double v;
-            if (mag) {
-                const double2 m = mag[i];
-                v = m.x * m.x + m.y * m.y;
-            } else {
-                v = vals[i];
-            }
-            a += v;
-        }
-    }
-    s[threadIdx.x] = a;
-    __syncthreads();
-    for (int stride = kSumThreads / 2; stride > 0; stride >>= 1) {
-        if (threadIdx.x < stride) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + stride];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[blockIdx.x] = s[0];
-}
-
-__global__ __launch_bounds__(256) void k_scale(double2 *mag, u64 n, double f) {
-    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x) {
-        double2 m = mag[i];
-        m.x = m.x / f;
-        m.y = m.y / f;
-        mag[i] = m;
-    }
-}
-
-// ===========================================================================
-// modifiers
-// ===========================================================================
-__global__ __launch_bounds__(256) void k_modifier(u8 *obj, const u64 *begin, const u32 *size, double2 *mag, u64 n,
-                                                  qd_modifier m, u32 *err) {
-    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x) {
-        u8 *o = obj + begin[i];
-        const u32 sz = size[i];
-        if (m.kind <= QD_MOD_ZGATE) {
-            if (m.bit >= sz || (m.kind == QD_MOD_CNOT && m.control_bit >= sz)) {
-                atomicOr(err, ERR_BIT);
-                continue;
-            }
-        } else if (!qcgd_valid(o, sz)) {
-            atomicOr(err, ERR_OBJECT);
-            continue;
-        }
-        double2 mm = mag[i];
-        cplx c{mm.x, mm.y};
-        modifier_apply(m, o, sz, c);
-        mag[i] = make_double2(c.re, c.im);
-    }
 }
 
 // ===========================================================================
@@ -1104,6 +222,8 @@ static int check_err_word(u32 w) {
     if (w & ERR_BIT) return fail(QD_ERR_INVALID, "rule or modifier bit index beyond an object's size");
     if (w & ERR_OBJECT) return fail(QD_ERR_INVALID, "malformed QCGD object (size != 4 + 4n + 16m, or n == 0)");
     if (w & ERR_OVERFLOW) return fail(QD_ERR_CAPACITY, "a dedup bucket exceeded its LDS capacity");
+    (void)ERR_WIDE;
+    (void)ERR_SPILL;
     return QD_OK;
 }
 
@@ -1200,7 +320,7 @@ static int select_loop(qd_ctx *ctx, SelItem *S0, SelItem *S1, u64 ns, u64 r, u32
 
 
 // Buckets beyond the LDS capacity: one workgroup each on global scratch.
-static int dedup_big_buckets(qd_ctx *ctx, const Entry *D, const u64 *boff, u64 nbig, double tol, u64 *UH, u64 *UR,
+static int dedup_big_buckets(qd_ctx *ctx, const Entry *D, u64 rstride, const u64 *boff, u64 nbig, double tol, u64 *UH, u64 *UR,
                              double2 *UM, u64 *U) {
     hipStream_t s = ctx->stream;
     std::vector<u64> list(nbig), soff(nbig + 1, 0);
@@ -1220,7 +340,7 @@ static int dedup_big_buckets(qd_ctx *ctx, const Entry *D, const u64 *boff, u64 n
     QD_TRY(ctx->bigseg.ensure(tot * sizeof(u32)));
     QD_HIP(hipMemcpyAsync(ctx->bigsoff.p, soff.data(), (nbig + 1) * sizeof(u64), hipMemcpyHostToDevice, s));
     u64 *counters = ctx->counters.as<u64>();
-    hipLaunchKernelGGL(k_dedup_big, dim3(nbig), dim3(1024), 0, s, D, boff, ctx->biglist.as<u64>(),
+    hipLaunchKernelGGL(k_dedup_big, dim3(nbig), dim3(1024), 0, s, D, rstride, boff, ctx->biglist.as<u64>(),
                        ctx->bigsoff.as<u64>(), ctx->bigkey.as<u64>(), ctx->bigcand.as<u64>(), ctx->bigmag.as<double2>(),
                        ctx->bigseg.as<u32>(), tol, UH, UR, UM, counters, ctx->nhist.as<u64>());
     QD_HIP(hipGetLastError());
@@ -1303,92 +423,142 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
     if (KIND == QD_RULE_QCGD_SPLIT_MERGE) QD_TRY(ctx->tab.ensure((in->bytes / 2 + 16) * sizeof(u64)));
     u64 *counters = ctx->counters.as<u64>();
     // counters: [0] unique count, [1] select count A, [2] select count B,
-    // [3] low: max child size, [4] low: error word
+    // [3] low: max child size, [4] low: error word, [5] oversized buckets
     QD_HIP(hipMemsetAsync(counters, 0, 64 * sizeof(u64), s));
     u32 *maxsz = reinterpret_cast<u32 *>(counters + 3);
     u32 *errw = reinterpret_cast<u32 *>(counters + 4);
-    prof_begin(ctx, QD_PHASE_PREP);
-    hipLaunchKernelGGL(k_prep<KIND>, dim3(grid_for(N)), dim3(256), 0, s, P, keep, rule, ctx->nc.as<u32>(),
-                       ctx->pre.as<u64>(), ctx->tab.as<u64>(), maxsz, errw);
-    QD_HIP(hipGetLastError());
-
-    // ---- 2 scan -> child_begin, C
-    prof_end(ctx, QD_PHASE_PREP);
-    prof_begin(ctx, QD_PHASE_SCAN);
-    QD_TRY(run_scan(ctx, LoadU32{ctx->nc.as<u32>()}, N, ctx->cb.as<u64>()));
-    prof_end(ctx, QD_PHASE_SCAN);
+    bool fast = KIND != QD_RULE_HADAMARD && !ctx->tune.generic_path;
     u64 *hb = ctx->host;
-    QD_HIP(hipMemcpyAsync(hb, ctx->cb.as<u64>() + N, sizeof(u64), hipMemcpyDeviceToHost, s));
-    QD_HIP(hipMemcpyAsync(hb + 1, counters + 3, 2 * sizeof(u64), hipMemcpyDeviceToHost, s));
-    QD_HIP(hipStreamSynchronize(s));
+    u64 nitems = 0;  // work items of the fast generate
+    if (fast) {
+        QD_TRY(ctx->pdesc.ensure(N * sizeof(PDesc)));
+        QD_TRY(ctx->wi.ensure(N * sizeof(u32)));
+        QD_TRY(ctx->wb.ensure((N + 1) * sizeof(u64)));
+        prof_begin(ctx, QD_PHASE_PREP);
+        hipLaunchKernelGGL(k_prep_fast<KIND>, dim3(grid_for(N, 4, 65536)), dim3(256), 0, s, P, keep,
+                           ctx->pdesc.as<PDesc>(), ctx->nc.as<u32>(), ctx->wi.as<u32>(), ctx->tab.as<u64>(), maxsz,
+                           errw);
+        QD_HIP(hipGetLastError());
+        prof_end(ctx, QD_PHASE_PREP);
+        prof_begin(ctx, QD_PHASE_SCAN);
+        QD_TRY(run_scan(ctx, LoadU32{ctx->nc.as<u32>()}, N, ctx->cb.as<u64>()));
+        QD_TRY(run_scan(ctx, LoadU32{ctx->wi.as<u32>()}, N, ctx->wb.as<u64>()));
+        prof_end(ctx, QD_PHASE_SCAN);
+        QD_HIP(hipMemcpyAsync(hb, ctx->cb.as<u64>() + N, sizeof(u64), hipMemcpyDeviceToHost, s));
+        QD_HIP(hipMemcpyAsync(hb + 1, counters + 3, 2 * sizeof(u64), hipMemcpyDeviceToHost, s));
+        QD_HIP(hipMemcpyAsync(hb + 3, ctx->wb.as<u64>() + N, sizeof(u64), hipMemcpyDeviceToHost, s));
+        QD_HIP(hipStreamSynchronize(s));
+        prof_collect(ctx);
+        nitems = hb[3];
+        if (u32(hb[2]) & ERR_WIDE) {  // a graph wider than 64 nodes: generic kernels
+            fast = false;
+            QD_HIP(hipMemsetAsync(counters + 3, 0, 2 * sizeof(u64), s));
+        }
+    }
+    if (!fast) {
+        prof_begin(ctx, QD_PHASE_PREP);
+        hipLaunchKernelGGL(k_prep<KIND>, dim3(grid_for(N)), dim3(256), 0, s, P, keep, rule, ctx->nc.as<u32>(),
+                           ctx->pre.as<u64>(), ctx->tab.as<u64>(), maxsz, errw);
+        QD_HIP(hipGetLastError());
+        prof_end(ctx, QD_PHASE_PREP);
+        prof_begin(ctx, QD_PHASE_SCAN);
+        QD_TRY(run_scan(ctx, LoadU32{ctx->nc.as<u32>()}, N, ctx->cb.as<u64>()));
+        prof_end(ctx, QD_PHASE_SCAN);
+        QD_HIP(hipMemcpyAsync(hb, ctx->cb.as<u64>() + N, sizeof(u64), hipMemcpyDeviceToHost, s));
+        QD_HIP(hipMemcpyAsync(hb + 1, counters + 3, 2 * sizeof(u64), hipMemcpyDeviceToHost, s));
+        QD_HIP(hipStreamSynchronize(s));
+        prof_collect(ctx);
+    }
     const u64 C = hb[0];
     QD_TRY(check_err_word(u32(hb[2])));
     st.num_candidates = C;
     if (C == 0) return finish_empty();
 
-    // ---- 3 generate
+    // ---- 3 generate: hash and magnitude at the candidate index
     u32 b1, b2;
     dedup_layout(C, ctx->tune, b1, b2);
     st.dedup_levels = (b1 ? 1 : 0) + (b2 ? 1 : 0);
-    QD_TRY(ctx->ebuf0.ensure(C * sizeof(Entry)));
-    QD_TRY(ctx->ebuf1.ensure(C * sizeof(Entry)));
+    QD_TRY(ctx->ebuf0.ensure(C * 32));
+    QD_TRY(ctx->ebuf1.ensure(C * 32));
     QD_TRY(ctx->hist1.ensure(kHistBins * sizeof(u64)));
     QD_TRY(ctx->off1.ensure((kHistBins + 1) * sizeof(u64)));
     QD_TRY(ctx->cur1.ensure(kHistBins * sizeof(u64)));
     QD_TRY(ctx->nhist.ensure(kHistBins * sizeof(u64)));
-    Entry *E0 = ctx->ebuf0.as<Entry>(), *E1 = ctx->ebuf1.as<Entry>();
+    u64 *H0 = ctx->ebuf0.as<u64>();
+    double2 *M0 = reinterpret_cast<double2 *>(H0 + C);
+    Entry *E1 = ctx->ebuf1.as<Entry>();
     u64 *hist1 = ctx->hist1.as<u64>();
     QD_HIP(hipMemsetAsync(hist1, 0, kHistBins * sizeof(u64), s));
     QD_HIP(hipMemsetAsync(ctx->nhist.p, 0, kHistBins * sizeof(u64), s));
     prof_begin(ctx, QD_PHASE_GENERATE);
-    {
+    if (fast) {
+        const unsigned gg =
+            unsigned(std::min<u64>((nitems + kGenThreads - 1) / kGenThreads, ctx->tune.generate_grid));
+        hipLaunchKernelGGL(k_generate_fast<KIND>, dim3(std::max(gg, 1u)), dim3(kGenThreads), 0, s, ctx->pdesc.as<PDesc>(),
+                           P.mag, ctx->cb.as<u64>(), ctx->wb.as<u64>(), N, ctx->tab.as<u64>(), P.begin, rule, nitems, H0, M0,
+                           b1, hist1);
+    } else {
         const unsigned gb = ctx->tune.generate_block;
         const unsigned gg = grid_for(C, gb, ctx->tune.generate_grid);
         hipLaunchKernelGGL(k_generate<KIND>, dim3(gg), dim3(gb), 0, s, P, ctx->cb.as<u64>(), rule, ctx->pre.as<u64>(),
-                           ctx->tab.as<u64>(), C, E0, b1, hist1);
+                           ctx->tab.as<u64>(), C, H0, M0, b1, hist1);
     }
     QD_HIP(hipGetLastError());
-
     prof_end(ctx, QD_PHASE_GENERATE);
+
     // ---- 4 partition
-    const Entry *D = E0;         // entries the dedup reads
-    const u64 *boff = nullptr;   // bucket offsets (nbk + 1)
+    const Entry *D = E1;       // candidates the dedup reads
+    u64 rstride = 0;           // fixed-capacity read stride of D (0: compact)
+    const u64 *woff = nullptr; // bucket output offsets (nbk + 1)
     u64 nbk = 1;
-    void *ubuf = ctx->ebuf1.p;   // where the uniques go (the buffer D does not read)
+    void *ubuf = ctx->ebuf0.p; // where the uniques go (H0/M0 are dead by then)
+    const u64 nb1 = 1ull << b1;
+    prof_begin(ctx, QD_PHASE_PARTITION);
     if (b1 == 0) {
-        QD_TRY(ctx->off1.ensure(2 * sizeof(u64)));
         u64 two[2] = {0, C};
         QD_HIP(hipMemcpyAsync(ctx->off1.p, two, sizeof two, hipMemcpyHostToDevice, s));
+        QD_HIP(hipMemcpyAsync(ctx->cur1.p, two, sizeof(u64), hipMemcpyHostToDevice, s));
         QD_HIP(hipStreamSynchronize(s));
-        boff = ctx->off1.as<u64>();
     } else {
-        const u64 nb1 = 1ull << b1;
-        prof_begin(ctx, QD_PHASE_PARTITION);
         QD_TRY(run_scan(ctx, LoadU64{hist1}, nb1, ctx->off1.as<u64>()));
         QD_HIP(hipMemcpyAsync(ctx->cur1.p, ctx->off1.p, nb1 * sizeof(u64), hipMemcpyDeviceToDevice, s));
-        hipLaunchKernelGGL(k_scatter1, dim3((C + kTile1 - 1) / kTile1), dim3(256), 0, s, (const Entry *)E0, C, b1,
-                           ctx->cur1.as<u64>(), E1);
+    }
+    hipLaunchKernelGGL(k_scatter1, dim3((C + kTile1 - 1) / kTile1), dim3(256), 0, s, (const u64 *)H0,
+                       (const double2 *)M0, C, b1, ctx->cur1.as<u64>(), E1);
+    QD_HIP(hipGetLastError());
+    if (b2 == 0) {
+        woff = ctx->off1.as<u64>();
+        nbk = nb1;
+    } else {
+        const u64 nb = 1ull << (b1 + b2);
+        const u64 mean = (C + nb - 1) / nb;
+        const u64 cap2 = ((mean + mean / 2 + 128) + 63) / 64 * 64;
+        QD_TRY(ctx->off2.ensure((nb + 1) * sizeof(u64)));
+        QD_TRY(ctx->cnt2.ensure(nb * sizeof(u64)));
+        QD_TRY(ctx->ebuf2.ensure(std::max<u64>(nb * cap2, C) * sizeof(Entry)));
+        Entry *E2 = ctx->ebuf2.as<Entry>();
+        hipLaunchKernelGGL(k_level2_fixed, dim3(nb1), dim3(1024), 0, s, (const Entry *)E1, ctx->off1.as<u64>(), b1, b2,
+                           cap2, E2, ctx->cnt2.as<u64>(), errw);
         QD_HIP(hipGetLastError());
-        if (b2 == 0) {
-            D = E1;
-            boff = ctx->off1.as<u64>();
-            nbk = nb1;
-            ubuf = ctx->ebuf0.p;
-        } else {
-            const u64 nb = 1ull << (b1 + b2);
-            QD_TRY(ctx->off2.ensure((nb + 1) * sizeof(u64)));
+        QD_TRY(run_scan(ctx, LoadU64{ctx->cnt2.as<u64>()}, nb, ctx->off2.as<u64>()));
+        u32 ew = 0;
+        QD_TRY(sync_read(ctx, errw, sizeof(u32), &ew));
+        rstride = cap2;
+        if (ew & ERR_SPILL) {  // a sub-bucket overflowed its capacity: exact offsets
+            QD_HIP(hipMemsetAsync(errw, 0, sizeof(u32), s));
             QD_HIP(hipMemcpyAsync(ctx->off2.as<u64>() + nb, ctx->cb.as<u64>() + N, sizeof(u64),
                                   hipMemcpyDeviceToDevice, s));
-            hipLaunchKernelGGL(k_level2, dim3(nb1), dim3(1024), 0, s, (const Entry *)E1, ctx->off1.as<u64>(), b1, b2,
-                               E0, ctx->off2.as<u64>());
+            hipLaunchKernelGGL(k_level2_exact, dim3(nb1), dim3(1024), 0, s, (const Entry *)E1, ctx->off1.as<u64>(), b1,
+                               b2, E2, ctx->off2.as<u64>());
             QD_HIP(hipGetLastError());
-            D = E0;
-            boff = ctx->off2.as<u64>();
-            nbk = nb;
-            ubuf = ctx->ebuf1.p;
+            rstride = 0;
+            st.reserved |= 1;  // spill fallback used
         }
+        D = E2;
+        woff = ctx->off2.as<u64>();
+        nbk = nb;
     }
-    if (b1) prof_end(ctx, QD_PHASE_PARTITION);
+    prof_end(ctx, QD_PHASE_PARTITION);
     const u32 L = b1 + b2;
 
     // ---- 5 dedup
@@ -1401,8 +571,8 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
         int ncu = 256;
         (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device);
         const u64 grid = std::min<u64>(nbk, u64(ncu));
-        hipLaunchKernelGGL((k_dedup<1024, 2048, 4096>), dim3(grid), dim3(1024), 0, s, D, boff, nbk, L, prm.tolerance, UH,
-                           UR, UM, counters, ctx->nhist.as<u64>(), ctx->biglist.as<u64>());
+        hipLaunchKernelGGL((k_dedup<1024, 2048, 4096>), dim3(grid), dim3(1024), 0, s, D, rstride, woff, nbk, L,
+                           prm.tolerance, UH, UR, UM, counters, ctx->nhist.as<u64>(), ctx->biglist.as<u64>());
         QD_HIP(hipGetLastError());
     }
     prof_end(ctx, QD_PHASE_DEDUP);
@@ -1411,7 +581,7 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
     prof_collect(ctx);
     QD_TRY(check_err_word(u32(hb[4])));
     u64 U = hb[0];
-    if (hb[5]) QD_TRY(dedup_big_buckets(ctx, D, boff, hb[5], prm.tolerance, UH, UR, UM, &U));
+    if (hb[5]) QD_TRY(dedup_big_buckets(ctx, D, rstride, woff, hb[5], prm.tolerance, UH, UR, UM, &U));
     st.num_unique = U;
 
     // memory-driven cap (max_num_object == 0, quids.hpp:459-486, 510-536):
@@ -1625,6 +795,7 @@ int qd_ctx_set_tuning(qd_ctx *c, const qd_tuning *t) {
         n.generate_block = g;
     }
     if (t->generate_grid) n.generate_grid = t->generate_grid;
+    n.generic_path = t->generic_path;  // 0 = fast kernels where they apply
     c->tune = n;
     return QD_OK;
 }

@@ -150,3 +150,25 @@ def test_empty_state(ctx):
     hs = HostState()
     out, st, _ = gpu_step(hs, qcgd.split_merge(1.0))
     assert out.num_object == 0 and st["num_candidates"] == 0 and out.total_proba == 0
+
+
+@pytest.mark.parametrize("generic", [0, 1])
+def test_qcgd_generate_paths_agree(ctx, generic):
+    """The QCGD fast generate (work items sharing walk prefixes) and the
+    generic byte-level generate both match the oracle."""
+    old = ctx.get_tuning()
+    ctx.set_tuning(generic_path=generic)
+    try:
+        hs_g = hs_o = qcgd_initial(7, 13)
+        rules = qcgd_iteration_rules() + [qcgd.coin(0.7, 0.4, 0.1)]
+        for it in range(3):
+            for r in rules:
+                if r.is_modifier:
+                    hs_g = gpu_modifier(hs_g, r)
+                    hs_o = orc.apply_modifier(hs_o, r)
+                else:
+                    hs_g, st, _ = gpu_step(hs_g, r, max_num_object=20000)
+                    hs_o, _ = orc.simulate(hs_o, r, max_num_object=20000)
+                    assert_states_identical(hs_g, hs_o, f"generic={generic} it {it} {type(r).__name__}")
+    finally:
+        ctx.set_tuning(generic_path=old["generic_path"])
