@@ -47,9 +47,10 @@ def alg_bytes(st):
     return {"generate": gen, "dedup": dedup, "finalize": fin, "total": gen + dedup + fin}
 
 
-def cpu_baseline(nodes, seed, cap, budget_s):
+def cpu_baseline(nodes, seed, cap, budget_s, iterations=2):
     """The compiled reference (oracle/_ref/ref_driver) on the same workload's
-    first iteration, on this host's cores."""
+    first `iterations` iterations (the second one is the first large step:
+    ~1.7e7 candidates truncated to 2^22), on this host's cores."""
     from oracle import oracle as orc
     from quids_amd.rules import qcgd
     from quids_amd.state import HostState
@@ -59,6 +60,7 @@ def cpu_baseline(nodes, seed, cap, budget_s):
     hs = HostState.from_objects([qcgd.seeded_graph(nodes, seed)])
     rules = iteration_rules()
     ops = [orc.mod_op(rules[0]), orc.rule_op(rules[1], cap, dump=False), orc.rule_op(rules[2], cap, dump=False)]
+    ops = ops * iterations
     try:
         _, out = orc.run_reference(1, hs, ops, threads=threads, timeout=budget_s)
     except Exception as e:  # noqa: BLE001
@@ -70,7 +72,8 @@ def cpu_baseline(nodes, seed, cap, budget_s):
         C += float(f["C"])
         secs += float(f["seconds"])
     return {"value": C / secs if secs else None, "unit": "children/s", "cores": threads, "kind": "reference",
-            "sample": f"reference simulate() on iteration 1 of the same workload ({nodes} nodes, cap 2^{cap.bit_length() - 1}), "
+            "sample": f"reference simulate() on iterations 1..{iterations} of the same workload "
+                      f"({nodes} nodes, cap 2^{cap.bit_length() - 1}), "
                       f"{int(C)} candidates in {secs:.2f} s, OMP_NUM_THREADS={threads}"}
 
 
@@ -84,6 +87,8 @@ def main():
     ap.add_argument("--log2-cap", type=int, default=22)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=120.0)
+    ap.add_argument("--copy-probe", type=float, default=0,
+                    help="only run the library's plain copy of this many GiB (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
@@ -93,6 +98,12 @@ def main():
     import quids_amd as q
     from quids_amd.rules import qcgd
     from quids_amd.state import HostState
+
+    if args.copy_probe:
+        nbytes = int(args.copy_probe * (1 << 30))
+        g = q.default_context().copy_probe(nbytes, reps=3)
+        print(json.dumps({"copy_probe_bytes": nbytes, "gbps_read_plus_write": g}), flush=True)
+        return
 
     cap = 1 << args.log2_cap
     rules = iteration_rules()

@@ -150,6 +150,12 @@ typedef struct qd_tuning {
 int qd_ctx_get_tuning(qd_ctx *ctx, qd_tuning *tuning);
 int qd_ctx_set_tuning(qd_ctx *ctx, const qd_tuning *tuning);
 
+/* Copies `bytes` of device memory `reps` times with a plain streaming
+ * kernel (no reference counterpart): the achievable-bandwidth reference of
+ * the autotuner and the calibration kernel of hardware byte counters.
+ * gbps receives (read + write bytes) / time of the fastest repetition. */
+int qd_copy_probe(qd_ctx *ctx, uint64_t bytes, int reps, double *gbps);
+
 /* ---- state: quids::it_t (src/quids.hpp:149-335) ------------------------ */
 int qd_state_create(qd_ctx *ctx, qd_state **state);
 int qd_state_destroy(qd_state *state);

@@ -63,6 +63,12 @@ class Context:
             setattr(t, k, int(v))
         check(lib().qd_ctx_set_tuning(self.handle, ctypes.byref(t)), "qd_ctx_set_tuning")
 
+    def copy_probe(self, nbytes, reps=3):
+        """(read + write) GB/s of a plain device copy of nbytes."""
+        g = ctypes.c_double()
+        check(lib().qd_copy_probe(self.handle, int(nbytes), int(reps), ctypes.byref(g)), "qd_copy_probe")
+        return g.value
+
     def set_profiling(self, enable=True):
         """Per-phase GPU timing of simulate() with HIP events."""
         check(lib().qd_ctx_set_profiling(self.handle, 1 if enable else 0), "qd_ctx_set_profiling")

@@ -121,6 +121,7 @@ SIGNATURES = {
     "qd_ctx_set_tuning": [_vp, ctypes.POINTER(QdTuning)],
     "qd_ctx_reset_phase_times": [_vp],
     "qd_ctx_phase_times": [_vp, _pd, _pu64],
+    "qd_copy_probe": [_vp, _u64, ctypes.c_int, _pd],
     "qd_state_create": [_vp, _pvp],
     "qd_state_destroy": [_vp],
     "qd_state_upload": [_vp, _u64, _vp, _pu64, _pu32, _pd, ctypes.c_double],

@@ -10,18 +10,21 @@ namespace qd {
 // ===========================================================================
 // phase 7/8: order, sizes, populate
 // ===========================================================================
-__global__ __launch_bounds__(256) void k_place(const u64 *UR, const double2 *UM, u64 U, const u64 *bitmap,
-                                               const u64 *wprefix, u64 *orep, double2 *omag) {
+// Survivors listed by their dedup output slot: rank = number of smaller
+// representatives (popcount prefix of the representative bitmap).
+// Survivors enumerated from the slot bitmap (one wave per 64 dedup output
+// slots); each goes to the rank of its representative among the marked
+// representatives.
+__global__ __launch_bounds__(256) void k_place(const u64 *UR, const double2 *UM, const u64 *sbm, u64 U,
+                                               const u64 *bitmap, const u64 *wprefix, u64 *orep, double2 *omag) {
     for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < U; i += u64(gridDim.x) * blockDim.x) {
+        if (!((sbm[i >> 6] >> (i & 63)) & 1)) continue;
         const u64 rep = UR[i];
-        if (rep == kNoRep) continue;
         const u64 word = bitmap[rep >> 6];
         const u32 bit = u32(rep & 63);
-        if ((word >> bit) & 1) {
-            const u64 r = wprefix[rep >> 6] + __popcll(word & ((1ull << bit) - 1));
-            orep[r] = rep;
-            omag[r] = UM[i];
-        }
+        const u64 r = wprefix[rep >> 6] + __popcll(word & ((1ull << bit) - 1));
+        orep[r] = rep;
+        omag[r] = UM[i];
     }
 }
 
@@ -51,29 +54,231 @@ __global__ __launch_bounds__(256) void k_child_size(Parents P, const u64 *cb, qd
     }
 }
 
+// One wavefront per survivor, everything lane-parallel:
+//  * erase_create / coin / hadamard: the child is the parent with some
+//    particle (qubit) bytes flipped -- one coalesced copy, the flips applied
+//    in registers by the lanes whose words cover those bytes;
+//  * split_merge: the parent is staged in LDS; lane 0 lists the child's
+//    nodes (kind, parent node) from the decisions (qcgd.hpp:850-1035); each
+//    lane then sizes its child node's name tree, a wave scan places them,
+//    and the lanes write the header and copy their name segments into an LDS
+//    child buffer, stored with coalesced 4-byte writes.
+// Objects beyond the LDS buffers (or not 4-byte aligned) are written by one
+// lane straight from global memory.
+constexpr int kPopParent = 4096;
+constexpr int kPopChild = 16384;
+constexpr int kPopNodes = 256;
+
+__device__ inline u32 wave_excl_sum(u32 v, u32 &total) {
+    const int lane = threadIdx.x & (kWave - 1);
+    u32 inc = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const u32 o = __shfl_up(inc, d, kWave);
+        if (lane >= d) inc += o;
+    }
+    total = __shfl(inc, kWave - 1, kWave);
+    return inc - v;
+}
+
+// Copies the parent's words to the child with the flips of child_id applied
+// (erase_create / coin: both particle bytes of every flipped node; hadamard:
+// byte `bit` xor !child_id).
 template <int KIND>
-__global__ __launch_bounds__(256) void k_populate(Parents P, const u64 *cb, qd_rule rule, const u64 *orep, u64 K,
-                                                  const u64 *obegin, const u32 *osize, u8 *oobj) {
-    for (u64 k = u64(blockIdx.x) * blockDim.x + threadIdx.x; k < K; k += u64(gridDim.x) * blockDim.x) {
+__device__ inline void populate_flip_wave(const u8 *o, u32 psz, u8 *d, u64 span, u32 j, const qd_rule &rule) {
+    const u32 lane = threadIdx.x & (kWave - 1);
+    u64 flipmask = 0;  // nodes whose particles flip (n <= 64)
+    u32 n = 0;
+    if (KIND != QD_RULE_HADAMARD) {
+        n = ld16(o);
+        const bool l = lane < n && o[2 + lane], r = lane < n && o[2 + n + lane];
+        const bool elig = lane < n && flip_eligible(KIND == QD_RULE_QCGD_COIN, l, r);
+        const u64 em = __ballot(elig);
+        const u32 rank = __popcll(em & ((1ull << lane) - 1));
+        flipmask = __ballot(elig && rank < 32 && ((j >> rank) & 1));
+    }
+    for (u64 w = lane; w < (span + 3) / 4; w += kWave) {
+        const u32 at = u32(4 * w);
+        u32 v = 0;
+        if (at + 4 <= psz) v = *reinterpret_cast<const u32 *>(o + at);
+        else
+            for (u32 q = 0; q < 4; ++q)
+                if (at + q < psz) v |= u32(o[at + q]) << (8 * q);
+        for (u32 q = 0; q < 4; ++q) {
+            const u32 pos = at + q;
+            bool flip = false;
+            if (KIND == QD_RULE_HADAMARD) flip = pos == rule.bit && j == 0;
+            else if (pos >= 2 && pos < 2 + 2 * n) {
+                const u32 node = (pos - 2) < n ? pos - 2 : pos - 2 - n;
+                flip = (flipmask >> node) & 1;
+            }
+            if (flip) {
+                const u32 byte = (v >> (8 * q)) & 0xFF;
+                const u32 nb = KIND == QD_RULE_HADAMARD ? (byte ^ 1u) : (byte ? 0u : 1u);
+                v = (v & ~(0xFFu << (8 * q))) | (nb << (8 * q));
+            }
+        }
+        if (at + 4 <= span) *reinterpret_cast<u32 *>(d + at) = v;
+        else
+            for (u32 q = 0; q < 4 && at + q < span; ++q) d[at + q] = u8(v >> (8 * q));
+    }
+}
+
+template <int KIND>
+__global__ __launch_bounds__(64) void k_populate(Parents P, const u64 *cb, qd_rule rule, const u64 *orep, u64 K,
+                                                 const u64 *obegin, const u32 *osize, u8 *oobj) {
+    __shared__ __align__(16) u8 pbuf[kPopParent];
+    __shared__ __align__(16) u8 cbuf[kPopChild];
+    __shared__ u8 ckind[kPopNodes], cl[kPopNodes], cr[kPopNodes];
+    __shared__ u16 cnode[kPopNodes];
+    __shared__ u32 nnodes;
+    const u32 lane = threadIdx.x;
+    for (u64 k = blockIdx.x; k < K; k += gridDim.x) {
         const u64 c = orep[k];
         const u64 p = find_parent(cb, P.n, c);
         const u32 j = u32(c - cb[p]);
         const u8 *o = P.obj + P.begin[p];
-        u8 *d = oobj + obegin[k];
-        u32 s;
-        if (KIND == QD_RULE_HADAMARD) {
-            hadamard_child_write(o, P.size[p], rule.bit, j, d);
-            s = P.size[p];
-        } else if (KIND == QD_RULE_QCGD_SPLIT_MERGE) {
-            Graph g(o);
-            s = sm_child_write(g, sm_first_hmlz(g), j, d);
-        } else {
-            flip_child_write(o, P.size[p], KIND == QD_RULE_QCGD_COIN, j, d);
-            s = P.size[p];
+        const u32 psz = P.size[p];
+        const u32 csz = osize[k];
+        const u64 ob = obegin[k], oe = obegin[k + 1];
+        const u64 span = oe - ob;  // child bytes + zero padding
+        u8 *d = oobj + ob;
+        const bool aligned = ((P.begin[p] | ob) & 3) == 0;
+        if (KIND != QD_RULE_QCGD_SPLIT_MERGE) {
+            const u32 n = KIND == QD_RULE_HADAMARD ? 0 : ld16(o);
+            if (aligned && n <= 64) {
+                populate_flip_wave<KIND>(o, psz, d, span, j, rule);
+            } else if (lane == 0) {
+                if (KIND == QD_RULE_HADAMARD) hadamard_child_write(o, psz, rule.bit, j, d);
+                else flip_child_write(o, psz, KIND == QD_RULE_QCGD_COIN, j, d);
+                for (u64 q = psz; q < span; ++q) d[q] = 0;
+            }
+            continue;
         }
-        const u64 e = obegin[k + 1];
-        for (u64 q = obegin[k] + s; q < e; ++q) oobj[q] = 0;
-        (void)osize;
+        if (!(aligned && psz <= kPopParent && csz <= kPopChild)) {
+            if (lane == 0) {
+                Graph g(o);
+                const u32 s = sm_child_write(g, sm_first_hmlz(g), j, d);
+                for (u64 q = s; q < span; ++q) d[q] = 0;
+            }
+            continue;
+        }
+        for (u32 w = lane; w < psz / 4; w += 64) reinterpret_cast<u32 *>(pbuf)[w] = *reinterpret_cast<const u32 *>(o + 4 * w);
+        __syncthreads();
+        Graph g(pbuf);
+        if (lane == 0) {
+            u32 cnt = 0;
+            cplx dm{0, 0};
+            QcgdCoef kk{};
+            sm_walk(g, sm_first_hmlz(g), j, kk, dm, false, [&](int kind, u32 i, bool l, bool r) {
+                if (cnt < kPopNodes) {
+                    ckind[cnt] = u8(kind);
+                    cnode[cnt] = u16(i);
+                    cl[cnt] = l;
+                    cr[cnt] = r;
+                }
+                ++cnt;
+            });
+            nnodes = cnt;
+        }
+        __syncthreads();
+        const u32 nc = nnodes;
+        if (nc > kPopNodes) {  // more child nodes than the list holds
+            if (lane == 0) {
+                const u32 s = sm_child_write(g, sm_first_hmlz(g), j, d);
+                for (u64 q = s; q < span; ++q) d[q] = 0;
+            }
+            __syncthreads();
+            continue;
+        }
+        // name tree of every child node: [src_b, src_e) copied after an
+        // optional new root
+        u32 base = 0;
+        for (u32 c0 = 0; c0 < nc; c0 += kWave) {
+            const u32 cn = c0 + lane;
+            u32 len = 0, sb = 0, se = 0, sb2 = 0, se2 = 0;
+            bool newroot = false;
+            i16 nh = 0, nt = 0;
+            u64 nhash = 0;
+            if (cn < nc) {
+                const u32 kind = ckind[cn], i = cnode[cn];
+                const u32 b = g.name_begin(i), e = g.name_begin(i + 1), m = e - b;
+                if (kind == 0) {
+                    sb = b;
+                    se = e;
+                } else if (kind == 1 || kind == 2) {
+                    const i16 t = g.sub_type(b);
+                    if (t >= 0) {
+                        sb = kind == 1 ? b + 1 : b + u32(t);
+                        se = kind == 1 ? b + u32(t) : e;
+                    } else {
+                        newroot = true;
+                        nt = kind == 1 ? QCGD_DOT_L : QCGD_DOT_R;
+                        nh = (nt == QCGD_DOT_L && g.sub_hmlz(b) < 0) ? i16(-1) : i16(1);
+                        nhash = kind == 1 ? split_left_hash(g, b) : split_right_hash(g, b);
+                        sb = b;
+                        se = e;
+                    }
+                } else {
+                    const u32 jn = i + 1 < g.n ? i + 1 : 0;
+                    const u32 bb = g.name_begin(jn), eb = g.name_begin(jn + 1);
+                    if (merge_collapses(g, b, bb)) {
+                        sb = b + 1;
+                        se = e;
+                    } else {
+                        newroot = true;
+                        nt = i16(m + 1);
+                        nh = (g.sub_hmlz(b) < 0 || g.sub_hmlz(bb) < 0) ? i16(-1) : i16(1);
+                        nhash = hash_combine(g.sub_hash(b), g.sub_hash(bb));
+                        sb = b;
+                        se = e;
+                        sb2 = bb;
+                        se2 = eb;
+                    }
+                }
+                len = u32(newroot) + (se - sb) + (se2 - sb2);
+            }
+            u32 tot;
+            const u32 off = base + wave_excl_sum(len, tot);
+            if (cn < nc) {
+                u8 *hdr = cbuf;
+                hdr[2 + cn] = cl[cn];
+                hdr[2 + nc + cn] = cr[cn];
+                st16(hdr + 2 + 2 * nc + 2 * cn, u16(off));
+                u8 *dst = cbuf + 4 + 4 * nc + 16 * size_t(off);
+                if (newroot) {
+                    put_sub(dst, nh, nt, nhash);
+                    dst += 16;
+                }
+                const u32 *src = reinterpret_cast<const u32 *>(g.sub(sb));
+                for (u32 q = 0; q < 4 * (se - sb); ++q) reinterpret_cast<u32 *>(dst)[q] = src[q];
+                dst += 16 * (se - sb);
+                if (se2 > sb2) {
+                    const u32 *src2 = reinterpret_cast<const u32 *>(g.sub(sb2));
+                    for (u32 q = 0; q < 4 * (se2 - sb2); ++q) reinterpret_cast<u32 *>(dst)[q] = src2[q];
+                }
+            }
+            base += tot;
+        }
+        if (lane == 0) {
+            st16(cbuf, u16(nc));
+            st16(cbuf + 2 + 2 * nc + 2 * nc, u16(base));
+        }
+        __syncthreads();
+        for (u64 w = lane; w < (span + 3) / 4; w += 64) {
+            const u64 at = 4 * w;
+            if (at + 4 <= span) {
+                u32 v = 0;
+                if (at + 4 <= csz) v = reinterpret_cast<const u32 *>(cbuf)[w];
+                else
+                    for (u32 q = 0; q < 4; ++q)
+                        if (at + q < csz) v |= u32(cbuf[at + q]) << (8 * q);
+                *reinterpret_cast<u32 *>(d + at) = v;
+            } else {
+                for (u64 q = at; q < span; ++q) d[q] = q < csz ? cbuf[q] : 0;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -145,5 +350,10 @@ __global__ __launch_bounds__(256) void k_modifier(u8 *obj, const u64 *begin, con
     }
 }
 
+
+// Plain streaming copy, 16 bytes per thread per iteration.
+__global__ __launch_bounds__(256) void k_copy_probe(const uint4 *src, uint4 *dst, u64 n) {
+    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x) dst[i] = src[i];
+}
 
 }  // namespace qd

@@ -10,8 +10,11 @@ namespace qd {
 // ===========================================================================
 // phase 6: top-K select on the 128-bit key (|m|^2 bits, ~hash)
 // ===========================================================================
+// A key under selection: (hi, lo) = (|m|^2 bits, ~hash) for uniques, whose
+// `slot` is their dedup output slot; (|m|^2 bits, ~index) for parents, whose
+// `slot` is their index.
 struct SelItem {
-    u64 hi, lo, rep;
+    u64 hi, lo, slot;
 };
 
 __device__ inline u32 key_digit(u64 hi, u64 lo, u32 pos) {
@@ -39,6 +42,24 @@ __device__ inline void mark(u64 *bitmap, u64 rep) {
     atomicOr(reinterpret_cast<unsigned long long *>(bitmap + (rep >> 6)), 1ull << (rep & 63));
 }
 
+// Marks a selected item: bit `key` of the bitmap (the representative, or
+// the parent index when UR is null) and, with a slot bitmap, bit `slot`.
+__device__ inline void select_item(bool pred, u64 slot, const u64 *UR, u64 *bitmap, u64 *sbm) {
+    if (!pred) return;
+    mark(bitmap, UR ? UR[slot] : slot);
+    if (sbm) mark(sbm, slot);
+}
+
+// The same for a wave whose lanes hold the 64 consecutive slots of word
+// `first >> 6` (first a multiple of 64, below U): the slot word is written
+// whole, so the slot bitmap needs no clearing for the words these kernels
+// cover.
+__device__ inline void select_word(bool pred, u64 slot, u64 first, const u64 *UR, u64 *bitmap, u64 *sbm) {
+    if (pred) mark(bitmap, UR[slot]);
+    const u64 m = __ballot(pred);
+    if ((threadIdx.x & (kWave - 1)) == 0) sbm[first >> 6] = m;
+}
+
 // First select stage over the dedup output slots: histogram of |m|^2 bits
 // 51..40 among the uniques whose bits 63..52 equal d1.
 __global__ __launch_bounds__(256) void k_sel_hist_u(const u64 *UR, const double2 *UM, u64 U, u32 d1, u64 *hist) {
@@ -59,23 +80,26 @@ __global__ __launch_bounds__(256) void k_sel_hist_u(const u64 *UR, const double2
 // Marks the uniques whose top 24 |m|^2 bits exceed prefix24 and collects
 // the ones equal to it.
 __global__ __launch_bounds__(256) void k_sel_first(const u64 *UH, const u64 *UR, const double2 *UM, u64 U,
-                                                   u32 prefix24, SelItem *S, u64 *scount, u64 *bitmap) {
+                                                   u32 prefix24, SelItem *S, u64 *scount, u64 *bitmap,
+                                                   u64 *sbm) {
     const u64 stride = u64(gridDim.x) * blockDim.x;
-    // uniform trip count per wave so that wave_append sees every lane
+    // uniform trip count per wave so that the appends see every lane
     for (u64 base = u64(blockIdx.x) * blockDim.x; base < U; base += stride) {
         const u64 i = base + threadIdx.x;
-        bool take = false;
+        bool take = false, keep = false;
         SelItem it{};
         if (i < U && UR[i] != kNoRep) {
             const double2 m = UM[i];
             const u64 bits = u64(__double_as_longlong(m.x * m.x + m.y * m.y));
             const u32 t = u32(bits >> 40);
-            if (t > prefix24) mark(bitmap, UR[i]);
+            if (t > prefix24) keep = true;
             else if (t == prefix24) {
                 take = true;
-                it = SelItem{bits, ~UH[i], UR[i]};
+                it = SelItem{bits, ~UH[i], i};
             }
         }
+        const u64 first = i & ~u64(kWave - 1);
+        if (first < U) select_word(keep, i, first, UR, bitmap, sbm);
         const u64 at = wave_append(take, scount);
         if (take) S[at] = it;
     }
@@ -93,32 +117,37 @@ __global__ __launch_bounds__(256) void k_sel_hist(const SelItem *S, u64 n, u32 p
 }
 
 __global__ __launch_bounds__(256) void k_sel_split(const SelItem *S, u64 n, u32 pos, u32 d, SelItem *out,
-                                                   u64 *ocount, u64 *bitmap) {
+                                                   u64 *ocount, const u64 *UR, u64 *bitmap, u64 *sbm) {
     const u64 stride = u64(gridDim.x) * blockDim.x;
     for (u64 base = u64(blockIdx.x) * blockDim.x; base < n; base += stride) {
         const u64 i = base + threadIdx.x;
-        bool take = false;
+        bool take = false, keep = false;
         SelItem it{};
         if (i < n) {
             it = S[i];
             const u32 t = key_digit(it.hi, it.lo, pos);
-            if (t > d) mark(bitmap, it.rep);
-            else take = t == d;
+            keep = t > d;
+            take = t == d;
         }
+        select_item(keep, it.slot, UR, bitmap, sbm);
         const u64 at = wave_append(take, ocount);
         if (take) out[at] = it;
     }
 }
 
-__global__ __launch_bounds__(256) void k_sel_mark_items(const SelItem *S, u64 n, u64 *bitmap) {
+__global__ __launch_bounds__(256) void k_sel_mark_items(const SelItem *S, u64 n, const u64 *UR, u64 *bitmap,
+                                                        u64 *sbm) {
     for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
-        mark(bitmap, S[i].rep);
+        select_item(true, S[i].slot, UR, bitmap, sbm);
 }
 
-__global__ __launch_bounds__(256) void k_mark_all(const u64 *UR, u64 U, u64 *bitmap) {
-    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < U; i += u64(gridDim.x) * blockDim.x)
-        if (UR[i] != kNoRep) mark(bitmap, UR[i]);
+__global__ __launch_bounds__(256) void k_mark_all(const u64 *UR, u64 U, u64 *bitmap, u64 *sbm) {
+    const u64 stride = u64(gridDim.x) * blockDim.x;
+    for (u64 base = u64(blockIdx.x) * blockDim.x; base < U; base += stride) {
+        const u64 i = base + threadIdx.x;
+        const u64 first = i & ~u64(kWave - 1);
+        if (first < U) select_word(i < U && UR[i] != kNoRep, i, first, UR, bitmap, sbm);
+    }
 }
-
 
 }  // namespace qd

@@ -136,11 +136,11 @@ struct qd_ctx {
     DBuf sel0, sel1;
     DBuf bitmap, wprefix, padded, chunks, scanblk;
     DBuf biglist, bigsoff, bigkey, bigcand, bigmag, bigseg;
-    DBuf pdesc, wi, wb, ebuf2, cnt2;
+    DBuf pdesc, wi, wb, ebuf2, cnt2, slist;
     std::vector<DBuf *> all() {
         return {&nc, &cb, &pre, &tab, &kept, &keys, &ebuf0, &ebuf1, &hist1, &off1, &cur1, &off2,
                 &nhist, &counters, &sel0, &sel1, &bitmap, &wprefix, &padded, &chunks, &scanblk,
-                &biglist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &ebuf2, &cnt2};
+                &biglist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &ebuf2, &cnt2, &slist};
     }
 };
 
@@ -286,7 +286,8 @@ __global__ __launch_bounds__(256) void k_parent_items(const double2 *mag, u64 n,
 // leading `pos` key bits are already known to tie; marks their rep bits in
 // bitmap.  Items above the chosen digit are marked as soon as it is known;
 // the tied ones are carried to the next 12-bit digit.
-static int select_loop(qd_ctx *ctx, SelItem *S0, SelItem *S1, u64 ns, u64 r, u32 pos, u64 *bitmap) {
+static int select_loop(qd_ctx *ctx, SelItem *S0, SelItem *S1, u64 ns, u64 r, u32 pos, u64 *bitmap,
+                       const u64 *UR, u64 *sbm) {
     hipStream_t s = ctx->stream;
     u64 *hist = ctx->hist1.as<u64>();
     u64 *cnt = ctx->counters.as<u64>() + 2;
@@ -306,14 +307,14 @@ static int select_loop(qd_ctx *ctx, SelItem *S0, SelItem *S1, u64 ns, u64 r, u32
         }
         r -= cum;
         hipLaunchKernelGGL(k_sel_split, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, s, S0, ns, pos, u32(d), S1, cnt,
-                           bitmap);
+                           UR, bitmap, sbm);
         QD_HIP(hipGetLastError());
         ns = h[d];
         std::swap(S0, S1);
         pos += 12;
     }
     if (r > 0 && r == ns)
-        hipLaunchKernelGGL(k_sel_mark_items, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, s, S0, ns, bitmap);
+        hipLaunchKernelGGL(k_sel_mark_items, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, s, S0, ns, UR, bitmap, sbm);
     QD_HIP(hipGetLastError());
     return QD_OK;
 }
@@ -409,7 +410,7 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
         hipLaunchKernelGGL(k_parent_items, dim3(grid_for(N)), dim3(256), 0, s, P.mag, N, ctx->sel0.as<SelItem>());
         QD_HIP(hipGetLastError());
         SelItem *S0 = ctx->sel0.as<SelItem>(), *S1 = ctx->sel1.as<SelItem>();
-        QD_TRY(select_loop(ctx, S0, S1, N, max_obj, 0, pbits));
+        QD_TRY(select_loop(ctx, S0, S1, N, max_obj, 0, pbits, nullptr, nullptr));
         keep = pbits;
         nkept = max_obj;
     }
@@ -599,10 +600,13 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
     QD_TRY(ctx->bitmap.ensure(W * sizeof(u64)));
     QD_TRY(ctx->wprefix.ensure((W + 1) * sizeof(u64)));
     u64 *bitmap = ctx->bitmap.as<u64>();
+    QD_TRY(ctx->slist.ensure(W * sizeof(u64)));
+    u64 *sbm = ctx->slist.as<u64>();
     prof_begin(ctx, QD_PHASE_SELECT);
     QD_HIP(hipMemsetAsync(bitmap, 0, W * sizeof(u64), s));
     if (U <= max_obj) {
-        if (U) hipLaunchKernelGGL(k_mark_all, dim3(grid_for(C, 256, 16384)), dim3(256), 0, s, UR, C, bitmap);
+        if (U)
+            hipLaunchKernelGGL(k_mark_all, dim3(grid_for(C, 256, 16384)), dim3(256), 0, s, UR, C, bitmap, sbm);
         QD_HIP(hipGetLastError());
     } else {
         if (!prm.simple_truncation)
@@ -633,9 +637,9 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
         QD_TRY(ctx->sel1.ensure(std::max<u64>(ns, 1) * sizeof(SelItem)));
         SelItem *S0 = ctx->sel0.as<SelItem>(), *S1 = ctx->sel1.as<SelItem>();
         hipLaunchKernelGGL(k_sel_first, dim3(grid_for(C, 256, 16384)), dim3(256), 0, s, UH, UR, UM, C,
-                           (d1 << 12) | u32(d), S0, counters + 1, bitmap);
+                           (d1 << 12) | u32(d), S0, counters + 1, bitmap, sbm);
         QD_HIP(hipGetLastError());
-        QD_TRY(select_loop(ctx, S0, S1, ns, r, 24, bitmap));
+        QD_TRY(select_loop(ctx, S0, S1, ns, r, 24, bitmap, UR, sbm));
     }
     prof_end(ctx, QD_PHASE_SELECT);
     prof_begin(ctx, QD_PHASE_ORDER);
@@ -654,7 +658,7 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
     QD_TRY(ctx->keys.ensure(K * sizeof(u64)));
     u64 *orep = ctx->keys.as<u64>();
     prof_begin(ctx, QD_PHASE_ORDER);
-    hipLaunchKernelGGL(k_place, dim3(grid_for(C, 256, 16384)), dim3(256), 0, s, UR, UM, C, bitmap,
+    hipLaunchKernelGGL(k_place, dim3(grid_for(C, 256, 16384)), dim3(256), 0, s, UR, UM, (const u64 *)sbm, C, bitmap,
                        ctx->wprefix.as<u64>(), orep, out->mag.as<double2>());
     prof_end(ctx, QD_PHASE_ORDER);
     prof_begin(ctx, QD_PHASE_FINALIZE);
@@ -667,8 +671,9 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
     const u64 bytes = hb[0];
     QD_TRY(out->objects.ensure(std::max<u64>(bytes, 1)));
     prof_begin(ctx, QD_PHASE_FINALIZE);
-    hipLaunchKernelGGL(k_populate<KIND>, dim3(grid_for(K)), dim3(256), 0, s, P, ctx->cb.as<u64>(), rule, orep, K,
-                       out->begin.as<u64>(), out->size.as<u32>(), out->objects.as<u8>());
+    hipLaunchKernelGGL(k_populate<KIND>, dim3(unsigned(std::min<u64>(K, 65536))), dim3(64), 0, s, P,
+                       ctx->cb.as<u64>(), rule, orep, K, out->begin.as<u64>(), out->size.as<u32>(),
+                       out->objects.as<u8>());
     QD_HIP(hipGetLastError());
     out->n = K;
     out->bytes = bytes;
@@ -830,6 +835,42 @@ int qd_ctx_phase_times(qd_ctx *c, double *ms, uint64_t *calls) {
         if (ms) ms[i] = c->phase_ms[i];
         if (calls) calls[i] = c->phase_calls[i];
     }
+    return QD_OK;
+}
+
+int qd_copy_probe(qd_ctx *c, uint64_t bytes, int reps, double *gbps) {
+    if (!c || !gbps || bytes < 16 || reps < 1) return fail(QD_ERR_INVALID, "bad argument");
+    QD_HIP(hipSetDevice(c->device));
+    const u64 n = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    QD_HIP(hipMalloc(&a, n * 16));
+    hipError_t e = hipMalloc(&b, n * 16);
+    if (e != hipSuccess) {
+        (void)hipFree(a);
+        return fail(QD_ERR_OOM, "hipMalloc of the probe buffers failed");
+    }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipMemsetAsync(a, 1, n * 16, c->stream);
+    float best = 0;
+    for (int r = 0; r < reps; ++r) {
+        (void)hipEventRecord(e0, c->stream);
+        hipLaunchKernelGGL(k_copy_probe, dim3(grid_for(n, 256, 16384)), dim3(256), 0, c->stream,
+                           static_cast<const uint4 *>(a), static_cast<uint4 *>(b), n);
+        (void)hipEventRecord(e1, c->stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (r == 0 || ms < best) best = ms;
+    }
+    e = hipGetLastError();
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(QD_ERR_HIP, std::string("copy probe: ") + hipGetErrorString(e));
+    *gbps = 2.0 * double(n * 16) / (double(best) * 1e6);
     return QD_OK;
 }
 
