@@ -77,6 +77,85 @@ def cpu_baseline(nodes, seed, cap, budget_s, iterations=2):
                       f"{int(C)} candidates in {secs:.2f} s, OMP_NUM_THREADS={threads}"}
 
 
+KERNEL_OF_PHASE = {"generate": "k_generate_fast", "dedup": "k_dedup<", "finalize": "k_populate"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_qcgd12.json")
+
+
+def _csv(path_glob):
+    import csv
+    import glob
+    paths = sorted(glob.glob(path_glob, recursive=True))
+    if not paths:
+        raise SystemExit(f"no file matches {path_glob}")
+    with open(paths[0]) as f:
+        return list(csv.DictReader(f))
+
+
+def _counter_per_dispatch(rows, kernel, counter):
+    """{dispatch id: counter value (summed over the rows of the dispatch)}"""
+    out = {}
+    for r in rows:
+        if kernel in r["Kernel_Name"] and r["Counter_Name"] == counter:
+            d = int(r["Dispatch_Id"])
+            out[d] = out.get(d, 0.0) + float(r["Counter_Value"])
+    return [out[d] for d in sorted(out)]
+
+
+def rocprof_summary(args):
+    """Formats the rocprofv3 output of this script's own runs: kernel-trace
+    stats, the timed launches of the dominant kernel (the last 2*steps
+    launches: one per simulate() call), and FETCH_SIZE / WRITE_SIZE per
+    launch scaled by what the same counters read for the library's plain
+    copy (qd_copy_probe) of a known size.  Prints markdown and writes the
+    traffic figures to --traffic-out."""
+    kt, fe, wr, cf, cw = args.rocprof_summary
+    stats = _csv(os.path.join(kt, "**", "*kernel_stats.csv"))
+    trace = _csv(os.path.join(kt, "**", "*kernel_trace.csv"))
+    nt = 2 * args.steps
+    lines = ["| kernel | calls | total ms | avg us | % |", "|---|---:|---:|---:|---:|"]
+    for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:16]:
+        name = r["Name"].split("(")[0].replace("void ", "")
+        lines.append(f"| `{name}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.2f} | "
+                     f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
+    calib_bytes = None
+    with open(args.calib_json) as f:
+        calib_bytes = json.load(f)["copy_probe_bytes"]
+    cfetch = _counter_per_dispatch(_csv(os.path.join(cf, "**", "*counter_collection.csv")), "k_copy_probe", "FETCH_SIZE")
+    cwrite = _counter_per_dispatch(_csv(os.path.join(cw, "**", "*counter_collection.csv")), "k_copy_probe", "WRITE_SIZE")
+    # the copy reads and writes calib_bytes each; counters are in KiB
+    f_scale = calib_bytes / (1024.0 * cfetch[-1])
+    w_scale = calib_bytes / (1024.0 * cwrite[-1])
+    fetch = _csv(os.path.join(fe, "**", "*counter_collection.csv"))
+    write = _csv(os.path.join(wr, "**", "*counter_collection.csv"))
+    traffic = {"workload": args.workload_key, "timed_launches": nt, "fetch_scale": f_scale, "write_scale": w_scale,
+               "kernels": {}}
+    lines += ["", "Timed launches (the last 2 x steps of each kernel, one per simulate() call):", "",
+              "| kernel | launches | avg us (trace) | FETCH_SIZE KiB/launch | WRITE_SIZE KiB/launch | "
+              "HBM bytes/launch (scaled) |", "|---|---:|---:|---:|---:|---:|"]
+    for phase, kname in KERNEL_OF_PHASE.items():
+        durs = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) for r in trace if kname in r["Kernel_Name"]]
+        fv = _counter_per_dispatch(fetch, kname, "FETCH_SIZE")
+        wv = _counter_per_dispatch(write, kname, "WRITE_SIZE")
+        if len(durs) < nt or len(fv) < nt or len(wv) < nt:
+            continue
+        durs, fv, wv = durs[-nt:], fv[-nt:], wv[-nt:]
+        avg_us = sum(durs) / nt / 1e3
+        fk, wk = sum(fv) / nt, sum(wv) / nt
+        hbm = 1024.0 * (fk * f_scale + wk * w_scale)
+        traffic["kernels"][phase] = {"kernel": kname.rstrip("<"), "avg_us": avg_us, "fetch_kib": fk,
+                                     "write_kib": wk, "hbm_bytes_per_launch": hbm}
+        lines.append(f"| `{kname.rstrip('<')}` | {nt} | {avg_us:.1f} | {fk:.0f} | {wk:.0f} | {hbm:.4g} |")
+    lines += ["", f"Counter scale (copy of {calib_bytes} B read + written): FETCH_SIZE x{f_scale:.3f}, "
+                  f"WRITE_SIZE x{w_scale:.3f}"]
+    with open(args.traffic_out, "w") as f:
+        json.dump(traffic, f, indent=1)
+    print("\n".join(lines))
+
+
+def workload_key(args):
+    return f"qcgd nodes={args.nodes} seed={args.seed} cap=2^{args.log2_cap} warmup={args.warmup} steps={args.steps}"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,7 +168,15 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=120.0)
     ap.add_argument("--copy-probe", type=float, default=0,
                     help="only run the library's plain copy of this many GiB (counter calibration) and exit")
+    ap.add_argument("--rocprof-summary", nargs=5, metavar=("KT", "FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE"),
+                    help="summarize rocprofv3 output folders of this script's runs and exit")
+    ap.add_argument("--calib-json", help="the --copy-probe JSON line of the calibration run")
+    ap.add_argument("--traffic-out", default=TRAFFIC_FILE)
     args = ap.parse_args()
+    if args.rocprof_summary:
+        args.workload_key = workload_key(args)
+        rocprof_summary(args)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
               file=sys.stderr)
@@ -148,11 +235,22 @@ def main():
     ab = {k: sum(alg_bytes(s)[k] for s in stats) for k in ("generate", "dedup", "finalize", "total")}
     kernel_ms = {k: phases[k][0] for k in ("generate", "dedup", "finalize")}
     dom = max(kernel_ms, key=kernel_ms.get)
-    dom_ms = kernel_ms[dom]
-    achieved = ab[dom] / (dom_ms / 1e3) if dom_ms > 0 else None
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9 if achieved else None,
+    launches = len(stats)  # each timed simulate() launches the phase's main kernel once
+    dom_ms = kernel_ms[dom] / launches
+    per_launch = ab[dom] / launches
+    achieved = per_launch / (dom_ms / 1e3) if dom_ms > 0 else None
+    traffic = None
+    try:
+        with open(TRAFFIC_FILE) as f:
+            tr = json.load(f)
+        if tr.get("workload") == workload_key(args) and dom in tr.get("kernels", {}):
+            traffic = tr["kernels"][dom]["hbm_bytes_per_launch"]
+    except (OSError, ValueError):
+        pass
+    roofline = {"bound": "hbm", "kernel": KERNEL_OF_PHASE[dom].rstrip("<"), "achieved": achieved / 1e9 if achieved else None,
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK if achieved else None,
-                "traffic": None}
+                "traffic": traffic, "alg_bytes_per_launch": per_launch, "ms_per_launch": dom_ms,
+                "launches": launches}
     step_frac = ab["total"] / elapsed / HBM_PEAK
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.nodes, args.seed, cap, args.cpu_budget)
