@@ -152,6 +152,57 @@ def rocprof_summary(args):
     print("\n".join(lines))
 
 
+def adjoint_bench(nodes=10, seed=42, n_iter=6, log2_cap=18, checkpoint_every=2, gd_steps=0, lr=0.05, reps=3):
+    """QCGD adjoint: loss = expected particle count after n_iter iterations
+    of step -> split_merge(t1, p1) -> erase_create(t2, p2) from a seeded
+    graph, gradient w.r.t. (t1, p1, t2, p2) with a checkpoint every
+    `checkpoint_every` iterations.  Untruncated runs of 10 nodes do not fit
+    past iteration 2, so every step keeps the top 2^log2_cap objects (the
+    adjoint holds that mask fixed).  Times the forward pass alone and the
+    gradient (forward with checkpoints + recompute + adjoint steps)."""
+    import numpy as np
+    import torch
+    import quids_amd as q
+    from quids_amd import adjoint as adj
+    from quids_amd.rules import qcgd
+    from quids_amd.state import HostState
+    it = q.it_t()
+    it.upload(HostState.from_objects([qcgd.seeded_graph(nodes, seed)]))
+    prog = adj.qcgd_program(n_iter, 1 << log2_cap)
+    p = np.array([math.pi / 4, 0.0, math.pi / 3, math.pi / 5])
+    adj.gradient(prog, p, it, adj.OBS_PARTICLES, 0, checkpoint_every)  # warm-up (scratch allocation)
+    fwd = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        adj.loss(prog, p, it, adj.OBS_PARTICLES)
+        torch.cuda.synchronize()
+        fwd.append(time.perf_counter() - t)
+    grad_t, bwd_t = [], []
+    for _ in range(reps):
+        tm = {}
+        value, g = adj.gradient(prog, p, it, adj.OBS_PARTICLES, 0, checkpoint_every, timings=tm)
+        grad_t.append(tm["forward"] + tm["backward"])
+        bwd_t.append(tm["backward"])
+    f = min(fwd)
+    out = {"workload": f"QCGD adjoint, {nodes} nodes seed {seed}, {n_iter} iterations, cap 2^{log2_cap}, "
+                       f"checkpoint every {checkpoint_every}, loss = expected particle count",
+           "forward_ms": f * 1e3, "gradient_ms": min(grad_t) * 1e3, "backward_ms": min(bwd_t) * 1e3,
+           "gradient_over_forward": min(grad_t) / f, "backward_over_forward": min(bwd_t) / f,
+           "loss": value, "grad": [float(x) for x in g]}
+    if gd_steps:
+        losses = []
+        t = time.perf_counter()
+        for _ in range(gd_steps):
+            value, g = adj.gradient(prog, p, it, adj.OBS_PARTICLES, 0, checkpoint_every)
+            losses.append(value)
+            p = p - lr * g
+        out.update({"gd_steps": gd_steps, "lr": lr, "gd_seconds": time.perf_counter() - t,
+                    "loss_first": losses[0], "loss_last": losses[-1], "params_last": [float(x) for x in p]})
+    it.close()
+    return out
+
+
 def workload_key(args):
     return f"qcgd nodes={args.nodes} seed={args.seed} cap=2^{args.log2_cap} warmup={args.warmup} steps={args.steps}"
 
@@ -172,6 +223,10 @@ def main():
                     help="summarize rocprofv3 output folders of this script's runs and exit")
     ap.add_argument("--calib-json", help="the --copy-probe JSON line of the calibration run")
     ap.add_argument("--traffic-out", default=TRAFFIC_FILE)
+    ap.add_argument("--workload", choices=["plain", "adjoint"], default="plain",
+                    help="adjoint: only the QCGD adjoint workload (with --gd-steps of gradient descent)")
+    ap.add_argument("--gd-steps", type=int, default=50)
+    ap.add_argument("--no-extras", action="store_true", help="skip the adjoint summary of the plain line")
     args = ap.parse_args()
     if args.rocprof_summary:
         args.workload_key = workload_key(args)
@@ -185,6 +240,13 @@ def main():
     import quids_amd as q
     from quids_amd.rules import qcgd
     from quids_amd.state import HostState
+
+    if args.workload == "adjoint":
+        res = adjoint_bench(gd_steps=args.gd_steps)
+        res.update({"metric": "adjoint gradient cost over one forward pass", "value": res["gradient_over_forward"],
+                    "unit": "x", "higher_is_better": False, "n_gpus": 1, "dtype": "f64", "data": "synthetic"})
+        print(json.dumps(res), flush=True)
+        return
 
     if args.copy_probe:
         nbytes = int(args.copy_probe * (1 << 30))
@@ -254,6 +316,12 @@ def main():
     step_frac = ab["total"] / elapsed / HBM_PEAK
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.nodes, args.seed, cap, args.cpu_budget)
+    extras = {}
+    if not args.no_extras:
+        try:
+            extras["adjoint"] = adjoint_bench()
+        except Exception as e:  # noqa: BLE001
+            extras["adjoint"] = {"error": str(e)}
 
     out = {
         "metric": METRIC,
@@ -276,6 +344,7 @@ def main():
         "step_hbm_frac": step_frac,
         "phase_ms": {k: round(v[0], 3) for k, v in phases.items()},
         "cpu_baseline": cpu,
+        **extras,
     }
     print(json.dumps(out), flush=True)
 

@@ -174,6 +174,10 @@ int qd_state_info(qd_state *state, uint64_t *num_object, uint64_t *object_bytes,
  * NULL to skip that array. */
 int qd_state_download(qd_state *state, void *objects, uint64_t *object_begin,
                       uint32_t *object_size, double *magnitude);
+/* Device-to-device copy of src into dst (same context).  The reference's
+ * it_t is not copyable; this serves checkpoints of the adjoint's forward
+ * pass. */
+int qd_state_copy(qd_state *dst, const qd_state *src);
 /* Device pointers of the state arrays (valid until the state is modified). */
 int qd_state_device_ptrs(qd_state *state, void **objects, uint64_t **object_begin,
                          uint32_t **object_size, double **magnitude);
@@ -199,6 +203,35 @@ int qd_apply_modifier(qd_ctx *ctx, qd_state *state, const qd_modifier *mod);
 
 /* ---- normalisation: iteration::normalize (src/quids.hpp:985-1017) ------- */
 int qd_normalize(qd_ctx *ctx, qd_state *state, double *total_proba);
+
+/* ---- expectation values: iteration::average_value (src/quids.hpp:208-234)
+ *      for the QCGD observables 0 = number of nodes, 1 = particle count
+ *      sum(L + R), 2 = [number of nodes == arg].  value = sum obs |m|^2 in a
+ *      fixed summation order.  cot (device, optional, 2 doubles per object)
+ *      receives d value / d Re m + i d value / d Im m = 2 obs m, the cotangent
+ *      the adjoint starts from. */
+int qd_average_value(qd_ctx *ctx, qd_state *state, int observable, int arg, double *value,
+                     double *cot);
+
+/* ---- adjoint of the step (no counterpart in the reference: gradients of a
+ *      loss of the final state with respect to the rules' real parameters,
+ *      for fitting QCGD angles).  `out` must be what qd_simulate returned for
+ *      this `in` and `rule` (normalised magnitudes, total_proba before
+ *      normalisation); the forward truncation is held as a fixed mask, and a
+ *      step whose parents were pre-truncated (max_num_object < in's size) is
+ *      rejected.  Device arrays of complex pairs (re, im):
+ *        out_cot  cotangent dL/dRe m + i dL/dIm m of out's magnitudes (read);
+ *        in_cot   the same for in's magnitudes (written).
+ *      dcoef[i], i < ndir <= 3: a rule of the same kind whose coefficient
+ *      fields hold the derivatives of rule's coefficients along real
+ *      parameter i (e.g. theta, phi); grad[i] (host) receives dL/dparam_i. */
+int qd_simulate_adjoint(qd_ctx *ctx, qd_state *in, const qd_rule *rule, const qd_rule *dcoef,
+                        int ndir, qd_state *out, const double *out_cot, double *in_cot,
+                        double *grad, uint64_t max_num_object);
+/* Cotangent through simulate(it, modifier): cot (device, 2 doubles per
+ * object) is multiplied in place by the conjugate of the factor the modifier
+ * applied; `after` is the state after the modifier. */
+int qd_modifier_adjoint(qd_ctx *ctx, qd_state *after, const qd_modifier *mod, double *cot);
 
 /* ---- single-object rule calls on the host (the rule_t virtual methods,
  *      src/quids.hpp:115-145), run by the same functor code as the device

@@ -226,6 +226,37 @@ struct QcgdCoef {
     cplx d, dn, dc, dnc;  // do_, do_not, do_conj, do_not_conj
 };
 
+// Forward-mode tangents (the reference gradient the adjoint is checked
+// against): a complex value with its derivatives along up to kDir real
+// parameters, multiplied through the rules in the same factor order as the
+// plain magnitudes.
+constexpr int kDir = 8;
+struct Dual {
+    cplx v;
+    cplx t[kDir];
+};
+static inline Dual cmul(const Dual &a, const Dual &b) {
+    Dual r;
+    r.v = cmul(a.v, b.v);
+    for (int i = 0; i < kDir; ++i) r.t[i] = cadd(cmul(a.t[i], b.v), cmul(a.v, b.t[i]));
+    return r;
+}
+static inline Dual cneg(const Dual &a) {
+    Dual r;
+    r.v = cneg(a.v);
+    for (int i = 0; i < kDir; ++i) r.t[i] = cneg(a.t[i]);
+    return r;
+}
+static inline Dual cscale(const Dual &a, double f) {
+    Dual r;
+    r.v = cscale(a.v, f);
+    for (int i = 0; i < kDir; ++i) r.t[i] = cscale(a.t[i], f);
+    return r;
+}
+struct DualCoef {
+    Dual d, dn, dc, dnc;
+};
+
 // split_merge::get_num_child, qcgd.hpp:623-642
 static u32 split_merge_num_child(const u8 *obj) {
     const u16 n = rd16(obj);
@@ -244,7 +275,8 @@ static u32 split_merge_num_child(const u8 *obj) {
 
 // split_merge::populate_child, qcgd.hpp:643-848, restated on decoded node
 // lists.  Returns the child graph, multiplies *mag in the reference's order.
-static Graph split_merge_child(const Graph &g, u32 child_id, const QcgdCoef &k, cplx *mag) {
+template <class K, class M>
+static Graph split_merge_child(const Graph &g, u32 child_id, const K &k, M *mag) {
     const int n = int(g.left.size());
     const auto &L = g.left, &R = g.right;
     bool first_split = L[0] && R[0];
@@ -341,7 +373,8 @@ static u32 flip_num_child(const u8 *obj, bool coin) {
     }
     return nc;
 }
-static void flip_child(u8 *child, u32 child_id, bool coin, const QcgdCoef &k, cplx *mag) {
+template <class K, class M>
+static void flip_child(u8 *child, u32 child_id, bool coin, const K &k, M *mag) {
     const u16 n = rd16(child);
     u8 *L = child + 2, *R = child + 2 + n;
     u32 cid = child_id;
@@ -388,8 +421,8 @@ static bool num_child(const qd_rule &r, const u8 *p, u32 psize, u32 &nc, u32 &ma
 
 // populate_child: writes the child into out (capacity >= max child size),
 // returns its size; multiplies *mag (may be null).
-static u32 populate(const qd_rule &r, const u8 *p, u32 psize, u32 child_id, u8 *out, cplx *mag) {
-    Rule rr{r};
+template <class K, class M>
+static u32 populate_t(const qd_rule &r, const K &k, const u8 *p, u32 psize, u32 child_id, u8 *out, M *mag) {
     switch (r.kind) {
     case QD_RULE_HADAMARD: {
         const u8 b = p[r.bit];
@@ -401,16 +434,37 @@ static u32 populate(const qd_rule &r, const u8 *p, u32 psize, u32 child_id, u8 *
     case QD_RULE_QCGD_ERASE_CREATE:
     case QD_RULE_QCGD_COIN:
         std::memcpy(out, p, psize);
-        flip_child(out, child_id, r.kind == QD_RULE_QCGD_COIN, rr.k(), mag);
+        flip_child(out, child_id, r.kind == QD_RULE_QCGD_COIN, k, mag);
         return psize;
     case QD_RULE_QCGD_SPLIT_MERGE: {
         Graph g = decode(p);
-        Graph c = split_merge_child(g, child_id, rr.k(), mag);
+        Graph c = split_merge_child(g, child_id, k, mag);
         encode(c, out);
         return u32(encoded_size(c));
     }
     }
     return 0;
+}
+static u32 populate(const qd_rule &r, const u8 *p, u32 psize, u32 child_id, u8 *out, cplx *mag) {
+    return populate_t(r, Rule{r}.k(), p, psize, child_id, out, mag);
+}
+// dcoef[t]: rule whose coefficient fields are the derivatives of r's
+// coefficients along parameter t (t < D).
+static DualCoef dual_coef(const qd_rule &r, const qd_rule *dcoef, int D) {
+    DualCoef k{};
+    const QcgdCoef v = Rule{r}.k();
+    k.d.v = v.d;
+    k.dn.v = v.dn;
+    k.dc.v = v.dc;
+    k.dnc.v = v.dnc;
+    for (int t = 0; t < D; ++t) {
+        const QcgdCoef d = Rule{dcoef[t]}.k();
+        k.d.t[t] = d.d;
+        k.dn.t[t] = d.dn;
+        k.dc.t[t] = d.dc;
+        k.dnc.t[t] = d.dnc;
+    }
+    return k;
 }
 
 static u64 rule_hash(const qd_rule &r, const u8 *obj, u32 size) {
@@ -513,8 +567,13 @@ static inline u32 align_pad(u32 size, u32 align) {
 // ---------------------------------------------------------------------------
 // simulate, src/quids.hpp:448-543 (simple truncation, explicit max)
 // ---------------------------------------------------------------------------
+// With D > 0, in_tan holds D tangents per input object (object-major) and
+// out_tan receives the output's: the forward-mode derivative of the step
+// along D real parameters, dcoef[t] holding the derivatives of the rule's
+// coefficients along parameter t.
 static int simulate(const State &in, const qd_rule &rule, State &out, u64 max_num_object,
-                    const qd_params &prm, qd_step_stats &st) {
+                    const qd_params &prm, qd_step_stats &st, const qd_rule *dcoef = nullptr, int D = 0,
+                    const std::vector<cplx> *in_tan = nullptr, std::vector<cplx> *out_tan = nullptr) {
     const size_t N = in.n();
     std::memset(&st, 0, sizeof st);
     st.num_parent = N;
@@ -554,7 +613,9 @@ static int simulate(const State &in, const qd_rule &rule, State &out, u64 max_nu
 
     std::vector<u64> H(C);
     std::vector<cplx> M(C);
+    std::vector<cplx> T(D ? C * D : 0);
     std::vector<u32> pidx(C), cid(C);
+    const DualCoef dk = D ? dual_coef(rule, dcoef, D) : DualCoef{};
 #pragma omp parallel
     {
         std::vector<u8> buf;
@@ -567,10 +628,20 @@ static int simulate(const State &in, const qd_rule &rule, State &out, u64 max_nu
             if (buf.size() < ms + 64) buf.resize(ms + 64);
             for (u32 j = 0; j < nchild[p]; ++j) {
                 const u64 c = cbegin[i] + j;
-                cplx m = in.mag[p];
-                u32 sz = populate(rule, po, in.size[p], j, buf.data(), &m);
+                u32 sz;
+                if (D) {
+                    Dual m{};
+                    m.v = in.mag[p];
+                    for (int t = 0; t < D; ++t) m.t[t] = (*in_tan)[p * D + t];
+                    sz = populate_t(rule, dk, po, in.size[p], j, buf.data(), &m);
+                    M[c] = m.v;
+                    for (int t = 0; t < D; ++t) T[c * D + t] = m.t[t];
+                } else {
+                    cplx m = in.mag[p];
+                    sz = populate(rule, po, in.size[p], j, buf.data(), &m);
+                    M[c] = m;
+                }
                 H[c] = rule_hash(rule, buf.data(), sz);
-                M[c] = m;
                 pidx[c] = u32(i);
                 cid[c] = j;
             }
@@ -593,24 +664,35 @@ static int simulate(const State &in, const qd_rule &rule, State &out, u64 max_nu
     struct Uniq {
         u64 hash, rep;
         cplx mag;
+        std::vector<cplx> tan;
     };
     std::vector<std::vector<Uniq>> per_bucket(B);
 #pragma omp parallel for schedule(dynamic, 4)
     for (size_t b = 0; b < B; ++b) {
         std::unordered_map<u64, size_t> idx;
         std::vector<std::vector<cplx>> members;
+        std::vector<std::vector<u64>> mc;
         auto &v = per_bucket[b];
         idx.reserve(bcount[b + 1] - bcount[b]);
         for (u64 k = bcount[b]; k < bcount[b + 1]; ++k) {
             const u64 c = order[k];
             auto [it, fresh] = idx.try_emplace(H[c], v.size());
             if (fresh) {
-                v.push_back({H[c], c, M[c]});
+                v.push_back({H[c], c, M[c], {}});
                 members.emplace_back();
+                mc.emplace_back();
             }
             members[it->second].push_back(M[c]);  // ascending candidate order
+            mc[it->second].push_back(c);
         }
-        for (size_t g = 0; g < v.size(); ++g) v[g].mag = pairwise_sum(members[g]);
+        for (size_t g = 0; g < v.size(); ++g) {
+            v[g].mag = pairwise_sum(members[g]);
+            for (int t = 0; t < D; ++t) {
+                std::vector<cplx> tv;
+                for (u64 c : mc[g]) tv.push_back(T[c * D + t]);
+                v[g].tan.push_back(pairwise_sum(tv));
+            }
+        }
     }
     std::vector<Uniq> uniq;
     for (auto &v : per_bucket)
@@ -663,6 +745,20 @@ static int simulate(const State &in, const qd_rule &rule, State &out, u64 max_nu
     for (size_t k = 0; k < K; ++k) nv[k] = cnorm(out.mag[k]);
     out.total_proba = canonical_sum(nv);
     const double f = std::sqrt(out.total_proba);
+    if (D && out_tan) {
+        // d(y / |y|) = (dy - y Re<y, dy> / |y|^2) / |y|
+        out_tan->assign(K * D, cplx{0, 0});
+        for (int t = 0; t < D; ++t) {
+            double sig = 0;
+            for (size_t k = 0; k < K; ++k)
+                sig += out.mag[k].re * uniq[k].tan[t].re + out.mag[k].im * uniq[k].tan[t].im;
+            for (size_t k = 0; k < K; ++k) {
+                const cplx d = uniq[k].tan[t];
+                (*out_tan)[k * D + t] = {(d.re - out.mag[k].re * sig / out.total_proba) / f,
+                                         (d.im - out.mag[k].im * sig / out.total_proba) / f};
+            }
+        }
+    }
     if (K > 0 && f != 1)
         for (auto &m : out.mag) m = {m.re / f, m.im / f};
     if (K == 0) out.total_proba = 0;
@@ -679,6 +775,8 @@ extern "C" {
 
 struct or_state {
     State s;
+    int D = 0;               // tangent directions carried (forward mode)
+    std::vector<cplx> tan;   // D per object, object-major
 };
 
 or_state *or_state_new(uint64_t n, const void *objects, const uint64_t *begin, const uint32_t *size,
@@ -731,9 +829,48 @@ int or_modifier(or_state *s, const qd_modifier *m) {
     bool ok = true;
     const size_t n = s->s.n();
 #pragma omp parallel for reduction(&& : ok)
-    for (size_t i = 0; i < n; ++i)
-        ok = apply_modifier(*m, s->s.objects.data() + s->s.begin[i], s->s.size[i], s->s.mag[i]) && ok;
+    for (size_t i = 0; i < n; ++i) {
+        u8 *o = s->s.objects.data() + s->s.begin[i];
+        if (s->D) {  // tangents take the factor the modifier applies to the magnitude
+            std::vector<u8> tmp(o, o + s->s.size[i]);
+            cplx f{1, 0};
+            apply_modifier(*m, tmp.data(), s->s.size[i], f);
+            for (int t = 0; t < s->D; ++t) s->tan[i * s->D + t] = cmul(s->tan[i * s->D + t], f);
+        }
+        ok = apply_modifier(*m, o, s->s.size[i], s->s.mag[i]) && ok;
+    }
     return ok ? QD_OK : QD_ERR_UNSUPPORTED;
+}
+
+// Forward-mode tangents: set / get D tangents per object (re, im pairs,
+// object-major), and a step carrying them (dcoef: D derivative rules).
+int or_state_set_tangent(or_state *s, int D, const double *tan) {
+    if (D < 0 || D > kDir) return QD_ERR_INVALID;
+    s->D = D;
+    s->tan.resize(s->s.n() * D);
+    for (size_t i = 0; i < s->tan.size(); ++i) s->tan[i] = {tan[2 * i], tan[2 * i + 1]};
+    return QD_OK;
+}
+
+void or_state_get_tangent(const or_state *s, double *tan) {
+    for (size_t i = 0; i < s->tan.size(); ++i) {
+        tan[2 * i] = s->tan[i].re;
+        tan[2 * i + 1] = s->tan[i].im;
+    }
+}
+
+int or_simulate_jvp(const or_state *in, const qd_rule *rule, const qd_rule *dcoef, or_state **out,
+                    uint64_t max_num_object, const qd_params *prm, qd_step_stats *st) {
+    auto *o = new or_state;
+    o->D = in->D;
+    int rc = simulate(in->s, *rule, o->s, max_num_object, *prm, *st, dcoef, in->D, &in->tan, &o->tan);
+    if (rc) {
+        delete o;
+        *out = nullptr;
+        return rc;
+    }
+    *out = o;
+    return QD_OK;
 }
 
 int or_num_child(const qd_rule *r, const void *obj, uint32_t size, uint32_t *nc, uint32_t *max_size) {
@@ -767,6 +904,28 @@ double or_canonical_sum(const double *v, uint64_t n) { return canonical_sum(std:
 // iteration::average_value (src/quids.hpp:208-234) for the QCGD observables
 // the adjoint loss uses: 0 = num_nodes, 1 = particle count sum(L+R),
 // 2 = [num_nodes == arg].  Summed sequentially in object order.
+static double observable_of(const u8 *o, int observable, int arg) {
+    const u16 n = rd16(o);
+    if (observable == 0) return n;
+    if (observable == 2) return (n == arg) ? 1.0 : 0.0;
+    double v = 0;
+    for (int k = 0; k < n; ++k) v += double(o[2 + k] != 0) + double(o[2 + n + k] != 0);
+    return v;
+}
+
+// d average_value / d parameter t = sum obs 2 Re(conj(m) dm_t), into dv[t].
+void or_average_value_jvp(const or_state *s, int observable, int arg, double *dv) {
+    for (int t = 0; t < s->D; ++t) dv[t] = 0;
+    for (size_t i = 0; i < s->s.n(); ++i) {
+        const double v = observable_of(s->s.objects.data() + s->s.begin[i], observable, arg);
+        const cplx m = s->s.mag[i];
+        for (int t = 0; t < s->D; ++t) {
+            const cplx d = s->tan[i * s->D + t];
+            dv[t] += 2 * v * (m.re * d.re + m.im * d.im);
+        }
+    }
+}
+
 double or_average_value(const or_state *s, int observable, int arg) {
     double avg = 0;
     for (size_t i = 0; i < s->s.n(); ++i) {

@@ -53,6 +53,11 @@ def _load():
         L.or_canonical_sum.restype = ctypes.c_double
         L.or_average_value.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.or_average_value.restype = ctypes.c_double
+        L.or_state_set_tangent.argtypes = [vp, ctypes.c_int, pd]
+        L.or_state_get_tangent.argtypes = [vp, pd]
+        L.or_simulate_jvp.argtypes = [vp, ctypes.POINTER(QdRule), ctypes.POINTER(QdRule), ctypes.POINTER(vp), u64,
+                                      ctypes.POINTER(QdParams), ctypes.POINTER(QdStepStats)]
+        L.or_average_value_jvp.argtypes = [vp, ctypes.c_int, ctypes.c_int, pd]
         _lib = L
     return _lib
 
@@ -231,3 +236,120 @@ def run_reference(family, hs, ops, tolerance=1e-30, align=8, threads=None, timeo
         if r.returncode != 0:
             raise RuntimeError(f"ref_driver failed ({r.returncode}): {r.stderr}")
         return read_dumps(out), r.stdout
+
+
+def program_jvp(hs0, program, params, observable, arg=0, tolerance=1e-30, align=8):
+    """Forward-mode derivative of a quids_amd.adjoint.Program's loss
+    (average_value of `observable` after the program) with respect to every
+    parameter: the value and tangents are carried together through each
+    step (oracle simulate with dual magnitudes, the truncation chosen by
+    value as in the plain step).  Returns (loss, gradient array, final
+    HostState)."""
+    from quids_amd._abi import QdParams, QdRule, QdStepStats
+    from quids_amd.adjoint import RuleOp
+    L = _load()
+    params = [float(p) for p in params]
+    D = len(params)
+    h = _to_or(hs0)
+    zeros = (ctypes.c_double * (2 * D * max(hs0.num_object, 1)))()
+    L.or_state_set_tangent(h, D, zeros)
+    prm = QdParams(tolerance, align, 1)
+    try:
+        for _ in range(program.n_iter):
+            for op in program.ops:
+                if isinstance(op, RuleOp):
+                    rule = op.make(params)
+                    tans = rule.tangent_descs(len(op.index))
+                    dcoef = (QdRule * max(D, 1))()
+                    for t in range(D):
+                        dcoef[t] = tans[op.index.index(t)] if t in op.index else QdRule(kind=rule.kind)
+                    out = ctypes.c_void_p()
+                    st = QdStepStats()
+                    d = rule.desc()
+                    rc = L.or_simulate_jvp(h, ctypes.byref(d), dcoef, ctypes.byref(out), int(program.max_num_object),
+                                           ctypes.byref(prm), ctypes.byref(st))
+                    if rc:
+                        raise RuntimeError(f"oracle simulate_jvp failed: {rc}")
+                    L.or_state_free(h)
+                    h = out.value
+                else:
+                    d = op.desc()
+                    if L.or_modifier(h, ctypes.byref(d)):
+                        raise RuntimeError("oracle modifier failed")
+        value = L.or_average_value(h, int(observable), int(arg))
+        dv = (ctypes.c_double * max(D, 1))()
+        L.or_average_value_jvp(h, int(observable), int(arg), dv)
+        final = _from_or(h)
+    finally:
+        L.or_state_free(h)
+    return value, np.array([dv[i] for i in range(D)]), final
+
+
+def program_loss(hs0, program, params, observable, arg=0, tolerance=1e-30, align=8):
+    """The loss alone, through the plain oracle steps (finite differences)."""
+    hs = hs0
+    for _ in range(program.n_iter):
+        for op in program.ops:
+            if hasattr(op, "make"):
+                hs, _ = simulate(hs, op.make(params), program.max_num_object, tolerance, align)
+            else:
+                hs = apply_modifier(hs, op)
+    return average_value(hs, observable, arg)
+
+
+
+def _set_tan(h, tan):
+    L = _load()
+    tan = np.ascontiguousarray(tan, np.complex128)
+    D = tan.shape[1] if tan.ndim == 2 else 0
+    flat = np.ascontiguousarray(tan.reshape(-1).view(np.float64)) if tan.size else np.zeros(1)
+    L.or_state_set_tangent(h, D, flat.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return D
+
+
+def _get_tan(h, n, D):
+    out = np.zeros(max(n * D, 1) * 2, np.float64)
+    _load().or_state_get_tangent(h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out[:2 * n * D].view(np.complex128).reshape(n, D)
+
+
+def simulate_jvp(hs, tan, rule, dcoef=None, max_num_object=(1 << 64) - 1, tolerance=1e-30, align=8):
+    """One oracle step carrying tangents: tan is (N, D) complex (derivatives
+    of the input magnitudes along D directions), dcoef D rule descriptors
+    (derivatives of the rule's coefficients; zero rules when None).
+    Returns (HostState, (K, D) tangents of the normalised output)."""
+    from quids_amd._abi import QdParams, QdRule, QdStepStats
+    L = _load()
+    h = _to_or(hs)
+    D = _set_tan(h, tan)
+    arr = (QdRule * max(D, 1))()
+    for t in range(D):
+        arr[t] = dcoef[t] if dcoef is not None else QdRule(kind=rule.desc().kind)
+    out = ctypes.c_void_p()
+    st = QdStepStats()
+    prm = QdParams(tolerance, align, 1)
+    d = rule.desc()
+    rc = L.or_simulate_jvp(h, ctypes.byref(d), arr, ctypes.byref(out), int(max_num_object), ctypes.byref(prm),
+                           ctypes.byref(st))
+    L.or_state_free(h)
+    if rc:
+        raise RuntimeError(f"oracle simulate_jvp failed: {rc}")
+    res = _from_or(out.value)
+    t_out = _get_tan(out.value, res.num_object, D)
+    L.or_state_free(out.value)
+    return res, t_out
+
+
+def modifier_jvp(hs, tan, mod):
+    """A modifier applied to a state carrying tangents: (HostState, tangents)."""
+    L = _load()
+    h = _to_or(hs)
+    D = _set_tan(h, tan)
+    d = mod.desc()
+    if L.or_modifier(h, ctypes.byref(d)):
+        L.or_state_free(h)
+        raise RuntimeError("oracle modifier failed")
+    res = _from_or(h)
+    t_out = _get_tan(h, res.num_object, D)
+    L.or_state_free(h)
+    return res, t_out

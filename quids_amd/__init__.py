@@ -173,6 +173,14 @@ class it_t:
         self._host_valid = False
         self._dev_valid = True
 
+    def copy(self):
+        """A device-to-device copy of this state (qd_state_copy)."""
+        self._sync_to_device()
+        c = it_t(ctx=self.ctx)
+        check(lib().qd_state_copy(c.handle, self.handle), "qd_state_copy")
+        c._device_changed()
+        return c
+
     # -- reference API ------------------------------------------------------
     @property
     def num_object(self):

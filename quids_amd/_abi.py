@@ -139,6 +139,11 @@ SIGNATURES = {
     "qd_rule_populate_child": [ctypes.POINTER(QdRule), _vp, _u32, _vp, _u32, _u32, _pu32, _pd],
     "qd_rule_hash": [ctypes.POINTER(QdRule), _vp, _u32, _pu64],
     "qd_modifier_apply_host": [ctypes.POINTER(QdModifier), _vp, _u32, _pd],
+    "qd_state_copy": [_vp, _vp],
+    "qd_average_value": [_vp, _vp, ctypes.c_int, ctypes.c_int, _pd, _vp],
+    "qd_simulate_adjoint": [_vp, _vp, ctypes.POINTER(QdRule), ctypes.POINTER(QdRule), ctypes.c_int, _vp, _vp, _vp,
+                            _pd, _u64],
+    "qd_modifier_adjoint": [_vp, _vp, ctypes.POINTER(QdModifier), _vp],
 }
 RESTYPES = {"qd_status_string": ctypes.c_char_p, "qd_last_error": ctypes.c_char_p}
 

@@ -250,7 +250,8 @@ QD_HD u32 flip_num_child(const Graph &g, bool coin) {
 
 // Child hash + magnitude: erase_create::populate_child (qcgd.hpp:487-510),
 // coin::populate_child (562-584) followed by hash_graph; nf = name_fold.
-QD_HD u64 flip_child_hash(const Graph &g, bool coin, u64 nf, u32 child_id, const QcgdCoef &k, cplx &mag) {
+template <class Coef, class Mag>
+QD_HD u64 flip_child_hash(const Graph &g, bool coin, u64 nf, u32 child_id, const Coef &k, Mag &mag) {
     u32 cid = child_id;
     u64 lh = 0, rh = 0;
     for (u32 i = 0; i < g.n; ++i) {
@@ -344,8 +345,10 @@ QD_HD void sm_fill_table(const Graph &g, u64 *tab) {
 // order, kind 0 = copy of node i, 1 = left half of i, 2 = right half of i,
 // 3 = merge of i with (i+1) mod n; multiplies mag with the reference's
 // factors in the reference's order when do_mag.
-template <class Emit>
-QD_HD void sm_walk(const Graph &g, bool first_hmlz, u32 child_id, const QcgdCoef &k, cplx &mag, bool do_mag,
+// Coef / Mag may be the dual-number types of the adjoint (value plus
+// derivatives), for which cmul / cneg are overloaded.
+template <class Coef, class Mag, class Emit>
+QD_HD void sm_walk(const Graph &g, bool first_hmlz, u32 child_id, const Coef &k, Mag &mag, bool do_mag,
                    Emit &&emit) {
     const u32 n = g.n;
     SmHeader h = sm_header(g);
@@ -409,8 +412,8 @@ QD_HD void sm_walk(const Graph &g, bool first_hmlz, u32 child_id, const QcgdCoef
 }
 
 // Child hash + magnitude from the per-parent table.
-QD_HD u64 sm_child_hash(const Graph &g, bool first_hmlz, const u64 *tab, u32 child_id, const QcgdCoef &k,
-                        cplx &mag) {
+template <class Coef, class Mag>
+QD_HD u64 sm_child_hash(const Graph &g, bool first_hmlz, const u64 *tab, u32 child_id, const Coef &k, Mag &mag) {
     u64 nh = 0, lh = 0, rh = 0;
     u32 c = 0;
     sm_walk(g, first_hmlz, child_id, k, mag, true, [&](int kind, u32 i, bool l, bool r) {
@@ -648,6 +651,69 @@ QD_HD void modifier_apply(const qd_modifier &m, u8 *obj, u32 size, cplx &mag) {
     }
     }
     (void)size;
+}
+
+// Magnitude factor a modifier applied, read from the object AFTER the
+// modifier (Y and Z flip their bit, so the bit before is the negation of the
+// bit after): the adjoint multiplies cotangents by its conjugate.
+QD_HD cplx modifier_factor_after(const qd_modifier &m, const u8 *obj) {
+    switch (m.kind) {
+    case QD_MOD_YGATE:  // i, times -1 when the bit was set
+        return obj[m.bit] ? cplx{0.0, 1.0} : cplx{-0.0, -1.0};
+    case QD_MOD_ZGATE:  // -1 when the bit was set
+        return obj[m.bit] ? cplx{1.0, 0.0} : cplx{-1.0, 0.0};
+    default:
+        return cplx{1.0, 0.0};
+    }
+}
+
+// ---- dual numbers for the adjoint ----------------------------------------
+// A complex value with its derivatives along up to kMaxDir real parameters
+// of a rule (theta, phi, xi).  The rule walks above multiply these in the
+// same factor order as the plain magnitudes.
+constexpr int kMaxDir = 3;
+struct DualC {
+    cplx v;
+    cplx t[kMaxDir];
+};
+QD_HD cplx cadd(cplx a, cplx b) { return {a.re + b.re, a.im + b.im}; }
+QD_HD DualC cneg(const DualC &a) {
+    DualC r;
+    r.v = cneg(a.v);
+    for (int i = 0; i < kMaxDir; ++i) r.t[i] = cneg(a.t[i]);
+    return r;
+}
+QD_HD DualC cmul(const DualC &a, const DualC &b) {
+    DualC r;
+    r.v = cmul(a.v, b.v);
+    for (int i = 0; i < kMaxDir; ++i) r.t[i] = cadd(cmul(a.t[i], b.v), cmul(a.v, b.t[i]));
+    return r;
+}
+struct DualCoef {
+    DualC d, dn, dc, dnc;  // do_, do_not, do_conj, do_not_conj with derivatives
+};
+// dr[i] holds, in its coefficient fields, the derivatives of r's
+// coefficients along parameter i (i < ndir); further directions are zero.
+QD_HD DualCoef dual_coef_of(const qd_rule &r, const qd_rule *dr, int ndir) {
+    DualCoef k;
+    const QcgdCoef v = coef_of(r);
+    k.d.v = v.d;
+    k.dn.v = v.dn;
+    k.dc.v = v.dc;
+    k.dnc.v = v.dnc;
+    for (int i = 0; i < kMaxDir; ++i) {
+        const QcgdCoef t = i < ndir ? coef_of(dr[i]) : QcgdCoef{};
+        k.d.t[i] = t.d;
+        k.dn.t[i] = t.dn;
+        k.dc.t[i] = t.dc;
+        k.dnc.t[i] = t.dnc;
+    }
+    return k;
+}
+QD_HD DualC dual_one() {
+    DualC r{};
+    r.v = cplx{1.0, 0.0};
+    return r;
 }
 
 }  // namespace qd

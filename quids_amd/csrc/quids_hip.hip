@@ -50,6 +50,7 @@
 #include "qd_k_dedup.hpp"
 #include "qd_k_select.hpp"
 #include "qd_k_finalize.hpp"
+#include "qd_k_adjoint.hpp"
 
 using namespace qd;
 
@@ -137,10 +138,12 @@ struct qd_ctx {
     DBuf bitmap, wprefix, padded, chunks, scanblk;
     DBuf biglist, bigsoff, bigkey, bigcand, bigmag, bigseg;
     DBuf pdesc, wi, wb, ebuf2, cnt2, slist;
+    DBuf adjk, adjv, adjg, adjd, adjp;
     std::vector<DBuf *> all() {
         return {&nc, &cb, &pre, &tab, &kept, &keys, &ebuf0, &ebuf1, &hist1, &off1, &cur1, &off2,
                 &nhist, &counters, &sel0, &sel1, &bitmap, &wprefix, &padded, &chunks, &scanblk,
-                &biglist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &ebuf2, &cnt2, &slist};
+                &biglist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &ebuf2, &cnt2, &slist,
+                &adjk, &adjv, &adjg, &adjd, &adjp};
     }
 };
 
@@ -228,7 +231,7 @@ static int check_err_word(u32 w) {
 }
 
 // canonical sum of |m|^2 (mag != null) or of vals
-static int canonical_sum(qd_ctx *ctx, const double2 *mag, u64 n, double *result) {
+static int canonical_sum(qd_ctx *ctx, const double2 *mag, const double *vals, u64 n, double *result) {
     if (n == 0) {
         *result = 0.0;
         return QD_OK;
@@ -239,8 +242,7 @@ static int canonical_sum(qd_ctx *ctx, const double2 *mag, u64 n, double *result)
     QD_TRY(ctx->chunks.ensure(2 * cap * sizeof(double)));
     double *a = ctx->chunks.as<double>(), *b = a + cap;
     prof_begin(ctx, QD_PHASE_NORMALIZE);
-    hipLaunchKernelGGL(k_sum_chunks, dim3(nchunk), dim3(kSumThreads), 0, ctx->stream, mag, (const double *)nullptr, n,
-                       a);
+    hipLaunchKernelGGL(k_sum_chunks, dim3(nchunk), dim3(kSumThreads), 0, ctx->stream, mag, vals, n, a);
     u64 m = nchunk;
     while (m > 1) {
         const u64 nc2 = (m + kSumChunk - 1) / kSumChunk;
@@ -257,7 +259,7 @@ static int canonical_sum(qd_ctx *ctx, const double2 *mag, u64 n, double *result)
 
 static int normalize_state(qd_ctx *ctx, qd_state *s) {
     double total = 0;
-    QD_TRY(canonical_sum(ctx, s->mag.as<double2>(), s->n, &total));
+    QD_TRY(canonical_sum(ctx, s->mag.as<double2>(), nullptr, s->n, &total));
     s->total_proba = total;
     if (s->n == 0) {
         s->total_proba = 0;
@@ -692,6 +694,83 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
 }
 
 // ===========================================================================
+// adjoint of one step (see qd_k_adjoint.hpp for the algebra)
+// ===========================================================================
+template <int KIND>
+static int adjoint_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, const qd_rule *dcoef, int ndir, qd_state *out,
+                        const double2 *gout, double2 *gin, double *grad, u64 max_obj) {
+    hipStream_t s = ctx->stream;
+    const u64 N = in->n, K = out->n;
+    for (int t = 0; t < ndir; ++t) grad[t] = 0.0;
+    if (N == 0) return QD_OK;
+    if (max_obj < N)
+        return fail(QD_ERR_UNSUPPORTED, "adjoint of a step whose parents were pre-truncated (max_num_object < N)");
+    if (K == 0) {
+        QD_HIP(hipMemsetAsync(gin, 0, N * sizeof(double2), s));
+        QD_HIP(hipStreamSynchronize(s));
+        return QD_OK;
+    }
+    if (K > 0xFFFFFFFFull) return fail(QD_ERR_CAPACITY, "adjoint output state beyond 2^32 objects");
+    Parents P{in->objects.as<u8>(), in->begin.as<u64>(), in->size.as<u32>(), in->mag.as<double2>(), N};
+
+    // per-parent prep (num_child, split_merge root tables / name folds)
+    QD_TRY(ctx->nc.ensure(N * sizeof(u32)));
+    QD_TRY(ctx->cb.ensure((N + 1) * sizeof(u64)));
+    QD_TRY(ctx->pre.ensure(N * sizeof(u64)));
+    QD_TRY(ctx->counters.ensure(64 * sizeof(u64)));
+    if (KIND == QD_RULE_QCGD_SPLIT_MERGE) QD_TRY(ctx->tab.ensure((in->bytes / 2 + 16) * sizeof(u64)));
+    u64 *counters = ctx->counters.as<u64>();
+    QD_HIP(hipMemsetAsync(counters, 0, 64 * sizeof(u64), s));
+    u32 *maxsz = reinterpret_cast<u32 *>(counters + 3);
+    u32 *errw = reinterpret_cast<u32 *>(counters + 4);
+    hipLaunchKernelGGL(k_prep<KIND>, dim3(grid_for(N)), dim3(256), 0, s, P, (const u64 *)nullptr, rule,
+                       ctx->nc.as<u32>(), ctx->pre.as<u64>(), ctx->tab.as<u64>(), maxsz, errw);
+    QD_HIP(hipGetLastError());
+    QD_TRY(run_scan(ctx, LoadU32{ctx->nc.as<u32>()}, N, ctx->cb.as<u64>()));
+
+    // g_y from the cotangent of the normalised output
+    QD_TRY(ctx->adjd.ensure(K * sizeof(double)));
+    QD_TRY(ctx->adjg.ensure(K * sizeof(double2)));
+    hipLaunchKernelGGL(k_adj_dot, dim3(grid_for(K)), dim3(256), 0, s, gout, out->mag.as<double2>(), K,
+                       ctx->adjd.as<double>());
+    QD_HIP(hipGetLastError());
+    double alpha = 0;
+    QD_TRY(canonical_sum(ctx, nullptr, ctx->adjd.as<double>(), K, &alpha));
+    const double sn = std::sqrt(out->total_proba);
+    if (!(sn > 0)) return fail(QD_ERR_INVALID, "output state has zero total probability");
+    hipLaunchKernelGGL(k_adj_gy, dim3(grid_for(K)), dim3(256), 0, s, gout, out->mag.as<double2>(), K, alpha, sn,
+                       ctx->adjg.as<double2>());
+    QD_HIP(hipGetLastError());
+
+    // hash -> output index table, load factor <= 1/2
+    u64 T = 1024;
+    while (T < 2 * K) T <<= 1;
+    QD_TRY(ctx->adjk.ensure(T * sizeof(u64)));
+    QD_TRY(ctx->adjv.ensure(T * sizeof(u32)));
+    QD_HIP(hipMemsetAsync(ctx->adjk.p, 0xFF, T * sizeof(u64), s));
+    u64 *special = counters + 8;
+    hipLaunchKernelGGL(k_adj_table<KIND>, dim3(grid_for(K)), dim3(256), 0, s, out->objects.as<u8>(),
+                       out->begin.as<u64>(), out->size.as<u32>(), K, rule, ctx->adjk.as<u64>(), ctx->adjv.as<u32>(),
+                       T - 1, special);
+    QD_HIP(hipGetLastError());
+    u64 hb[16];
+    QD_TRY(sync_read(ctx, counters, 16 * sizeof(u64), hb));
+    QD_TRY(check_err_word(u32(hb[4])));
+
+    // gather over all candidates, one wavefront per parent
+    const int nd = std::max(ndir, 0);
+    QD_TRY(ctx->adjp.ensure(std::max<u64>(u64(nd) * N, 1) * sizeof(double)));
+    const DualCoef dk = dual_coef_of(rule, dcoef, nd);
+    hipLaunchKernelGGL(k_adjoint<KIND>, dim3(grid_for(N, 4, 65536)), dim3(256), 0, s, P, ctx->cb.as<u64>(), rule, dk,
+                       nd, ctx->pre.as<u64>(), ctx->tab.as<u64>(), ctx->adjk.as<u64>(), ctx->adjv.as<u32>(), T - 1,
+                       hb[8], ctx->adjg.as<double2>(), gin, ctx->adjp.as<double>());
+    QD_HIP(hipGetLastError());
+    for (int t = 0; t < nd; ++t) QD_TRY(canonical_sum(ctx, nullptr, ctx->adjp.as<double>() + u64(t) * N, N, &grad[t]));
+    QD_HIP(hipStreamSynchronize(s));
+    return QD_OK;
+}
+
+// ===========================================================================
 // C ABI
 // ===========================================================================
 extern "C" {
@@ -930,6 +1009,28 @@ int qd_state_upload(qd_state *s, uint64_t n, const void *objects, const uint64_t
     return QD_OK;
 }
 
+int qd_state_copy(qd_state *dst, const qd_state *src) {
+    if (!dst || !src) return fail(QD_ERR_INVALID, "null argument");
+    if (dst == src) return QD_OK;
+    if (dst->ctx != src->ctx) return fail(QD_ERR_INVALID, "states belong to different contexts");
+    qd_ctx *c = dst->ctx;
+    QD_HIP(hipSetDevice(c->device));
+    const u64 n = src->n, bytes = src->bytes;
+    QD_TRY(state_reserve(dst, n, bytes));
+    hipStream_t s = c->stream;
+    if (bytes) QD_HIP(hipMemcpyAsync(dst->objects.p, src->objects.p, bytes, hipMemcpyDeviceToDevice, s));
+    QD_HIP(hipMemcpyAsync(dst->begin.p, src->begin.p, (n + 1) * sizeof(u64), hipMemcpyDeviceToDevice, s));
+    if (n) {
+        QD_HIP(hipMemcpyAsync(dst->size.p, src->size.p, n * sizeof(u32), hipMemcpyDeviceToDevice, s));
+        QD_HIP(hipMemcpyAsync(dst->mag.p, src->mag.p, n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+    }
+    QD_HIP(hipStreamSynchronize(s));
+    dst->n = n;
+    dst->bytes = bytes;
+    dst->total_proba = src->total_proba;
+    return QD_OK;
+}
+
 int qd_state_info(qd_state *s, uint64_t *n, uint64_t *bytes, double *total) {
     if (!s) return fail(QD_ERR_INVALID, "null state");
     if (n) *n = s->n;
@@ -1032,6 +1133,69 @@ int qd_normalize(qd_ctx *c, qd_state *s, double *total_proba) {
     QD_TRY(normalize_state(c, s));
     if (total_proba) *total_proba = s->total_proba;
     return QD_OK;
+}
+
+int qd_simulate_adjoint(qd_ctx *c, qd_state *in, const qd_rule *rule, const qd_rule *dcoef, int ndir, qd_state *out,
+                        const double *out_cot, double *in_cot, double *grad, uint64_t max_num_object) {
+    if (!c || !in || !rule || !out) return fail(QD_ERR_INVALID, "null argument");
+    if (ndir < 0 || ndir > kMaxDir) return fail(QD_ERR_INVALID, "ndir must be in [0, 3]");
+    if (ndir > 0 && (!dcoef || !grad)) return fail(QD_ERR_INVALID, "null derivative rules or gradient");
+    if (in->n && !in_cot) return fail(QD_ERR_INVALID, "null input cotangent");
+    if (out->n && !out_cot) return fail(QD_ERR_INVALID, "null output cotangent");
+    if (in == out) return fail(QD_ERR_INVALID, "input and output states must differ");
+    if (in->ctx != c || out->ctx != c) return fail(QD_ERR_INVALID, "state belongs to another context");
+    for (int t = 0; t < ndir; ++t)
+        if (dcoef[t].kind != rule->kind) return fail(QD_ERR_INVALID, "derivative rule of another kind");
+    QD_HIP(hipSetDevice(c->device));
+    const double2 *go = reinterpret_cast<const double2 *>(out_cot);
+    double2 *gi = reinterpret_cast<double2 *>(in_cot);
+    double dummy[kMaxDir];
+    double *gr = grad ? grad : dummy;
+    switch (rule->kind) {
+    case QD_RULE_HADAMARD:
+        return adjoint_kind<QD_RULE_HADAMARD>(c, in, *rule, dcoef, 0, out, go, gi, gr, max_num_object);
+    case QD_RULE_QCGD_ERASE_CREATE:
+        return adjoint_kind<QD_RULE_QCGD_ERASE_CREATE>(c, in, *rule, dcoef, ndir, out, go, gi, gr, max_num_object);
+    case QD_RULE_QCGD_COIN:
+        return adjoint_kind<QD_RULE_QCGD_COIN>(c, in, *rule, dcoef, ndir, out, go, gi, gr, max_num_object);
+    case QD_RULE_QCGD_SPLIT_MERGE:
+        return adjoint_kind<QD_RULE_QCGD_SPLIT_MERGE>(c, in, *rule, dcoef, ndir, out, go, gi, gr, max_num_object);
+    }
+    return fail(QD_ERR_UNSUPPORTED, "rule kind has no device implementation");
+}
+
+int qd_modifier_adjoint(qd_ctx *c, qd_state *after, const qd_modifier *m, double *cot) {
+    if (!c || !after || !m) return fail(QD_ERR_INVALID, "null argument");
+    if (!modifier_known(m->kind)) return fail(QD_ERR_UNSUPPORTED, "modifier kind has no device implementation");
+    if (after->ctx != c) return fail(QD_ERR_INVALID, "state belongs to another context");
+    if (after->n == 0) return QD_OK;
+    if (!cot) return fail(QD_ERR_INVALID, "null cotangent");
+    QD_HIP(hipSetDevice(c->device));
+    QD_TRY(c->counters.ensure(64 * sizeof(u64)));
+    u32 *errw = reinterpret_cast<u32 *>(c->counters.as<u64>() + 4);
+    QD_HIP(hipMemsetAsync(errw, 0, sizeof(u32), c->stream));
+    hipLaunchKernelGGL(k_modifier_adjoint, dim3(grid_for(after->n)), dim3(256), 0, c->stream,
+                       after->objects.as<u8>(), after->begin.as<u64>(), after->size.as<u32>(), after->n, *m,
+                       reinterpret_cast<double2 *>(cot), errw);
+    QD_HIP(hipGetLastError());
+    u32 w = 0;
+    QD_TRY(sync_read(c, errw, sizeof(u32), &w));
+    return check_err_word(w);
+}
+
+int qd_average_value(qd_ctx *c, qd_state *s, int observable, int arg, double *value, double *cot) {
+    if (!c || !s || !value) return fail(QD_ERR_INVALID, "null argument");
+    if (observable < 0 || observable > 2) return fail(QD_ERR_INVALID, "unknown observable");
+    if (s->ctx != c) return fail(QD_ERR_INVALID, "state belongs to another context");
+    QD_HIP(hipSetDevice(c->device));
+    *value = 0.0;
+    if (s->n == 0) return QD_OK;
+    QD_TRY(c->adjd.ensure(s->n * sizeof(double)));
+    hipLaunchKernelGGL(k_observable, dim3(grid_for(s->n)), dim3(256), 0, c->stream, s->objects.as<u8>(),
+                       s->begin.as<u64>(), s->size.as<u32>(), s->mag.as<double2>(), s->n, observable, arg,
+                       c->adjd.as<double>(), reinterpret_cast<double2 *>(cot));
+    QD_HIP(hipGetLastError());
+    return canonical_sum(c, nullptr, c->adjd.as<double>(), s->n, value);
 }
 
 int qd_rule_num_child(const qd_rule *r, const void *parent, uint32_t size, uint32_t *num_child,

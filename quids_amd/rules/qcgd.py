@@ -44,6 +44,27 @@ class _CoefRule(DeviceRule):
                       do_not_conj_re=self.do_not_conj.real, do_not_conj_im=self.do_not_conj.imag)
 
 
+    #: names of the real parameters, in the order of tangent_descs
+    params = ("theta", "phi", "xi")
+
+    def tangent_descs(self, ndir=2):
+        """Rule descriptors whose coefficient fields hold the derivatives of
+        this rule's coefficients along theta, phi (and xi when ndir == 3):
+        d do_/d theta = polar(cos theta, phi), d do_/d phi = i do_,
+        d do_not/d theta = polar(-sin theta, xi), d do_not/d xi = i do_not,
+        and the conjugates of these for do_conj / do_not_conj."""
+        th, ph, xi = self.theta, self.phi, self.xi
+        d_do = [_polar(math.cos(th), ph), 1j * self.do_, 0j]
+        d_dn = [_polar(-math.sin(th), xi), 0j, 1j * self.do_not]
+        out = []
+        for t in range(ndir):
+            a, b = complex(d_do[t]), complex(d_dn[t])
+            out.append(QdRule(kind=self.kind, bit=0, coef=0.0, do_re=a.real, do_im=a.imag,
+                              do_not_re=b.real, do_not_im=b.imag, do_conj_re=a.real, do_conj_im=-a.imag,
+                              do_not_conj_re=b.real, do_not_conj_im=-b.imag))
+        return out
+
+
 class erase_create(_CoefRule):
     """qcgd.hpp:459-532: at every node with left == right, either flip both
     particle bits (factor do_, or do_conj when both were set) or not (do_not,
