@@ -197,6 +197,24 @@ int qd_simulate(qd_ctx *ctx, qd_state *in, const qd_rule *rule, qd_state *out,
                 qd_symbolic *sy, uint64_t max_num_object, const qd_params *params,
                 qd_step_stats *stats);
 
+/* ---- batched step (no counterpart in the reference: many independent
+ *      QCGD simulations advanced by one launch sequence).  `in` holds the
+ *      instances back to back: instance i owns objects [in_seg[i],
+ *      in_seg[i+1]) (host array, num_instances + 1 entries, in_seg[0] = 0,
+ *      in_seg[num_instances] = the state size).  Instance i runs rules[i]
+ *      (all of one QCGD kind, each with its own coefficients) with its own
+ *      max_num_object[i] (explicit: 0 is not accepted, and an instance may not
+ *      hold more objects than its cap).  The result is what num_instances
+ *      separate qd_simulate calls return, bit for bit, written back to back
+ *      into `out`; out_seg (host, num_instances + 1) receives the instances'
+ *      object offsets and stats (host, optional, num_instances) their
+ *      counters.  Equal objects of different instances never merge: the
+ *      instance index is folded into the dedup key. */
+int qd_simulate_batched(qd_ctx *ctx, qd_state *in, const uint64_t *in_seg, uint32_t num_instances,
+                        const qd_rule *rules, qd_state *out, uint64_t *out_seg,
+                        const uint64_t *max_num_object, const qd_params *params,
+                        qd_step_stats *stats);
+
 /* ---- modifier: quids::simulate(it, modifier) (src/quids.hpp:436-438,
  *      973-980) ----------------------------------------------------------- */
 int qd_apply_modifier(qd_ctx *ctx, qd_state *state, const qd_modifier *mod);

@@ -140,6 +140,8 @@ SIGNATURES = {
     "qd_rule_hash": [ctypes.POINTER(QdRule), _vp, _u32, _pu64],
     "qd_modifier_apply_host": [ctypes.POINTER(QdModifier), _vp, _u32, _pd],
     "qd_state_copy": [_vp, _vp],
+    "qd_simulate_batched": [_vp, _vp, _pu64, _u32, ctypes.POINTER(QdRule), _vp, _pu64, _pu64, ctypes.POINTER(QdParams),
+                            ctypes.POINTER(QdStepStats)],
     "qd_average_value": [_vp, _vp, ctypes.c_int, ctypes.c_int, _pd, _vp],
     "qd_simulate_adjoint": [_vp, _vp, ctypes.POINTER(QdRule), ctypes.POINTER(QdRule), ctypes.c_int, _vp, _vp, _vp,
                             _pd, _u64],

@@ -1,0 +1,317 @@
+#pragma once
+// Batched step: many independent simulations (instances) in one launch
+// sequence.  The instances' parents are consecutive in one state, so their
+// candidates are consecutive too; instance i adds i * kFold to its children's
+// hashes (the dedup key), so equal objects of different instances never merge
+// while the merge inside an instance is the plain step's.  Truncation and
+// normalisation are per instance, in the plain step's canonical orders, so
+// every instance comes out exactly as its own plain simulate() call.
+#include <hip/hip_runtime.h>
+
+#include "qd_device.hpp"
+#include "qd_k_dedup.hpp"
+#include "qd_k_finalize.hpp"
+#include "qd_k_select.hpp"
+#include "qd_rules.hpp"
+
+namespace qd {
+
+constexpr u64 kFold = 0x9E3779B97F4A7C15ull;  // odd: key = hash + instance * kFold
+constexpr u32 kMaxInst = 4096;                // instances per batched call (LDS counters)
+constexpr u32 kNoInst = ~0u;
+
+// largest i < n with seg[i] <= x (seg holds n + 1 ascending offsets)
+__device__ inline u32 find_seg(const u64 *seg, u32 n, u64 x) {
+    u32 lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const u32 mid = (lo + hi + 1) >> 1;
+        if (seg[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// out[i] = v[idx[i]] for i < m
+__global__ __launch_bounds__(256) void k_gather_u64(const u64 *v, const u64 *idx, u32 m, u64 *out) {
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) out[i] = v[idx[i]];
+}
+
+// out[i] = number of set bits of the bitmap below position idx[i] (the
+// survivor rank of candidate idx[i]), i < m
+__global__ __launch_bounds__(256) void k_rank_at(const u64 *bitmap, const u64 *wprefix, const u64 *idx, u32 m,
+                                                 u64 W, u64 *out) {
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+        const u64 c = idx[i];
+        const u64 w = c >> 6;
+        out[i] = w >= W ? wprefix[W] : wprefix[w] + __popcll(bitmap[w] & ((1ull << (c & 63)) - 1));
+    }
+}
+
+// Per dedup slot: the instance of its representative (kNoInst for empty
+// slots), and per-instance unique counts.
+__global__ __launch_bounds__(256) void k_seg_count(const u64 *UR, u64 nslots, const u64 *cseg, u32 n, u32 *sinst,
+                                                   u64 *ucount) {
+    __shared__ u32 cnt[kMaxInst];
+    for (u32 i = threadIdx.x; i < n; i += blockDim.x) cnt[i] = 0;
+    __syncthreads();
+    for (u64 s = u64(blockIdx.x) * blockDim.x + threadIdx.x; s < nslots; s += u64(gridDim.x) * blockDim.x) {
+        const u64 r = UR[s];
+        u32 inst = kNoInst;
+        if (r != kNoRep) {
+            inst = find_seg(cseg, n, r);
+            atomicAdd(&cnt[inst], 1u);
+        }
+        sinst[s] = inst;
+    }
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < n; i += blockDim.x)
+        if (cnt[i]) atomicAdd(reinterpret_cast<unsigned long long *>(ucount + i), (unsigned long long)cnt[i]);
+}
+
+// Groups the slots of truncated instances by instance: each block ranks its
+// tile's slots per instance in LDS and reserves one range per instance.
+constexpr int kSegTile = 4096;
+__global__ __launch_bounds__(256) void k_seg_scatter(const u32 *sinst, u64 nslots, const u32 *trunc, u32 n,
+                                                     u64 *cursor, u64 *list) {
+    __shared__ u32 cnt[kMaxInst];
+    __shared__ u64 base[kMaxInst];
+    constexpr int kPer = kSegTile / 256;
+    for (u64 t0 = u64(blockIdx.x) * kSegTile; t0 < nslots; t0 += u64(gridDim.x) * kSegTile) {
+        for (u32 i = threadIdx.x; i < n; i += blockDim.x) cnt[i] = 0;
+        __syncthreads();
+        u32 inst[kPer], rank[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const u64 s = t0 + threadIdx.x + u64(q) * 256;
+            inst[q] = kNoInst;
+            if (s < nslots) {
+                const u32 i = sinst[s];
+                if (i != kNoInst && trunc[i]) {
+                    inst[q] = i;
+                    rank[q] = atomicAdd(&cnt[i], 1u);
+                }
+            }
+        }
+        __syncthreads();
+        for (u32 i = threadIdx.x; i < n; i += blockDim.x)
+            if (cnt[i]) base[i] = atomicAdd(reinterpret_cast<unsigned long long *>(cursor + i), (unsigned long long)cnt[i]);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kPer; ++q)
+            if (inst[q] != kNoInst) list[base[inst[q]] + rank[q]] = t0 + threadIdx.x + u64(q) * 256;
+        __syncthreads();
+    }
+}
+
+// Radix-select state of one instance: the K-th largest 128-bit key
+// (|m|^2 bits, ~hash) is being located 12 bits at a time; `prefix` holds
+// the `pos` bits found so far (lower bits zero), r how many keys with that
+// prefix are still to be kept.  When active is 0 the instance keeps every
+// key >= prefix (keepall: every key).
+struct SegSel {
+    u64 hi, lo, r;
+    u32 pos, active, keepall, err;
+};
+
+__device__ inline void seg_key(const u64 *UH, const double2 *UM, u64 slot, u32 inst, u64 &hi, u64 &lo) {
+    const double2 m = UM[slot];
+    hi = u64(__double_as_longlong(m.x * m.x + m.y * m.y));
+    lo = ~(UH[slot] - u64(inst) * kFold);
+}
+
+__device__ inline bool prefix_eq(u64 hi, u64 lo, const SegSel &q) {
+    if (q.pos == 0) return true;
+    if (q.pos <= 64) return (hi >> (64 - q.pos)) == (q.hi >> (64 - q.pos));
+    return hi == q.hi && (lo >> (128 - q.pos)) == (q.lo >> (128 - q.pos));
+}
+
+// Tile of the grouped slot list: items [begin, end) of instance inst.
+struct SegTile {
+    u64 begin, end;
+    u32 inst, pad;
+};
+
+// One workgroup per tile: LDS histogram of the next 12 key bits among the
+// keys matching the instance's prefix, added into the instance's global row.
+__global__ __launch_bounds__(256) void k_seg_hist(const u64 *list, const SegTile *tiles, u32 ntiles, const u64 *UH,
+                                                  const double2 *UM, const SegSel *sel, u32 *hist) {
+    __shared__ u32 h[kHistBins];
+    for (u32 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const SegTile tl = tiles[t];
+        const SegSel q = sel[tl.inst];
+        if (!q.active) continue;  // uniform over the block
+        for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) h[i] = 0;
+        __syncthreads();
+        for (u64 k = tl.begin + threadIdx.x; k < tl.end; k += blockDim.x) {
+            u64 hi, lo;
+            seg_key(UH, UM, list[k], tl.inst, hi, lo);
+            if (prefix_eq(hi, lo, q)) atomicAdd(&h[key_digit(hi, lo, q.pos)], 1u);
+        }
+        __syncthreads();
+        u32 *row = hist + u64(tl.inst) * kHistBins;
+        for (int i = threadIdx.x; i < kHistBins; i += blockDim.x)
+            if (h[i]) atomicAdd(&row[i], h[i]);
+        __syncthreads();
+    }
+}
+
+// One workgroup (256 threads) per instance: picks the digit holding the
+// r-th largest key from the instance's histogram row, updates the state and
+// clears the row for the next round; sets *more when the instance needs
+// another round.
+__global__ __launch_bounds__(256) void k_seg_pick(SegSel *sel, u32 *hist, u32 n, u32 *more) {
+    __shared__ u64 part[256];
+    __shared__ u32 found_d;
+    __shared__ u64 found_above, found_cnt;
+    const u32 inst = blockIdx.x;
+    if (inst >= n) return;
+    SegSel q = sel[inst];
+    if (!q.active) return;
+    u32 *row = hist + u64(inst) * kHistBins;
+    constexpr int kPer = kHistBins / 256;  // 16 bins per thread, thread 0 holds the top ones
+    const u32 t = threadIdx.x;
+    if (t == 0) found_cnt = ~0ull;
+    u64 local = 0;
+    for (int j = 0; j < kPer; ++j) local += row[kHistBins - 1 - (t * kPer + j)];
+    part[t] = local;
+    __syncthreads();
+    // exclusive prefix over threads in order (bins descending)
+    for (u32 d = 1; d < 256; d <<= 1) {
+        const u64 v = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    const u64 before = part[t] - local;
+    if (before < q.r && q.r <= before + local) {
+        u64 cum = before;
+        for (int j = 0; j < kPer; ++j) {
+            const u32 d = kHistBins - 1 - (t * kPer + j);
+            const u64 c = row[d];
+            if (cum + c >= q.r) {
+                found_d = d;
+                found_above = cum;
+                found_cnt = c;
+                break;
+            }
+            cum += c;
+        }
+    }
+    __syncthreads();
+    for (int i = t; i < kHistBins; i += 256) row[i] = 0;
+    if (t == 0) {
+        if (found_cnt == ~0ull) {  // fewer keys than r: inconsistent counts
+            q.active = 0;
+            q.err = 1;
+            sel[inst] = q;
+            return;
+        }
+        // the digit covers key bits [116 - pos, 128 - pos) (bit 127 = top of hi)
+        const u64 d = found_d;
+        if (q.pos < 64) {
+            const u32 sh = 64 - q.pos;
+            if (sh >= 12) q.hi |= d << (sh - 12);
+            else {
+                q.hi |= d >> (12 - sh);
+                q.lo |= d << (52 + sh);
+            }
+        } else if (q.pos <= 116) {
+            q.lo |= d << (116 - q.pos);
+        } else {
+            q.lo |= d >> (q.pos - 116);  // pos 120: the last 8 key bits
+        }
+        q.pos = min(q.pos + 12, 128u);
+        q.r -= found_above;
+        if (found_cnt == q.r) q.active = 0;
+        else if (q.pos >= 128) {
+            q.active = 0;
+            q.err = 1;  // equal keys: cannot happen, hashes are distinct in an instance
+        } else atomicOr(more, 1u);
+        sel[inst] = q;
+    }
+}
+
+// Marks the survivors: every unique of an instance that is not truncated,
+// and the keys >= prefix of a truncated one.  Lanes hold consecutive slots.
+__global__ __launch_bounds__(256) void k_seg_mark(const u32 *sinst, const u64 *UH, const u64 *UR, const double2 *UM,
+                                                  u64 nslots, const SegSel *sel, u64 *bitmap, u64 *sbm) {
+    const u64 stride = u64(gridDim.x) * blockDim.x;
+    for (u64 base = u64(blockIdx.x) * blockDim.x; base < nslots; base += stride) {
+        const u64 s = base + threadIdx.x;
+        bool keep = false;
+        if (s < nslots) {
+            const u32 inst = sinst[s];
+            if (inst != kNoInst) {
+                const SegSel q = sel[inst];
+                if (q.keepall) keep = true;
+                else {
+                    u64 hi, lo;
+                    seg_key(UH, UM, s, inst, hi, lo);
+                    keep = hi > q.hi || (hi == q.hi && lo >= q.lo);
+                }
+            }
+        }
+        const u64 first = s & ~u64(kWave - 1);
+        if (first < nslots) select_word(keep, s, first, UR, bitmap, sbm);
+    }
+}
+
+// Segmented canonical sum, level 1: chunk c (global) of instance i covers
+// survivors [kseg[i] + j * kSumChunk, ...) where j = c - chunk0[i]; sums
+// |m|^2 exactly as k_sum_chunks does.
+__global__ __launch_bounds__(kSumThreads) void k_seg_sum_chunks(const double2 *mag, const u64 *kseg,
+                                                                const u64 *chunk0, u32 n, double *out) {
+    __shared__ double s[kSumThreads];
+    const u64 c = blockIdx.x;
+    const u32 inst = find_seg(chunk0, n, c);
+    const u64 beg = kseg[inst] + (c - chunk0[inst]) * kSumChunk, end = kseg[inst + 1];
+    double a = 0.0;
+    for (int j = 0; j < kSumChunk / kSumThreads; ++j) {
+        const u64 i = beg + threadIdx.x + u64(kSumThreads) * j;
+        if (i < end) {
+            const double2 m = mag[i];
+            a += m.x * m.x + m.y * m.y;
+        }
+    }
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int stride = kSumThreads / 2; stride > 0; stride >>= 1) {
+        if (threadIdx.x < stride) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + stride];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = s[0];
+}
+
+// Level 2, one block per instance over its (at most kSumChunk) chunk sums.
+__global__ __launch_bounds__(kSumThreads) void k_seg_sum_final(const double *csum, const u64 *chunk0, u32 n,
+                                                               double *total) {
+    __shared__ double s[kSumThreads];
+    const u32 inst = blockIdx.x;
+    const u64 beg = chunk0[inst], end = chunk0[inst + 1];
+    double a = 0.0;
+    for (int j = 0; j < kSumChunk / kSumThreads; ++j) {
+        const u64 i = beg + threadIdx.x + u64(kSumThreads) * j;
+        if (i < end) a += csum[i];
+    }
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int stride = kSumThreads / 2; stride > 0; stride >>= 1) {
+        if (threadIdx.x < stride) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + stride];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) total[inst] = s[0];
+}
+
+// m /= sqrt(total of its instance), skipped where that factor is exactly 1.
+__global__ __launch_bounds__(256) void k_seg_scale(double2 *mag, u64 K, const u64 *kseg, u32 n, const double *total) {
+    for (u64 k = u64(blockIdx.x) * blockDim.x + threadIdx.x; k < K; k += u64(gridDim.x) * blockDim.x) {
+        const u32 inst = find_seg(kseg, n, k);
+        const double f = sqrt(total[inst]);
+        if (f != 1.0) {
+            const double2 m = mag[k];
+            mag[k] = make_double2(m.x / f, m.y / f);
+        }
+    }
+}
+
+}  // namespace qd

@@ -272,17 +272,21 @@ constexpr int kGenThreads = 256;
 // One thread per work item; writes hash and magnitude of its children at
 // their candidate index (coalesced for each of the up-to-8 children) and
 // histograms the top b1 hash bits for the first partition pass.
-template <int KIND>
+// BATCH: parents belong to instances (iseg: ninst + 1 parent offsets); each
+// takes its instance's rule coefficients and adds instance * fold to its
+// children's hashes.
+template <int KIND, bool BATCH>
 __global__ __launch_bounds__(kGenThreads) void k_generate_fast(const PDesc *D, const double2 *pmag, const u64 *cb,
                                                               const u64 *wb, u64 N, const u64 *tab, const u64 *pbegin,
                                                               qd_rule rule, u64 W, u64 *H0, double2 *M0, u32 b1,
-                                                              u64 *hist1) {
+                                                              u64 *hist1, const qd_rule *rules, const u64 *iseg,
+                                                              u32 ninst, u64 fold) {
     __shared__ u32 hist[kHistBins];
     __shared__ u64 swb[kGenThreads + 1];
     __shared__ u64 pfirst;
     if (b1)
         for (int i = threadIdx.x; i < (1 << b1); i += blockDim.x) hist[i] = 0;
-    const QcgdCoef k = coef_of(rule);
+    QcgdCoef k = coef_of(rule);
     for (u64 w0 = u64(blockIdx.x) * kGenThreads; w0 < W; w0 += u64(gridDim.x) * kGenThreads) {
         __syncthreads();
         if (threadIdx.x == 0) pfirst = find_parent(wb, N, w0);
@@ -305,6 +309,17 @@ __global__ __launch_bounds__(kGenThreads) void k_generate_fast(const PDesc *D, c
         } else {
             p = find_parent(wb, N, w);
         }
+        u64 hadd = 0;
+        if (BATCH) {
+            u32 lo = 0, hi = ninst - 1;  // instance of parent p
+            while (lo < hi) {
+                const u32 mid = (lo + hi + 1) >> 1;
+                if (iseg[mid] <= p) lo = mid;
+                else hi = mid - 1;
+            }
+            k = coef_of(rules[lo]);
+            hadd = u64(lo) * fold;
+        }
         const PDesc pd = D[p];
         const u32 j0 = u32(w - wb[p]);
         const u32 t = min(u32(pd.k), kItemBits);
@@ -326,6 +341,7 @@ __global__ __launch_bounds__(kGenThreads) void k_generate_fast(const PDesc *D, c
                 flip_run<KIND == QD_RULE_QCGD_COIN>(x, pd, u, pd.k, k);
                 h = hash_combine(hash_combine(pd.nf, x.lh), x.rh);
             }
+            if (BATCH) h += hadd;
             const u64 c = cbase + (u64(u) << s);
             H0[c] = h;
             M0[c] = make_double2(x.mag.re, x.mag.im);

@@ -51,6 +51,7 @@
 #include "qd_k_select.hpp"
 #include "qd_k_finalize.hpp"
 #include "qd_k_adjoint.hpp"
+#include "qd_k_batch.hpp"
 
 using namespace qd;
 
@@ -139,11 +140,13 @@ struct qd_ctx {
     DBuf biglist, bigsoff, bigkey, bigcand, bigmag, bigseg;
     DBuf pdesc, wi, wb, ebuf2, cnt2, slist;
     DBuf adjk, adjv, adjg, adjd, adjp;
+    DBuf bseg, boseg, brules, bsinst, bcount, bsel, btrunc, blist, bcur, btile, bhist, bkseg, bchunk, bcsum, btot;
     std::vector<DBuf *> all() {
         return {&nc, &cb, &pre, &tab, &kept, &keys, &ebuf0, &ebuf1, &hist1, &off1, &cur1, &off2,
                 &nhist, &counters, &sel0, &sel1, &bitmap, &wprefix, &padded, &chunks, &scanblk,
                 &biglist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &ebuf2, &cnt2, &slist,
-                &adjk, &adjv, &adjg, &adjd, &adjp};
+                &adjk, &adjv, &adjg, &adjd, &adjp, &bseg, &boseg, &brules, &bsinst, &bcount, &bsel, &btrunc,
+                &blist, &bcur, &btile, &bhist, &bkseg, &bchunk, &bcsum, &btot};
     }
 };
 
@@ -371,13 +374,183 @@ static void dedup_layout(u64 C, const qd_tuning &t, u32 &b1, u32 &b2) {
     b2 = L - b1;
 }
 
-template <int KIND>
-static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_state *out, qd_symbolic *sy, u64 max_obj,
-                         const qd_params &prm, qd_step_stats &st) {
+// ===========================================================================
+// batched step helpers (qd_k_batch.hpp)
+// ===========================================================================
+// A batch: instance i owns parents [oseg[i], oseg[i+1]) of the input view;
+// the step fills its candidate offsets (cseg), unique and survivor counts,
+// output object offsets (kseg) and total probabilities.
+struct Batch {
+    u32 n = 0;
+    const u64 *d_oseg = nullptr;      // device, n + 1
+    const qd_rule *d_rules = nullptr; // device, n
+    const u64 *caps = nullptr;        // host, n
+    std::vector<u64> cseg, U, kseg;
+    std::vector<double> total;
+    void set_empty() {
+        if (cseg.size() != n + 1) cseg.assign(n + 1, 0);
+        if (U.size() != n) U.assign(n, 0);
+        kseg.assign(n + 1, 0);
+        total.assign(n, 0.0);
+    }
+};
+
+static int batch_cseg(qd_ctx *ctx, Batch *bt) {
+    QD_TRY(ctx->bseg.ensure((bt->n + 1) * sizeof(u64)));
+    hipLaunchKernelGGL(k_gather_u64, dim3(grid_for(bt->n + 1)), dim3(256), 0, ctx->stream, ctx->cb.as<u64>(),
+                       bt->d_oseg, bt->n + 1, ctx->bseg.as<u64>());
+    QD_HIP(hipGetLastError());
+    bt->cseg.resize(bt->n + 1);
+    return sync_read(ctx, ctx->bseg.p, (bt->n + 1) * sizeof(u64), bt->cseg.data());
+}
+
+static int upload(qd_ctx *ctx, DBuf &b, const void *src, size_t bytes) {
+    QD_TRY(b.ensure(std::max<size_t>(bytes, 8)));
+    if (bytes) QD_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    QD_HIP(hipStreamSynchronize(ctx->stream));  // the host source may go away
+    return QD_OK;
+}
+
+// Per-instance top-K over the nslots dedup output slots; marks survivors.
+static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const u64 *UR, const double2 *UM,
+                        u64 *bitmap, u64 *sbm) {
     hipStream_t s = ctx->stream;
-    const u64 N = in->n;
+    const u32 n = bt->n;
+    QD_TRY(ctx->bsinst.ensure(nslots * sizeof(u32)));
+    QD_TRY(ctx->bcount.ensure(n * sizeof(u64)));
+    u32 *sinst = ctx->bsinst.as<u32>();
+    QD_HIP(hipMemsetAsync(ctx->bcount.p, 0, n * sizeof(u64), s));
+    hipLaunchKernelGGL(k_seg_count, dim3(grid_for(nslots, 256, 8192)), dim3(256), 0, s, UR, nslots,
+                       ctx->bseg.as<u64>(), n, sinst, ctx->bcount.as<u64>());
+    QD_HIP(hipGetLastError());
+    bt->U.resize(n);
+    QD_TRY(sync_read(ctx, ctx->bcount.p, n * sizeof(u64), bt->U.data()));
+    std::vector<SegSel> sel(n);
+    std::vector<u32> trunc(n, 0);
+    std::vector<u64> loff(n, 0);
+    std::vector<SegTile> tiles;
+    u64 ntrunc = 0;
+    constexpr u64 kTileItems = 16384;
+    for (u32 i = 0; i < n; ++i) {
+        SegSel q{};
+        if (bt->U[i] > bt->caps[i]) {
+            q.r = bt->caps[i];
+            q.active = 1;
+            trunc[i] = 1;
+            loff[i] = ntrunc;
+            for (u64 b = 0; b < bt->U[i]; b += kTileItems)
+                tiles.push_back(SegTile{ntrunc + b, ntrunc + std::min(bt->U[i], b + kTileItems), i, 0});
+            ntrunc += bt->U[i];
+        } else {
+            q.keepall = 1;
+        }
+        sel[i] = q;
+    }
+    QD_TRY(upload(ctx, ctx->bsel, sel.data(), n * sizeof(SegSel)));
+    SegSel *dsel = ctx->bsel.as<SegSel>();
+    if (ntrunc) {
+        QD_TRY(upload(ctx, ctx->btrunc, trunc.data(), n * sizeof(u32)));
+        QD_TRY(upload(ctx, ctx->bcur, loff.data(), n * sizeof(u64)));
+        QD_TRY(upload(ctx, ctx->btile, tiles.data(), tiles.size() * sizeof(SegTile)));
+        QD_TRY(ctx->blist.ensure(ntrunc * sizeof(u64)));
+        QD_TRY(ctx->bhist.ensure(u64(n) * kHistBins * sizeof(u32)));
+        QD_TRY(ctx->counters.ensure(64 * sizeof(u64)));
+        u32 *more = reinterpret_cast<u32 *>(ctx->counters.as<u64>() + 10);
+        const unsigned sg = unsigned(std::min<u64>((nslots + kSegTile - 1) / kSegTile, 16384));
+        hipLaunchKernelGGL(k_seg_scatter, dim3(std::max(sg, 1u)), dim3(256), 0, s, sinst, nslots,
+                           ctx->btrunc.as<u32>(), n, ctx->bcur.as<u64>(), ctx->blist.as<u64>());
+        QD_HIP(hipMemsetAsync(ctx->bhist.p, 0, u64(n) * kHistBins * sizeof(u32), s));
+        const u32 nt = u32(tiles.size());
+        for (int round = 0; round < 12; ++round) {
+            QD_HIP(hipMemsetAsync(more, 0, sizeof(u32), s));
+            hipLaunchKernelGGL(k_seg_hist, dim3(std::min<u32>(nt, 65536)), dim3(256), 0, s, ctx->blist.as<u64>(),
+                               ctx->btile.as<SegTile>(), nt, UH, UM, (const SegSel *)dsel, ctx->bhist.as<u32>());
+            hipLaunchKernelGGL(k_seg_pick, dim3(n), dim3(256), 0, s, dsel, ctx->bhist.as<u32>(), n, more);
+            QD_HIP(hipGetLastError());
+            u32 m = 0;
+            QD_TRY(sync_read(ctx, more, sizeof(u32), &m));
+            if (!m) break;
+        }
+        QD_TRY(sync_read(ctx, dsel, n * sizeof(SegSel), sel.data()));
+        for (u32 i = 0; i < n; ++i)
+            if (sel[i].err || sel[i].active) return fail(QD_ERR_HIP, "batched select did not converge");
+    }
+    hipLaunchKernelGGL(k_seg_mark, dim3(grid_for(nslots, 256, 16384)), dim3(256), 0, s, sinst, UH, UR, UM, nslots,
+                       (const SegSel *)dsel, bitmap, sbm);
+    QD_HIP(hipGetLastError());
+    return QD_OK;
+}
+
+// Output object offsets of the instances: survivor ranks at their first
+// candidates.
+static int batch_kseg(qd_ctx *ctx, Batch *bt, const u64 *bitmap, u64 W) {
+    QD_TRY(ctx->bkseg.ensure((bt->n + 1) * sizeof(u64)));
+    hipLaunchKernelGGL(k_rank_at, dim3(grid_for(bt->n + 1)), dim3(256), 0, ctx->stream, bitmap,
+                       ctx->wprefix.as<u64>(), ctx->bseg.as<u64>(), bt->n + 1, W, ctx->bkseg.as<u64>());
+    QD_HIP(hipGetLastError());
+    bt->kseg.resize(bt->n + 1);
+    return sync_read(ctx, ctx->bkseg.p, (bt->n + 1) * sizeof(u64), bt->kseg.data());
+}
+
+// Per-instance canonical sums of |m|^2 and scaling, in the plain step's
+// order relative to each instance's first survivor.
+static int batch_normalize(qd_ctx *ctx, Batch *bt, qd_state *out) {
+    hipStream_t s = ctx->stream;
+    const u32 n = bt->n;
+    std::vector<u64> chunk0(n + 1, 0);
+    bool big = false;
+    for (u32 i = 0; i < n; ++i) {
+        const u64 k = bt->kseg[i + 1] - bt->kseg[i];
+        const u64 c = (k + kSumChunk - 1) / kSumChunk;
+        if (c > u64(kSumChunk)) big = true;
+        chunk0[i + 1] = chunk0[i] + c;
+    }
+    const u64 M = chunk0[n];
+    QD_TRY(upload(ctx, ctx->bchunk, chunk0.data(), (n + 1) * sizeof(u64)));
+    QD_TRY(ctx->bcsum.ensure(std::max<u64>(M, 1) * sizeof(double)));
+    QD_TRY(ctx->btot.ensure(n * sizeof(double)));
+    prof_begin(ctx, QD_PHASE_NORMALIZE);
+    if (M) {
+        hipLaunchKernelGGL(k_seg_sum_chunks, dim3(unsigned(M)), dim3(kSumThreads), 0, s, out->mag.as<double2>(),
+                           ctx->bkseg.as<u64>(), ctx->bchunk.as<u64>(), n, ctx->bcsum.as<double>());
+        hipLaunchKernelGGL(k_seg_sum_final, dim3(n), dim3(kSumThreads), 0, s, ctx->bcsum.as<double>(),
+                           ctx->bchunk.as<u64>(), n, ctx->btot.as<double>());
+        QD_HIP(hipGetLastError());
+    } else {
+        QD_HIP(hipMemsetAsync(ctx->btot.p, 0, n * sizeof(double), s));
+    }
+    prof_end(ctx, QD_PHASE_NORMALIZE);
+    bt->total.resize(n);
+    QD_TRY(sync_read(ctx, ctx->btot.p, n * sizeof(double), bt->total.data()));
+    if (big) {  // instances beyond kSumChunk^2 survivors: the plain multi-level sum
+        for (u32 i = 0; i < n; ++i) {
+            const u64 k = bt->kseg[i + 1] - bt->kseg[i];
+            if ((k + kSumChunk - 1) / kSumChunk > u64(kSumChunk))
+                QD_TRY(canonical_sum(ctx, out->mag.as<double2>() + bt->kseg[i], nullptr, k, &bt->total[i]));
+        }
+        QD_TRY(upload(ctx, ctx->btot, bt->total.data(), n * sizeof(double)));
+    }
+    double sum = 0;
+    for (u32 i = 0; i < n; ++i) sum += bt->total[i];
+    out->total_proba = sum;
+    if (out->n) {
+        prof_begin(ctx, QD_PHASE_NORMALIZE);
+        hipLaunchKernelGGL(k_seg_scale, dim3(grid_for(out->n)), dim3(256), 0, s, out->mag.as<double2>(), out->n,
+                           ctx->bkseg.as<u64>(), n, ctx->btot.as<double>());
+        QD_HIP(hipGetLastError());
+        prof_end(ctx, QD_PHASE_NORMALIZE);
+    }
+    return QD_OK;
+}
+
+// One step over the parents P (of a state whose objects span in_bytes).  With
+// BATCH, the parents are the instances of bt, each with its own rule and cap.
+template <int KIND, bool BATCH>
+static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule &rule, Batch *bt, qd_state *out,
+                     qd_symbolic *sy, u64 max_obj, const qd_params &prm, qd_step_stats &st) {
+    hipStream_t s = ctx->stream;
+    const u64 N = P.n;
     st.num_parent = N;
-    Parents P{in->objects.as<u8>(), in->begin.as<u64>(), in->size.as<u32>(), in->mag.as<double2>(), N};
 
     auto finish_empty = [&]() -> int {
         QD_TRY(state_reserve(out, 0, 0));
@@ -390,6 +563,7 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
             sy->num_object = st.num_candidates;
             sy->num_object_after_interferences = st.num_unique;
         }
+        if (BATCH) bt->set_empty();
         QD_HIP(hipStreamSynchronize(s));
         return QD_OK;
     };
@@ -399,7 +573,7 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
     // ties by ascending index, through the 128-bit select on (|m|^2, ~index).
     const u64 *keep = nullptr;
     u64 nkept = N;
-    if (max_obj != 0 && max_obj < N) {
+    if (!BATCH && max_obj != 0 && max_obj < N) {
         if (!prm.simple_truncation)
             return fail(QD_ERR_UNSUPPORTED, "probabilistic truncation is not implemented; set simple_truncation");
         QD_TRY(ctx->sel0.ensure(N * sizeof(SelItem)));
@@ -423,7 +597,7 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
     QD_TRY(ctx->cb.ensure((N + 1) * sizeof(u64)));
     QD_TRY(ctx->pre.ensure(N * sizeof(u64)));
     QD_TRY(ctx->counters.ensure(64 * sizeof(u64)));
-    if (KIND == QD_RULE_QCGD_SPLIT_MERGE) QD_TRY(ctx->tab.ensure((in->bytes / 2 + 16) * sizeof(u64)));
+    if (KIND == QD_RULE_QCGD_SPLIT_MERGE) QD_TRY(ctx->tab.ensure((in_bytes / 2 + 16) * sizeof(u64)));
     u64 *counters = ctx->counters.as<u64>();
     // counters: [0] unique count, [1] select count A, [2] select count B,
     // [3] low: max child size, [4] low: error word, [5] oversized buckets
@@ -431,6 +605,7 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
     u32 *maxsz = reinterpret_cast<u32 *>(counters + 3);
     u32 *errw = reinterpret_cast<u32 *>(counters + 4);
     bool fast = KIND != QD_RULE_HADAMARD && !ctx->tune.generic_path;
+    if (BATCH && !fast) return fail(QD_ERR_UNSUPPORTED, "the batched step needs the QCGD fast path");
     u64 *hb = ctx->host;
     u64 nitems = 0;  // work items of the fast generate
     if (fast) {
@@ -453,6 +628,8 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
         QD_HIP(hipStreamSynchronize(s));
         prof_collect(ctx);
         nitems = hb[3];
+        if (BATCH && (u32(hb[2]) & ERR_WIDE))
+            return fail(QD_ERR_UNSUPPORTED, "the batched step handles graphs of at most 64 nodes");
         if (u32(hb[2]) & ERR_WIDE) {  // a graph wider than 64 nodes: generic kernels
             fast = false;
             QD_HIP(hipMemsetAsync(counters + 3, 0, 2 * sizeof(u64), s));
@@ -475,6 +652,7 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
     const u64 C = hb[0];
     QD_TRY(check_err_word(u32(hb[2])));
     st.num_candidates = C;
+    if (BATCH) QD_TRY(batch_cseg(ctx, bt));
     if (C == 0) return finish_empty();
 
     // ---- 3 generate: hash and magnitude at the candidate index
@@ -497,9 +675,16 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
     if (fast) {
         const unsigned gg =
             unsigned(std::min<u64>((nitems + kGenThreads - 1) / kGenThreads, ctx->tune.generate_grid));
-        hipLaunchKernelGGL(k_generate_fast<KIND>, dim3(std::max(gg, 1u)), dim3(kGenThreads), 0, s, ctx->pdesc.as<PDesc>(),
-                           P.mag, ctx->cb.as<u64>(), ctx->wb.as<u64>(), N, ctx->tab.as<u64>(), P.begin, rule, nitems, H0, M0,
-                           b1, hist1);
+        if (BATCH)
+            hipLaunchKernelGGL((k_generate_fast<KIND, true>), dim3(std::max(gg, 1u)), dim3(kGenThreads), 0, s,
+                               ctx->pdesc.as<PDesc>(), P.mag, ctx->cb.as<u64>(), ctx->wb.as<u64>(), N,
+                               ctx->tab.as<u64>(), P.begin, rule, nitems, H0, M0, b1, hist1, bt->d_rules, bt->d_oseg,
+                               bt->n, kFold);
+        else
+            hipLaunchKernelGGL((k_generate_fast<KIND, false>), dim3(std::max(gg, 1u)), dim3(kGenThreads), 0, s,
+                               ctx->pdesc.as<PDesc>(), P.mag, ctx->cb.as<u64>(), ctx->wb.as<u64>(), N,
+                               ctx->tab.as<u64>(), P.begin, rule, nitems, H0, M0, b1, hist1,
+                               (const qd_rule *)nullptr, (const u64 *)nullptr, 0u, u64(0));
     } else {
         const unsigned gb = ctx->tune.generate_block;
         const unsigned gg = grid_for(C, gb, ctx->tune.generate_grid);
@@ -590,7 +775,7 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
     // memory-driven cap (max_num_object == 0, quids.hpp:459-486, 510-536):
     // keep as many survivors as the free device memory holds for the next
     // state, at the largest child size seen, with a 20% safety margin.
-    if (max_obj == 0) {
+    if (!BATCH && max_obj == 0) {
         size_t freeb = 0, totalb = 0;
         QD_HIP(hipMemGetInfo(&freeb, &totalb));
         const u64 per = u64(u32(hb[3])) + 8 + sizeof(u64) + sizeof(u32) + sizeof(double2);
@@ -606,7 +791,9 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
     u64 *sbm = ctx->slist.as<u64>();
     prof_begin(ctx, QD_PHASE_SELECT);
     QD_HIP(hipMemsetAsync(bitmap, 0, W * sizeof(u64), s));
-    if (U <= max_obj) {
+    if (BATCH) {
+        QD_TRY(batch_select(ctx, bt, C, UH, UR, UM, bitmap, sbm));
+    } else if (U <= max_obj) {
         if (U)
             hipLaunchKernelGGL(k_mark_all, dim3(grid_for(C, 256, 16384)), dim3(256), 0, s, UR, C, bitmap, sbm);
         QD_HIP(hipGetLastError());
@@ -651,6 +838,7 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
     const u64 K = hb[0];
     st.num_survivors = K;
     if (K == 0) return finish_empty();
+    if (BATCH) QD_TRY(batch_kseg(ctx, bt, bitmap, W));
 
     // ---- 8 finalize
     QD_TRY(out->begin.ensure((K + 1) * sizeof(u64)));
@@ -683,7 +871,8 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
 
     prof_end(ctx, QD_PHASE_FINALIZE);
     // ---- 9 normalize
-    QD_TRY(normalize_state(ctx, out));
+    if (BATCH) QD_TRY(batch_normalize(ctx, bt, out));
+    else QD_TRY(normalize_state(ctx, out));
     prof_collect(ctx);
     st.total_proba = out->total_proba;
     if (sy) {
@@ -691,6 +880,13 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
         sy->num_object_after_interferences = U;
     }
     return QD_OK;
+}
+
+template <int KIND>
+static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_state *out, qd_symbolic *sy, u64 max_obj,
+                         const qd_params &prm, qd_step_stats &st) {
+    const Parents P{in->objects.as<u8>(), in->begin.as<u64>(), in->size.as<u32>(), in->mag.as<double2>(), in->n};
+    return step_impl<KIND, false>(ctx, P, in->bytes, rule, nullptr, out, sy, max_obj, prm, st);
 }
 
 // ===========================================================================
@@ -1132,6 +1328,83 @@ int qd_normalize(qd_ctx *c, qd_state *s, double *total_proba) {
     QD_HIP(hipSetDevice(c->device));
     QD_TRY(normalize_state(c, s));
     if (total_proba) *total_proba = s->total_proba;
+    return QD_OK;
+}
+
+int qd_simulate_batched(qd_ctx *c, qd_state *in, const uint64_t *in_seg, uint32_t num_instances,
+                        const qd_rule *rules, qd_state *out, uint64_t *out_seg, const uint64_t *max_num_object,
+                        const qd_params *params, qd_step_stats *stats) {
+    if (!c || !in || !in_seg || !rules || !out || !out_seg || !max_num_object)
+        return fail(QD_ERR_INVALID, "null argument");
+    if (in == out) return fail(QD_ERR_INVALID, "input and output states must differ");
+    if (in->ctx != c || out->ctx != c) return fail(QD_ERR_INVALID, "state belongs to another context");
+    const u32 n = num_instances;
+    if (n == 0 || n > kMaxInst) return fail(QD_ERR_INVALID, "num_instances must be in [1, 4096]");
+    if (in_seg[0] != 0 || in_seg[n] != in->n) return fail(QD_ERR_INVALID, "in_seg must run from 0 to the state size");
+    const int kind = rules[0].kind;
+    if (kind != QD_RULE_QCGD_ERASE_CREATE && kind != QD_RULE_QCGD_COIN && kind != QD_RULE_QCGD_SPLIT_MERGE)
+        return fail(QD_ERR_UNSUPPORTED, "the batched step runs the QCGD rules");
+    for (u32 i = 0; i < n; ++i) {
+        if (in_seg[i + 1] < in_seg[i]) return fail(QD_ERR_INVALID, "in_seg must be ascending");
+        if (rules[i].kind != kind) return fail(QD_ERR_INVALID, "all instances of a batch run the same rule kind");
+        if (max_num_object[i] == 0)
+            return fail(QD_ERR_UNSUPPORTED, "memory-driven truncation (max_num_object 0) is not batched");
+        if (in_seg[i + 1] - in_seg[i] > max_num_object[i])
+            return fail(QD_ERR_UNSUPPORTED, "batched instances cannot be pre-truncated (more parents than max)");
+    }
+    qd_params prm{1e-30, 8, 1};
+    if (params) prm = *params;
+    if (!prm.simple_truncation)
+        return fail(QD_ERR_UNSUPPORTED, "probabilistic truncation is not implemented; set simple_truncation");
+    QD_HIP(hipSetDevice(c->device));
+    QD_TRY(upload(c, c->boseg, in_seg, (n + 1) * sizeof(u64)));
+    QD_TRY(upload(c, c->brules, rules, n * sizeof(qd_rule)));
+    Batch bt;
+    bt.n = n;
+    bt.d_oseg = c->boseg.as<u64>();
+    bt.d_rules = c->brules.as<qd_rule>();
+    bt.caps = max_num_object;
+    qd_step_stats st;
+    std::memset(&st, 0, sizeof st);
+    const Parents P{in->objects.as<u8>(), in->begin.as<u64>(), in->size.as<u32>(), in->mag.as<double2>(), in->n};
+    int rc;
+    switch (kind) {
+    case QD_RULE_QCGD_ERASE_CREATE:
+        rc = step_impl<QD_RULE_QCGD_ERASE_CREATE, true>(c, P, in->bytes, rules[0], &bt, out, nullptr, UINT64_MAX, prm, st);
+        break;
+    case QD_RULE_QCGD_COIN:
+        rc = step_impl<QD_RULE_QCGD_COIN, true>(c, P, in->bytes, rules[0], &bt, out, nullptr, UINT64_MAX, prm, st);
+        break;
+    default:
+        rc = step_impl<QD_RULE_QCGD_SPLIT_MERGE, true>(c, P, in->bytes, rules[0], &bt, out, nullptr, UINT64_MAX, prm, st);
+        break;
+    }
+    if (rc) return rc;
+    for (u32 i = 0; i <= n; ++i) out_seg[i] = bt.kseg[i];
+    if (stats) {
+        // output bytes per instance: object_begin at the instance boundaries
+        std::vector<u64> ob(n + 1, 0);
+        if (out->n) {
+            QD_TRY(c->bcount.ensure((n + 1) * sizeof(u64)));
+            QD_TRY(upload(c, c->bchunk, bt.kseg.data(), (n + 1) * sizeof(u64)));
+            hipLaunchKernelGGL(k_gather_u64, dim3(grid_for(n + 1)), dim3(256), 0, c->stream, out->begin.as<u64>(),
+                               c->bchunk.as<u64>(), n + 1, c->bcount.as<u64>());
+            QD_HIP(hipGetLastError());
+            QD_TRY(sync_read(c, c->bcount.p, (n + 1) * sizeof(u64), ob.data()));
+        }
+        for (u32 i = 0; i < n; ++i) {
+            qd_step_stats &x = stats[i];
+            std::memset(&x, 0, sizeof x);
+            x.num_parent = x.num_parent_kept = in_seg[i + 1] - in_seg[i];
+            x.num_candidates = bt.cseg[i + 1] - bt.cseg[i];
+            x.num_unique = bt.U[i];
+            x.num_survivors = bt.kseg[i + 1] - bt.kseg[i];
+            x.output_bytes = ob[i + 1] - ob[i];
+            x.total_proba = bt.total[i];
+            x.dedup_levels = st.dedup_levels;
+            x.reserved = st.reserved;
+        }
+    }
     return QD_OK;
 }
 
