@@ -203,6 +203,96 @@ def adjoint_bench(nodes=10, seed=42, n_iter=6, log2_cap=18, checkpoint_every=2, 
     return out
 
 
+def _dev_free_gb():
+    import torch
+    return torch.cuda.mem_get_info()[0] / 1e9
+
+
+def batched_bench(n_inst=256, n_iter=8, log2_cap=18, seed=4242, lo_nodes=6, hi_nodes=16, sequential=True):
+    """QCGD batch: n_inst instances with node counts uniform in
+    [lo_nodes, hi_nodes], particle bits from per-instance seeds and (theta,
+    phi) uniform in [0, pi/2] x [0, pi] per instance (the same angles drive
+    split_merge and erase_create of an instance), n_iter iterations of
+    step -> split_merge -> erase_create with a per-instance cap 2^log2_cap.
+    Times the batched steps and, when `sequential`, the same work as n_inst
+    separate plain simulate() calls per rule; checks that every instance's
+    counters and total probability agree."""
+    import numpy as np
+    import quids_amd as q
+    from quids_amd.batch import apply_modifier, batch_t, simulate_batched
+    from quids_amd.rules import qcgd
+    from quids_amd.state import HostState
+    rng = np.random.default_rng(seed)
+    nodes = rng.integers(lo_nodes, hi_nodes + 1, n_inst)
+    seeds = rng.integers(0, 2 ** 31 - 1, n_inst)
+    theta = rng.uniform(0, math.pi / 2, n_inst)
+    phi = rng.uniform(0, math.pi, n_inst)
+    states = [HostState.from_objects([qcgd.seeded_graph(int(nodes[i]), int(seeds[i]))]) for i in range(n_inst)]
+    cap = 1 << log2_cap
+    ctx = q.default_context()
+
+    def rules(which):
+        cls = qcgd.split_merge if which == 0 else qcgd.erase_create
+        return [cls(float(theta[i]), float(phi[i])) for i in range(n_inst)]
+
+    # batched
+    a, b = batch_t.from_states(states), batch_t()
+    bstats = []
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for it in range(n_iter):
+        apply_modifier(a, qcgd.step)
+        for which in (0, 1):
+            bstats.append(simulate_batched(a, rules(which), b, cap))
+            a, b = b, a
+            log(f"batched iteration {it + 1} rule {which}: C={sum(x['num_candidates'] for x in bstats[-1])} "
+                f"objects={int(a.seg[-1])} chunked={bool(bstats[-1][0]['reserved'] & 2)} "
+                f"scratch={ctx.scratch_bytes() / 1e9:.1f} GB free={_dev_free_gb():.1f} GB")
+    ctx.synchronize()
+    t_batch = time.perf_counter() - t0
+    C = sum(st["num_candidates"] for step in bstats for st in step)
+    out = {"workload": f"batched QCGD, {n_inst} instances, nodes uniform in {lo_nodes}..{hi_nodes}, "
+                       f"theta in [0, pi/2], phi in [0, pi] per instance (same angles for both rules), "
+                       f"{n_iter} iterations, per-instance cap 2^{log2_cap}, seed {seed}",
+           "candidates": C, "batched_seconds": t_batch, "batched_children_per_s": C / t_batch}
+    a.close()
+    b.close()
+    if sequential:
+        its = []
+        for hs in states:
+            it = q.it_t()
+            it.upload(hs)
+            its.append(it)
+        sstats = []
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for it in range(n_iter):
+            for i in range(n_inst):
+                q.simulate(its[i], qcgd.step)
+            for which in (0, 1):
+                rs = rules(which)
+                row = []
+                for i in range(n_inst):
+                    nxt = q.it_t()
+                    row.append(q.simulate(its[i], rs[i], nxt, None, cap))
+                    its[i].close()
+                    its[i] = nxt
+                sstats.append(row)
+            log(f"sequential iteration {it + 1}")
+        ctx.synchronize()
+        t_seq = time.perf_counter() - t0
+        mismatch = 0
+        for bs, ss in zip(bstats, sstats):
+            for x, y in zip(bs, ss):
+                if any(x[k] != y[k] for k in ("num_candidates", "num_unique", "num_survivors", "total_proba")):
+                    mismatch += 1
+        out.update({"sequential_seconds": t_seq, "sequential_children_per_s": C / t_seq,
+                    "speedup_vs_sequential": t_seq / t_batch, "instance_steps_mismatched": mismatch})
+        for it in its:
+            it.close()
+    return out
+
+
 def workload_key(args):
     return f"qcgd nodes={args.nodes} seed={args.seed} cap=2^{args.log2_cap} warmup={args.warmup} steps={args.steps}"
 
@@ -223,8 +313,12 @@ def main():
                     help="summarize rocprofv3 output folders of this script's runs and exit")
     ap.add_argument("--calib-json", help="the --copy-probe JSON line of the calibration run")
     ap.add_argument("--traffic-out", default=TRAFFIC_FILE)
-    ap.add_argument("--workload", choices=["plain", "adjoint"], default="plain",
-                    help="adjoint: only the QCGD adjoint workload (with --gd-steps of gradient descent)")
+    ap.add_argument("--workload", choices=["plain", "adjoint", "batched"], default="plain",
+                    help="adjoint: the QCGD adjoint workload (with --gd-steps of gradient descent); "
+                         "batched: the 256-instance batch against sequential plain calls")
+    ap.add_argument("--batch-log2-cap", type=int, default=18)
+    ap.add_argument("--batch-iters", type=int, default=8)
+    ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--gd-steps", type=int, default=50)
     ap.add_argument("--no-extras", action="store_true", help="skip the adjoint summary of the plain line")
     args = ap.parse_args()
@@ -245,6 +339,14 @@ def main():
         res = adjoint_bench(gd_steps=args.gd_steps)
         res.update({"metric": "adjoint gradient cost over one forward pass", "value": res["gradient_over_forward"],
                     "unit": "x", "higher_is_better": False, "n_gpus": 1, "dtype": "f64", "data": "synthetic"})
+        print(json.dumps(res), flush=True)
+        return
+
+    if args.workload == "batched":
+        res = batched_bench(n_inst=args.batch_instances, n_iter=args.batch_iters, log2_cap=args.batch_log2_cap)
+        res.update({"metric": "batched children/s over sequential plain simulate() calls",
+                    "value": res.get("speedup_vs_sequential"), "unit": "x", "higher_is_better": True, "n_gpus": 1,
+                    "dtype": "f64", "data": "synthetic"})
         print(json.dumps(res), flush=True)
         return
 

@@ -93,7 +93,8 @@ typedef struct qd_step_stats {
     uint64_t output_bytes;      /* padded object bytes of the next state */
     double total_proba;         /* next.total_proba (before normalisation) */
     uint32_t dedup_levels;      /* radix partition passes used by the dedup */
-    uint32_t reserved;          /* bit 0: the second pass fell back to exact offsets */
+    uint32_t reserved;          /* bit 0: the second pass fell back to exact offsets;
+                                   bit 1: a batched step ran in several chunks */
 } qd_step_stats;
 
 typedef struct qd_ctx qd_ctx;
@@ -145,7 +146,10 @@ typedef struct qd_tuning {
     uint32_t generate_grid;       /* max generate blocks (default 2048) */
     uint32_t dedup_cap;           /* LDS bucket capacity of the dedup (2048; larger buckets take the global-memory path) */
     uint32_t generic_path;        /* 1: byte-level generate kernels for every rule (default 0: QCGD fast path) */
-    uint32_t reserved[2];
+    uint32_t chunk_log2;          /* at most 2^k candidates per pass, 10..40 (default 0: as many as
+                                     free device memory holds): a batched step splits its instances
+                                     into chunks, and a step beyond it runs in hash-range passes */
+    uint32_t reserved;
 } qd_tuning;
 int qd_ctx_get_tuning(qd_ctx *ctx, qd_tuning *tuning);
 int qd_ctx_set_tuning(qd_ctx *ctx, const qd_tuning *tuning);

@@ -57,6 +57,7 @@ class Context:
     def set_tuning(self, **kw):
         """Sets launch / layout parameters (fields of qd_tuning); others keep their value."""
         t = _abi.QdTuning()
+        check(lib().qd_ctx_get_tuning(self.handle, ctypes.byref(t)), "qd_ctx_get_tuning")
         for k, v in kw.items():
             if not hasattr(t, k) or k == "reserved":
                 raise KeyError(k)

@@ -80,7 +80,10 @@ class QdStepStats(ctypes.Structure):
     ]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        """Counters by field name; 'reserved' holds flags (bit 0: the second
+        partition pass fell back to exact offsets; bit 1: a batched step ran
+        in several chunks)."""
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class QdTuning(ctypes.Structure):
@@ -91,7 +94,8 @@ class QdTuning(ctypes.Structure):
         ("generate_grid", ctypes.c_uint32),
         ("dedup_cap", ctypes.c_uint32),
         ("generic_path", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 2),
+        ("chunk_log2", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
     def as_dict(self):
@@ -167,6 +171,15 @@ def lib():
             raise RuntimeError(
                 f"quids_amd: HIP library {LIB_PATH} is missing; build it with "
                 "`make -C quids_amd/csrc` (or __graft_entry__.build())")
+        # One HIP runtime per process: torch's HIP library asks for the
+        # runtime by its unversioned name, so if this library (linked to
+        # libamdhip64.so.7) loaded first, torch would load a second runtime
+        # that sees no device.  Loading torch first makes this library reuse
+        # torch's runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         handle = ctypes.CDLL(LIB_PATH)
         for name, args in SIGNATURES.items():
             fn = getattr(handle, name)

@@ -355,4 +355,108 @@ __global__ __launch_bounds__(kGenThreads) void k_generate_fast(const PDesc *D, c
     }
 }
 
+// ---------------------------------------------------------------------------
+// Hash-range passes, for steps whose candidates exceed the scratch memory.
+// COUNT: histogram of the top 12 bits of every child hash into hist (4096
+// bins).  Otherwise: only the children whose top 12 hash bits lie in
+// [rlo, rhi) are kept, compacted (a block reserves one range per iteration)
+// as key = hash rotated left by 12 (so the partition and the dedup, which
+// bucket on the top key bits, see well spread bits), magnitude and candidate
+// index, and histogrammed on the top b1 key bits for the partition.
+// ---------------------------------------------------------------------------
+__device__ inline u64 rotl12(u64 h) { return (h << 12) | (h >> 52); }
+__device__ inline u64 rotr12(u64 k) { return (k >> 12) | (k << 52); }
+
+template <int KIND, bool COUNT>
+__global__ __launch_bounds__(kGenThreads) void k_generate_range(const PDesc *D, const double2 *pmag, const u64 *cb,
+                                                               const u64 *wb, u64 N, const u64 *tab, const u64 *pbegin,
+                                                               qd_rule rule, u64 W, u32 rlo, u32 rhi, u64 *HK,
+                                                               double2 *MK, u64 *CK, u64 *rcount, u32 b1, u64 *hist) {
+    __shared__ u32 lh[kHistBins];
+    __shared__ u64 swb[kGenThreads + 1];
+    __shared__ u64 pfirst, wbase;
+    __shared__ u64 tmp[kGenThreads / kWave + 1];
+    const u32 hb = COUNT ? 12 : b1;
+    if (hb)
+        for (int i = threadIdx.x; i < (1 << hb); i += blockDim.x) lh[i] = 0;
+    const QcgdCoef k = coef_of(rule);
+    for (u64 w0 = u64(blockIdx.x) * kGenThreads; w0 < W; w0 += u64(gridDim.x) * kGenThreads) {
+        __syncthreads();
+        if (threadIdx.x == 0) pfirst = find_parent(wb, N, w0);
+        __syncthreads();
+        const u64 pf = pfirst;
+        for (u32 q = threadIdx.x; q <= kGenThreads; q += blockDim.x) swb[q] = pf + q <= N ? wb[pf + q] : ~0ull;
+        __syncthreads();
+        const u64 w = w0 + threadIdx.x;
+        u64 hs[1 << kItemBits];
+        double2 ms[1 << kItemBits];
+        u64 cs[1 << kItemBits];
+        u32 nk = 0;
+        if (w < W) {
+            u64 p;
+            if (swb[kGenThreads] > w) {
+                u32 lo = 0, hi = kGenThreads;
+                while (lo + 1 < hi) {
+                    const u32 mid = (lo + hi) >> 1;
+                    if (swb[mid] <= w) lo = mid;
+                    else hi = mid;
+                }
+                p = pf + lo;
+            } else {
+                p = find_parent(wb, N, w);
+            }
+            const PDesc pd = D[p];
+            const u32 j0 = u32(w - wb[p]);
+            const u32 t = min(u32(pd.k), kItemBits);
+            const u32 s = pd.k - t;
+            const double2 m0 = pmag[p];
+            Walk pre{};
+            pre.mag = cplx{m0.x, m0.y};
+            const u64 *ptab = tab + pbegin[p] / 2;
+            if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(pre, pd, ptab, j0, s, k);
+            else flip_run<KIND == QD_RULE_QCGD_COIN>(pre, pd, j0, s, k);
+            const u64 cbase = cb[p] + j0;
+            for (u32 u = 0; u < (1u << t); ++u) {
+                Walk x = pre;
+                u64 h;
+                if (KIND == QD_RULE_QCGD_SPLIT_MERGE) {
+                    sm_run(x, pd, ptab, u, pd.k, k);
+                    h = sm_finish(x, ptab);
+                } else {
+                    flip_run<KIND == QD_RULE_QCGD_COIN>(x, pd, u, pd.k, k);
+                    h = hash_combine(hash_combine(pd.nf, x.lh), x.rh);
+                }
+                const u32 top = u32(h >> 52);
+                if (COUNT) {
+                    atomicAdd(&lh[top], 1u);
+                } else if (top >= rlo && top < rhi) {
+                    hs[nk] = rotl12(h);
+                    ms[nk] = make_double2(x.mag.re, x.mag.im);
+                    cs[nk] = cbase + (u64(u) << s);
+                    ++nk;
+                }
+            }
+        }
+        if (!COUNT) {
+            u64 total;
+            const u64 off = block_excl_scan(u64(nk), tmp, total);
+            if (threadIdx.x == 0 && total)
+                wbase = atomicAdd(reinterpret_cast<unsigned long long *>(rcount), (unsigned long long)total);
+            __syncthreads();
+            const u64 at = wbase + off;
+            for (u32 q = 0; q < nk; ++q) {
+                HK[at + q] = hs[q];
+                MK[at + q] = ms[q];
+                CK[at + q] = cs[q];
+                if (hb) atomicAdd(&lh[hs[q] >> (64 - hb)], 1u);
+            }
+        }
+    }
+    if (hb) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < (1 << hb); i += blockDim.x)
+            if (lh[i]) atomicAdd(reinterpret_cast<unsigned long long *>(hist + i), (unsigned long long)lh[i]);
+    }
+}
+
 }  // namespace qd

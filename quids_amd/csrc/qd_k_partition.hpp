@@ -14,8 +14,10 @@ constexpr int kTile1 = 4096;  // entries per workgroup in the first pass
 
 // First pass: from candidate order (index = candidate, hash and magnitude
 // as written by generate) to 2^b1 buckets at exact offsets.
+// CI, when not null, holds the candidate index of every input position
+// (compacted inputs of the hash-range passes).
 __global__ __launch_bounds__(256) void k_scatter1(const u64 *H0, const double2 *M0, u64 n, u32 b1, u64 *cursor,
-                                                  Entry *out) {
+                                                  Entry *out, const u64 *CI) {
     __shared__ u32 cnt[kHistBins];
     __shared__ u64 gbase[kHistBins];
     const int nbin = 1 << b1;
@@ -44,7 +46,7 @@ __global__ __launch_bounds__(256) void k_scatter1(const u64 *H0, const double2 *
             const double2 m = M0[i];
             Entry e;
             e.hash = H0[i];
-            e.cand = i;
+            e.cand = CI ? CI[i] : i;
             e.re = m.x;
             e.im = m.y;
             out[gbase[dr[k] >> 16] + (dr[k] & 0xFFFF)] = e;

@@ -52,6 +52,7 @@
 #include "qd_k_finalize.hpp"
 #include "qd_k_adjoint.hpp"
 #include "qd_k_batch.hpp"
+#include "qd_k_range.hpp"
 
 using namespace qd;
 
@@ -101,7 +102,10 @@ struct DBuf {
             if (e != hipSuccess) {
                 (void)hipGetLastError();
                 p = nullptr;
-                return fail(QD_ERR_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+                size_t freeb = 0, totalb = 0;
+                (void)hipMemGetInfo(&freeb, &totalb);
+                return fail(QD_ERR_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed (device free " +
+                                            std::to_string(freeb) + " of " + std::to_string(totalb) + " bytes)");
             }
             want = std::max<size_t>(bytes, 256);
         }
@@ -119,6 +123,8 @@ struct DBuf {
     }
 };
 
+constexpr u64 kHostN = 8191;  // slot of the pinned area that stages a value for the device
+
 struct qd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -127,7 +133,7 @@ struct qd_ctx {
     hipEvent_t ev[2 * QD_NUM_PHASES] = {};
     bool pending[QD_NUM_PHASES] = {};
     double phase_ms[QD_NUM_PHASES] = {};
-    qd_tuning tune = {768, 12, 256, 2048, 2048, 0, {0, 0}};
+    qd_tuning tune = {768, 12, 256, 2048, 2048, 0, 0, 0};
     u64 phase_calls[QD_NUM_PHASES] = {};
     u64 *host = nullptr;  // pinned readback area, 8192 u64
     // scratch of the step
@@ -140,13 +146,14 @@ struct qd_ctx {
     DBuf biglist, bigsoff, bigkey, bigcand, bigmag, bigseg;
     DBuf pdesc, wi, wb, ebuf2, cnt2, slist;
     DBuf adjk, adjv, adjg, adjd, adjp;
+    DBuf pool0, pool1, pwpre;
     DBuf bseg, boseg, brules, bsinst, bcount, bsel, btrunc, blist, bcur, btile, bhist, bkseg, bchunk, bcsum, btot;
     std::vector<DBuf *> all() {
         return {&nc, &cb, &pre, &tab, &kept, &keys, &ebuf0, &ebuf1, &hist1, &off1, &cur1, &off2,
                 &nhist, &counters, &sel0, &sel1, &bitmap, &wprefix, &padded, &chunks, &scanblk,
                 &biglist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &ebuf2, &cnt2, &slist,
                 &adjk, &adjv, &adjg, &adjd, &adjp, &bseg, &boseg, &brules, &bsinst, &bcount, &bsel, &btrunc,
-                &blist, &bcur, &btile, &bhist, &bkseg, &bchunk, &bcsum, &btot};
+                &blist, &bcur, &btile, &bhist, &bkseg, &bchunk, &bcsum, &btot, &pool0, &pool1, &pwpre};
     }
 };
 
@@ -325,6 +332,45 @@ static int select_loop(qd_ctx *ctx, SelItem *S0, SelItem *S1, u64 ns, u64 r, u32
 }
 
 
+// Marks the top r slots of nslots (empty slots have UR == kNoRep) by the
+// key (|m|^2 bits desc, hash asc): their bits in sbm (whole words written)
+// and their representatives in bitmap.  h is the histogram of the top 12
+// |m|^2 bits over the slots (it is overwritten).
+static int select_topk(qd_ctx *ctx, const u64 *UH, const u64 *UR, const double2 *UM, u64 nslots, u64 r,
+                       std::vector<u64> &h, u64 *bitmap, u64 *sbm) {
+    hipStream_t s = ctx->stream;
+    u64 *hist1 = ctx->hist1.as<u64>();
+    u64 *counters = ctx->counters.as<u64>();
+    u64 cum = 0;
+    int d = kHistBins - 1;
+    for (; d >= 0; --d) {
+        if (cum + h[d] >= r) break;
+        cum += h[d];
+    }
+    r -= cum;
+    // second 12 bits inside bin d
+    const u32 d1 = u32(d);
+    QD_HIP(hipMemsetAsync(hist1, 0, kHistBins * sizeof(u64), s));
+    QD_HIP(hipMemsetAsync(counters + 1, 0, sizeof(u64), s));
+    hipLaunchKernelGGL(k_sel_hist_u, dim3(grid_for(nslots, 256, 8192)), dim3(256), 0, s, UR, UM, nslots, d1, hist1);
+    QD_HIP(hipGetLastError());
+    QD_TRY(sync_read(ctx, hist1, kHistBins * sizeof(u64), h.data()));
+    cum = 0;
+    for (d = kHistBins - 1; d >= 0; --d) {
+        if (cum + h[d] >= r) break;
+        cum += h[d];
+    }
+    r -= cum;
+    const u64 ns = h[d];
+    QD_TRY(ctx->sel0.ensure(std::max<u64>(ns, 1) * sizeof(SelItem)));
+    QD_TRY(ctx->sel1.ensure(std::max<u64>(ns, 1) * sizeof(SelItem)));
+    SelItem *S0 = ctx->sel0.as<SelItem>(), *S1 = ctx->sel1.as<SelItem>();
+    hipLaunchKernelGGL(k_sel_first, dim3(grid_for(nslots, 256, 16384)), dim3(256), 0, s, UH, UR, UM, nslots,
+                       (d1 << 12) | u32(d), S0, counters + 1, bitmap, sbm);
+    QD_HIP(hipGetLastError());
+    return select_loop(ctx, S0, S1, ns, r, 24, bitmap, UR, sbm);
+}
+
 // Buckets beyond the LDS capacity: one workgroup each on global scratch.
 static int dedup_big_buckets(qd_ctx *ctx, const Entry *D, u64 rstride, const u64 *boff, u64 nbig, double tol, u64 *UH, u64 *UR,
                              double2 *UM, u64 *U) {
@@ -372,6 +418,308 @@ static void dedup_layout(u64 C, const qd_tuning &t, u32 &b1, u32 &b2) {
     b1 = std::min<u32>(L, t.first_pass_bits);
     if (L - b1 > 12) b1 = L - 12;
     b2 = L - b1;
+}
+
+// Partition (one or two radix passes on the top key bits) and dedup of n
+// candidates with keys H0 and magnitudes M0 (CI: candidate index of each
+// position, null when the position is the candidate index).  hist1 holds the
+// histogram of the top b1 key bits.  The uniques land in the ebuf0 space
+// (H0 / M0 / CI are dead by then) as n slots UH / UR / UM; U counts them.
+// The context's counters must have been cleared (unique count, error word,
+// oversized-bucket count).
+static int partition_dedup(qd_ctx *ctx, const u64 *H0, const double2 *M0, const u64 *CI, u64 n, u32 b1, u32 b2,
+                           double tol, qd_step_stats &st, u64 *&UH, u64 *&UR, double2 *&UM, u64 &U) {
+    hipStream_t s = ctx->stream;
+    u64 *counters = ctx->counters.as<u64>();
+    u32 *errw = reinterpret_cast<u32 *>(counters + 4);
+    u64 *hist1 = ctx->hist1.as<u64>();
+    Entry *E1 = ctx->ebuf1.as<Entry>();
+    u64 *hb = ctx->host;
+    const Entry *D = E1;       // candidates the dedup reads
+    u64 rstride = 0;           // fixed-capacity read stride of D (0: compact)
+    const u64 *woff = nullptr; // bucket output offsets (nbk + 1)
+    u64 nbk = 1;
+    void *ubuf = ctx->ebuf0.p; // where the uniques go (H0/M0 are dead by then)
+    const u64 nb1 = 1ull << b1;
+    prof_begin(ctx, QD_PHASE_PARTITION);
+    if (b1 == 0) {
+        u64 two[2] = {0, n};
+        QD_HIP(hipMemcpyAsync(ctx->off1.p, two, sizeof two, hipMemcpyHostToDevice, s));
+        QD_HIP(hipMemcpyAsync(ctx->cur1.p, two, sizeof(u64), hipMemcpyHostToDevice, s));
+        QD_HIP(hipStreamSynchronize(s));
+    } else {
+        QD_TRY(run_scan(ctx, LoadU64{hist1}, nb1, ctx->off1.as<u64>()));
+        QD_HIP(hipMemcpyAsync(ctx->cur1.p, ctx->off1.p, nb1 * sizeof(u64), hipMemcpyDeviceToDevice, s));
+    }
+    hipLaunchKernelGGL(k_scatter1, dim3(unsigned((n + kTile1 - 1) / kTile1)), dim3(256), 0, s, H0, M0, n, b1,
+                       ctx->cur1.as<u64>(), E1, CI);
+    QD_HIP(hipGetLastError());
+    if (b2 == 0) {
+        woff = ctx->off1.as<u64>();
+        nbk = nb1;
+    } else {
+        const u64 nb = 1ull << (b1 + b2);
+        const u64 mean = (n + nb - 1) / nb;
+        const u64 cap2 = ((mean + mean / 2 + 128) + 63) / 64 * 64;
+        QD_TRY(ctx->off2.ensure((nb + 1) * sizeof(u64)));
+        QD_TRY(ctx->cnt2.ensure(nb * sizeof(u64)));
+        QD_TRY(ctx->ebuf2.ensure(std::max<u64>(nb * cap2, n) * sizeof(Entry)));
+        Entry *E2 = ctx->ebuf2.as<Entry>();
+        hipLaunchKernelGGL(k_level2_fixed, dim3(nb1), dim3(1024), 0, s, (const Entry *)E1, ctx->off1.as<u64>(), b1, b2,
+                           cap2, E2, ctx->cnt2.as<u64>(), errw);
+        QD_HIP(hipGetLastError());
+        QD_TRY(run_scan(ctx, LoadU64{ctx->cnt2.as<u64>()}, nb, ctx->off2.as<u64>()));
+        u32 ew = 0;
+        QD_TRY(sync_read(ctx, errw, sizeof(u32), &ew));
+        rstride = cap2;
+        if (ew & ERR_SPILL) {  // a sub-bucket overflowed its capacity: exact offsets
+            QD_HIP(hipMemsetAsync(errw, 0, sizeof(u32), s));
+            ctx->host[kHostN] = n;
+            QD_HIP(hipMemcpyAsync(ctx->off2.as<u64>() + nb, ctx->host + kHostN, sizeof(u64), hipMemcpyHostToDevice,
+                                  s));
+            hipLaunchKernelGGL(k_level2_exact, dim3(nb1), dim3(1024), 0, s, (const Entry *)E1, ctx->off1.as<u64>(), b1,
+                               b2, E2, ctx->off2.as<u64>());
+            QD_HIP(hipGetLastError());
+            rstride = 0;
+            st.reserved |= 1;  // spill fallback used
+        }
+        D = E2;
+        woff = ctx->off2.as<u64>();
+        nbk = nb;
+    }
+    prof_end(ctx, QD_PHASE_PARTITION);
+    const u32 L = b1 + b2;
+
+    // ---- 5 dedup
+    UH = static_cast<u64 *>(ubuf);
+    UR = UH + n;
+    UM = reinterpret_cast<double2 *>(UR + n);
+    prof_begin(ctx, QD_PHASE_DEDUP);
+    QD_TRY(ctx->biglist.ensure((n / 2048 + 2) * sizeof(u64)));
+    {
+        int ncu = 256;
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device);
+        const u64 grid = std::min<u64>(nbk, u64(ncu));
+        hipLaunchKernelGGL((k_dedup<1024, 2048, 4096>), dim3(grid), dim3(1024), 0, s, D, rstride, woff, nbk, L,
+                           tol, UH, UR, UM, counters, ctx->nhist.as<u64>(), ctx->biglist.as<u64>());
+        QD_HIP(hipGetLastError());
+    }
+    prof_end(ctx, QD_PHASE_DEDUP);
+    QD_HIP(hipMemcpyAsync(hb, counters, 6 * sizeof(u64), hipMemcpyDeviceToHost, s));
+    QD_HIP(hipStreamSynchronize(s));
+    prof_collect(ctx);
+    QD_TRY(check_err_word(u32(hb[4])));
+    U = hb[0];
+    if (hb[5]) QD_TRY(dedup_big_buckets(ctx, D, rstride, woff, hb[5], tol, UH, UR, UM, &U));
+    return QD_OK;
+}
+
+// Regenerates the K survivors (candidate indices orep, ascending; their
+// magnitudes already in out->mag) into out: sizes, padded offsets, bytes.
+template <int KIND>
+static int populate_out(qd_ctx *ctx, const Parents &P, const qd_rule &rule, const u64 *orep, u64 K,
+                        const qd_params &prm, qd_state *out, qd_step_stats &st) {
+    hipStream_t s = ctx->stream;
+    u64 *hb = ctx->host;
+    prof_begin(ctx, QD_PHASE_FINALIZE);
+    hipLaunchKernelGGL(k_child_size<KIND>, dim3(grid_for(K)), dim3(256), 0, s, P, ctx->cb.as<u64>(), rule, orep, K,
+                       prm.align_byte_length, out->size.as<u32>(), ctx->padded.as<u64>());
+    QD_HIP(hipGetLastError());
+    QD_TRY(run_scan(ctx, LoadU64{ctx->padded.as<u64>()}, K, out->begin.as<u64>()));
+    prof_end(ctx, QD_PHASE_FINALIZE);
+    QD_TRY(sync_read(ctx, out->begin.as<u64>() + K, sizeof(u64), hb));
+    const u64 bytes = hb[0];
+    QD_TRY(out->objects.ensure(std::max<u64>(bytes, 1)));
+    prof_begin(ctx, QD_PHASE_FINALIZE);
+    hipLaunchKernelGGL(k_populate<KIND>, dim3(unsigned(std::min<u64>(K, 65536))), dim3(64), 0, s, P,
+                       ctx->cb.as<u64>(), rule, orep, K, out->begin.as<u64>(), out->size.as<u32>(),
+                       out->objects.as<u8>());
+    QD_HIP(hipGetLastError());
+    out->n = K;
+    out->bytes = bytes;
+    st.output_bytes = bytes;
+
+    prof_end(ctx, QD_PHASE_FINALIZE);
+    return QD_OK;
+}
+
+// Largest candidate count one pass of a step (a batched chunk or a hash
+// range) may take: 2^chunk_log2,
+// or what free device memory (plus the context's cached scratch) holds at
+// about 150 bytes of scratch per candidate (entries, partition buffers, dedup
+// output, bitmaps, growth slack), with room left for the states and for
+// buffers that grow one at a time.
+static u64 step_budget(qd_ctx *ctx) {
+    if (ctx->tune.chunk_log2) return 1ull << ctx->tune.chunk_log2;
+    size_t freeb = 0, totalb = 0;
+    if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) {
+        (void)hipGetLastError();
+        return 1ull << 28;
+    }
+    u64 cached = 0;
+    for (DBuf *b : ctx->all()) cached += b->cap;
+    return std::max<u64>(u64((double(freeb) + double(cached)) * 0.45 / 150.0), 1ull << 20);
+}
+
+// ===========================================================================
+// hash-range passes (qd_k_range.hpp): steps whose candidates exceed the
+// scratch memory
+// ===========================================================================
+template <int KIND>
+static int range_step(qd_ctx *ctx, const Parents &P, const qd_rule &rule, u64 C, u64 nitems, u64 max_obj, u64 budget,
+                      const qd_params &prm, qd_state *out, qd_symbolic *sy, qd_step_stats &st) {
+    hipStream_t s = ctx->stream;
+    const u64 N = P.n;
+    u64 *counters = ctx->counters.as<u64>();
+    u64 *hb = ctx->host;
+    QD_TRY(ctx->hist1.ensure(kHistBins * sizeof(u64)));
+    QD_TRY(ctx->off1.ensure((kHistBins + 1) * sizeof(u64)));
+    QD_TRY(ctx->cur1.ensure(kHistBins * sizeof(u64)));
+    QD_TRY(ctx->nhist.ensure(kHistBins * sizeof(u64)));
+    u64 *hist1 = ctx->hist1.as<u64>();
+    const unsigned gg = unsigned(std::max<u64>(
+        std::min<u64>((nitems + kGenThreads - 1) / kGenThreads, ctx->tune.generate_grid), 1));
+    // 1. how the candidates spread over the 4096 values of the top 12 hash bits
+    std::vector<u64> hc(kHistBins);
+    QD_HIP(hipMemsetAsync(hist1, 0, kHistBins * sizeof(u64), s));
+    prof_begin(ctx, QD_PHASE_GENERATE);
+    hipLaunchKernelGGL((k_generate_range<KIND, true>), dim3(gg), dim3(kGenThreads), 0, s, ctx->pdesc.as<PDesc>(), P.mag,
+                       ctx->cb.as<u64>(), ctx->wb.as<u64>(), N, ctx->tab.as<u64>(), P.begin, rule, nitems, 0u, 0u,
+                       (u64 *)nullptr, (double2 *)nullptr, (u64 *)nullptr, (u64 *)nullptr, 0u, hist1);
+    QD_HIP(hipGetLastError());
+    prof_end(ctx, QD_PHASE_GENERATE);
+    QD_TRY(sync_read(ctx, hist1, kHistBins * sizeof(u64), hc.data()));
+    // 2. contiguous ranges of bins within the budget
+    std::vector<u32> cut{0};
+    std::vector<u64> rn;
+    u64 acc = 0, nmax = 0;
+    for (u32 d = 0; d < u32(kHistBins); ++d) {
+        if (acc && acc + hc[d] > budget) {
+            cut.push_back(d);
+            rn.push_back(acc);
+            nmax = std::max(nmax, acc);
+            acc = 0;
+        }
+        acc += hc[d];
+    }
+    cut.push_back(kHistBins);
+    rn.push_back(acc);
+    nmax = std::max(nmax, acc);
+    // 3. scratch of one range, the pool and the candidate bitmap
+    const u64 pcap = std::max<u64>(max_obj == UINT64_MAX ? C : std::min<u64>(C, max_obj + nmax), 1);
+    QD_TRY(ctx->ebuf0.ensure(nmax * 32));
+    QD_TRY(ctx->ebuf1.ensure(nmax * 32));
+    QD_TRY(ctx->pool0.ensure(pcap * 32));
+    QD_TRY(ctx->pool1.ensure(pcap * 32));
+    const u64 W = (C + 63) / 64;
+    QD_TRY(ctx->bitmap.ensure(W * sizeof(u64)));
+    QD_TRY(ctx->wprefix.ensure((W + 1) * sizeof(u64)));
+    QD_TRY(ctx->slist.ensure(((pcap + 63) / 64) * sizeof(u64)));
+    QD_TRY(ctx->pwpre.ensure(((pcap + 63) / 64 + 1) * sizeof(u64)));
+    auto pool = [&](DBuf &b, u64 *&h, u64 *&r, double2 *&m) {
+        h = b.as<u64>();
+        r = h + pcap;
+        m = reinterpret_cast<double2 *>(r + pcap);
+    };
+    u64 *Ph, *Pr, *Qh, *Qr;
+    double2 *Pm, *Qm;
+    pool(ctx->pool0, Ph, Pr, Pm);
+    pool(ctx->pool1, Qh, Qr, Qm);
+    u64 ns = 0, Utot = 0;
+    std::vector<u64> h(kHistBins);
+    for (size_t r = 0; r + 1 < cut.size(); ++r) {
+        const u64 n = rn[r];
+        if (n == 0) continue;
+        u32 b1, b2;
+        dedup_layout(n, ctx->tune, b1, b2);
+        st.dedup_levels = (b1 ? 1 : 0) + (b2 ? 1 : 0);
+        QD_HIP(hipMemsetAsync(counters, 0, 16 * sizeof(u64), s));
+        QD_HIP(hipMemsetAsync(hist1, 0, kHistBins * sizeof(u64), s));
+        QD_HIP(hipMemsetAsync(ctx->nhist.p, 0, kHistBins * sizeof(u64), s));
+        u64 *HK = ctx->ebuf0.as<u64>();
+        double2 *MK = reinterpret_cast<double2 *>(HK + n);
+        u64 *CK = reinterpret_cast<u64 *>(MK + n);
+        prof_begin(ctx, QD_PHASE_GENERATE);
+        hipLaunchKernelGGL((k_generate_range<KIND, false>), dim3(gg), dim3(kGenThreads), 0, s, ctx->pdesc.as<PDesc>(),
+                           P.mag, ctx->cb.as<u64>(), ctx->wb.as<u64>(), N, ctx->tab.as<u64>(), P.begin, rule, nitems,
+                           cut[r], cut[r + 1], HK, MK, CK, counters + 7, b1, hist1);
+        QD_HIP(hipGetLastError());
+        prof_end(ctx, QD_PHASE_GENERATE);
+        u64 *UH, *UR;
+        double2 *UM;
+        u64 U;
+        QD_TRY(partition_dedup(ctx, HK, MK, CK, n, b1, b2, prm.tolerance, st, UH, UR, UM, U));
+        u64 kept = 0;
+        QD_TRY(sync_read(ctx, counters + 7, sizeof(u64), &kept));
+        if (kept != n) return fail(QD_ERR_HIP, "hash-range pass kept a different candidate count than it counted");
+        Utot += U;
+        if (!U) continue;
+        // append the range's uniques to the pool
+        hb[kHostN - 1] = ns;
+        QD_HIP(hipMemcpyAsync(counters + 8, hb + kHostN - 1, sizeof(u64), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_pool_append, dim3(unsigned(std::min<u64>((n + 256 * kPoolPer - 1) / (256 * kPoolPer), 16384))),
+                           dim3(256), 0, s, UH, UR, UM, n, Ph, Pr, Pm, counters + 8);
+        QD_HIP(hipGetLastError());
+        ns += U;
+        if (ns <= max_obj) continue;
+        // keep the best max_obj of the pool
+        prof_begin(ctx, QD_PHASE_SELECT);
+        QD_HIP(hipMemsetAsync(hist1, 0, kHistBins * sizeof(u64), s));
+        hipLaunchKernelGGL(k_pool_hist0, dim3(grid_for(ns, 256, 4096)), dim3(256), 0, s, (const double2 *)Pm, ns, hist1);
+        QD_HIP(hipGetLastError());
+        QD_TRY(sync_read(ctx, hist1, kHistBins * sizeof(u64), h.data()));
+        u64 *keep = ctx->slist.as<u64>();
+        QD_TRY(select_topk(ctx, Ph, Pr, Pm, ns, max_obj, h, ctx->bitmap.as<u64>(), keep));
+        const u64 Wp = (ns + 63) / 64;
+        QD_TRY(run_scan(ctx, LoadPopc{keep}, Wp, ctx->pwpre.as<u64>()));
+        hipLaunchKernelGGL(k_pool_compact, dim3(grid_for(ns)), dim3(256), 0, s, (const u64 *)Ph, (const u64 *)Pr,
+                           (const double2 *)Pm, ns, (const u64 *)keep, (const u64 *)ctx->pwpre.as<u64>(), Qh, Qr, Qm);
+        QD_HIP(hipGetLastError());
+        prof_end(ctx, QD_PHASE_SELECT);
+        std::swap(Ph, Qh);
+        std::swap(Pr, Qr);
+        std::swap(Pm, Qm);
+        ns = max_obj;
+    }
+    st.num_unique = Utot;
+    const u64 K = ns;
+    st.num_survivors = K;
+    if (sy) {
+        sy->num_object = C;
+        sy->num_object_after_interferences = Utot;
+    }
+    if (K == 0) {
+        QD_TRY(state_reserve(out, 0, 0));
+        QD_HIP(hipMemsetAsync(out->begin.p, 0, sizeof(u64), s));
+        QD_HIP(hipStreamSynchronize(s));
+        out->n = 0;
+        out->bytes = 0;
+        out->total_proba = 0;
+        st.total_proba = 0;
+        return QD_OK;
+    }
+    // order: survivors in ascending representative order
+    u64 *bitmap = ctx->bitmap.as<u64>();
+    prof_begin(ctx, QD_PHASE_ORDER);
+    QD_HIP(hipMemsetAsync(bitmap, 0, W * sizeof(u64), s));
+    hipLaunchKernelGGL(k_mark_reps, dim3(grid_for(K)), dim3(256), 0, s, (const u64 *)Pr, K, bitmap);
+    QD_TRY(run_scan(ctx, LoadPopc{bitmap}, W, ctx->wprefix.as<u64>()));
+    QD_TRY(out->begin.ensure((K + 1) * sizeof(u64)));
+    QD_TRY(out->size.ensure(K * sizeof(u32)));
+    QD_TRY(out->mag.ensure(K * sizeof(double2)));
+    QD_TRY(ctx->padded.ensure(K * sizeof(u64)));
+    QD_TRY(ctx->keys.ensure(K * sizeof(u64)));
+    u64 *orep = ctx->keys.as<u64>();
+    hipLaunchKernelGGL(k_place_list, dim3(grid_for(K)), dim3(256), 0, s, (const u64 *)Pr, (const double2 *)Pm, K,
+                       (const u64 *)bitmap, (const u64 *)ctx->wprefix.as<u64>(), orep, out->mag.as<double2>());
+    QD_HIP(hipGetLastError());
+    prof_end(ctx, QD_PHASE_ORDER);
+    QD_TRY(populate_out<KIND>(ctx, P, rule, orep, K, prm, out, st));
+    QD_TRY(normalize_state(ctx, out));
+    prof_collect(ctx);
+    st.total_proba = out->total_proba;
+    st.reserved |= 4;  // hash-range passes
+    return QD_OK;
 }
 
 // ===========================================================================
@@ -654,6 +1002,18 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
     st.num_candidates = C;
     if (BATCH) QD_TRY(batch_cseg(ctx, bt));
     if (C == 0) return finish_empty();
+    if (!BATCH && fast) {
+        const u64 budget = step_budget(ctx);
+        if (C > budget) {
+            if (max_obj == 0) {  // memory-driven cap, as below
+                size_t freeb = 0, totalb = 0;
+                QD_HIP(hipMemGetInfo(&freeb, &totalb));
+                const u64 per = u64(u32(hb[1])) + 8 + sizeof(u64) + sizeof(u32) + sizeof(double2);
+                max_obj = std::max<u64>(1, u64(double(freeb) * 0.8) / per);
+            }
+            return range_step<KIND>(ctx, P, rule, C, nitems, max_obj, budget, prm, out, sy, st);
+        }
+    }
 
     // ---- 3 generate: hash and magnitude at the candidate index
     u32 b1, b2;
@@ -694,82 +1054,10 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
     QD_HIP(hipGetLastError());
     prof_end(ctx, QD_PHASE_GENERATE);
 
-    // ---- 4 partition
-    const Entry *D = E1;       // candidates the dedup reads
-    u64 rstride = 0;           // fixed-capacity read stride of D (0: compact)
-    const u64 *woff = nullptr; // bucket output offsets (nbk + 1)
-    u64 nbk = 1;
-    void *ubuf = ctx->ebuf0.p; // where the uniques go (H0/M0 are dead by then)
-    const u64 nb1 = 1ull << b1;
-    prof_begin(ctx, QD_PHASE_PARTITION);
-    if (b1 == 0) {
-        u64 two[2] = {0, C};
-        QD_HIP(hipMemcpyAsync(ctx->off1.p, two, sizeof two, hipMemcpyHostToDevice, s));
-        QD_HIP(hipMemcpyAsync(ctx->cur1.p, two, sizeof(u64), hipMemcpyHostToDevice, s));
-        QD_HIP(hipStreamSynchronize(s));
-    } else {
-        QD_TRY(run_scan(ctx, LoadU64{hist1}, nb1, ctx->off1.as<u64>()));
-        QD_HIP(hipMemcpyAsync(ctx->cur1.p, ctx->off1.p, nb1 * sizeof(u64), hipMemcpyDeviceToDevice, s));
-    }
-    hipLaunchKernelGGL(k_scatter1, dim3((C + kTile1 - 1) / kTile1), dim3(256), 0, s, (const u64 *)H0,
-                       (const double2 *)M0, C, b1, ctx->cur1.as<u64>(), E1);
-    QD_HIP(hipGetLastError());
-    if (b2 == 0) {
-        woff = ctx->off1.as<u64>();
-        nbk = nb1;
-    } else {
-        const u64 nb = 1ull << (b1 + b2);
-        const u64 mean = (C + nb - 1) / nb;
-        const u64 cap2 = ((mean + mean / 2 + 128) + 63) / 64 * 64;
-        QD_TRY(ctx->off2.ensure((nb + 1) * sizeof(u64)));
-        QD_TRY(ctx->cnt2.ensure(nb * sizeof(u64)));
-        QD_TRY(ctx->ebuf2.ensure(std::max<u64>(nb * cap2, C) * sizeof(Entry)));
-        Entry *E2 = ctx->ebuf2.as<Entry>();
-        hipLaunchKernelGGL(k_level2_fixed, dim3(nb1), dim3(1024), 0, s, (const Entry *)E1, ctx->off1.as<u64>(), b1, b2,
-                           cap2, E2, ctx->cnt2.as<u64>(), errw);
-        QD_HIP(hipGetLastError());
-        QD_TRY(run_scan(ctx, LoadU64{ctx->cnt2.as<u64>()}, nb, ctx->off2.as<u64>()));
-        u32 ew = 0;
-        QD_TRY(sync_read(ctx, errw, sizeof(u32), &ew));
-        rstride = cap2;
-        if (ew & ERR_SPILL) {  // a sub-bucket overflowed its capacity: exact offsets
-            QD_HIP(hipMemsetAsync(errw, 0, sizeof(u32), s));
-            QD_HIP(hipMemcpyAsync(ctx->off2.as<u64>() + nb, ctx->cb.as<u64>() + N, sizeof(u64),
-                                  hipMemcpyDeviceToDevice, s));
-            hipLaunchKernelGGL(k_level2_exact, dim3(nb1), dim3(1024), 0, s, (const Entry *)E1, ctx->off1.as<u64>(), b1,
-                               b2, E2, ctx->off2.as<u64>());
-            QD_HIP(hipGetLastError());
-            rstride = 0;
-            st.reserved |= 1;  // spill fallback used
-        }
-        D = E2;
-        woff = ctx->off2.as<u64>();
-        nbk = nb;
-    }
-    prof_end(ctx, QD_PHASE_PARTITION);
-    const u32 L = b1 + b2;
-
-    // ---- 5 dedup
-    u64 *UH = static_cast<u64 *>(ubuf);
-    u64 *UR = UH + C;
-    double2 *UM = reinterpret_cast<double2 *>(UR + C);
-    prof_begin(ctx, QD_PHASE_DEDUP);
-    QD_TRY(ctx->biglist.ensure((C / 2048 + 2) * sizeof(u64)));
-    {
-        int ncu = 256;
-        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device);
-        const u64 grid = std::min<u64>(nbk, u64(ncu));
-        hipLaunchKernelGGL((k_dedup<1024, 2048, 4096>), dim3(grid), dim3(1024), 0, s, D, rstride, woff, nbk, L,
-                           prm.tolerance, UH, UR, UM, counters, ctx->nhist.as<u64>(), ctx->biglist.as<u64>());
-        QD_HIP(hipGetLastError());
-    }
-    prof_end(ctx, QD_PHASE_DEDUP);
-    QD_HIP(hipMemcpyAsync(hb, counters, 6 * sizeof(u64), hipMemcpyDeviceToHost, s));
-    QD_HIP(hipStreamSynchronize(s));
-    prof_collect(ctx);
-    QD_TRY(check_err_word(u32(hb[4])));
-    u64 U = hb[0];
-    if (hb[5]) QD_TRY(dedup_big_buckets(ctx, D, rstride, woff, hb[5], prm.tolerance, UH, UR, UM, &U));
+    u64 *UH, *UR;
+    double2 *UM;
+    u64 U;
+    QD_TRY(partition_dedup(ctx, H0, M0, nullptr, C, b1, b2, prm.tolerance, st, UH, UR, UM, U));
     st.num_unique = U;
 
     // memory-driven cap (max_num_object == 0, quids.hpp:459-486, 510-536):
@@ -802,33 +1090,7 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
             return fail(QD_ERR_UNSUPPORTED, "probabilistic truncation is not implemented; set simple_truncation");
         std::vector<u64> h(kHistBins);
         QD_TRY(sync_read(ctx, ctx->nhist.p, kHistBins * sizeof(u64), h.data()));
-        u64 r = max_obj, cum = 0;
-        int d = kHistBins - 1;
-        for (; d >= 0; --d) {
-            if (cum + h[d] >= r) break;
-            cum += h[d];
-        }
-        r -= cum;
-        // second 12 bits inside bin d
-        const u32 d1 = u32(d);
-        QD_HIP(hipMemsetAsync(hist1, 0, kHistBins * sizeof(u64), s));
-        hipLaunchKernelGGL(k_sel_hist_u, dim3(grid_for(C, 256, 8192)), dim3(256), 0, s, UR, UM, C, d1, hist1);
-        QD_HIP(hipGetLastError());
-        QD_TRY(sync_read(ctx, hist1, kHistBins * sizeof(u64), h.data()));
-        cum = 0;
-        for (d = kHistBins - 1; d >= 0; --d) {
-            if (cum + h[d] >= r) break;
-            cum += h[d];
-        }
-        r -= cum;
-        u64 ns = h[d];
-        QD_TRY(ctx->sel0.ensure(std::max<u64>(ns, 1) * sizeof(SelItem)));
-        QD_TRY(ctx->sel1.ensure(std::max<u64>(ns, 1) * sizeof(SelItem)));
-        SelItem *S0 = ctx->sel0.as<SelItem>(), *S1 = ctx->sel1.as<SelItem>();
-        hipLaunchKernelGGL(k_sel_first, dim3(grid_for(C, 256, 16384)), dim3(256), 0, s, UH, UR, UM, C,
-                           (d1 << 12) | u32(d), S0, counters + 1, bitmap, sbm);
-        QD_HIP(hipGetLastError());
-        QD_TRY(select_loop(ctx, S0, S1, ns, r, 24, bitmap, UR, sbm));
+        QD_TRY(select_topk(ctx, UH, UR, UM, C, max_obj, h, bitmap, sbm));
     }
     prof_end(ctx, QD_PHASE_SELECT);
     prof_begin(ctx, QD_PHASE_ORDER);
@@ -851,25 +1113,7 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
     hipLaunchKernelGGL(k_place, dim3(grid_for(C, 256, 16384)), dim3(256), 0, s, UR, UM, (const u64 *)sbm, C, bitmap,
                        ctx->wprefix.as<u64>(), orep, out->mag.as<double2>());
     prof_end(ctx, QD_PHASE_ORDER);
-    prof_begin(ctx, QD_PHASE_FINALIZE);
-    hipLaunchKernelGGL(k_child_size<KIND>, dim3(grid_for(K)), dim3(256), 0, s, P, ctx->cb.as<u64>(), rule, orep, K,
-                       prm.align_byte_length, out->size.as<u32>(), ctx->padded.as<u64>());
-    QD_HIP(hipGetLastError());
-    QD_TRY(run_scan(ctx, LoadU64{ctx->padded.as<u64>()}, K, out->begin.as<u64>()));
-    prof_end(ctx, QD_PHASE_FINALIZE);
-    QD_TRY(sync_read(ctx, out->begin.as<u64>() + K, sizeof(u64), hb));
-    const u64 bytes = hb[0];
-    QD_TRY(out->objects.ensure(std::max<u64>(bytes, 1)));
-    prof_begin(ctx, QD_PHASE_FINALIZE);
-    hipLaunchKernelGGL(k_populate<KIND>, dim3(unsigned(std::min<u64>(K, 65536))), dim3(64), 0, s, P,
-                       ctx->cb.as<u64>(), rule, orep, K, out->begin.as<u64>(), out->size.as<u32>(),
-                       out->objects.as<u8>());
-    QD_HIP(hipGetLastError());
-    out->n = K;
-    out->bytes = bytes;
-    st.output_bytes = bytes;
-
-    prof_end(ctx, QD_PHASE_FINALIZE);
+    QD_TRY(populate_out<KIND>(ctx, P, rule, orep, K, prm, out, st));
     // ---- 9 normalize
     if (BATCH) QD_TRY(batch_normalize(ctx, bt, out));
     else QD_TRY(normalize_state(ctx, out));
@@ -963,6 +1207,190 @@ static int adjoint_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, const qd
     QD_HIP(hipGetLastError());
     for (int t = 0; t < nd; ++t) QD_TRY(canonical_sum(ctx, nullptr, ctx->adjp.as<double>() + u64(t) * N, N, &grad[t]));
     QD_HIP(hipStreamSynchronize(s));
+    return QD_OK;
+}
+
+// ===========================================================================
+// batched step over instance chunks
+// ===========================================================================
+__global__ __launch_bounds__(256) void k_add_offset(const u64 *src, u64 n, u64 add, u64 *dst) {
+    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+        dst[i] = src[i] + add;
+}
+
+// Candidates per instance of the whole batch (prep + scan only): cseg.
+template <int KIND>
+static int batch_plan(qd_ctx *ctx, const Parents &P, u64 in_bytes, Batch *bt) {
+    hipStream_t s = ctx->stream;
+    const u64 N = P.n;
+    if (N == 0) {
+        bt->cseg.assign(bt->n + 1, 0);
+        return QD_OK;
+    }
+    QD_TRY(ctx->nc.ensure(N * sizeof(u32)));
+    QD_TRY(ctx->cb.ensure((N + 1) * sizeof(u64)));
+    QD_TRY(ctx->counters.ensure(64 * sizeof(u64)));
+    QD_TRY(ctx->pdesc.ensure(N * sizeof(PDesc)));
+    QD_TRY(ctx->wi.ensure(N * sizeof(u32)));
+    if (KIND == QD_RULE_QCGD_SPLIT_MERGE) QD_TRY(ctx->tab.ensure((in_bytes / 2 + 16) * sizeof(u64)));
+    u64 *counters = ctx->counters.as<u64>();
+    QD_HIP(hipMemsetAsync(counters, 0, 64 * sizeof(u64), s));
+    hipLaunchKernelGGL(k_prep_fast<KIND>, dim3(grid_for(N, 4, 65536)), dim3(256), 0, s, P, (const u64 *)nullptr,
+                       ctx->pdesc.as<PDesc>(), ctx->nc.as<u32>(), ctx->wi.as<u32>(), ctx->tab.as<u64>(),
+                       reinterpret_cast<u32 *>(counters + 3), reinterpret_cast<u32 *>(counters + 4));
+    QD_HIP(hipGetLastError());
+    QD_TRY(run_scan(ctx, LoadU32{ctx->nc.as<u32>()}, N, ctx->cb.as<u64>()));
+    u64 hb[2];
+    QD_TRY(sync_read(ctx, counters + 3, 2 * sizeof(u64), hb));
+    if (u32(hb[1]) & ERR_WIDE) return fail(QD_ERR_UNSUPPORTED, "the batched step handles graphs of at most 64 nodes");
+    QD_TRY(check_err_word(u32(hb[1])));
+    return batch_cseg(ctx, bt);
+}
+
+template <int KIND>
+static int batched_kind(qd_ctx *c, qd_state *in, const u64 *in_seg, u32 n, const qd_rule *rules, qd_state *out,
+                        u64 *out_seg, const u64 *caps, const qd_params &prm, qd_step_stats *stats) {
+    hipStream_t s = c->stream;
+    const Parents P{in->objects.as<u8>(), in->begin.as<u64>(), in->size.as<u32>(), in->mag.as<double2>(), in->n};
+    QD_TRY(upload(c, c->boseg, in_seg, (n + 1) * sizeof(u64)));
+    QD_TRY(upload(c, c->brules, rules, n * sizeof(qd_rule)));
+    Batch plan;
+    plan.n = n;
+    plan.d_oseg = c->boseg.as<u64>();
+    QD_TRY(batch_plan<KIND>(c, P, in->bytes, &plan));
+    // chunks of consecutive instances within the candidate budget
+    const u64 budget = step_budget(c);
+    std::vector<u32> cuts{0};
+    u64 acc = 0;
+    for (u32 i = 0; i < n; ++i) {
+        const u64 ci = plan.cseg[i + 1] - plan.cseg[i];
+        if (acc && acc + ci > budget) {
+            cuts.push_back(i);
+            acc = 0;
+        }
+        acc += ci;
+    }
+    cuts.push_back(n);
+    const size_t nchunk = cuts.size() - 1;
+
+    std::vector<u64> U(n, 0), kseg(n + 1, 0);
+    std::vector<double> total(n, 0.0);
+    qd_step_stats st;
+    std::memset(&st, 0, sizeof st);
+    std::vector<qd_state *> parts;
+    auto cleanup = [&]() {
+        for (qd_state *t : parts) {
+            t->objects.release();
+            t->begin.release();
+            t->size.release();
+            t->mag.release();
+            delete t;
+        }
+        parts.clear();
+    };
+    for (size_t k = 0; k < nchunk; ++k) {
+        const u32 i0 = cuts[k], i1 = cuts[k + 1];
+        const u64 p0 = in_seg[i0], p1 = in_seg[i1];
+        std::vector<u64> sub(i1 - i0 + 1);
+        for (u32 i = i0; i <= i1; ++i) sub[i - i0] = in_seg[i] - p0;
+        int rc = upload(c, c->boseg, sub.data(), sub.size() * sizeof(u64));
+        if (rc) {
+            cleanup();
+            return rc;
+        }
+        Batch bt;
+        bt.n = i1 - i0;
+        bt.d_oseg = c->boseg.as<u64>();
+        bt.d_rules = c->brules.as<qd_rule>() + i0;
+        bt.caps = caps + i0;
+        const Parents Pk{P.obj, P.begin + p0, P.size + p0, P.mag + p0, p1 - p0};
+        qd_state *dst = out;
+        if (nchunk > 1) {
+            dst = new qd_state;
+            dst->ctx = c;
+            parts.push_back(dst);
+        }
+        if (bt.n == 1) {
+            // one instance alone: its plain step (which runs in hash-range
+            // passes when it exceeds the budget by itself)
+            rc = step_impl<KIND, false>(c, Pk, in->bytes, rules[i0], nullptr, dst, nullptr, caps[i0], prm, st);
+            bt.U.assign(1, st.num_unique);
+            bt.kseg = {0, dst->n};
+            bt.total.assign(1, dst->total_proba);
+        } else {
+            rc = step_impl<KIND, true>(c, Pk, in->bytes, rules[i0], &bt, dst, nullptr, UINT64_MAX, prm, st);
+        }
+        if (rc) {
+            cleanup();
+            return rc;
+        }
+        const u64 kbase = kseg[i0];
+        for (u32 i = i0; i < i1; ++i) {
+            U[i] = bt.U[i - i0];
+            total[i] = bt.total[i - i0];
+            kseg[i + 1] = kbase + bt.kseg[i - i0 + 1];
+        }
+    }
+    if (nchunk > 1) {  // back to back into out
+        u64 K = 0, bytes = 0;
+        for (qd_state *t : parts) {
+            K += t->n;
+            bytes += t->bytes;
+        }
+        int rc = state_reserve(out, K, bytes);
+        if (rc) {
+            cleanup();
+            return rc;
+        }
+        u64 kb = 0, bb = 0;
+        double sum = 0;
+        for (qd_state *t : parts) {
+            if (t->n) {
+                QD_HIP(hipMemcpyAsync(out->objects.as<u8>() + bb, t->objects.p, t->bytes, hipMemcpyDeviceToDevice, s));
+                QD_HIP(hipMemcpyAsync(out->size.as<u32>() + kb, t->size.p, t->n * sizeof(u32), hipMemcpyDeviceToDevice,
+                                      s));
+                QD_HIP(hipMemcpyAsync(out->mag.as<double2>() + kb, t->mag.p, t->n * sizeof(double2),
+                                      hipMemcpyDeviceToDevice, s));
+                hipLaunchKernelGGL(k_add_offset, dim3(grid_for(t->n + 1)), dim3(256), 0, s, t->begin.as<u64>(), t->n + 1,
+                                   bb, out->begin.as<u64>() + kb);
+                QD_HIP(hipGetLastError());
+            }
+            kb += t->n;
+            bb += t->bytes;
+            sum += t->total_proba;
+        }
+        if (K == 0) QD_HIP(hipMemsetAsync(out->begin.p, 0, sizeof(u64), s));
+        QD_HIP(hipStreamSynchronize(s));
+        out->n = K;
+        out->bytes = bytes;
+        out->total_proba = sum;
+        cleanup();
+    }
+    for (u32 i = 0; i <= n; ++i) out_seg[i] = kseg[i];
+    if (stats) {
+        // output bytes per instance: object_begin at the instance boundaries
+        std::vector<u64> ob(n + 1, 0);
+        if (out->n) {
+            QD_TRY(c->bcount.ensure((n + 1) * sizeof(u64)));
+            QD_TRY(upload(c, c->bchunk, kseg.data(), (n + 1) * sizeof(u64)));
+            hipLaunchKernelGGL(k_gather_u64, dim3(grid_for(n + 1)), dim3(256), 0, s, out->begin.as<u64>(),
+                               c->bchunk.as<u64>(), n + 1, c->bcount.as<u64>());
+            QD_HIP(hipGetLastError());
+            QD_TRY(sync_read(c, c->bcount.p, (n + 1) * sizeof(u64), ob.data()));
+        }
+        for (u32 i = 0; i < n; ++i) {
+            qd_step_stats &x = stats[i];
+            std::memset(&x, 0, sizeof x);
+            x.num_parent = x.num_parent_kept = in_seg[i + 1] - in_seg[i];
+            x.num_candidates = plan.cseg[i + 1] - plan.cseg[i];
+            x.num_unique = U[i];
+            x.num_survivors = kseg[i + 1] - kseg[i];
+            x.output_bytes = ob[i + 1] - ob[i];
+            x.total_proba = total[i];
+            x.dedup_levels = st.dedup_levels;
+            x.reserved = st.reserved | (nchunk > 1 ? 2u : 0u);
+        }
+    }
     return QD_OK;
 }
 
@@ -1076,6 +1504,9 @@ int qd_ctx_set_tuning(qd_ctx *c, const qd_tuning *t) {
     }
     if (t->generate_grid) n.generate_grid = t->generate_grid;
     n.generic_path = t->generic_path;  // 0 = fast kernels where they apply
+    if (t->chunk_log2 && (t->chunk_log2 < 10 || t->chunk_log2 > 40))
+        return fail(QD_ERR_INVALID, "chunk_log2 must be 0 or in [10, 40]");
+    n.chunk_log2 = t->chunk_log2;  // 0 = from free device memory
     c->tune = n;
     return QD_OK;
 }
@@ -1357,55 +1788,15 @@ int qd_simulate_batched(qd_ctx *c, qd_state *in, const uint64_t *in_seg, uint32_
     if (!prm.simple_truncation)
         return fail(QD_ERR_UNSUPPORTED, "probabilistic truncation is not implemented; set simple_truncation");
     QD_HIP(hipSetDevice(c->device));
-    QD_TRY(upload(c, c->boseg, in_seg, (n + 1) * sizeof(u64)));
-    QD_TRY(upload(c, c->brules, rules, n * sizeof(qd_rule)));
-    Batch bt;
-    bt.n = n;
-    bt.d_oseg = c->boseg.as<u64>();
-    bt.d_rules = c->brules.as<qd_rule>();
-    bt.caps = max_num_object;
-    qd_step_stats st;
-    std::memset(&st, 0, sizeof st);
-    const Parents P{in->objects.as<u8>(), in->begin.as<u64>(), in->size.as<u32>(), in->mag.as<double2>(), in->n};
-    int rc;
     switch (kind) {
     case QD_RULE_QCGD_ERASE_CREATE:
-        rc = step_impl<QD_RULE_QCGD_ERASE_CREATE, true>(c, P, in->bytes, rules[0], &bt, out, nullptr, UINT64_MAX, prm, st);
-        break;
+        return batched_kind<QD_RULE_QCGD_ERASE_CREATE>(c, in, in_seg, n, rules, out, out_seg, max_num_object, prm, stats);
     case QD_RULE_QCGD_COIN:
-        rc = step_impl<QD_RULE_QCGD_COIN, true>(c, P, in->bytes, rules[0], &bt, out, nullptr, UINT64_MAX, prm, st);
-        break;
+        return batched_kind<QD_RULE_QCGD_COIN>(c, in, in_seg, n, rules, out, out_seg, max_num_object, prm, stats);
     default:
-        rc = step_impl<QD_RULE_QCGD_SPLIT_MERGE, true>(c, P, in->bytes, rules[0], &bt, out, nullptr, UINT64_MAX, prm, st);
-        break;
+        return batched_kind<QD_RULE_QCGD_SPLIT_MERGE>(c, in, in_seg, n, rules, out, out_seg, max_num_object, prm,
+                                                      stats);
     }
-    if (rc) return rc;
-    for (u32 i = 0; i <= n; ++i) out_seg[i] = bt.kseg[i];
-    if (stats) {
-        // output bytes per instance: object_begin at the instance boundaries
-        std::vector<u64> ob(n + 1, 0);
-        if (out->n) {
-            QD_TRY(c->bcount.ensure((n + 1) * sizeof(u64)));
-            QD_TRY(upload(c, c->bchunk, bt.kseg.data(), (n + 1) * sizeof(u64)));
-            hipLaunchKernelGGL(k_gather_u64, dim3(grid_for(n + 1)), dim3(256), 0, c->stream, out->begin.as<u64>(),
-                               c->bchunk.as<u64>(), n + 1, c->bcount.as<u64>());
-            QD_HIP(hipGetLastError());
-            QD_TRY(sync_read(c, c->bcount.p, (n + 1) * sizeof(u64), ob.data()));
-        }
-        for (u32 i = 0; i < n; ++i) {
-            qd_step_stats &x = stats[i];
-            std::memset(&x, 0, sizeof x);
-            x.num_parent = x.num_parent_kept = in_seg[i + 1] - in_seg[i];
-            x.num_candidates = bt.cseg[i + 1] - bt.cseg[i];
-            x.num_unique = bt.U[i];
-            x.num_survivors = bt.kseg[i + 1] - bt.kseg[i];
-            x.output_bytes = ob[i + 1] - ob[i];
-            x.total_proba = bt.total[i];
-            x.dedup_levels = st.dedup_levels;
-            x.reserved = st.reserved;
-        }
-    }
-    return QD_OK;
 }
 
 int qd_simulate_adjoint(qd_ctx *c, qd_state *in, const qd_rule *rule, const qd_rule *dcoef, int ndir, qd_state *out,

@@ -82,3 +82,17 @@ def test_many_small_instances():
     specs = [(int(rng.integers(2, 7)), int(rng.integers(0, 10 ** 6)), float(rng.uniform(0, 1.5)),
               float(rng.uniform(0, 3)), float(rng.uniform(0, 1.5)), float(rng.uniform(0, 3))) for _ in range(256)]
     _run(specs, 2, 256)
+
+
+def test_batch_split_into_chunks():
+    """A candidate budget of 2^10 per launch sequence splits the batch into
+    many chunks (concatenated on the device): results are unchanged."""
+    ctx = q.default_context()
+    rng = np.random.default_rng(7)
+    specs = [(int(rng.integers(3, 8)), int(rng.integers(0, 1000)), float(rng.uniform(0, 1.5)),
+              float(rng.uniform(0, 3)), float(rng.uniform(0, 1.5)), float(rng.uniform(0, 3))) for _ in range(20)]
+    ctx.set_tuning(chunk_log2=10)
+    try:
+        _run(specs, 3, 500)
+    finally:
+        ctx.set_tuning(chunk_log2=0)
