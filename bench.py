@@ -238,6 +238,8 @@ def batched_bench(n_inst=256, n_iter=8, log2_cap=18, seed=4242, lo_nodes=6, hi_n
     # batched
     a, b = batch_t.from_states(states), batch_t()
     bstats = []
+    ctx.set_profiling(True)
+    ctx.reset_phase_times()
     ctx.synchronize()
     t0 = time.perf_counter()
     for it in range(n_iter):
@@ -250,11 +252,14 @@ def batched_bench(n_inst=256, n_iter=8, log2_cap=18, seed=4242, lo_nodes=6, hi_n
                 f"scratch={ctx.scratch_bytes() / 1e9:.1f} GB free={_dev_free_gb():.1f} GB")
     ctx.synchronize()
     t_batch = time.perf_counter() - t0
+    bphase = {k: round(v[0], 2) for k, v in ctx.phase_times().items()}
+    ctx.reset_phase_times()
     C = sum(st["num_candidates"] for step in bstats for st in step)
     out = {"workload": f"batched QCGD, {n_inst} instances, nodes uniform in {lo_nodes}..{hi_nodes}, "
                        f"theta in [0, pi/2], phi in [0, pi] per instance (same angles for both rules), "
                        f"{n_iter} iterations, per-instance cap 2^{log2_cap}, seed {seed}",
-           "candidates": C, "batched_seconds": t_batch, "batched_children_per_s": C / t_batch}
+           "candidates": C, "batched_seconds": t_batch, "batched_children_per_s": C / t_batch,
+           "batched_phase_ms": bphase}
     a.close()
     b.close()
     if sequential:
@@ -281,6 +286,7 @@ def batched_bench(n_inst=256, n_iter=8, log2_cap=18, seed=4242, lo_nodes=6, hi_n
             log(f"sequential iteration {it + 1}")
         ctx.synchronize()
         t_seq = time.perf_counter() - t0
+        out["sequential_phase_ms"] = {k: round(v[0], 2) for k, v in ctx.phase_times().items()}
         mismatch = 0
         for bs, ss in zip(bstats, sstats):
             for x, y in zip(bs, ss):
@@ -290,6 +296,7 @@ def batched_bench(n_inst=256, n_iter=8, log2_cap=18, seed=4242, lo_nodes=6, hi_n
                     "speedup_vs_sequential": t_seq / t_batch, "instance_steps_mismatched": mismatch})
         for it in its:
             it.close()
+    ctx.set_profiling(False)
     return out
 
 
@@ -316,9 +323,10 @@ def main():
     ap.add_argument("--workload", choices=["plain", "adjoint", "batched"], default="plain",
                     help="adjoint: the QCGD adjoint workload (with --gd-steps of gradient descent); "
                          "batched: the 256-instance batch against sequential plain calls")
-    ap.add_argument("--batch-log2-cap", type=int, default=18)
-    ap.add_argument("--batch-iters", type=int, default=8)
+    ap.add_argument("--batch-log2-cap", type=int, default=14)
+    ap.add_argument("--batch-iters", type=int, default=4)
     ap.add_argument("--batch-instances", type=int, default=256)
+    ap.add_argument("--no-sequential", action="store_true", help="batched workload without the sequential run")
     ap.add_argument("--gd-steps", type=int, default=50)
     ap.add_argument("--no-extras", action="store_true", help="skip the adjoint summary of the plain line")
     args = ap.parse_args()
@@ -343,7 +351,8 @@ def main():
         return
 
     if args.workload == "batched":
-        res = batched_bench(n_inst=args.batch_instances, n_iter=args.batch_iters, log2_cap=args.batch_log2_cap)
+        res = batched_bench(n_inst=args.batch_instances, n_iter=args.batch_iters, log2_cap=args.batch_log2_cap,
+                            sequential=not args.no_sequential)
         res.update({"metric": "batched children/s over sequential plain simulate() calls",
                     "value": res.get("speedup_vs_sequential"), "unit": "x", "higher_is_better": True, "n_gpus": 1,
                     "dtype": "f64", "data": "synthetic"})

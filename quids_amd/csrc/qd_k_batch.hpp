@@ -47,69 +47,42 @@ __global__ __launch_bounds__(256) void k_rank_at(const u64 *bitmap, const u64 *w
     }
 }
 
-// Per dedup slot: the instance of its representative (kNoInst for empty
-// slots), and per-instance unique counts.
-__global__ __launch_bounds__(256) void k_seg_count(const u64 *UR, u64 nslots, const u64 *cseg, u32 n, u32 *sinst,
-                                                   u64 *ucount) {
+// flags[i] = 1 when instance i holds a graph of more than 64 nodes (the
+// batched kernels' limit; such an instance steps on its own).
+__global__ __launch_bounds__(256) void k_wide_flags(const u8 *obj, const u64 *begin, const u32 *size, u64 N,
+                                                    const u64 *oseg, u32 n, u32 *flags) {
+    for (u64 p = u64(blockIdx.x) * blockDim.x + threadIdx.x; p < N; p += u64(gridDim.x) * blockDim.x)
+        if (size[p] >= 2 && ld16(obj + begin[p]) > 64) flags[find_seg(oseg, n, p)] = 1;
+}
+
+// Instance of a dedup slot's representative (kNoInst for empty slots).
+__device__ inline u32 slot_inst(const u64 *UR, u64 s, const u64 *cseg, u32 n) {
+    const u64 r = UR[s];
+    return r == kNoRep ? kNoInst : find_seg(cseg, n, r);
+}
+
+// Per-instance unique counts.
+__global__ __launch_bounds__(256) void k_seg_count(const u64 *UR, u64 nslots, const u64 *cseg, u32 n, u64 *ucount) {
     __shared__ u32 cnt[kMaxInst];
     for (u32 i = threadIdx.x; i < n; i += blockDim.x) cnt[i] = 0;
     __syncthreads();
     for (u64 s = u64(blockIdx.x) * blockDim.x + threadIdx.x; s < nslots; s += u64(gridDim.x) * blockDim.x) {
-        const u64 r = UR[s];
-        u32 inst = kNoInst;
-        if (r != kNoRep) {
-            inst = find_seg(cseg, n, r);
-            atomicAdd(&cnt[inst], 1u);
-        }
-        sinst[s] = inst;
+        const u32 inst = slot_inst(UR, s, cseg, n);
+        if (inst != kNoInst) atomicAdd(&cnt[inst], 1u);
     }
     __syncthreads();
     for (u32 i = threadIdx.x; i < n; i += blockDim.x)
         if (cnt[i]) atomicAdd(reinterpret_cast<unsigned long long *>(ucount + i), (unsigned long long)cnt[i]);
 }
 
-// Groups the slots of truncated instances by instance: each block ranks its
-// tile's slots per instance in LDS and reserves one range per instance.
-constexpr int kSegTile = 4096;
-__global__ __launch_bounds__(256) void k_seg_scatter(const u32 *sinst, u64 nslots, const u32 *trunc, u32 n,
-                                                     u64 *cursor, u64 *list) {
-    __shared__ u32 cnt[kMaxInst];
-    __shared__ u64 base[kMaxInst];
-    constexpr int kPer = kSegTile / 256;
-    for (u64 t0 = u64(blockIdx.x) * kSegTile; t0 < nslots; t0 += u64(gridDim.x) * kSegTile) {
-        for (u32 i = threadIdx.x; i < n; i += blockDim.x) cnt[i] = 0;
-        __syncthreads();
-        u32 inst[kPer], rank[kPer];
-#pragma unroll
-        for (int q = 0; q < kPer; ++q) {
-            const u64 s = t0 + threadIdx.x + u64(q) * 256;
-            inst[q] = kNoInst;
-            if (s < nslots) {
-                const u32 i = sinst[s];
-                if (i != kNoInst && trunc[i]) {
-                    inst[q] = i;
-                    rank[q] = atomicAdd(&cnt[i], 1u);
-                }
-            }
-        }
-        __syncthreads();
-        for (u32 i = threadIdx.x; i < n; i += blockDim.x)
-            if (cnt[i]) base[i] = atomicAdd(reinterpret_cast<unsigned long long *>(cursor + i), (unsigned long long)cnt[i]);
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < kPer; ++q)
-            if (inst[q] != kNoInst) list[base[inst[q]] + rank[q]] = t0 + threadIdx.x + u64(q) * 256;
-        __syncthreads();
-    }
-}
-
 // Radix-select state of one instance: the K-th largest 128-bit key
-// (|m|^2 bits, ~hash) is being located 12 bits at a time; `prefix` holds
-// the `pos` bits found so far (lower bits zero), r how many keys with that
-// prefix are still to be kept.  When active is 0 the instance keeps every
-// key >= prefix (keepall: every key).
+// (|m|^2 bits, ~hash) is being located 12 bits at a time; `prefix` (hi, lo)
+// holds the `pos` bits found so far (lower bits zero), r how many keys with
+// that prefix are still to be kept, tied how many keys carry that prefix.
+// When active is 0 the instance keeps every key >= prefix (keepall: every
+// key).
 struct SegSel {
-    u64 hi, lo, r;
+    u64 hi, lo, r, tied;
     u32 pos, active, keepall, err;
 };
 
@@ -125,16 +98,60 @@ __device__ inline bool prefix_eq(u64 hi, u64 lo, const SegSel &q) {
     return hi == q.hi && (lo >> (128 - q.pos)) == (q.lo >> (128 - q.pos));
 }
 
-// Tile of the grouped slot list: items [begin, end) of instance inst.
+// Groups the uniques of truncated instances by instance as key records
+// {hi, lo, slot} (read once, in slot order): each block ranks its tile's
+// slots per instance in LDS and reserves one range per instance.
+constexpr int kSegTile = 4096;
+__global__ __launch_bounds__(256) void k_seg_scatter(const u64 *UR, const u64 *cseg, u64 nslots, const u64 *UH,
+                                                     const double2 *UM, const u32 *trunc, u32 n, u64 *cursor,
+                                                     SelItem *list) {
+    __shared__ u32 cnt[kMaxInst];
+    __shared__ u64 base[kMaxInst];
+    constexpr int kPer = kSegTile / 256;
+    for (u64 t0 = u64(blockIdx.x) * kSegTile; t0 < nslots; t0 += u64(gridDim.x) * kSegTile) {
+        for (u32 i = threadIdx.x; i < n; i += blockDim.x) cnt[i] = 0;
+        __syncthreads();
+        u32 inst[kPer], rank[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const u64 s = t0 + threadIdx.x + u64(q) * 256;
+            inst[q] = kNoInst;
+            if (s < nslots) {
+                const u32 i = slot_inst(UR, s, cseg, n);
+                if (i != kNoInst && trunc[i]) {
+                    inst[q] = i;
+                    rank[q] = atomicAdd(&cnt[i], 1u);
+                }
+            }
+        }
+        __syncthreads();
+        for (u32 i = threadIdx.x; i < n; i += blockDim.x)
+            if (cnt[i]) base[i] = atomicAdd(reinterpret_cast<unsigned long long *>(cursor + i), (unsigned long long)cnt[i]);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kPer; ++q)
+            if (inst[q] != kNoInst) {
+                const u64 s = t0 + threadIdx.x + u64(q) * 256;
+                SelItem it;
+                seg_key(UH, UM, s, inst[q], it.hi, it.lo);
+                it.slot = s;
+                list[base[inst[q]] + rank[q]] = it;
+            }
+        __syncthreads();
+    }
+}
+
+// Tile of a key-record list: records [begin, end) of instance inst.
 struct SegTile {
     u64 begin, end;
     u32 inst, pad;
 };
 
-// One workgroup per tile: LDS histogram of the next 12 key bits among the
-// keys matching the instance's prefix, added into the instance's global row.
-__global__ __launch_bounds__(256) void k_seg_hist(const u64 *list, const SegTile *tiles, u32 ntiles, const u64 *UH,
-                                                  const double2 *UM, const SegSel *sel, u32 *hist) {
+// One workgroup per tile of an active instance: LDS histogram of the 12
+// key bits at the instance's position, added into its global row.  Every
+// record of the list carries the instance's current prefix.
+__global__ __launch_bounds__(256) void k_seg_hist(const SelItem *list, const SegTile *tiles, u32 ntiles,
+                                                  const SegSel *sel, u32 *hist) {
     __shared__ u32 h[kHistBins];
     for (u32 t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const SegTile tl = tiles[t];
@@ -143,15 +160,44 @@ __global__ __launch_bounds__(256) void k_seg_hist(const u64 *list, const SegTile
         for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) h[i] = 0;
         __syncthreads();
         for (u64 k = tl.begin + threadIdx.x; k < tl.end; k += blockDim.x) {
-            u64 hi, lo;
-            seg_key(UH, UM, list[k], tl.inst, hi, lo);
-            if (prefix_eq(hi, lo, q)) atomicAdd(&h[key_digit(hi, lo, q.pos)], 1u);
+            const SelItem it = list[k];
+            atomicAdd(&h[key_digit(it.hi, it.lo, q.pos)], 1u);
         }
         __syncthreads();
         u32 *row = hist + u64(tl.inst) * kHistBins;
         for (int i = threadIdx.x; i < kHistBins; i += blockDim.x)
             if (h[i]) atomicAdd(&row[i], h[i]);
         __syncthreads();
+    }
+}
+
+// After a pick: carries the records that match the instance's new prefix
+// (the tied ones) into the next list, at the instance's cursor; one
+// reservation per tile.
+__global__ __launch_bounds__(256) void k_seg_collect(const SelItem *list, const SegTile *tiles, u32 ntiles,
+                                                     const SegSel *sel, u64 *cursor, SelItem *out) {
+    __shared__ u64 tmp[256 / kWave + 1];
+    __shared__ u64 base;
+    for (u32 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const SegTile tl = tiles[t];
+        const SegSel q = sel[tl.inst];
+        if (!q.active) continue;  // uniform over the block
+        for (u64 k0 = tl.begin; k0 < tl.end; k0 += blockDim.x) {
+            const u64 k = k0 + threadIdx.x;
+            bool take = false;
+            SelItem it{};
+            if (k < tl.end) {
+                it = list[k];
+                take = prefix_eq(it.hi, it.lo, q);
+            }
+            u64 total;
+            const u64 off = block_excl_scan(take ? 1 : 0, tmp, total);
+            if (threadIdx.x == 0 && total)
+                base = atomicAdd(reinterpret_cast<unsigned long long *>(cursor + tl.inst), (unsigned long long)total);
+            __syncthreads();
+            if (take) out[base + off] = it;
+            __syncthreads();
+        }
     }
 }
 
@@ -175,7 +221,7 @@ __global__ __launch_bounds__(256) void k_seg_pick(SegSel *sel, u32 *hist, u32 n,
     for (int j = 0; j < kPer; ++j) local += row[kHistBins - 1 - (t * kPer + j)];
     part[t] = local;
     __syncthreads();
-    // exclusive prefix over threads in order (bins descending)
+    // inclusive prefix over threads in order (bins descending)
     for (u32 d = 1; d < 256; d <<= 1) {
         const u64 v = t >= d ? part[t - d] : 0;
         __syncthreads();
@@ -222,6 +268,7 @@ __global__ __launch_bounds__(256) void k_seg_pick(SegSel *sel, u32 *hist, u32 n,
         }
         q.pos = min(q.pos + 12, 128u);
         q.r -= found_above;
+        q.tied = found_cnt;
         if (found_cnt == q.r) q.active = 0;
         else if (q.pos >= 128) {
             q.active = 0;
@@ -233,14 +280,15 @@ __global__ __launch_bounds__(256) void k_seg_pick(SegSel *sel, u32 *hist, u32 n,
 
 // Marks the survivors: every unique of an instance that is not truncated,
 // and the keys >= prefix of a truncated one.  Lanes hold consecutive slots.
-__global__ __launch_bounds__(256) void k_seg_mark(const u32 *sinst, const u64 *UH, const u64 *UR, const double2 *UM,
-                                                  u64 nslots, const SegSel *sel, u64 *bitmap, u64 *sbm) {
+__global__ __launch_bounds__(256) void k_seg_mark(const u64 *cseg, u32 n, const u64 *UH, const u64 *UR,
+                                                  const double2 *UM, u64 nslots, const SegSel *sel, u64 *bitmap,
+                                                  u64 *sbm) {
     const u64 stride = u64(gridDim.x) * blockDim.x;
     for (u64 base = u64(blockIdx.x) * blockDim.x; base < nslots; base += stride) {
         const u64 s = base + threadIdx.x;
         bool keep = false;
         if (s < nslots) {
-            const u32 inst = sinst[s];
+            const u32 inst = slot_inst(UR, s, cseg, n);
             if (inst != kNoInst) {
                 const SegSel q = sel[inst];
                 if (q.keepall) keep = true;

@@ -83,17 +83,26 @@ static int fail(int code, const std::string &msg) {
 // ===========================================================================
 // device buffers
 // ===========================================================================
+constexpr size_t kBigAlloc = 48ull << 30;
+
 struct DBuf {
     void *p = nullptr;
     size_t cap = 0;
     int ensure(size_t bytes) {
         if (bytes <= cap && p) return QD_OK;
+        const size_t old = cap;
         if (p) {
             (void)hipFree(p);
             p = nullptr;
             cap = 0;
         }
+        // a buffer that grows takes at least 1.5x its old size, so steps of
+        // varying size settle on few reallocations; but no more than
+        // kBigAlloc unless asked for it: on MI355X, allocations of 64 GB and
+        // more take seconds to map and to free, smaller ones about 1 ms
         size_t want = std::max<size_t>(bytes + bytes / 8, 256);
+        if (old) want = std::max(want, old + old / 2);
+        want = std::min(want, std::max(bytes, kBigAlloc));
         hipError_t e = hipMalloc(&p, want);
         if (e != hipSuccess) {
             (void)hipGetLastError();
@@ -146,14 +155,22 @@ struct qd_ctx {
     DBuf biglist, bigsoff, bigkey, bigcand, bigmag, bigseg;
     DBuf pdesc, wi, wb, ebuf2, cnt2, slist;
     DBuf adjk, adjv, adjg, adjd, adjp;
-    DBuf pool0, pool1, pwpre;
+    DBuf pool0, pool1, pwpre, blist2;
+    // output buffers of batched chunks (objects, begin, size, mag per chunk),
+    // kept between calls
+    std::vector<DBuf> parts;
     DBuf bseg, boseg, brules, bsinst, bcount, bsel, btrunc, blist, bcur, btile, bhist, bkseg, bchunk, bcsum, btot;
     std::vector<DBuf *> all() {
+        std::vector<DBuf *> v = all_named();
+        for (DBuf &b : parts) v.push_back(&b);
+        return v;
+    }
+    std::vector<DBuf *> all_named() {
         return {&nc, &cb, &pre, &tab, &kept, &keys, &ebuf0, &ebuf1, &hist1, &off1, &cur1, &off2,
                 &nhist, &counters, &sel0, &sel1, &bitmap, &wprefix, &padded, &chunks, &scanblk,
                 &biglist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &ebuf2, &cnt2, &slist,
                 &adjk, &adjv, &adjg, &adjd, &adjp, &bseg, &boseg, &brules, &bsinst, &bcount, &bsel, &btrunc,
-                &blist, &bcur, &btile, &bhist, &bkseg, &bchunk, &bcsum, &btot, &pool0, &pool1, &pwpre};
+                &blist, &bcur, &btile, &bhist, &bkseg, &bchunk, &bcsum, &btot, &pool0, &pool1, &pwpre, &blist2};
     }
 };
 
@@ -420,6 +437,37 @@ static void dedup_layout(u64 C, const qd_tuning &t, u32 &b1, u32 &b2) {
     b2 = L - b1;
 }
 
+// Largest candidate count one pass of a step (a batched chunk or a hash
+// range) may take: 2^chunk_log2,
+// or what free device memory (plus the context's cached scratch) holds at
+// about 150 bytes of scratch per candidate (entries, partition buffers, dedup
+// output, bitmaps, growth slack), with room left for the states and for
+// buffers that grow one at a time.
+static u64 step_budget(qd_ctx *ctx) {
+    if (ctx->tune.chunk_log2) return 1ull << ctx->tune.chunk_log2;
+    size_t freeb = 0, totalb = 0;
+    if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) {
+        (void)hipGetLastError();
+        return 1ull << 28;
+    }
+    u64 cached = 0;
+    for (DBuf *b : ctx->all()) cached += b->cap;
+    // the largest buffer (the second partition pass, about 1.7 x 32 bytes per
+    // candidate) stays within kBigAlloc
+    const u64 by_alloc = u64(double(kBigAlloc) / (32.0 * 1.8));
+    return std::max<u64>(std::min<u64>(u64((double(freeb) + double(cached)) * 0.45 / 150.0), by_alloc), 1ull << 20);
+}
+
+// Grows a per-candidate scratch buffer; once a step needs an eighth of what
+// the pass budget needs (`full`), it takes `full` at once, so large runs map
+// each big buffer a single time (large allocations and frees cost up to
+// seconds on a busy device).
+static int ensure_big(DBuf &b, u64 bytes, u64 full) {
+    if (bytes <= b.cap && b.p) return QD_OK;
+    if (full > bytes && bytes >= full / 8) bytes = full;
+    return b.ensure(bytes);
+}
+
 // Partition (one or two radix passes on the top key bits) and dedup of n
 // candidates with keys H0 and magnitudes M0 (CI: candidate index of each
 // position, null when the position is the candidate index).  hist1 holds the
@@ -463,7 +511,8 @@ static int partition_dedup(qd_ctx *ctx, const u64 *H0, const double2 *M0, const 
         const u64 cap2 = ((mean + mean / 2 + 128) + 63) / 64 * 64;
         QD_TRY(ctx->off2.ensure((nb + 1) * sizeof(u64)));
         QD_TRY(ctx->cnt2.ensure(nb * sizeof(u64)));
-        QD_TRY(ctx->ebuf2.ensure(std::max<u64>(nb * cap2, n) * sizeof(Entry)));
+        QD_TRY(ensure_big(ctx->ebuf2, std::max<u64>(nb * cap2, n) * sizeof(Entry),
+                          u64(double(step_budget(ctx)) * 1.8) * sizeof(Entry)));
         Entry *E2 = ctx->ebuf2.as<Entry>();
         hipLaunchKernelGGL(k_level2_fixed, dim3(nb1), dim3(1024), 0, s, (const Entry *)E1, ctx->off1.as<u64>(), b1, b2,
                            cap2, E2, ctx->cnt2.as<u64>(), errw);
@@ -543,23 +592,6 @@ static int populate_out(qd_ctx *ctx, const Parents &P, const qd_rule &rule, cons
     return QD_OK;
 }
 
-// Largest candidate count one pass of a step (a batched chunk or a hash
-// range) may take: 2^chunk_log2,
-// or what free device memory (plus the context's cached scratch) holds at
-// about 150 bytes of scratch per candidate (entries, partition buffers, dedup
-// output, bitmaps, growth slack), with room left for the states and for
-// buffers that grow one at a time.
-static u64 step_budget(qd_ctx *ctx) {
-    if (ctx->tune.chunk_log2) return 1ull << ctx->tune.chunk_log2;
-    size_t freeb = 0, totalb = 0;
-    if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) {
-        (void)hipGetLastError();
-        return 1ull << 28;
-    }
-    u64 cached = 0;
-    for (DBuf *b : ctx->all()) cached += b->cap;
-    return std::max<u64>(u64((double(freeb) + double(cached)) * 0.45 / 150.0), 1ull << 20);
-}
 
 // ===========================================================================
 // hash-range passes (qd_k_range.hpp): steps whose candidates exceed the
@@ -760,34 +792,33 @@ static int upload(qd_ctx *ctx, DBuf &b, const void *src, size_t bytes) {
 }
 
 // Per-instance top-K over the nslots dedup output slots; marks survivors.
+// The keys of truncated instances are grouped by instance once; each radix
+// round histograms the current records and carries only the ones tied at
+// the chosen digit into the next round.
 static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const u64 *UR, const double2 *UM,
                         u64 *bitmap, u64 *sbm) {
     hipStream_t s = ctx->stream;
     const u32 n = bt->n;
-    QD_TRY(ctx->bsinst.ensure(nslots * sizeof(u32)));
     QD_TRY(ctx->bcount.ensure(n * sizeof(u64)));
-    u32 *sinst = ctx->bsinst.as<u32>();
+    const u64 *cseg = ctx->bseg.as<u64>();
     QD_HIP(hipMemsetAsync(ctx->bcount.p, 0, n * sizeof(u64), s));
-    hipLaunchKernelGGL(k_seg_count, dim3(grid_for(nslots, 256, 8192)), dim3(256), 0, s, UR, nslots,
-                       ctx->bseg.as<u64>(), n, sinst, ctx->bcount.as<u64>());
+    hipLaunchKernelGGL(k_seg_count, dim3(grid_for(nslots, 256, 8192)), dim3(256), 0, s, UR, nslots, cseg, n,
+                       ctx->bcount.as<u64>());
     QD_HIP(hipGetLastError());
     bt->U.resize(n);
     QD_TRY(sync_read(ctx, ctx->bcount.p, n * sizeof(u64), bt->U.data()));
     std::vector<SegSel> sel(n);
     std::vector<u32> trunc(n, 0);
-    std::vector<u64> loff(n, 0);
-    std::vector<SegTile> tiles;
+    std::vector<u64> off(n, 0), cnt(n, 0);
     u64 ntrunc = 0;
-    constexpr u64 kTileItems = 16384;
     for (u32 i = 0; i < n; ++i) {
         SegSel q{};
         if (bt->U[i] > bt->caps[i]) {
             q.r = bt->caps[i];
             q.active = 1;
             trunc[i] = 1;
-            loff[i] = ntrunc;
-            for (u64 b = 0; b < bt->U[i]; b += kTileItems)
-                tiles.push_back(SegTile{ntrunc + b, ntrunc + std::min(bt->U[i], b + kTileItems), i, 0});
+            off[i] = ntrunc;
+            cnt[i] = bt->U[i];
             ntrunc += bt->U[i];
         } else {
             q.keepall = 1;
@@ -798,32 +829,59 @@ static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const
     SegSel *dsel = ctx->bsel.as<SegSel>();
     if (ntrunc) {
         QD_TRY(upload(ctx, ctx->btrunc, trunc.data(), n * sizeof(u32)));
-        QD_TRY(upload(ctx, ctx->bcur, loff.data(), n * sizeof(u64)));
-        QD_TRY(upload(ctx, ctx->btile, tiles.data(), tiles.size() * sizeof(SegTile)));
-        QD_TRY(ctx->blist.ensure(ntrunc * sizeof(u64)));
+        QD_TRY(upload(ctx, ctx->bcur, off.data(), n * sizeof(u64)));
+        // the key records go where the partition's entries were (dead by now):
+        // the first list in the first pass's buffer (32 bytes per slot), the
+        // next rounds' in the second pass's
+        QD_TRY(ctx->ebuf2.ensure(ntrunc * sizeof(SelItem)));
         QD_TRY(ctx->bhist.ensure(u64(n) * kHistBins * sizeof(u32)));
         QD_TRY(ctx->counters.ensure(64 * sizeof(u64)));
         u32 *more = reinterpret_cast<u32 *>(ctx->counters.as<u64>() + 10);
+        SelItem *cur = ctx->ebuf1.as<SelItem>(), *nxt = ctx->ebuf2.as<SelItem>();
         const unsigned sg = unsigned(std::min<u64>((nslots + kSegTile - 1) / kSegTile, 16384));
-        hipLaunchKernelGGL(k_seg_scatter, dim3(std::max(sg, 1u)), dim3(256), 0, s, sinst, nslots,
-                           ctx->btrunc.as<u32>(), n, ctx->bcur.as<u64>(), ctx->blist.as<u64>());
+        hipLaunchKernelGGL(k_seg_scatter, dim3(std::max(sg, 1u)), dim3(256), 0, s, UR, cseg, nslots, UH, UM,
+                           ctx->btrunc.as<u32>(), n, ctx->bcur.as<u64>(), cur);
         QD_HIP(hipMemsetAsync(ctx->bhist.p, 0, u64(n) * kHistBins * sizeof(u32), s));
-        const u32 nt = u32(tiles.size());
+        constexpr u64 kTileItems = 16384;
+        std::vector<SegTile> tiles;
         for (int round = 0; round < 12; ++round) {
+            tiles.clear();
+            for (u32 i = 0; i < n; ++i)
+                if (sel[i].active)
+                    for (u64 b = 0; b < cnt[i]; b += kTileItems)
+                        tiles.push_back(SegTile{off[i] + b, off[i] + std::min(cnt[i], b + kTileItems), i, 0});
+            if (tiles.empty()) break;
+            QD_TRY(upload(ctx, ctx->btile, tiles.data(), tiles.size() * sizeof(SegTile)));
+            const u32 nt = u32(tiles.size());
             QD_HIP(hipMemsetAsync(more, 0, sizeof(u32), s));
-            hipLaunchKernelGGL(k_seg_hist, dim3(std::min<u32>(nt, 65536)), dim3(256), 0, s, ctx->blist.as<u64>(),
-                               ctx->btile.as<SegTile>(), nt, UH, UM, (const SegSel *)dsel, ctx->bhist.as<u32>());
+            hipLaunchKernelGGL(k_seg_hist, dim3(std::min<u32>(nt, 65536)), dim3(256), 0, s, (const SelItem *)cur,
+                               ctx->btile.as<SegTile>(), nt, (const SegSel *)dsel, ctx->bhist.as<u32>());
             hipLaunchKernelGGL(k_seg_pick, dim3(n), dim3(256), 0, s, dsel, ctx->bhist.as<u32>(), n, more);
             QD_HIP(hipGetLastError());
-            u32 m = 0;
-            QD_TRY(sync_read(ctx, more, sizeof(u32), &m));
-            if (!m) break;
+            QD_TRY(sync_read(ctx, dsel, n * sizeof(SegSel), sel.data()));
+            u64 next = 0;
+            bool any = false;
+            for (u32 i = 0; i < n; ++i) {
+                if (sel[i].active) {
+                    off[i] = next;
+                    cnt[i] = sel[i].tied;
+                    next += cnt[i];
+                    any = true;
+                } else {
+                    cnt[i] = 0;
+                }
+            }
+            if (!any) break;
+            QD_TRY(upload(ctx, ctx->bcur, off.data(), n * sizeof(u64)));
+            hipLaunchKernelGGL(k_seg_collect, dim3(std::min<u32>(nt, 65536)), dim3(256), 0, s, (const SelItem *)cur,
+                               ctx->btile.as<SegTile>(), nt, (const SegSel *)dsel, ctx->bcur.as<u64>(), nxt);
+            QD_HIP(hipGetLastError());
+            std::swap(cur, nxt);
         }
-        QD_TRY(sync_read(ctx, dsel, n * sizeof(SegSel), sel.data()));
         for (u32 i = 0; i < n; ++i)
             if (sel[i].err || sel[i].active) return fail(QD_ERR_HIP, "batched select did not converge");
     }
-    hipLaunchKernelGGL(k_seg_mark, dim3(grid_for(nslots, 256, 16384)), dim3(256), 0, s, sinst, UH, UR, UM, nslots,
+    hipLaunchKernelGGL(k_seg_mark, dim3(grid_for(nslots, 256, 16384)), dim3(256), 0, s, cseg, n, UH, UR, UM, nslots,
                        (const SegSel *)dsel, bitmap, sbm);
     QD_HIP(hipGetLastError());
     return QD_OK;
@@ -1019,8 +1077,11 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
     u32 b1, b2;
     dedup_layout(C, ctx->tune, b1, b2);
     st.dedup_levels = (b1 ? 1 : 0) + (b2 ? 1 : 0);
-    QD_TRY(ctx->ebuf0.ensure(C * 32));
-    QD_TRY(ctx->ebuf1.ensure(C * 32));
+    {
+        const u64 full = step_budget(ctx) * 32;
+        QD_TRY(ensure_big(ctx->ebuf0, C * 32, full));
+        QD_TRY(ensure_big(ctx->ebuf1, C * 32, full));
+    }
     QD_TRY(ctx->hist1.ensure(kHistBins * sizeof(u64)));
     QD_TRY(ctx->off1.ensure((kHistBins + 1) * sizeof(u64)));
     QD_TRY(ctx->cur1.ensure(kHistBins * sizeof(u64)));
@@ -1218,32 +1279,39 @@ __global__ __launch_bounds__(256) void k_add_offset(const u64 *src, u64 n, u64 a
         dst[i] = src[i] + add;
 }
 
-// Candidates per instance of the whole batch (prep + scan only): cseg.
+// Candidates per instance of the whole batch (prep + scan only): cseg;
+// wide[i] = 1 for the instances holding graphs beyond the batched kernels'
+// 64 nodes.
 template <int KIND>
-static int batch_plan(qd_ctx *ctx, const Parents &P, u64 in_bytes, Batch *bt) {
+static int batch_plan(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule &rule, Batch *bt,
+                      std::vector<u32> &wide) {
     hipStream_t s = ctx->stream;
     const u64 N = P.n;
+    wide.assign(bt->n, 0);
     if (N == 0) {
         bt->cseg.assign(bt->n + 1, 0);
         return QD_OK;
     }
     QD_TRY(ctx->nc.ensure(N * sizeof(u32)));
     QD_TRY(ctx->cb.ensure((N + 1) * sizeof(u64)));
+    QD_TRY(ctx->pre.ensure(N * sizeof(u64)));
     QD_TRY(ctx->counters.ensure(64 * sizeof(u64)));
-    QD_TRY(ctx->pdesc.ensure(N * sizeof(PDesc)));
-    QD_TRY(ctx->wi.ensure(N * sizeof(u32)));
+    QD_TRY(ctx->btrunc.ensure(bt->n * sizeof(u32)));
     if (KIND == QD_RULE_QCGD_SPLIT_MERGE) QD_TRY(ctx->tab.ensure((in_bytes / 2 + 16) * sizeof(u64)));
     u64 *counters = ctx->counters.as<u64>();
     QD_HIP(hipMemsetAsync(counters, 0, 64 * sizeof(u64), s));
-    hipLaunchKernelGGL(k_prep_fast<KIND>, dim3(grid_for(N, 4, 65536)), dim3(256), 0, s, P, (const u64 *)nullptr,
-                       ctx->pdesc.as<PDesc>(), ctx->nc.as<u32>(), ctx->wi.as<u32>(), ctx->tab.as<u64>(),
-                       reinterpret_cast<u32 *>(counters + 3), reinterpret_cast<u32 *>(counters + 4));
+    QD_HIP(hipMemsetAsync(ctx->btrunc.p, 0, bt->n * sizeof(u32), s));
+    hipLaunchKernelGGL(k_prep<KIND>, dim3(grid_for(N)), dim3(256), 0, s, P, (const u64 *)nullptr, rule,
+                       ctx->nc.as<u32>(), ctx->pre.as<u64>(), ctx->tab.as<u64>(), reinterpret_cast<u32 *>(counters + 3),
+                       reinterpret_cast<u32 *>(counters + 4));
+    hipLaunchKernelGGL(k_wide_flags, dim3(grid_for(N)), dim3(256), 0, s, P.obj, P.begin, P.size, N, bt->d_oseg, bt->n,
+                       ctx->btrunc.as<u32>());
     QD_HIP(hipGetLastError());
     QD_TRY(run_scan(ctx, LoadU32{ctx->nc.as<u32>()}, N, ctx->cb.as<u64>()));
     u64 hb[2];
     QD_TRY(sync_read(ctx, counters + 3, 2 * sizeof(u64), hb));
-    if (u32(hb[1]) & ERR_WIDE) return fail(QD_ERR_UNSUPPORTED, "the batched step handles graphs of at most 64 nodes");
     QD_TRY(check_err_word(u32(hb[1])));
+    QD_TRY(sync_read(ctx, ctx->btrunc.p, bt->n * sizeof(u32), wide.data()));
     return batch_cseg(ctx, bt);
 }
 
@@ -1257,18 +1325,28 @@ static int batched_kind(qd_ctx *c, qd_state *in, const u64 *in_seg, u32 n, const
     Batch plan;
     plan.n = n;
     plan.d_oseg = c->boseg.as<u64>();
-    QD_TRY(batch_plan<KIND>(c, P, in->bytes, &plan));
-    // chunks of consecutive instances within the candidate budget
+    std::vector<u32> wide;
+    prof_begin(c, QD_PHASE_PREP);
+    const int prc = batch_plan<KIND>(c, P, in->bytes, rules[0], &plan, wide);
+    prof_end(c, QD_PHASE_PREP);
+    QD_TRY(prc);
+    // chunks of consecutive instances within the candidate budget; an
+    // instance beyond the budget or with wide graphs steps alone
     const u64 budget = step_budget(c);
     std::vector<u32> cuts{0};
     u64 acc = 0;
     for (u32 i = 0; i < n; ++i) {
         const u64 ci = plan.cseg[i + 1] - plan.cseg[i];
-        if (acc && acc + ci > budget) {
+        const bool solo = wide[i] || ci > budget;
+        if (acc && (solo || acc + ci > budget)) {
             cuts.push_back(i);
             acc = 0;
         }
         acc += ci;
+        if (solo && i + 1 < n) {
+            cuts.push_back(i + 1);
+            acc = 0;
+        }
     }
     cuts.push_back(n);
     const size_t nchunk = cuts.size() - 1;
@@ -1277,17 +1355,24 @@ static int batched_kind(qd_ctx *c, qd_state *in, const u64 *in_seg, u32 n, const
     std::vector<double> total(n, 0.0);
     qd_step_stats st;
     std::memset(&st, 0, sizeof st);
+    // chunk outputs live in buffers the context keeps between calls
+    if (nchunk > 1 && c->parts.size() < 4 * nchunk) c->parts.resize(4 * nchunk);
+    std::vector<qd_state> partv(nchunk > 1 ? nchunk : 0);
+    for (size_t k = 0; k < partv.size(); ++k) {
+        partv[k].ctx = c;
+        partv[k].objects = c->parts[4 * k];
+        partv[k].begin = c->parts[4 * k + 1];
+        partv[k].size = c->parts[4 * k + 2];
+        partv[k].mag = c->parts[4 * k + 3];
+    }
     std::vector<qd_state *> parts;
-    auto cleanup = [&]() {
-        for (qd_state *t : parts) {
-            t->objects.release();
-            t->begin.release();
-            t->size.release();
-            t->mag.release();
-            delete t;
-        }
-        parts.clear();
+    auto give_back = [&](size_t k) {  // the (possibly regrown) buffers of chunk k
+        c->parts[4 * k] = partv[k].objects;
+        c->parts[4 * k + 1] = partv[k].begin;
+        c->parts[4 * k + 2] = partv[k].size;
+        c->parts[4 * k + 3] = partv[k].mag;
     };
+    auto cleanup = [&]() { parts.clear(); };
     for (size_t k = 0; k < nchunk; ++k) {
         const u32 i0 = cuts[k], i1 = cuts[k + 1];
         const u64 p0 = in_seg[i0], p1 = in_seg[i1];
@@ -1306,8 +1391,7 @@ static int batched_kind(qd_ctx *c, qd_state *in, const u64 *in_seg, u32 n, const
         const Parents Pk{P.obj, P.begin + p0, P.size + p0, P.mag + p0, p1 - p0};
         qd_state *dst = out;
         if (nchunk > 1) {
-            dst = new qd_state;
-            dst->ctx = c;
+            dst = &partv[k];
             parts.push_back(dst);
         }
         if (bt.n == 1) {
@@ -1320,6 +1404,7 @@ static int batched_kind(qd_ctx *c, qd_state *in, const u64 *in_seg, u32 n, const
         } else {
             rc = step_impl<KIND, true>(c, Pk, in->bytes, rules[i0], &bt, dst, nullptr, UINT64_MAX, prm, st);
         }
+        if (nchunk > 1) give_back(k);
         if (rc) {
             cleanup();
             return rc;

@@ -1,0 +1,45 @@
+"""The autotuner: picks a tuning from its grid, sets it on the context, and
+every tuning gives bit-identical steps."""
+import math
+
+import numpy as np
+import pytest
+
+import quids_amd as q
+from quids_amd import tuning
+from quids_amd.rules import qcgd
+from quids_amd.state import HostState
+
+pytestmark = pytest.mark.gpu
+
+
+def _state():
+    it, nxt = q.it_t(), q.it_t()
+    it.upload(HostState.from_objects([qcgd.seeded_graph(8, 5)]))
+    q.simulate(it, qcgd.split_merge(math.pi / 4), nxt, None, q.UINT64_MAX)
+    return nxt
+
+
+def test_autotune_sets_a_grid_point_and_keeps_results():
+    ctx = q.default_context()
+    before = ctx.get_tuning()
+    st = _state()
+    rules = [qcgd.erase_create(math.pi / 3, math.pi / 5)]
+    grid = {"dedup_bucket_target": (256, 768), "first_pass_bits": (6, 12)}
+    try:
+        ref = q.it_t()
+        q.simulate(st, rules[0], ref, None, 5000)
+        res = tuning.autotune(ctx, st, rules, 5000, grid=grid, reps=1, key="test")
+        assert res["tried"] == 4
+        t = ctx.get_tuning()
+        for k, v in res["tuning"].items():
+            assert t[k] == v and v in grid[k]
+        got = q.it_t()
+        q.simulate(st, rules[0], got, None, 5000)
+        a, b = ref.download(), got.download()
+        assert np.array_equal(a.objects, b.objects) and np.array_equal(a.magnitude, b.magnitude)
+        # memoised per (device, key)
+        again = tuning.autotune(ctx, st, rules, 5000, grid=grid, reps=1, key="test")
+        assert again is res
+    finally:
+        ctx.set_tuning(**{k: before[k] for k in ("dedup_bucket_target", "first_pass_bits", "generate_grid")})
