@@ -329,6 +329,9 @@ def main():
     ap.add_argument("--no-sequential", action="store_true", help="batched workload without the sequential run")
     ap.add_argument("--gd-steps", type=int, default=50)
     ap.add_argument("--no-extras", action="store_true", help="skip the adjoint summary of the plain line")
+    ap.add_argument("--autotune", action="store_true",
+                    help="before the timed iterations, let quids_amd.tuning pick the step's launch and bucket "
+                         "parameters on the warmed-up state")
     args = ap.parse_args()
     if args.rocprof_summary:
         args.workload_key = workload_key(args)
@@ -389,6 +392,13 @@ def main():
         ctx.synchronize()
         log(f"warmup iteration {i + 1}: {a.num_object} objects, last C={sy.num_object} ({time.time() - t:.2f} s)")
 
+    tuned = None
+    if args.autotune:
+        from quids_amd import tuning
+        q.simulate(a, rules[0])
+        tuned = tuning.autotune(ctx, a, rules[1:], cap, reps=1, verbose=False)
+        q.simulate(a, qcgd.reversed_step)  # back to the warmed-up state
+        log(f"autotune: {tuned}")
     ctx.set_profiling(True)
     ctx.reset_phase_times()
     stats = []
@@ -455,6 +465,7 @@ def main():
         "step_hbm_frac": step_frac,
         "phase_ms": {k: round(v[0], 3) for k, v in phases.items()},
         "cpu_baseline": cpu,
+        "tuning": ctx.get_tuning() if tuned is None else dict(tuned["tuning"], autotuned=True),
         **extras,
     }
     print(json.dumps(out), flush=True)

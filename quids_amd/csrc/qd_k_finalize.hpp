@@ -65,9 +65,9 @@ __global__ __launch_bounds__(256) void k_child_size(Parents P, const u64 *cb, qd
 //    child buffer, stored with coalesced 4-byte writes.
 // Objects beyond the LDS buffers (or not 4-byte aligned) are written by one
 // lane straight from global memory.
-constexpr int kPopParent = 4096;
-constexpr int kPopChild = 16384;
-constexpr int kPopNodes = 256;
+constexpr int kPopParent = 2048;  // LDS bytes per wave: small, so ~20 waves fit a CU
+constexpr int kPopChild = 4096;
+constexpr int kPopNodes = 128;
 
 __device__ inline u32 wave_excl_sum(u32 v, u32 &total) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -124,8 +124,38 @@ __device__ inline void populate_flip_wave(const u8 *o, u32 psz, u8 *d, u64 span,
     }
 }
 
+// erase_create / coin / hadamard: one wavefront per survivor, four per
+// block, no LDS.
 template <int KIND>
-__global__ __launch_bounds__(64) void k_populate(Parents P, const u64 *cb, qd_rule rule, const u64 *orep, u64 K,
+__global__ __launch_bounds__(256) void k_populate_flip(Parents P, const u64 *cb, qd_rule rule, const u64 *orep, u64 K,
+                                                       const u64 *obegin, const u32 *osize, u8 *oobj) {
+    const u32 lane = threadIdx.x & (kWave - 1);
+    const u64 wave = (u64(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+    const u64 nwaves = u64(gridDim.x) * blockDim.x / kWave;
+    for (u64 k = wave; k < K; k += nwaves) {
+        const u64 c = orep[k];
+        const u64 p = find_parent(cb, P.n, c);
+        const u32 j = u32(c - cb[p]);
+        const u8 *o = P.obj + P.begin[p];
+        const u32 psz = P.size[p];
+        const u64 ob = obegin[k], oe = obegin[k + 1];
+        const u64 span = oe - ob;  // child bytes + zero padding
+        u8 *d = oobj + ob;
+        const bool aligned = ((P.begin[p] | ob) & 3) == 0;
+        const u32 n = KIND == QD_RULE_HADAMARD ? 0 : ld16(o);
+        if (aligned && n <= 64) {
+            populate_flip_wave<KIND>(o, psz, d, span, j, rule);
+        } else if (lane == 0) {
+            if (KIND == QD_RULE_HADAMARD) hadamard_child_write(o, psz, rule.bit, j, d);
+            else flip_child_write(o, psz, KIND == QD_RULE_QCGD_COIN, j, d);
+            for (u64 q = psz; q < span; ++q) d[q] = 0;
+        }
+        (void)osize;
+    }
+}
+
+// split_merge: one single-wave block per survivor (see above).
+__global__ __launch_bounds__(64) void k_populate_sm(Parents P, const u64 *cb, qd_rule rule, const u64 *orep, u64 K,
                                                  const u64 *obegin, const u32 *osize, u8 *oobj) {
     __shared__ __align__(16) u8 pbuf[kPopParent];
     __shared__ __align__(16) u8 cbuf[kPopChild];
@@ -144,17 +174,6 @@ __global__ __launch_bounds__(64) void k_populate(Parents P, const u64 *cb, qd_ru
         const u64 span = oe - ob;  // child bytes + zero padding
         u8 *d = oobj + ob;
         const bool aligned = ((P.begin[p] | ob) & 3) == 0;
-        if (KIND != QD_RULE_QCGD_SPLIT_MERGE) {
-            const u32 n = KIND == QD_RULE_HADAMARD ? 0 : ld16(o);
-            if (aligned && n <= 64) {
-                populate_flip_wave<KIND>(o, psz, d, span, j, rule);
-            } else if (lane == 0) {
-                if (KIND == QD_RULE_HADAMARD) hadamard_child_write(o, psz, rule.bit, j, d);
-                else flip_child_write(o, psz, KIND == QD_RULE_QCGD_COIN, j, d);
-                for (u64 q = psz; q < span; ++q) d[q] = 0;
-            }
-            continue;
-        }
         if (!(aligned && psz <= kPopParent && csz <= kPopChild)) {
             if (lane == 0) {
                 Graph g(o);

@@ -142,7 +142,8 @@ struct qd_ctx {
     hipEvent_t ev[2 * QD_NUM_PHASES] = {};
     bool pending[QD_NUM_PHASES] = {};
     double phase_ms[QD_NUM_PHASES] = {};
-    qd_tuning tune = {768, 12, 256, 2048, 2048, 0, 0, 0};
+    // defaults: what quids_amd.tuning picked on the 12-node QCGD workload
+    qd_tuning tune = {1024, 8, 256, 4096, 2048, 0, 0, 0};
     u64 phase_calls[QD_NUM_PHASES] = {};
     u64 *host = nullptr;  // pinned readback area, 8192 u64
     // scratch of the step
@@ -580,9 +581,14 @@ static int populate_out(qd_ctx *ctx, const Parents &P, const qd_rule &rule, cons
     const u64 bytes = hb[0];
     QD_TRY(out->objects.ensure(std::max<u64>(bytes, 1)));
     prof_begin(ctx, QD_PHASE_FINALIZE);
-    hipLaunchKernelGGL(k_populate<KIND>, dim3(unsigned(std::min<u64>(K, 65536))), dim3(64), 0, s, P,
-                       ctx->cb.as<u64>(), rule, orep, K, out->begin.as<u64>(), out->size.as<u32>(),
-                       out->objects.as<u8>());
+    if (KIND == QD_RULE_QCGD_SPLIT_MERGE)
+        hipLaunchKernelGGL(k_populate_sm, dim3(unsigned(std::min<u64>(K, 65536))), dim3(64), 0, s, P,
+                           ctx->cb.as<u64>(), rule, orep, K, out->begin.as<u64>(), out->size.as<u32>(),
+                           out->objects.as<u8>());
+    else
+        hipLaunchKernelGGL(k_populate_flip<KIND>, dim3(unsigned(std::min<u64>((K + 3) / 4, 65536))), dim3(256), 0, s,
+                           P, ctx->cb.as<u64>(), rule, orep, K, out->begin.as<u64>(), out->size.as<u32>(),
+                           out->objects.as<u8>());
     QD_HIP(hipGetLastError());
     out->n = K;
     out->bytes = bytes;
