@@ -78,7 +78,12 @@ typedef struct qd_modifier {
 typedef struct qd_params {
     double tolerance;           /* quids::tolerance, default 1e-30 */
     uint32_t align_byte_length; /* quids::align_byte_length, default 8 */
-    uint32_t simple_truncation; /* quids::simple_truncation; must be 1 */
+    uint32_t simple_truncation; /* quids::simple_truncation (default 1): keep the largest |m|^2;
+                                   0: keep the smallest u / |m|^2 (quids.hpp:594-608, 829-845) with
+                                   u uniform in (0, 1] from Philox2x64-10 keyed by (object hash,
+                                   seed) -- reproducible, unlike the reference's rand() streams */
+    uint32_t reserved;
+    uint64_t seed;              /* key of the probabilistic truncation's random draws */
 } qd_params;
 
 /* Counters of one simulate() call (reference: sy_it.num_object and

@@ -558,6 +558,33 @@ struct State {
     size_t n() const { return size.size(); }
 };
 
+// Probabilistic truncation (src/quids.hpp:594-608, 829-845, 884-895): keep
+// the smallest u / |m|^2, i.e. the largest |m|^2 / u.  The reference draws u
+// from per-thread generators seeded by rand() (random.hpp:12-15); this
+// restatement, like the library, draws it from Philox2x64-10 (Salmon et
+// al., SC 2011: 10 rounds of (c0, c1) <- (hi(M c0) ^ k ^ c1, lo(M c0)),
+// k += W) with counter (hash, 0) and key = seed, and takes the top 53 bits
+// as u in (0, 1].
+static void philox2x64_10(u64 &c0, u64 &c1, u64 k) {
+    for (int r = 0; r < 10; ++r) {
+        const unsigned __int128 p = (unsigned __int128)0xD2B74407B1CE6E93ull * c0;
+        const u64 hi = u64(p >> 64), lo = u64(p);
+        c0 = hi ^ k ^ c1;
+        c1 = lo;
+        k += 0x9E3779B97F4A7C15ull;
+    }
+}
+static double trunc_uniform(u64 hash, u64 seed) {
+    u64 c0 = hash, c1 = 0;
+    philox2x64_10(c0, c1, seed);
+    return double((c0 >> 11) + 1) * 0x1p-53;
+}
+// the value whose larger entries are kept
+static double trunc_key(cplx m, u64 hash, const qd_params &prm) {
+    const double v = cnorm(m);
+    return prm.simple_truncation ? v : v / trunc_uniform(hash, prm.seed);
+}
+
 static inline u32 align_pad(u32 size, u32 align) {
     if (align <= 1) return 0;
     u32 off = align - size % align;
@@ -592,9 +619,13 @@ static int simulate(const State &in, const qd_rule &rule, State &out, u64 max_nu
     if (max_num_object < N) {
         std::vector<u64> ord(N);
         for (size_t p = 0; p < N; ++p) ord[p] = p;
+        std::vector<double> key(N);
+        for (size_t p = 0; p < N; ++p)
+            key[p] = trunc_key(in.mag[p],
+                               prm.simple_truncation ? 0 : rule_hash(rule, in.objects.data() + in.begin[p], in.size[p]),
+                               prm);
         std::nth_element(ord.begin(), ord.begin() + max_num_object, ord.end(), [&](u64 a, u64 b) {
-            double na = cnorm(in.mag[a]), nb = cnorm(in.mag[b]);
-            return na != nb ? na > nb : a < b;
+            return key[a] != key[b] ? key[a] > key[b] : a < b;
         });
         ord.resize(max_num_object);
         std::sort(ord.begin(), ord.end());
@@ -703,8 +734,8 @@ static int simulate(const State &in, const qd_rule &rule, State &out, u64 max_nu
     // truncation (src/quids.hpp:866-900): top max by |m|^2 desc, hash asc
     if (max_num_object < uniq.size()) {
         std::nth_element(uniq.begin(), uniq.begin() + max_num_object, uniq.end(),
-                         [](const Uniq &a, const Uniq &b) {
-                             double na = cnorm(a.mag), nb = cnorm(b.mag);
+                         [&](const Uniq &a, const Uniq &b) {
+                             const double na = trunc_key(a.mag, a.hash, prm), nb = trunc_key(b.mag, b.hash, prm);
                              return na != nb ? na > nb : a.hash < b.hash;
                          });
         uniq.resize(max_num_object);
@@ -897,6 +928,12 @@ uint64_t or_std_hash(const void *bytes, uint64_t n) { return std_hash_bytes(stat
 uint64_t or_hash_combine(uint64_t seed, uint64_t v) {
     hash_combine(seed, v);
     return seed;
+}
+
+void or_philox2x64_10(uint64_t c0, uint64_t c1, uint64_t key, uint64_t *out) {
+    philox2x64_10(c0, c1, key);
+    out[0] = c0;
+    out[1] = c1;
 }
 
 double or_canonical_sum(const double *v, uint64_t n) { return canonical_sum(std::vector<double>(v, v + n)); }

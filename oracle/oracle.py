@@ -86,14 +86,14 @@ def _from_or(h):
     return HostState(objects, begin, size, mag.view(np.complex128).reshape(-1), t.value)
 
 
-def simulate(hs, rule, max_num_object=(1 << 64) - 1, tolerance=1e-30, align=8):
+def simulate(hs, rule, max_num_object=(1 << 64) - 1, tolerance=1e-30, align=8, simple_truncation=True, seed=0):
     """One oracle step on a HostState -> (HostState, stats dict)."""
     from quids_amd._abi import QdParams, QdStepStats
     L = _load()
     h = _to_or(hs)
     out = ctypes.c_void_p()
     st = QdStepStats()
-    prm = QdParams(tolerance, align, 1)
+    prm = QdParams(tolerance, align, 1 if simple_truncation else 0, 0, seed)
     d = rule.desc()
     rc = L.or_simulate(h, ctypes.byref(d), ctypes.byref(out), int(max_num_object), ctypes.byref(prm),
                        ctypes.byref(st))

@@ -23,10 +23,14 @@ __all__ = ["tolerance", "align_byte_length", "simple_truncation", "it_t", "sy_it
 tolerance = 1e-30
 align_byte_length = 8
 simple_truncation = True
+#: key of the probabilistic truncation's draws (simple_truncation False):
+#: Philox2x64-10 keyed by (object hash, truncation_seed)
+truncation_seed = 0
 
 
 def _params():
-    return QdParams(float(tolerance), int(align_byte_length), 1 if simple_truncation else 0)
+    return QdParams(float(tolerance), int(align_byte_length), 1 if simple_truncation else 0, 0,
+                    int(truncation_seed))
 
 
 class Context:

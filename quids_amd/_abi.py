@@ -63,6 +63,8 @@ class QdParams(ctypes.Structure):
         ("tolerance", ctypes.c_double),
         ("align_byte_length", ctypes.c_uint32),
         ("simple_truncation", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("seed", ctypes.c_uint64),
     ]
 
 

@@ -86,10 +86,12 @@ struct SegSel {
     u32 pos, active, keepall, err;
 };
 
-__device__ inline void seg_key(const u64 *UH, const double2 *UM, u64 slot, u32 inst, u64 &hi, u64 &lo) {
+__device__ inline void seg_key(const u64 *UH, const double2 *UM, u64 slot, u32 inst, const SelKey &sk, u64 &hi,
+                               u64 &lo) {
     const double2 m = UM[slot];
-    hi = u64(__double_as_longlong(m.x * m.x + m.y * m.y));
-    lo = ~(UH[slot] - u64(inst) * kFold);
+    const u64 h = UH[slot] - u64(inst) * kFold;  // the instance fold taken back off
+    hi = sel_hi(m.x, m.y, h, sk);
+    lo = ~h;
 }
 
 __device__ inline bool prefix_eq(u64 hi, u64 lo, const SegSel &q) {
@@ -104,7 +106,7 @@ __device__ inline bool prefix_eq(u64 hi, u64 lo, const SegSel &q) {
 constexpr int kSegTile = 4096;
 __global__ __launch_bounds__(256) void k_seg_scatter(const u64 *UR, const u64 *cseg, u64 nslots, const u64 *UH,
                                                      const double2 *UM, const u32 *trunc, u32 n, u64 *cursor,
-                                                     SelItem *list) {
+                                                     SelItem *list, SelKey sk) {
     __shared__ u32 cnt[kMaxInst];
     __shared__ u64 base[kMaxInst];
     constexpr int kPer = kSegTile / 256;
@@ -133,7 +135,7 @@ __global__ __launch_bounds__(256) void k_seg_scatter(const u64 *UR, const u64 *c
             if (inst[q] != kNoInst) {
                 const u64 s = t0 + threadIdx.x + u64(q) * 256;
                 SelItem it;
-                seg_key(UH, UM, s, inst[q], it.hi, it.lo);
+                seg_key(UH, UM, s, inst[q], sk, it.hi, it.lo);
                 it.slot = s;
                 list[base[inst[q]] + rank[q]] = it;
             }
@@ -282,7 +284,7 @@ __global__ __launch_bounds__(256) void k_seg_pick(SegSel *sel, u32 *hist, u32 n,
 // and the keys >= prefix of a truncated one.  Lanes hold consecutive slots.
 __global__ __launch_bounds__(256) void k_seg_mark(const u64 *cseg, u32 n, const u64 *UH, const u64 *UR,
                                                   const double2 *UM, u64 nslots, const SegSel *sel, u64 *bitmap,
-                                                  u64 *sbm) {
+                                                  u64 *sbm, SelKey sk) {
     const u64 stride = u64(gridDim.x) * blockDim.x;
     for (u64 base = u64(blockIdx.x) * blockDim.x; base < nslots; base += stride) {
         const u64 s = base + threadIdx.x;
@@ -294,7 +296,7 @@ __global__ __launch_bounds__(256) void k_seg_mark(const u64 *cseg, u32 n, const 
                 if (q.keepall) keep = true;
                 else {
                     u64 hi, lo;
-                    seg_key(UH, UM, s, inst, hi, lo);
+                    seg_key(UH, UM, s, inst, sk, hi, lo);
                     keep = hi > q.hi || (hi == q.hi && lo >= q.lo);
                 }
             }

@@ -60,16 +60,33 @@ __device__ inline void select_word(bool pred, u64 slot, u64 first, const u64 *UR
     if ((threadIdx.x & (kWave - 1)) == 0) sbm[first >> 6] = m;
 }
 
-// First select stage over the dedup output slots: histogram of |m|^2 bits
-// 51..40 among the uniques whose bits 63..52 equal d1.
-__global__ __launch_bounds__(256) void k_sel_hist_u(const u64 *UR, const double2 *UM, u64 U, u32 d1, u64 *hist) {
+// Histogram of the top 12 key bits over the slots holding a unique.
+__global__ __launch_bounds__(256) void k_key_hist0(const u64 *UH, const u64 *UR, const double2 *UM, u64 U, u64 *hist,
+                                                   SelKey sk) {
     __shared__ u32 h[kHistBins];
     for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) h[i] = 0;
     __syncthreads();
     for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < U; i += u64(gridDim.x) * blockDim.x) {
         if (UR[i] == kNoRep) continue;
         const double2 m = UM[i];
-        const u64 bits = u64(__double_as_longlong(m.x * m.x + m.y * m.y));
+        atomicAdd(&h[u32(sel_hi(m.x, m.y, UH[i], sk) >> 52)], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x)
+        if (h[i]) atomicAdd(reinterpret_cast<unsigned long long *>(hist + i), (unsigned long long)h[i]);
+}
+
+// First select stage over the dedup output slots: histogram of |m|^2 bits
+// 51..40 among the uniques whose bits 63..52 equal d1.
+__global__ __launch_bounds__(256) void k_sel_hist_u(const u64 *UH, const u64 *UR, const double2 *UM, u64 U, u32 d1,
+                                                    u64 *hist, SelKey sk) {
+    __shared__ u32 h[kHistBins];
+    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < U; i += u64(gridDim.x) * blockDim.x) {
+        if (UR[i] == kNoRep) continue;
+        const double2 m = UM[i];
+        const u64 bits = sel_hi(m.x, m.y, UH[i], sk);
         if (u32(bits >> 52) == d1) atomicAdd(&h[u32(bits >> 40) & 0xFFF], 1u);
     }
     __syncthreads();
@@ -81,7 +98,7 @@ __global__ __launch_bounds__(256) void k_sel_hist_u(const u64 *UR, const double2
 // the ones equal to it.
 __global__ __launch_bounds__(256) void k_sel_first(const u64 *UH, const u64 *UR, const double2 *UM, u64 U,
                                                    u32 prefix24, SelItem *S, u64 *scount, u64 *bitmap,
-                                                   u64 *sbm) {
+                                                   u64 *sbm, SelKey sk) {
     const u64 stride = u64(gridDim.x) * blockDim.x;
     // uniform trip count per wave so that the appends see every lane
     for (u64 base = u64(blockIdx.x) * blockDim.x; base < U; base += stride) {
@@ -90,7 +107,7 @@ __global__ __launch_bounds__(256) void k_sel_first(const u64 *UH, const u64 *UR,
         SelItem it{};
         if (i < U && UR[i] != kNoRep) {
             const double2 m = UM[i];
-            const u64 bits = u64(__double_as_longlong(m.x * m.x + m.y * m.y));
+            const u64 bits = sel_hi(m.x, m.y, UH[i], sk);
             const u32 t = u32(bits >> 40);
             if (t > prefix24) keep = true;
             else if (t == prefix24) {

@@ -28,6 +28,7 @@
 #pragma once
 
 #include <stdint.h>
+#include <string.h>
 
 #include "../../include/quids_hip.h"
 
@@ -665,6 +666,51 @@ QD_HD cplx modifier_factor_after(const qd_modifier &m, const u8 *obj) {
     default:
         return cplx{1.0, 0.0};
     }
+}
+
+// ---- random keys of probabilistic truncation -----------------------------
+// The reference keeps the objects with the smallest rng() / |m|^2
+// (quids.hpp:594-608, 829-845, 884-895), rng() from per-thread generators
+// seeded with rand().  Here u is drawn from Philox2x64-10 (Salmon et al.,
+// "Parallel random numbers: as easy as 1, 2, 3", SC 2011) with counter
+// (object hash, 0) and key `seed`, so the draw depends only on the object
+// and the seed.  Keeping the smallest u / |m|^2 is keeping the largest
+// |m|^2 / u, which the top-K select handles like |m|^2.
+QD_HD u64 mulhi64(u64 a, u64 b) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __umul64hi(a, b);
+#else
+    return u64((unsigned __int128)a * b >> 64);
+#endif
+}
+QD_HD u64 philox2x64_10(u64 c0, u64 c1, u64 key) {
+    for (int r = 0; r < 10; ++r) {
+        const u64 hi = mulhi64(0xD2B74407B1CE6E93ull, c0);
+        const u64 lo = 0xD2B74407B1CE6E93ull * c0;
+        c0 = hi ^ key ^ c1;
+        c1 = lo;
+        key += 0x9E3779B97F4A7C15ull;
+    }
+    return c0;
+}
+// uniform in (0, 1] with 53 random bits
+QD_HD double trunc_uniform(u64 hash, u64 seed) {
+    return double((philox2x64_10(hash, 0, seed) >> 11) + 1) * 0x1p-53;
+}
+// The high word of the select key: the bits of |m|^2, or of |m|^2 / u when
+// truncation is probabilistic (both non-negative doubles, so their bit
+// patterns order like their values).
+struct SelKey {
+    u32 prob;
+    u32 pad;
+    u64 seed;
+};
+QD_HD u64 sel_hi(double re, double im, u64 hash, const SelKey &k) {
+    double v = re * re + im * im;
+    if (k.prob) v = v / trunc_uniform(hash, k.seed);
+    u64 b;
+    memcpy(&b, &v, 8);
+    return b;
 }
 
 // ---- dual numbers for the adjoint ----------------------------------------

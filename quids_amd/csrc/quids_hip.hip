@@ -304,11 +304,13 @@ static int normalize_state(qd_ctx *ctx, qd_state *s) {
 }
 
 
-__global__ __launch_bounds__(256) void k_parent_items(const double2 *mag, u64 n, SelItem *S) {
-    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x) {
-        const double2 m = mag[i];
-        const double nrm = m.x * m.x + m.y * m.y;
-        S[i] = SelItem{u64(__double_as_longlong(nrm)), ~i, i};
+// Parent keys of the pre-truncation: (|m|^2 bits, or the random key of the
+// parent's hash, ~index).
+__global__ __launch_bounds__(256) void k_parent_items(Parents P, qd_rule rule, SelKey sk, SelItem *S) {
+    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < P.n; i += u64(gridDim.x) * blockDim.x) {
+        const double2 m = P.mag[i];
+        const u64 h = sk.prob ? rule_hash_object(rule, P.obj + P.begin[i], P.size[i]) : 0;
+        S[i] = SelItem{sel_hi(m.x, m.y, h, sk), ~i, i};
     }
 }
 
@@ -355,7 +357,7 @@ static int select_loop(qd_ctx *ctx, SelItem *S0, SelItem *S1, u64 ns, u64 r, u32
 // and their representatives in bitmap.  h is the histogram of the top 12
 // |m|^2 bits over the slots (it is overwritten).
 static int select_topk(qd_ctx *ctx, const u64 *UH, const u64 *UR, const double2 *UM, u64 nslots, u64 r,
-                       std::vector<u64> &h, u64 *bitmap, u64 *sbm) {
+                       std::vector<u64> &h, u64 *bitmap, u64 *sbm, const SelKey &sk) {
     hipStream_t s = ctx->stream;
     u64 *hist1 = ctx->hist1.as<u64>();
     u64 *counters = ctx->counters.as<u64>();
@@ -370,7 +372,8 @@ static int select_topk(qd_ctx *ctx, const u64 *UH, const u64 *UR, const double2 
     const u32 d1 = u32(d);
     QD_HIP(hipMemsetAsync(hist1, 0, kHistBins * sizeof(u64), s));
     QD_HIP(hipMemsetAsync(counters + 1, 0, sizeof(u64), s));
-    hipLaunchKernelGGL(k_sel_hist_u, dim3(grid_for(nslots, 256, 8192)), dim3(256), 0, s, UR, UM, nslots, d1, hist1);
+    hipLaunchKernelGGL(k_sel_hist_u, dim3(grid_for(nslots, 256, 8192)), dim3(256), 0, s, UH, UR, UM, nslots, d1, hist1,
+                       sk);
     QD_HIP(hipGetLastError());
     QD_TRY(sync_read(ctx, hist1, kHistBins * sizeof(u64), h.data()));
     cum = 0;
@@ -384,7 +387,7 @@ static int select_topk(qd_ctx *ctx, const u64 *UH, const u64 *UR, const double2 
     QD_TRY(ctx->sel1.ensure(std::max<u64>(ns, 1) * sizeof(SelItem)));
     SelItem *S0 = ctx->sel0.as<SelItem>(), *S1 = ctx->sel1.as<SelItem>();
     hipLaunchKernelGGL(k_sel_first, dim3(grid_for(nslots, 256, 16384)), dim3(256), 0, s, UH, UR, UM, nslots,
-                       (d1 << 12) | u32(d), S0, counters + 1, bitmap, sbm);
+                       (d1 << 12) | u32(d), S0, counters + 1, bitmap, sbm, sk);
     QD_HIP(hipGetLastError());
     return select_loop(ctx, S0, S1, ns, r, 24, bitmap, UR, sbm);
 }
@@ -608,6 +611,7 @@ static int range_step(qd_ctx *ctx, const Parents &P, const qd_rule &rule, u64 C,
                       const qd_params &prm, qd_state *out, qd_symbolic *sy, qd_step_stats &st) {
     hipStream_t s = ctx->stream;
     const u64 N = P.n;
+    const SelKey sk{prm.simple_truncation ? 0u : 1u, 0u, prm.seed};
     u64 *counters = ctx->counters.as<u64>();
     u64 *hb = ctx->host;
     QD_TRY(ctx->hist1.ensure(kHistBins * sizeof(u64)));
@@ -703,11 +707,12 @@ static int range_step(qd_ctx *ctx, const Parents &P, const qd_rule &rule, u64 C,
         // keep the best max_obj of the pool
         prof_begin(ctx, QD_PHASE_SELECT);
         QD_HIP(hipMemsetAsync(hist1, 0, kHistBins * sizeof(u64), s));
-        hipLaunchKernelGGL(k_pool_hist0, dim3(grid_for(ns, 256, 4096)), dim3(256), 0, s, (const double2 *)Pm, ns, hist1);
+        hipLaunchKernelGGL(k_key_hist0, dim3(grid_for(ns, 256, 4096)), dim3(256), 0, s, (const u64 *)Ph, (const u64 *)Pr,
+                           (const double2 *)Pm, ns, hist1, sk);
         QD_HIP(hipGetLastError());
         QD_TRY(sync_read(ctx, hist1, kHistBins * sizeof(u64), h.data()));
         u64 *keep = ctx->slist.as<u64>();
-        QD_TRY(select_topk(ctx, Ph, Pr, Pm, ns, max_obj, h, ctx->bitmap.as<u64>(), keep));
+        QD_TRY(select_topk(ctx, Ph, Pr, Pm, ns, max_obj, h, ctx->bitmap.as<u64>(), keep, sk));
         const u64 Wp = (ns + 63) / 64;
         QD_TRY(run_scan(ctx, LoadPopc{keep}, Wp, ctx->pwpre.as<u64>()));
         hipLaunchKernelGGL(k_pool_compact, dim3(grid_for(ns)), dim3(256), 0, s, (const u64 *)Ph, (const u64 *)Pr,
@@ -771,6 +776,7 @@ struct Batch {
     const u64 *d_oseg = nullptr;      // device, n + 1
     const qd_rule *d_rules = nullptr; // device, n
     const u64 *caps = nullptr;        // host, n
+    SelKey sk{0, 0, 0};               // truncation key (probabilistic or not)
     std::vector<u64> cseg, U, kseg;
     std::vector<double> total;
     void set_empty() {
@@ -846,7 +852,7 @@ static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const
         SelItem *cur = ctx->ebuf1.as<SelItem>(), *nxt = ctx->ebuf2.as<SelItem>();
         const unsigned sg = unsigned(std::min<u64>((nslots + kSegTile - 1) / kSegTile, 16384));
         hipLaunchKernelGGL(k_seg_scatter, dim3(std::max(sg, 1u)), dim3(256), 0, s, UR, cseg, nslots, UH, UM,
-                           ctx->btrunc.as<u32>(), n, ctx->bcur.as<u64>(), cur);
+                           ctx->btrunc.as<u32>(), n, ctx->bcur.as<u64>(), cur, bt->sk);
         QD_HIP(hipMemsetAsync(ctx->bhist.p, 0, u64(n) * kHistBins * sizeof(u32), s));
         constexpr u64 kTileItems = 16384;
         std::vector<SegTile> tiles;
@@ -888,7 +894,7 @@ static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const
             if (sel[i].err || sel[i].active) return fail(QD_ERR_HIP, "batched select did not converge");
     }
     hipLaunchKernelGGL(k_seg_mark, dim3(grid_for(nslots, 256, 16384)), dim3(256), 0, s, cseg, n, UH, UR, UM, nslots,
-                       (const SegSel *)dsel, bitmap, sbm);
+                       (const SegSel *)dsel, bitmap, sbm, bt->sk);
     QD_HIP(hipGetLastError());
     return QD_OK;
 }
@@ -963,6 +969,7 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
     hipStream_t s = ctx->stream;
     const u64 N = P.n;
     st.num_parent = N;
+    const SelKey sk{prm.simple_truncation ? 0u : 1u, 0u, prm.seed};
 
     auto finish_empty = [&]() -> int {
         QD_TRY(state_reserve(out, 0, 0));
@@ -986,8 +993,6 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
     const u64 *keep = nullptr;
     u64 nkept = N;
     if (!BATCH && max_obj != 0 && max_obj < N) {
-        if (!prm.simple_truncation)
-            return fail(QD_ERR_UNSUPPORTED, "probabilistic truncation is not implemented; set simple_truncation");
         QD_TRY(ctx->sel0.ensure(N * sizeof(SelItem)));
         QD_TRY(ctx->sel1.ensure(N * sizeof(SelItem)));
         QD_TRY(ctx->kept.ensure(((N + 63) / 64) * sizeof(u64)));
@@ -995,7 +1000,7 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
         QD_TRY(ctx->hist1.ensure(kHistBins * sizeof(u64)));
         u64 *pbits = ctx->kept.as<u64>();
         QD_HIP(hipMemsetAsync(pbits, 0, ((N + 63) / 64) * sizeof(u64), s));
-        hipLaunchKernelGGL(k_parent_items, dim3(grid_for(N)), dim3(256), 0, s, P.mag, N, ctx->sel0.as<SelItem>());
+        hipLaunchKernelGGL(k_parent_items, dim3(grid_for(N)), dim3(256), 0, s, P, rule, sk, ctx->sel0.as<SelItem>());
         QD_HIP(hipGetLastError());
         SelItem *S0 = ctx->sel0.as<SelItem>(), *S1 = ctx->sel1.as<SelItem>();
         QD_TRY(select_loop(ctx, S0, S1, N, max_obj, 0, pbits, nullptr, nullptr));
@@ -1153,11 +1158,15 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
             hipLaunchKernelGGL(k_mark_all, dim3(grid_for(C, 256, 16384)), dim3(256), 0, s, UR, C, bitmap, sbm);
         QD_HIP(hipGetLastError());
     } else {
-        if (!prm.simple_truncation)
-            return fail(QD_ERR_UNSUPPORTED, "probabilistic truncation is not implemented; set simple_truncation");
         std::vector<u64> h(kHistBins);
+        if (sk.prob) {  // the dedup's histogram is of |m|^2; the keys are random
+            QD_HIP(hipMemsetAsync(ctx->nhist.p, 0, kHistBins * sizeof(u64), s));
+            hipLaunchKernelGGL(k_key_hist0, dim3(grid_for(C, 256, 4096)), dim3(256), 0, s, UH, UR, UM, C,
+                               ctx->nhist.as<u64>(), sk);
+            QD_HIP(hipGetLastError());
+        }
         QD_TRY(sync_read(ctx, ctx->nhist.p, kHistBins * sizeof(u64), h.data()));
-        QD_TRY(select_topk(ctx, UH, UR, UM, C, max_obj, h, bitmap, sbm));
+        QD_TRY(select_topk(ctx, UH, UR, UM, C, max_obj, h, bitmap, sbm, sk));
     }
     prof_end(ctx, QD_PHASE_SELECT);
     prof_begin(ctx, QD_PHASE_ORDER);
@@ -1394,6 +1403,7 @@ static int batched_kind(qd_ctx *c, qd_state *in, const u64 *in_seg, u32 n, const
         bt.d_oseg = c->boseg.as<u64>();
         bt.d_rules = c->brules.as<qd_rule>() + i0;
         bt.caps = caps + i0;
+        bt.sk = SelKey{prm.simple_truncation ? 0u : 1u, 0u, prm.seed};
         const Parents Pk{P.obj, P.begin + p0, P.size + p0, P.mag + p0, p1 - p0};
         qd_state *dst = out;
         if (nchunk > 1) {
@@ -1808,7 +1818,7 @@ int qd_simulate(qd_ctx *c, qd_state *in, const qd_rule *rule, qd_state *out, qd_
     if (!c || !in || !rule || !out) return fail(QD_ERR_INVALID, "null argument");
     if (in == out) return fail(QD_ERR_INVALID, "input and output states must differ");
     if (in->ctx != c || out->ctx != c) return fail(QD_ERR_INVALID, "state belongs to another context");
-    qd_params prm{1e-30, 8, 1};
+    qd_params prm{1e-30, 8, 1, 0, 0};
     if (params) prm = *params;
     qd_step_stats st;
     std::memset(&st, 0, sizeof st);
@@ -1874,10 +1884,8 @@ int qd_simulate_batched(qd_ctx *c, qd_state *in, const uint64_t *in_seg, uint32_
         if (in_seg[i + 1] - in_seg[i] > max_num_object[i])
             return fail(QD_ERR_UNSUPPORTED, "batched instances cannot be pre-truncated (more parents than max)");
     }
-    qd_params prm{1e-30, 8, 1};
+    qd_params prm{1e-30, 8, 1, 0, 0};
     if (params) prm = *params;
-    if (!prm.simple_truncation)
-        return fail(QD_ERR_UNSUPPORTED, "probabilistic truncation is not implemented; set simple_truncation");
     QD_HIP(hipSetDevice(c->device));
     switch (kind) {
     case QD_RULE_QCGD_ERASE_CREATE:

@@ -14,8 +14,9 @@
 //  * results follow the canonical choices of the library (representative =
 //    smallest candidate index, pairwise merge order, ties at a truncation
 //    cut broken by hash, next state in ascending representative order);
-//  * simple_truncation must stay true (probabilistic truncation is not
-//    implemented);
+//  * with simple_truncation false, the truncation keeps the smallest
+//    u / |m|^2 as the reference does, but u is a Philox draw keyed by the
+//    object's hash and truncation_seed, so runs are reproducible;
 //  * it_t objects live in device memory; reading them (get_object,
 //    average_value) downloads a host copy.
 #pragma once
@@ -40,8 +41,10 @@ typedef unsigned uint;
 inline uint align_byte_length = 8;
 /// objects with a smaller probability are dropped (src/quids.hpp:62)
 inline double tolerance = 1e-30;
-/// top-|m|^2 truncation (src/quids.hpp:69-75); must be true here
+/// top-|m|^2 truncation (src/quids.hpp:69-75); false: probabilistic
 inline bool simple_truncation = true;
+/// key of the probabilistic truncation's draws (no reference counterpart)
+inline uint64_t truncation_seed = 0;
 /// memory safety margin of max_num_object == 0 (src/quids.hpp:64)
 inline float safety_margin = 0.2f;
 
@@ -71,6 +74,8 @@ inline qd_params params() {
     p.tolerance = tolerance;
     p.align_byte_length = align_byte_length;
     p.simple_truncation = simple_truncation ? 1 : 0;
+    p.reserved = 0;
+    p.seed = truncation_seed;
     return p;
 }
 }  // namespace detail

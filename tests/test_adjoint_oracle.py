@@ -66,3 +66,18 @@ def test_jvp_cap_at_state_size_is_untruncated():
     assert v0 == v1 and np.array_equal(g0, g1)
     v2, g2, fin2 = orc.program_jvp(hs, qcgd_program(2, max_num_object=n // 2), P0, OBS_PARTICLES)
     assert fin2.num_object <= n // 2 and not np.array_equal(g0, g2)
+
+
+def test_philox_known_answers():
+    """Philox2x64-10, the generator of the probabilistic truncation, against
+    the published known-answer vectors of Random123 (counter, key -> out)."""
+    import ctypes
+    L = orc._load()
+    L.or_philox2x64_10.argtypes = [ctypes.c_uint64] * 3 + [ctypes.POINTER(ctypes.c_uint64)]
+    for c0, c1, key, want in [
+        (0, 0, 0, (0xca00a0459843d731, 0x66c24222c9a845b5)),
+        (2 ** 64 - 1, 2 ** 64 - 1, 2 ** 64 - 1, (0x65b021d60cd8310f, 0x4d02f3222f86df20)),
+    ]:
+        out = (ctypes.c_uint64 * 2)()
+        L.or_philox2x64_10(c0, c1, key, out)
+        assert (out[0], out[1]) == want
