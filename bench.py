@@ -77,7 +77,9 @@ def cpu_baseline(nodes, seed, cap, budget_s, iterations=2):
                       f"{int(C)} candidates in {secs:.2f} s, OMP_NUM_THREADS={threads}"}
 
 
-KERNEL_OF_PHASE = {"generate": "k_generate_fast", "dedup": "k_dedup<", "finalize": "k_populate"}
+# The generate phase's kernel: k_generate_scatter (generate fused with the
+# first radix pass, the default) or k_generate_fast (split_generate=1).
+KERNEL_OF_PHASE = {"generate": "k_generate_scatter", "dedup": "k_dedup<", "finalize": "k_populate"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_qcgd12.json")
 
 
@@ -422,15 +424,19 @@ def main():
     dom_ms = kernel_ms[dom] / launches
     per_launch = ab[dom] / launches
     achieved = per_launch / (dom_ms / 1e3) if dom_ms > 0 else None
+    kname = KERNEL_OF_PHASE[dom].rstrip("<")
+    if dom == "generate" and ctx.get_tuning()["split_generate"]:
+        kname = "k_generate_fast"
     traffic = None
     try:
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
-        if tr.get("workload") == workload_key(args) and dom in tr.get("kernels", {}):
-            traffic = tr["kernels"][dom]["hbm_bytes_per_launch"]
+        kt = tr.get("kernels", {}).get(dom, {})
+        if tr.get("workload") == workload_key(args) and kt.get("kernel") == kname:
+            traffic = kt["hbm_bytes_per_launch"]
     except (OSError, ValueError):
         pass
-    roofline = {"bound": "hbm", "kernel": KERNEL_OF_PHASE[dom].rstrip("<"), "achieved": achieved / 1e9 if achieved else None,
+    roofline = {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9 if achieved else None,
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK if achieved else None,
                 "traffic": traffic, "alg_bytes_per_launch": per_launch, "ms_per_launch": dom_ms,
                 "launches": launches}

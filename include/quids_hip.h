@@ -154,7 +154,8 @@ typedef struct qd_tuning {
     uint32_t chunk_log2;          /* at most 2^k candidates per pass, 10..40 (default 0: as many as
                                      free device memory holds): a batched step splits its instances
                                      into chunks, and a step beyond it runs in hash-range passes */
-    uint32_t reserved;
+    uint32_t split_generate;      /* 1: generate, then the first radix pass as a kernel of its own
+                                     (default 0: fused when the first pass has at most 2^8 buckets) */
 } qd_tuning;
 int qd_ctx_get_tuning(qd_ctx *ctx, qd_tuning *tuning);
 int qd_ctx_set_tuning(qd_ctx *ctx, const qd_tuning *tuning);

@@ -97,7 +97,7 @@ class QdTuning(ctypes.Structure):
         ("dedup_cap", ctypes.c_uint32),
         ("generic_path", ctypes.c_uint32),
         ("chunk_log2", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("split_generate", ctypes.c_uint32),
     ]
 
     def as_dict(self):

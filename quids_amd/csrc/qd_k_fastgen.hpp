@@ -356,6 +356,122 @@ __global__ __launch_bounds__(kGenThreads) void k_generate_fast(const PDesc *D, c
 }
 
 // ---------------------------------------------------------------------------
+// Generate fused with the first radix pass (at most 2^8 buckets): a block
+// ranks its children per bucket of the top b1 key bits in LDS, reserves one
+// range per bucket and writes 32-byte entries {key, candidate, magnitude}
+// into fixed-capacity buckets (bucket d owns [d * cap1, (d + 1) * cap1) of
+// E1; cnt1[d] counts them).  A bucket that would pass cap1 drops the excess
+// and raises ERR_SPILL1: the step then redoes generate and partition the
+// two-kernel way.  BATCH as in k_generate_fast.
+// ---------------------------------------------------------------------------
+constexpr u32 ERR_SPILL1 = 32u;
+constexpr u32 kFusedBins = 256;
+
+template <int KIND, bool BATCH>
+__global__ __launch_bounds__(kGenThreads) void k_generate_scatter(const PDesc *D, const double2 *pmag, const u64 *cb,
+                                                                 const u64 *wb, u64 N, const u64 *tab,
+                                                                 const u64 *pbegin, qd_rule rule, u64 W, u32 b1,
+                                                                 u64 cap1, u64 *cnt1, Entry *E1, u32 *err,
+                                                                 const qd_rule *rules, const u64 *iseg, u32 ninst,
+                                                                 u64 fold) {
+    __shared__ u32 bcnt[kFusedBins];
+    __shared__ u64 bbase[kFusedBins];
+    __shared__ u64 swb[kGenThreads + 1];
+    __shared__ u64 pfirst;
+    const u32 nbin = 1u << b1;
+    QcgdCoef k = coef_of(rule);
+    bool spill = false;
+    for (u64 w0 = u64(blockIdx.x) * kGenThreads; w0 < W; w0 += u64(gridDim.x) * kGenThreads) {
+        __syncthreads();
+        if (threadIdx.x == 0) pfirst = find_parent(wb, N, w0);
+        for (u32 i = threadIdx.x; i < nbin; i += blockDim.x) bcnt[i] = 0;
+        __syncthreads();
+        const u64 pf = pfirst;
+        for (u32 q = threadIdx.x; q <= kGenThreads; q += blockDim.x) swb[q] = pf + q <= N ? wb[pf + q] : ~0ull;
+        __syncthreads();
+        const u64 w = w0 + threadIdx.x;
+        u64 hs[1 << kItemBits];
+        double2 ms[1 << kItemBits];
+        u64 cs[1 << kItemBits];
+        u32 br[1 << kItemBits];
+        u32 nk = 0;
+        if (w < W) {
+            u64 p;
+            if (swb[kGenThreads] > w) {
+                u32 lo = 0, hi = kGenThreads;
+                while (lo + 1 < hi) {
+                    const u32 mid = (lo + hi) >> 1;
+                    if (swb[mid] <= w) lo = mid;
+                    else hi = mid;
+                }
+                p = pf + lo;
+            } else {
+                p = find_parent(wb, N, w);
+            }
+            u64 hadd = 0;
+            if (BATCH) {
+                u32 lo = 0, hi = ninst - 1;  // instance of parent p
+                while (lo < hi) {
+                    const u32 mid = (lo + hi + 1) >> 1;
+                    if (iseg[mid] <= p) lo = mid;
+                    else hi = mid - 1;
+                }
+                k = coef_of(rules[lo]);
+                hadd = u64(lo) * fold;
+            }
+            const PDesc pd = D[p];
+            const u32 j0 = u32(w - wb[p]);
+            const u32 t = min(u32(pd.k), kItemBits);
+            const u32 s = pd.k - t;
+            const double2 m0 = pmag[p];
+            Walk pre{};
+            pre.mag = cplx{m0.x, m0.y};
+            const u64 *ptab = tab + pbegin[p] / 2;
+            if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(pre, pd, ptab, j0, s, k);
+            else flip_run<KIND == QD_RULE_QCGD_COIN>(pre, pd, j0, s, k);
+            const u64 cbase = cb[p] + j0;
+            for (u32 u = 0; u < (1u << t); ++u) {
+                Walk x = pre;
+                u64 h;
+                if (KIND == QD_RULE_QCGD_SPLIT_MERGE) {
+                    sm_run(x, pd, ptab, u, pd.k, k);
+                    h = sm_finish(x, ptab);
+                } else {
+                    flip_run<KIND == QD_RULE_QCGD_COIN>(x, pd, u, pd.k, k);
+                    h = hash_combine(hash_combine(pd.nf, x.lh), x.rh);
+                }
+                if (BATCH) h += hadd;
+                const u32 d = b1 ? u32(h >> (64 - b1)) : 0u;
+                hs[nk] = h;
+                ms[nk] = make_double2(x.mag.re, x.mag.im);
+                cs[nk] = cbase + (u64(u) << s);
+                br[nk] = (d << 16) | atomicAdd(&bcnt[d], 1u);
+                ++nk;
+            }
+        }
+        __syncthreads();
+        for (u32 i = threadIdx.x; i < nbin; i += blockDim.x)
+            if (bcnt[i]) bbase[i] = atomicAdd(reinterpret_cast<unsigned long long *>(cnt1 + i), (unsigned long long)bcnt[i]);
+        __syncthreads();
+        for (u32 q = 0; q < nk; ++q) {
+            const u32 d = br[q] >> 16;
+            const u64 slot = bbase[d] + (br[q] & 0xFFFF);
+            if (slot < cap1) {
+                Entry e;
+                e.hash = hs[q];
+                e.cand = cs[q];
+                e.re = ms[q].x;
+                e.im = ms[q].y;
+                E1[u64(d) * cap1 + slot] = e;
+            } else {
+                spill = true;
+            }
+        }
+    }
+    if (spill) atomicOr(err, ERR_SPILL1);
+}
+
+// ---------------------------------------------------------------------------
 // Hash-range passes, for steps whose candidates exceed the scratch memory.
 // COUNT: histogram of the top 12 bits of every child hash into hist (4096
 // bins).  Otherwise: only the children whose top 12 hash bits lie in

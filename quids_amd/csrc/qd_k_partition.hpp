@@ -61,12 +61,15 @@ __global__ __launch_bounds__(256) void k_scatter1(const u64 *H0, const double2 *
 // ERR_SPILL (the step then repeats this pass with exact counts).
 constexpr u32 ERR_SPILL = 16u;
 
-__global__ __launch_bounds__(1024) void k_level2_fixed(const Entry *in, const u64 *off1, u32 b1, u32 b2, u64 cap2,
-                                                       Entry *out, u64 *cnt2, u32 *err) {
+// Input bucket b holds off1[b+1] - off1[b] entries, read from in at
+// b * in_stride (fixed-capacity first pass) or at off1[b] (in_stride 0).
+__global__ __launch_bounds__(1024) void k_level2_fixed(const Entry *in, u64 in_stride, const u64 *off1, u32 b1, u32 b2,
+                                                       u64 cap2, Entry *out, u64 *cnt2, u32 *err) {
     __shared__ u32 ccnt[kHistBins];
     __shared__ u32 cur[kHistBins];
     const u64 b = blockIdx.x;
     const u64 beg = off1[b], end = off1[b + 1];
+    const u64 ibase = in_stride ? b * in_stride : beg;  // entry i of [beg, end) is in[ibase + i - beg]
     const u32 nb2 = 1u << b2;
     const u32 shift = 64 - b1 - b2;
     Entry *region = out + (b << b2) * cap2;
@@ -81,7 +84,7 @@ __global__ __launch_bounds__(1024) void k_level2_fixed(const Entry *in, const u6
         for (int k = 0; k < 4; ++k) {
             const u64 i = cbase + threadIdx.x + 1024 * k;
             if (i < end) {
-                e[k] = in[i];
+                e[k] = in[ibase + (i - beg)];
                 const u32 d = u32(e[k].hash >> shift) & (nb2 - 1);
                 dr[k] = (d << 16) | atomicAdd(&ccnt[d], 1u);
             }
@@ -107,19 +110,20 @@ __global__ __launch_bounds__(1024) void k_level2_fixed(const Entry *in, const u6
 
 // The same split with exact offsets (a counting pass first): used when a
 // sub-bucket spills its fixed capacity.
-__global__ __launch_bounds__(1024) void k_level2_exact(const Entry *in, const u64 *off1, u32 b1, u32 b2, Entry *out,
-                                                       u64 *off2) {
+__global__ __launch_bounds__(1024) void k_level2_exact(const Entry *in, u64 in_stride, const u64 *off1, u32 b1, u32 b2,
+                                                       Entry *out, u64 *off2) {
     __shared__ u32 cnt[kHistBins];
     __shared__ u32 ccnt[kHistBins];
     __shared__ u64 cur[kHistBins];
     __shared__ u64 tmp[1024 / kWave + 1];
     const u64 b = blockIdx.x;
     const u64 beg = off1[b], end = off1[b + 1];
+    const u64 ibase = in_stride ? b * in_stride : beg;
     const u32 nb2 = 1u << b2;
     const u32 shift = 64 - b1 - b2;
     for (u32 i = threadIdx.x; i < nb2; i += blockDim.x) cnt[i] = 0;
     __syncthreads();
-    for (u64 i = beg + threadIdx.x; i < end; i += blockDim.x) atomicAdd(&cnt[u32(in[i].hash >> shift) & (nb2 - 1)], 1u);
+    for (u64 i = beg + threadIdx.x; i < end; i += blockDim.x) atomicAdd(&cnt[u32(in[ibase + (i - beg)].hash >> shift) & (nb2 - 1)], 1u);
     __syncthreads();
     {
         u32 v[4];
@@ -148,14 +152,14 @@ __global__ __launch_bounds__(1024) void k_level2_exact(const Entry *in, const u6
         for (int k = 0; k < 4; ++k) {
             const u64 i = cbase + threadIdx.x + 1024 * k;
             if (i < end) {
-                const u32 d = u32(in[i].hash >> shift) & (nb2 - 1);
+                const u32 d = u32(in[ibase + (i - beg)].hash >> shift) & (nb2 - 1);
                 dr[k] = (d << 16) | atomicAdd(&ccnt[d], 1u);
             }
         }
         __syncthreads();
         for (int k = 0; k < 4; ++k) {
             const u64 i = cbase + threadIdx.x + 1024 * k;
-            if (i < end) out[cur[dr[k] >> 16] + (dr[k] & 0xFFFF)] = in[i];
+            if (i < end) out[cur[dr[k] >> 16] + (dr[k] & 0xFFFF)] = in[ibase + (i - beg)];
         }
         __syncthreads();
         for (u32 i = threadIdx.x; i < nb2; i += blockDim.x) cur[i] += ccnt[i];

@@ -479,8 +479,11 @@ static int ensure_big(DBuf &b, u64 bytes, u64 full) {
 // (H0 / M0 / CI are dead by then) as n slots UH / UR / UM; U counts them.
 // The context's counters must have been cleared (unique count, error word,
 // oversized-bucket count).
+// With in_stride != 0 the first pass is done already: E1 holds bucket d at
+// [d * in_stride, ...) with its count in cur1[d] (the fused generate).
 static int partition_dedup(qd_ctx *ctx, const u64 *H0, const double2 *M0, const u64 *CI, u64 n, u32 b1, u32 b2,
-                           double tol, qd_step_stats &st, u64 *&UH, u64 *&UR, double2 *&UM, u64 &U) {
+                           double tol, qd_step_stats &st, u64 *&UH, u64 *&UR, double2 *&UM, u64 &U,
+                           u64 in_stride = 0) {
     hipStream_t s = ctx->stream;
     u64 *counters = ctx->counters.as<u64>();
     u32 *errw = reinterpret_cast<u32 *>(counters + 4);
@@ -494,7 +497,9 @@ static int partition_dedup(qd_ctx *ctx, const u64 *H0, const double2 *M0, const 
     void *ubuf = ctx->ebuf0.p; // where the uniques go (H0/M0 are dead by then)
     const u64 nb1 = 1ull << b1;
     prof_begin(ctx, QD_PHASE_PARTITION);
-    if (b1 == 0) {
+    if (in_stride) {
+        QD_TRY(run_scan(ctx, LoadU64{ctx->cur1.as<u64>()}, nb1, ctx->off1.as<u64>()));
+    } else if (b1 == 0) {
         u64 two[2] = {0, n};
         QD_HIP(hipMemcpyAsync(ctx->off1.p, two, sizeof two, hipMemcpyHostToDevice, s));
         QD_HIP(hipMemcpyAsync(ctx->cur1.p, two, sizeof(u64), hipMemcpyHostToDevice, s));
@@ -503,12 +508,15 @@ static int partition_dedup(qd_ctx *ctx, const u64 *H0, const double2 *M0, const 
         QD_TRY(run_scan(ctx, LoadU64{hist1}, nb1, ctx->off1.as<u64>()));
         QD_HIP(hipMemcpyAsync(ctx->cur1.p, ctx->off1.p, nb1 * sizeof(u64), hipMemcpyDeviceToDevice, s));
     }
-    hipLaunchKernelGGL(k_scatter1, dim3(unsigned((n + kTile1 - 1) / kTile1)), dim3(256), 0, s, H0, M0, n, b1,
-                       ctx->cur1.as<u64>(), E1, CI);
-    QD_HIP(hipGetLastError());
+    if (!in_stride) {
+        hipLaunchKernelGGL(k_scatter1, dim3(unsigned((n + kTile1 - 1) / kTile1)), dim3(256), 0, s, H0, M0, n, b1,
+                           ctx->cur1.as<u64>(), E1, CI);
+        QD_HIP(hipGetLastError());
+    }
     if (b2 == 0) {
         woff = ctx->off1.as<u64>();
         nbk = nb1;
+        rstride = in_stride;
     } else {
         const u64 nb = 1ull << (b1 + b2);
         const u64 mean = (n + nb - 1) / nb;
@@ -518,8 +526,8 @@ static int partition_dedup(qd_ctx *ctx, const u64 *H0, const double2 *M0, const 
         QD_TRY(ensure_big(ctx->ebuf2, std::max<u64>(nb * cap2, n) * sizeof(Entry),
                           u64(double(step_budget(ctx)) * 1.8) * sizeof(Entry)));
         Entry *E2 = ctx->ebuf2.as<Entry>();
-        hipLaunchKernelGGL(k_level2_fixed, dim3(nb1), dim3(1024), 0, s, (const Entry *)E1, ctx->off1.as<u64>(), b1, b2,
-                           cap2, E2, ctx->cnt2.as<u64>(), errw);
+        hipLaunchKernelGGL(k_level2_fixed, dim3(nb1), dim3(1024), 0, s, (const Entry *)E1, in_stride, ctx->off1.as<u64>(),
+                           b1, b2, cap2, E2, ctx->cnt2.as<u64>(), errw);
         QD_HIP(hipGetLastError());
         QD_TRY(run_scan(ctx, LoadU64{ctx->cnt2.as<u64>()}, nb, ctx->off2.as<u64>()));
         u32 ew = 0;
@@ -530,8 +538,8 @@ static int partition_dedup(qd_ctx *ctx, const u64 *H0, const double2 *M0, const 
             ctx->host[kHostN] = n;
             QD_HIP(hipMemcpyAsync(ctx->off2.as<u64>() + nb, ctx->host + kHostN, sizeof(u64), hipMemcpyHostToDevice,
                                   s));
-            hipLaunchKernelGGL(k_level2_exact, dim3(nb1), dim3(1024), 0, s, (const Entry *)E1, ctx->off1.as<u64>(), b1,
-                               b2, E2, ctx->off2.as<u64>());
+            hipLaunchKernelGGL(k_level2_exact, dim3(nb1), dim3(1024), 0, s, (const Entry *)E1, in_stride,
+                               ctx->off1.as<u64>(), b1, b2, E2, ctx->off2.as<u64>());
             QD_HIP(hipGetLastError());
             rstride = 0;
             st.reserved |= 1;  // spill fallback used
@@ -1084,14 +1092,22 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
         }
     }
 
-    // ---- 3 generate: hash and magnitude at the candidate index
+    // ---- 3 generate: hash and magnitude at the candidate index, or (fused)
+    // straight into the first pass's fixed-capacity buckets
     u32 b1, b2;
     dedup_layout(C, ctx->tune, b1, b2);
     st.dedup_levels = (b1 ? 1 : 0) + (b2 ? 1 : 0);
+    bool fused = fast && b1 <= 8 && !ctx->tune.split_generate;
+    const u64 nbin1 = 1ull << b1;
+    u64 cap1 = C;  // bucket capacity of the fused first pass: mean + 8 sigma + 4096
+    if (b1) {
+        const double mean = double(C) / double(nbin1);
+        cap1 = std::min<u64>(C, u64(mean + 8.0 * std::sqrt(mean)) + 4096);
+    }
     {
         const u64 full = step_budget(ctx) * 32;
         QD_TRY(ensure_big(ctx->ebuf0, C * 32, full));
-        QD_TRY(ensure_big(ctx->ebuf1, C * 32, full));
+        QD_TRY(ensure_big(ctx->ebuf1, std::max(C, nbin1 * cap1) * 32, full + full / 32));
     }
     QD_TRY(ctx->hist1.ensure(kHistBins * sizeof(u64)));
     QD_TRY(ctx->off1.ensure((kHistBins + 1) * sizeof(u64)));
@@ -1103,8 +1119,38 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
     u64 *hist1 = ctx->hist1.as<u64>();
     QD_HIP(hipMemsetAsync(hist1, 0, kHistBins * sizeof(u64), s));
     QD_HIP(hipMemsetAsync(ctx->nhist.p, 0, kHistBins * sizeof(u64), s));
-    prof_begin(ctx, QD_PHASE_GENERATE);
-    if (fast) {
+    u64 in_stride = 0;
+    if (fused) {
+        const unsigned gg =
+            unsigned(std::min<u64>((nitems + kGenThreads - 1) / kGenThreads, ctx->tune.generate_grid));
+        QD_HIP(hipMemsetAsync(ctx->cur1.p, 0, nbin1 * sizeof(u64), s));
+        QD_HIP(hipMemsetAsync(errw, 0, sizeof(u32), s));
+        prof_begin(ctx, QD_PHASE_GENERATE);
+        if (BATCH)
+            hipLaunchKernelGGL((k_generate_scatter<KIND, true>), dim3(std::max(gg, 1u)), dim3(kGenThreads), 0, s,
+                               ctx->pdesc.as<PDesc>(), P.mag, ctx->cb.as<u64>(), ctx->wb.as<u64>(), N,
+                               ctx->tab.as<u64>(), P.begin, rule, nitems, b1, cap1, ctx->cur1.as<u64>(), E1, errw,
+                               bt->d_rules, bt->d_oseg, bt->n, kFold);
+        else
+            hipLaunchKernelGGL((k_generate_scatter<KIND, false>), dim3(std::max(gg, 1u)), dim3(kGenThreads), 0, s,
+                               ctx->pdesc.as<PDesc>(), P.mag, ctx->cb.as<u64>(), ctx->wb.as<u64>(), N,
+                               ctx->tab.as<u64>(), P.begin, rule, nitems, b1, cap1, ctx->cur1.as<u64>(), E1, errw,
+                               (const qd_rule *)nullptr, (const u64 *)nullptr, 0u, u64(0));
+        QD_HIP(hipGetLastError());
+        prof_end(ctx, QD_PHASE_GENERATE);
+        u32 ew = 0;
+        QD_TRY(sync_read(ctx, errw, sizeof(u32), &ew));
+        if (ew & ERR_SPILL1) {  // a bucket passed its capacity: generate and partition again, unfused
+            QD_HIP(hipMemsetAsync(errw, 0, sizeof(u32), s));
+            fused = false;
+            st.reserved |= 8;
+        } else {
+            in_stride = cap1;
+        }
+    }
+    if (!fused) prof_begin(ctx, QD_PHASE_GENERATE);
+    if (fused) {
+    } else if (fast) {
         const unsigned gg =
             unsigned(std::min<u64>((nitems + kGenThreads - 1) / kGenThreads, ctx->tune.generate_grid));
         if (BATCH)
@@ -1124,12 +1170,12 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
                            ctx->tab.as<u64>(), C, H0, M0, b1, hist1);
     }
     QD_HIP(hipGetLastError());
-    prof_end(ctx, QD_PHASE_GENERATE);
+    if (!fused) prof_end(ctx, QD_PHASE_GENERATE);
 
     u64 *UH, *UR;
     double2 *UM;
     u64 U;
-    QD_TRY(partition_dedup(ctx, H0, M0, nullptr, C, b1, b2, prm.tolerance, st, UH, UR, UM, U));
+    QD_TRY(partition_dedup(ctx, H0, M0, nullptr, C, b1, b2, prm.tolerance, st, UH, UR, UM, U, in_stride));
     st.num_unique = U;
 
     // memory-driven cap (max_num_object == 0, quids.hpp:459-486, 510-536):
@@ -1608,6 +1654,7 @@ int qd_ctx_set_tuning(qd_ctx *c, const qd_tuning *t) {
     if (t->chunk_log2 && (t->chunk_log2 < 10 || t->chunk_log2 > 40))
         return fail(QD_ERR_INVALID, "chunk_log2 must be 0 or in [10, 40]");
     n.chunk_log2 = t->chunk_log2;  // 0 = from free device memory
+    n.split_generate = t->split_generate ? 1 : 0;
     c->tune = n;
     return QD_OK;
 }

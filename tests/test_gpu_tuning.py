@@ -43,3 +43,38 @@ def test_autotune_sets_a_grid_point_and_keeps_results():
         assert again is res
     finally:
         ctx.set_tuning(**{k: before[k] for k in ("dedup_bucket_target", "first_pass_bits", "generate_grid")})
+
+
+@pytest.mark.parametrize("first_pass_bits", [8, 6])
+def test_fused_and_split_generate_agree(first_pass_bits):
+    """Generate fused with the first radix pass against generate followed by
+    the first pass as its own kernel: identical steps (plain and batched)."""
+    from quids_amd.batch import batch_t, simulate_batched
+    ctx = q.default_context()
+    before = ctx.get_tuning()
+    hs = HostState.from_objects([qcgd.seeded_graph(9, 4)])
+    rules = [qcgd.split_merge(0.7, 0.2), qcgd.erase_create(1.1, 0.5)]
+    out = {}
+    try:
+        for split in (0, 1):
+            ctx.set_tuning(split_generate=split, first_pass_bits=first_pass_bits, dedup_bucket_target=256)
+            a, b = q.it_t(), q.it_t()
+            a.upload(hs)
+            res = []
+            for _ in range(2):
+                q.simulate(a, qcgd.step)
+                for r in rules:
+                    q.simulate(a, r, b, None, 20000)
+                    a, b = b, a
+                    res.append(a.download())
+            bt, bb = batch_t.from_states([hs, HostState.from_objects([qcgd.seeded_graph(6, 1)])]), batch_t()
+            for r in rules:
+                simulate_batched(bt, [r, r], bb, 5000)
+                bt, bb = bb, bt
+            res.append(bt.state.download())
+            out[split] = res
+        for x, y in zip(out[0], out[1]):
+            assert np.array_equal(x.objects, y.objects) and np.array_equal(x.magnitude, y.magnitude)
+    finally:
+        ctx.set_tuning(split_generate=0, first_pass_bits=before["first_pass_bits"],
+                       dedup_bucket_target=before["dedup_bucket_target"])
