@@ -334,6 +334,8 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="before the timed iterations, let quids_amd.tuning pick the step's launch and bucket "
                          "parameters on the warmed-up state")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="set one qd_tuning field before the run (repeatable), e.g. --tune split_generate=1")
     args = ap.parse_args()
     if args.rocprof_summary:
         args.workload_key = workload_key(args)
@@ -373,6 +375,8 @@ def main():
     cap = 1 << args.log2_cap
     rules = iteration_rules()
     ctx = q.default_context()
+    if args.tune:
+        ctx.set_tuning(**{k: int(v) for k, v in (kv.split("=", 1) for kv in args.tune)})
     a, b = q.it_t(ctx=ctx), q.it_t(ctx=ctx)
     sy = q.sy_it_t(ctx=ctx)
     a.upload(HostState.from_objects([qcgd.seeded_graph(args.nodes, args.seed)]))

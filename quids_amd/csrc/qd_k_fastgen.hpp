@@ -356,7 +356,7 @@ __global__ __launch_bounds__(kGenThreads) void k_generate_fast(const PDesc *D, c
 }
 
 // ---------------------------------------------------------------------------
-// Generate fused with the first radix pass (at most 2^8 buckets): a block
+// Generate fused with the first radix pass (at most 2^9 buckets): a block
 // ranks its children per bucket of the top b1 key bits in LDS, reserves one
 // range per bucket and writes 32-byte entries {key, candidate, magnitude}
 // into fixed-capacity buckets (bucket d owns [d * cap1, (d + 1) * cap1) of
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(kGenThreads) void k_generate_fast(const PDesc *D, c
 // two-kernel way.  BATCH as in k_generate_fast.
 // ---------------------------------------------------------------------------
 constexpr u32 ERR_SPILL1 = 32u;
-constexpr u32 kFusedBins = 256;
+constexpr u32 kFusedBins = 512;
 
 template <int KIND, bool BATCH>
 __global__ __launch_bounds__(kGenThreads) void k_generate_scatter(const PDesc *D, const double2 *pmag, const u64 *cb,
@@ -374,6 +374,7 @@ __global__ __launch_bounds__(kGenThreads) void k_generate_scatter(const PDesc *D
                                                                  u64 cap1, u64 *cnt1, Entry *E1, u32 *err,
                                                                  const qd_rule *rules, const u64 *iseg, u32 ninst,
                                                                  u64 fold) {
+    constexpr u32 kKids = 1u << kItemBits;
     __shared__ u32 bcnt[kFusedBins];
     __shared__ u64 bbase[kFusedBins];
     __shared__ u64 swb[kGenThreads + 1];
@@ -390,11 +391,13 @@ __global__ __launch_bounds__(kGenThreads) void k_generate_scatter(const PDesc *D
         for (u32 q = threadIdx.x; q <= kGenThreads; q += blockDim.x) swb[q] = pf + q <= N ? wb[pf + q] : ~0ull;
         __syncthreads();
         const u64 w = w0 + threadIdx.x;
-        u64 hs[1 << kItemBits];
-        double2 ms[1 << kItemBits];
-        u64 cs[1 << kItemBits];
-        u32 br[1 << kItemBits];
-        u32 nk = 0;
+        // children u < nk of this item; the arrays are indexed by the
+        // unrolled loop counter only, so they stay in registers
+        u64 hs[kKids];
+        double2 ms[kKids];
+        u32 br[kKids];
+        u32 nk = 0, s = 0;
+        u64 cbase = 0;
         if (w < W) {
             u64 p;
             if (swb[kGenThreads] > w) {
@@ -422,49 +425,54 @@ __global__ __launch_bounds__(kGenThreads) void k_generate_scatter(const PDesc *D
             const PDesc pd = D[p];
             const u32 j0 = u32(w - wb[p]);
             const u32 t = min(u32(pd.k), kItemBits);
-            const u32 s = pd.k - t;
+            s = pd.k - t;
+            nk = 1u << t;
             const double2 m0 = pmag[p];
             Walk pre{};
             pre.mag = cplx{m0.x, m0.y};
             const u64 *ptab = tab + pbegin[p] / 2;
             if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(pre, pd, ptab, j0, s, k);
             else flip_run<KIND == QD_RULE_QCGD_COIN>(pre, pd, j0, s, k);
-            const u64 cbase = cb[p] + j0;
-            for (u32 u = 0; u < (1u << t); ++u) {
-                Walk x = pre;
-                u64 h;
-                if (KIND == QD_RULE_QCGD_SPLIT_MERGE) {
-                    sm_run(x, pd, ptab, u, pd.k, k);
-                    h = sm_finish(x, ptab);
-                } else {
-                    flip_run<KIND == QD_RULE_QCGD_COIN>(x, pd, u, pd.k, k);
-                    h = hash_combine(hash_combine(pd.nf, x.lh), x.rh);
+            cbase = cb[p] + j0;
+#pragma unroll
+            for (u32 u = 0; u < kKids; ++u) {
+                if (u < nk) {
+                    Walk x = pre;
+                    u64 h;
+                    if (KIND == QD_RULE_QCGD_SPLIT_MERGE) {
+                        sm_run(x, pd, ptab, u, pd.k, k);
+                        h = sm_finish(x, ptab);
+                    } else {
+                        flip_run<KIND == QD_RULE_QCGD_COIN>(x, pd, u, pd.k, k);
+                        h = hash_combine(hash_combine(pd.nf, x.lh), x.rh);
+                    }
+                    if (BATCH) h += hadd;
+                    const u32 d = b1 ? u32(h >> (64 - b1)) : 0u;
+                    hs[u] = h;
+                    ms[u] = make_double2(x.mag.re, x.mag.im);
+                    br[u] = (d << 16) | atomicAdd(&bcnt[d], 1u);
                 }
-                if (BATCH) h += hadd;
-                const u32 d = b1 ? u32(h >> (64 - b1)) : 0u;
-                hs[nk] = h;
-                ms[nk] = make_double2(x.mag.re, x.mag.im);
-                cs[nk] = cbase + (u64(u) << s);
-                br[nk] = (d << 16) | atomicAdd(&bcnt[d], 1u);
-                ++nk;
             }
         }
         __syncthreads();
         for (u32 i = threadIdx.x; i < nbin; i += blockDim.x)
             if (bcnt[i]) bbase[i] = atomicAdd(reinterpret_cast<unsigned long long *>(cnt1 + i), (unsigned long long)bcnt[i]);
         __syncthreads();
-        for (u32 q = 0; q < nk; ++q) {
-            const u32 d = br[q] >> 16;
-            const u64 slot = bbase[d] + (br[q] & 0xFFFF);
-            if (slot < cap1) {
-                Entry e;
-                e.hash = hs[q];
-                e.cand = cs[q];
-                e.re = ms[q].x;
-                e.im = ms[q].y;
-                E1[u64(d) * cap1 + slot] = e;
-            } else {
-                spill = true;
+#pragma unroll
+        for (u32 u = 0; u < kKids; ++u) {
+            if (u < nk) {
+                const u32 d = br[u] >> 16;
+                const u64 slot = bbase[d] + (br[u] & 0xFFFF);
+                if (slot < cap1) {
+                    Entry e;
+                    e.hash = hs[u];
+                    e.cand = cbase + (u64(u) << s);
+                    e.re = ms[u].x;
+                    e.im = ms[u].y;
+                    E1[u64(d) * cap1 + slot] = e;
+                } else {
+                    spill = true;
+                }
             }
         }
     }

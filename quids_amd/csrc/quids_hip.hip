@@ -143,7 +143,7 @@ struct qd_ctx {
     bool pending[QD_NUM_PHASES] = {};
     double phase_ms[QD_NUM_PHASES] = {};
     // defaults: what quids_amd.tuning picked on the 12-node QCGD workload
-    qd_tuning tune = {1024, 8, 256, 4096, 2048, 0, 0, 0};
+    qd_tuning tune = {512, 8, 256, 4096, 1024, 0, 0, 0};
     u64 phase_calls[QD_NUM_PHASES] = {};
     u64 *host = nullptr;  // pinned readback area, 8192 u64
     // scratch of the step
@@ -153,7 +153,7 @@ struct qd_ctx {
     DBuf nhist, counters;
     DBuf sel0, sel1;
     DBuf bitmap, wprefix, padded, chunks, scanblk;
-    DBuf biglist, bigsoff, bigkey, bigcand, bigmag, bigseg;
+    DBuf biglist, redolist, bigsoff, bigkey, bigcand, bigmag, bigseg;
     DBuf pdesc, wi, wb, ebuf2, cnt2, slist;
     DBuf adjk, adjv, adjg, adjd, adjp;
     DBuf pool0, pool1, pwpre, blist2;
@@ -169,7 +169,7 @@ struct qd_ctx {
     std::vector<DBuf *> all_named() {
         return {&nc, &cb, &pre, &tab, &kept, &keys, &ebuf0, &ebuf1, &hist1, &off1, &cur1, &off2,
                 &nhist, &counters, &sel0, &sel1, &bitmap, &wprefix, &padded, &chunks, &scanblk,
-                &biglist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &ebuf2, &cnt2, &slist,
+                &biglist, &redolist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &ebuf2, &cnt2, &slist,
                 &adjk, &adjv, &adjg, &adjd, &adjp, &bseg, &boseg, &brules, &bsinst, &bcount, &bsel, &btrunc,
                 &blist, &bcur, &btile, &bhist, &bkseg, &bchunk, &bcsum, &btot, &pool0, &pool1, &pwpre, &blist2};
     }
@@ -520,20 +520,28 @@ static int partition_dedup(qd_ctx *ctx, const u64 *H0, const double2 *M0, const 
     } else {
         const u64 nb = 1ull << (b1 + b2);
         const u64 mean = (n + nb - 1) / nb;
-        const u64 cap2 = ((mean + mean / 2 + 128) + 63) / 64 * 64;
+        // mean + 6 sigma + 128 (sub-bucket sizes are near Poisson; a spill
+        // takes the exact path)
+        const u64 cap2 = (mean + u64(6.0 * std::sqrt(double(mean))) + 128 + 63) / 64 * 64;
         QD_TRY(ctx->off2.ensure((nb + 1) * sizeof(u64)));
         QD_TRY(ctx->cnt2.ensure(nb * sizeof(u64)));
-        QD_TRY(ensure_big(ctx->ebuf2, std::max<u64>(nb * cap2, n) * sizeof(Entry),
-                          u64(double(step_budget(ctx)) * 1.8) * sizeof(Entry)));
+        // fixed-capacity sub-buckets unless their padding would pass the
+        // pass budget's buffer (small sub-buckets near the budget): then
+        // exact offsets from a counting pass
+        const u64 full2 = u64(double(step_budget(ctx)) * 1.8);
+        const bool fixed = nb * cap2 <= std::max<u64>(full2, n);
+        QD_TRY(ensure_big(ctx->ebuf2, std::max<u64>(fixed ? nb * cap2 : n, n) * sizeof(Entry), full2 * sizeof(Entry)));
         Entry *E2 = ctx->ebuf2.as<Entry>();
-        hipLaunchKernelGGL(k_level2_fixed, dim3(nb1), dim3(1024), 0, s, (const Entry *)E1, in_stride, ctx->off1.as<u64>(),
-                           b1, b2, cap2, E2, ctx->cnt2.as<u64>(), errw);
-        QD_HIP(hipGetLastError());
-        QD_TRY(run_scan(ctx, LoadU64{ctx->cnt2.as<u64>()}, nb, ctx->off2.as<u64>()));
-        u32 ew = 0;
-        QD_TRY(sync_read(ctx, errw, sizeof(u32), &ew));
-        rstride = cap2;
-        if (ew & ERR_SPILL) {  // a sub-bucket overflowed its capacity: exact offsets
+        u32 ew = ERR_SPILL;
+        if (fixed) {
+            hipLaunchKernelGGL(k_level2_fixed, dim3(nb1), dim3(1024), 0, s, (const Entry *)E1, in_stride,
+                               ctx->off1.as<u64>(), b1, b2, cap2, E2, ctx->cnt2.as<u64>(), errw);
+            QD_HIP(hipGetLastError());
+            QD_TRY(run_scan(ctx, LoadU64{ctx->cnt2.as<u64>()}, nb, ctx->off2.as<u64>()));
+            QD_TRY(sync_read(ctx, errw, sizeof(u32), &ew));
+            rstride = cap2;
+        }
+        if (ew & ERR_SPILL) {  // a sub-bucket overflowed its capacity (or no fixed layout): exact offsets
             QD_HIP(hipMemsetAsync(errw, 0, sizeof(u32), s));
             ctx->host[kHostN] = n;
             QD_HIP(hipMemcpyAsync(ctx->off2.as<u64>() + nb, ctx->host + kHostN, sizeof(u64), hipMemcpyHostToDevice,
@@ -542,7 +550,7 @@ static int partition_dedup(qd_ctx *ctx, const u64 *H0, const double2 *M0, const 
                                ctx->off1.as<u64>(), b1, b2, E2, ctx->off2.as<u64>());
             QD_HIP(hipGetLastError());
             rstride = 0;
-            st.reserved |= 1;  // spill fallback used
+            if (fixed) st.reserved |= 1;  // spill fallback used
         }
         D = E2;
         woff = ctx->off2.as<u64>();
@@ -556,19 +564,54 @@ static int partition_dedup(qd_ctx *ctx, const u64 *H0, const double2 *M0, const 
     UR = UH + n;
     UM = reinterpret_cast<double2 *>(UR + n);
     prof_begin(ctx, QD_PHASE_DEDUP);
-    QD_TRY(ctx->biglist.ensure((n / 2048 + 2) * sizeof(u64)));
-    {
-        int ncu = 256;
-        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device);
-        const u64 grid = std::min<u64>(nbk, u64(ncu));
-        hipLaunchKernelGGL((k_dedup<1024, 2048, 4096>), dim3(grid), dim3(1024), 0, s, D, rstride, woff, nbk, L,
-                           tol, UH, UR, UM, counters, ctx->nhist.as<u64>(), ctx->biglist.as<u64>());
+    const u32 dcap = ctx->tune.dedup_cap;
+    QD_TRY(ctx->biglist.ensure((n / dcap + 2) * sizeof(u64)));
+    QD_TRY(ctx->redolist.ensure((nbk + 1) * sizeof(u64)));
+    int ncu = 256;
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    // persistent grids of as many blocks as stay resident (LDS bound)
+    auto resident = [&](auto kern, int threads, u64 nwork) -> unsigned {
+        int per = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, threads, 0) != hipSuccess || per < 1) {
+            (void)hipGetLastError();
+            per = 1;
+        }
+        return unsigned(std::max<u64>(std::min<u64>(nwork, u64(ncu) * u64(per)), 1));
+    };
+    auto run_dedup = [&](auto kern, int threads) -> int {
+        hipLaunchKernelGGL(kern, dim3(resident(kern, threads, nbk)), dim3(threads), 0, s, D, rstride, woff, nbk, L,
+                           tol, UH, UR, UM, counters, ctx->nhist.as<u64>(), ctx->biglist.as<u64>(),
+                           ctx->redolist.as<u64>());
         QD_HIP(hipGetLastError());
-    }
+        return QD_OK;
+    };
+    // buckets holding a group too large for registers: the sort-based kernel
+    auto run_sort = [&](auto kern, int threads, u64 nred) -> int {
+        hipLaunchKernelGGL(kern, dim3(resident(kern, threads, nred)), dim3(threads), 0, s, D, rstride, woff, nred, L,
+                           tol, UH, UR, UM, counters, ctx->nhist.as<u64>(), ctx->biglist.as<u64>(),
+                           (const u64 *)ctx->redolist.as<u64>());
+        QD_HIP(hipGetLastError());
+        return QD_OK;
+    };
+    if (dcap == 256) QD_TRY(run_dedup(k_dedup<128, 256, 512>, 128));
+    else if (dcap == 512) QD_TRY(run_dedup(k_dedup<256, 512, 1024>, 256));
+    else if (dcap == 1024) QD_TRY(run_dedup(k_dedup<512, 1024, 2048>, 512));
+    else QD_TRY(run_dedup(k_dedup<1024, 2048, 4096>, 1024));
     prof_end(ctx, QD_PHASE_DEDUP);
-    QD_HIP(hipMemcpyAsync(hb, counters, 6 * sizeof(u64), hipMemcpyDeviceToHost, s));
+    QD_HIP(hipMemcpyAsync(hb, counters, 7 * sizeof(u64), hipMemcpyDeviceToHost, s));
     QD_HIP(hipStreamSynchronize(s));
     prof_collect(ctx);
+    if (hb[6]) {
+        prof_begin(ctx, QD_PHASE_DEDUP);
+        if (dcap == 256) QD_TRY(run_sort(k_dedup_sort<128, 256, 512>, 128, hb[6]));
+        else if (dcap == 512) QD_TRY(run_sort(k_dedup_sort<256, 512, 1024>, 256, hb[6]));
+        else if (dcap == 1024) QD_TRY(run_sort(k_dedup_sort<512, 1024, 2048>, 512, hb[6]));
+        else QD_TRY(run_sort(k_dedup_sort<1024, 2048, 4096>, 1024, hb[6]));
+        prof_end(ctx, QD_PHASE_DEDUP);
+        QD_HIP(hipMemcpyAsync(hb, counters, 7 * sizeof(u64), hipMemcpyDeviceToHost, s));
+        QD_HIP(hipStreamSynchronize(s));
+        prof_collect(ctx);
+    }
     QD_TRY(check_err_word(u32(hb[4])));
     U = hb[0];
     if (hb[5]) QD_TRY(dedup_big_buckets(ctx, D, rstride, woff, hb[5], tol, UH, UR, UM, &U));
@@ -1097,7 +1140,7 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
     u32 b1, b2;
     dedup_layout(C, ctx->tune, b1, b2);
     st.dedup_levels = (b1 ? 1 : 0) + (b2 ? 1 : 0);
-    bool fused = fast && b1 <= 8 && !ctx->tune.split_generate;
+    bool fused = fast && (1u << b1) <= kFusedBins && !ctx->tune.split_generate;
     const u64 nbin1 = 1ull << b1;
     u64 cap1 = C;  // bucket capacity of the fused first pass: mean + 8 sigma + 4096
     if (b1) {
@@ -1633,7 +1676,8 @@ int qd_ctx_set_tuning(qd_ctx *c, const qd_tuning *t) {
     if (!c || !t) return fail(QD_ERR_INVALID, "null argument");
     qd_tuning n = c->tune;
     if (t->dedup_cap) {
-        if (t->dedup_cap != 2048) return fail(QD_ERR_INVALID, "dedup_cap must be 2048");
+        if (t->dedup_cap != 256 && t->dedup_cap != 512 && t->dedup_cap != 1024 && t->dedup_cap != 2048)
+            return fail(QD_ERR_INVALID, "dedup_cap must be 256, 512, 1024 or 2048");
         n.dedup_cap = t->dedup_cap;
     }
     if (t->dedup_bucket_target) n.dedup_bucket_target = t->dedup_bucket_target;

@@ -1,8 +1,10 @@
 """Autotuner of the step's launch and layout parameters (qd_tuning).
 
-The knobs: the mean candidates per dedup bucket (how full a workgroup's LDS
-table runs: the load factor of the interference hash table), the bits of the
-first radix pass, and the generate kernel's grid.  ``autotune`` times a
+The knobs: the dedup workgroup's LDS table size and the mean candidates per
+dedup bucket (how full that table runs: the load factor of the interference
+hash table), the bits of the first radix pass, and the generate kernel's
+grid.  Combinations whose bucket target passes half the table size are
+skipped.  ``autotune`` times a
 representative step for each combination on the caller's context and keeps
 the fastest; results are remembered per (device, workload key) in memory and,
 when ``cache_path`` is given, in a JSON file.
@@ -15,9 +17,10 @@ import time
 from . import UINT64_MAX, it_t, simulate
 
 DEFAULT_GRID = {
-    "dedup_bucket_target": (384, 512, 768, 1024),
-    "first_pass_bits": (8, 10, 12),
-    "generate_grid": (1024, 2048, 4096),
+    "dedup_cap": (512, 1024, 2048),
+    "dedup_bucket_target": (256, 384, 512, 1024),
+    "first_pass_bits": (8, 9),
+    "generate_grid": (2048, 4096),
 }
 
 _memo = {}
@@ -70,6 +73,8 @@ def autotune(ctx, state, rules, max_num_object=UINT64_MAX, grid=None, reps=2, ke
     tried = 0
     for values in itertools.product(*[grid[k] for k in names]):
         cand = dict(zip(names, values))
+        if cand.get("dedup_bucket_target", start["dedup_bucket_target"]) * 2 > cand.get("dedup_cap", start["dedup_cap"]):
+            continue
         ctx.set_tuning(**cand)
         t = _time_step(ctx, state, rules, max_num_object, reps)
         tried += 1

@@ -25,7 +25,7 @@ def test_autotune_sets_a_grid_point_and_keeps_results():
     before = ctx.get_tuning()
     st = _state()
     rules = [qcgd.erase_create(math.pi / 3, math.pi / 5)]
-    grid = {"dedup_bucket_target": (256, 768), "first_pass_bits": (6, 12)}
+    grid = {"dedup_bucket_target": (256, 512), "first_pass_bits": (6, 12)}
     try:
         ref = q.it_t()
         q.simulate(st, rules[0], ref, None, 5000)
@@ -78,3 +78,42 @@ def test_fused_and_split_generate_agree(first_pass_bits):
     finally:
         ctx.set_tuning(split_generate=0, first_pass_bits=before["first_pass_bits"],
                        dedup_bucket_target=before["dedup_bucket_target"])
+
+
+@pytest.mark.parametrize("copies", [3, 6, 12, 1200])
+@pytest.mark.parametrize("cap,target", [(256, 128), (512, 256), (1024, 512), (2048, 1024)])
+def test_dedup_table_sizes_match_oracle(cap, target, copies):
+    """Every dedup table size (256 .. 2048 entries per workgroup) against
+    the oracle, on a state of `copies` copies of one graph (each child hash
+    then has that many members: groups summed in registers up to 8 members,
+    by the sort-based kernel past that, by the oversized-bucket kernel past
+    the table size) and on a QCGD chain."""
+    from oracle import oracle as orc
+    ctx = q.default_context()
+    before = ctx.get_tuning()
+    rng = np.random.default_rng(cap + copies)
+    g = qcgd.seeded_graph(5, 3)
+    mags = rng.normal(size=copies) + 1j * rng.normal(size=copies)
+    dup = HostState.from_objects([g] * copies, mags=mags / np.linalg.norm(mags))
+    tol0 = q.tolerance
+    try:
+        q.tolerance = 1e-30
+        ctx.set_tuning(dedup_cap=cap, dedup_bucket_target=target)
+        for hs, rule in ((dup, qcgd.erase_create(0.8, 0.3)), (dup, qcgd.split_merge(0.5, 0.1)),
+                         (HostState.from_objects([qcgd.seeded_graph(8, 7)]), qcgd.split_merge(0.9, 0.4))):
+            ref = hs
+            a = q.it_t()
+            a.upload(hs)
+            for _ in range(2):
+                b = q.it_t()
+                st = q.simulate(a, rule, b, None, q.UINT64_MAX)
+                ref, rst = orc.simulate(ref, rule)
+                a = b
+                got = a.download()
+                assert st["num_unique"] == rst["num_unique"]
+                assert np.array_equal(got.objects, ref.objects)
+                assert np.array_equal(got.magnitude, ref.magnitude)
+                assert got.total_proba == ref.total_proba
+    finally:
+        q.tolerance = tol0
+        ctx.set_tuning(dedup_cap=before["dedup_cap"], dedup_bucket_target=before["dedup_bucket_target"])
