@@ -267,6 +267,50 @@ __device__ inline void flip_run(Walk &w, const PDesc &pd, u32 bits, u32 dlimit, 
     }
 }
 
+// The children of a work item, depth first over its t = min(k, 3) trailing
+// decisions s, s+1, s+2 (bit j of the child's item index u is decision
+// s + j): each branch walks the nodes up to the next decision once and its
+// two continuations start from there, so siblings share every node before
+// the first decision in which they differ (2 + 4 + 8 segment walks in place
+// of 8 whole tails).  Each child still sees the same sequence of magnitude
+// products and hash folds as a walk of its own.  leaf(u, hash, magnitude)
+// runs once per child, u a compile-time constant after unrolling.
+static_assert(kItemBits == 3, "walk_item enumerates three decision levels");
+
+template <int KIND>
+__device__ __attribute__((always_inline)) inline void walk_step(Walk &w, const PDesc &pd, const u64 *ptab, u32 bit,
+                                                                u32 dlimit, const QcgdCoef &k) {
+    if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(w, pd, ptab, bit, dlimit, k);
+    else flip_run<KIND == QD_RULE_QCGD_COIN>(w, pd, bit, dlimit, k);
+}
+
+template <int KIND, typename Leaf>
+__device__ __attribute__((always_inline)) inline void walk_item(const Walk &pre, const PDesc &pd, const u64 *ptab, u32 s,
+                                                                u32 t, const QcgdCoef &k, Leaf &&leaf) {
+#pragma unroll
+    for (u32 b0 = 0; b0 < 2; ++b0) {
+        if (b0 && t < 1) break;
+        Walk w0 = pre;
+        walk_step<KIND>(w0, pd, ptab, b0, s + 1, k);
+#pragma unroll
+        for (u32 b1 = 0; b1 < 2; ++b1) {
+            if (b1 && t < 2) break;
+            Walk w1 = w0;
+            walk_step<KIND>(w1, pd, ptab, b1, s + 2, k);
+#pragma unroll
+            for (u32 b2 = 0; b2 < 2; ++b2) {
+                if (b2 && t < 3) break;
+                Walk w2 = w1;
+                walk_step<KIND>(w2, pd, ptab, b2, s + 3, k);
+                u64 h;
+                if (KIND == QD_RULE_QCGD_SPLIT_MERGE) h = sm_finish(w2, ptab);
+                else h = hash_combine(hash_combine(pd.nf, w2.lh), w2.rh);
+                leaf(b0 | (b1 << 1) | (b2 << 2), h, w2.mag);
+            }
+        }
+    }
+}
+
 constexpr int kGenThreads = 256;
 
 // One thread per work item; writes hash and magnitude of its children at
@@ -331,22 +375,13 @@ __global__ __launch_bounds__(kGenThreads) void k_generate_fast(const PDesc *D, c
         if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(pre, pd, ptab, j0, s, k);
         else flip_run<KIND == QD_RULE_QCGD_COIN>(pre, pd, j0, s, k);
         const u64 cbase = cb[p] + j0;
-        for (u32 u = 0; u < (1u << t); ++u) {
-            Walk x = pre;
-            u64 h;
-            if (KIND == QD_RULE_QCGD_SPLIT_MERGE) {
-                sm_run(x, pd, ptab, u, pd.k, k);
-                h = sm_finish(x, ptab);
-            } else {
-                flip_run<KIND == QD_RULE_QCGD_COIN>(x, pd, u, pd.k, k);
-                h = hash_combine(hash_combine(pd.nf, x.lh), x.rh);
-            }
+        walk_item<KIND>(pre, pd, ptab, s, t, k, [&](u32 u, u64 h, const cplx &m) {
             if (BATCH) h += hadd;
             const u64 c = cbase + (u64(u) << s);
             H0[c] = h;
-            M0[c] = make_double2(x.mag.re, x.mag.im);
+            M0[c] = make_double2(m.re, m.im);
             if (b1) atomicAdd(&hist[h >> (64 - b1)], 1u);
-        }
+        });
     }
     if (b1) {
         __syncthreads();
@@ -368,7 +403,7 @@ constexpr u32 ERR_SPILL1 = 32u;
 constexpr u32 kFusedBins = 512;
 
 template <int KIND, bool BATCH>
-__global__ __launch_bounds__(kGenThreads) void k_generate_scatter(const PDesc *D, const double2 *pmag, const u64 *cb,
+__global__ __launch_bounds__(kGenThreads, 3) void k_generate_scatter(const PDesc *D, const double2 *pmag, const u64 *cb,
                                                                  const u64 *wb, u64 N, const u64 *tab,
                                                                  const u64 *pbegin, qd_rule rule, u64 W, u32 b1,
                                                                  u64 cap1, u64 *cnt1, Entry *E1, u32 *err,
@@ -434,25 +469,13 @@ __global__ __launch_bounds__(kGenThreads) void k_generate_scatter(const PDesc *D
             if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(pre, pd, ptab, j0, s, k);
             else flip_run<KIND == QD_RULE_QCGD_COIN>(pre, pd, j0, s, k);
             cbase = cb[p] + j0;
-#pragma unroll
-            for (u32 u = 0; u < kKids; ++u) {
-                if (u < nk) {
-                    Walk x = pre;
-                    u64 h;
-                    if (KIND == QD_RULE_QCGD_SPLIT_MERGE) {
-                        sm_run(x, pd, ptab, u, pd.k, k);
-                        h = sm_finish(x, ptab);
-                    } else {
-                        flip_run<KIND == QD_RULE_QCGD_COIN>(x, pd, u, pd.k, k);
-                        h = hash_combine(hash_combine(pd.nf, x.lh), x.rh);
-                    }
-                    if (BATCH) h += hadd;
-                    const u32 d = b1 ? u32(h >> (64 - b1)) : 0u;
-                    hs[u] = h;
-                    ms[u] = make_double2(x.mag.re, x.mag.im);
-                    br[u] = (d << 16) | atomicAdd(&bcnt[d], 1u);
-                }
-            }
+            walk_item<KIND>(pre, pd, ptab, s, t, k, [&](u32 u, u64 h, const cplx &m) {
+                if (BATCH) h += hadd;
+                const u32 d = b1 ? u32(h >> (64 - b1)) : 0u;
+                hs[u] = h;
+                ms[u] = make_double2(m.re, m.im);
+                br[u] = (d << 16) | atomicAdd(&bcnt[d], 1u);
+            });
         }
         __syncthreads();
         for (u32 i = threadIdx.x; i < nbin; i += blockDim.x)
@@ -514,8 +537,8 @@ __global__ __launch_bounds__(kGenThreads) void k_generate_range(const PDesc *D, 
         const u64 w = w0 + threadIdx.x;
         u64 hs[1 << kItemBits];
         double2 ms[1 << kItemBits];
-        u64 cs[1 << kItemBits];
-        u32 nk = 0;
+        u32 keep = 0, s = 0;  // children kept (bit u), indexed by the unrolled child counter only
+        u64 cbase = 0;
         if (w < W) {
             u64 p;
             if (swb[kGenThreads] > w) {
@@ -532,47 +555,41 @@ __global__ __launch_bounds__(kGenThreads) void k_generate_range(const PDesc *D, 
             const PDesc pd = D[p];
             const u32 j0 = u32(w - wb[p]);
             const u32 t = min(u32(pd.k), kItemBits);
-            const u32 s = pd.k - t;
+            s = pd.k - t;
             const double2 m0 = pmag[p];
             Walk pre{};
             pre.mag = cplx{m0.x, m0.y};
             const u64 *ptab = tab + pbegin[p] / 2;
             if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(pre, pd, ptab, j0, s, k);
             else flip_run<KIND == QD_RULE_QCGD_COIN>(pre, pd, j0, s, k);
-            const u64 cbase = cb[p] + j0;
-            for (u32 u = 0; u < (1u << t); ++u) {
-                Walk x = pre;
-                u64 h;
-                if (KIND == QD_RULE_QCGD_SPLIT_MERGE) {
-                    sm_run(x, pd, ptab, u, pd.k, k);
-                    h = sm_finish(x, ptab);
-                } else {
-                    flip_run<KIND == QD_RULE_QCGD_COIN>(x, pd, u, pd.k, k);
-                    h = hash_combine(hash_combine(pd.nf, x.lh), x.rh);
-                }
+            cbase = cb[p] + j0;
+            walk_item<KIND>(pre, pd, ptab, s, t, k, [&](u32 u, u64 h, const cplx &m) {
                 const u32 top = u32(h >> 52);
                 if (COUNT) {
                     atomicAdd(&lh[top], 1u);
                 } else if (top >= rlo && top < rhi) {
-                    hs[nk] = rotl12(h);
-                    ms[nk] = make_double2(x.mag.re, x.mag.im);
-                    cs[nk] = cbase + (u64(u) << s);
-                    ++nk;
+                    hs[u] = rotl12(h);
+                    ms[u] = make_double2(m.re, m.im);
+                    keep |= 1u << u;
                 }
-            }
+            });
         }
         if (!COUNT) {
             u64 total;
-            const u64 off = block_excl_scan(u64(nk), tmp, total);
+            const u64 off = block_excl_scan(u64(__popc(keep)), tmp, total);
             if (threadIdx.x == 0 && total)
                 wbase = atomicAdd(reinterpret_cast<unsigned long long *>(rcount), (unsigned long long)total);
             __syncthreads();
-            const u64 at = wbase + off;
-            for (u32 q = 0; q < nk; ++q) {
-                HK[at + q] = hs[q];
-                MK[at + q] = ms[q];
-                CK[at + q] = cs[q];
-                if (hb) atomicAdd(&lh[hs[q] >> (64 - hb)], 1u);
+            u64 at = wbase + off;
+#pragma unroll
+            for (u32 u = 0; u < (1u << kItemBits); ++u) {
+                if (keep & (1u << u)) {
+                    HK[at] = hs[u];
+                    MK[at] = ms[u];
+                    CK[at] = cbase + (u64(u) << s);
+                    if (hb) atomicAdd(&lh[hs[u] >> (64 - hb)], 1u);
+                    ++at;
+                }
             }
         }
     }
