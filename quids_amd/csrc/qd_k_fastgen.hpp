@@ -313,6 +313,14 @@ __device__ __attribute__((always_inline)) inline void walk_item(const Walk &pre,
 
 constexpr int kGenThreads = 256;
 
+// Parent of the first work item of every chunk of kGenThreads items (the
+// chunk a generate workgroup takes per iteration): cpar[c] = the last p
+// with wb[p] <= c * kGenThreads.
+__global__ __launch_bounds__(256) void k_chunk_parent(const u64 *wb, u64 N, u64 nchunk, u64 *cpar) {
+    for (u64 c = u64(blockIdx.x) * blockDim.x + threadIdx.x; c < nchunk; c += u64(gridDim.x) * blockDim.x)
+        cpar[c] = find_parent(wb, N, c * kGenThreads);
+}
+
 // One thread per work item; writes hash and magnitude of its children at
 // their candidate index (coalesced for each of the up-to-8 children) and
 // histograms the top b1 hash bits for the first partition pass.
@@ -321,7 +329,8 @@ constexpr int kGenThreads = 256;
 // children's hashes.
 template <int KIND, bool BATCH>
 __global__ __launch_bounds__(kGenThreads) void k_generate_fast(const PDesc *D, const double2 *pmag, const u64 *cb,
-                                                              const u64 *wb, u64 N, const u64 *tab, const u64 *pbegin,
+                                                              const u64 *wb, u64 N, const u64 *cpar, const u64 *tab,
+                                                              const u64 *pbegin,
                                                               qd_rule rule, u64 W, u64 *H0, double2 *M0, u32 b1,
                                                               u64 *hist1, const qd_rule *rules, const u64 *iseg,
                                                               u32 ninst, u64 fold) {
@@ -333,7 +342,7 @@ __global__ __launch_bounds__(kGenThreads) void k_generate_fast(const PDesc *D, c
     QcgdCoef k = coef_of(rule);
     for (u64 w0 = u64(blockIdx.x) * kGenThreads; w0 < W; w0 += u64(gridDim.x) * kGenThreads) {
         __syncthreads();
-        if (threadIdx.x == 0) pfirst = find_parent(wb, N, w0);
+        if (threadIdx.x == 0) pfirst = cpar[w0 / kGenThreads];
         __syncthreads();
         const u64 pf = pfirst;
         for (u32 q = threadIdx.x; q <= kGenThreads; q += blockDim.x) swb[q] = pf + q <= N ? wb[pf + q] : ~0ull;
@@ -404,14 +413,15 @@ constexpr u32 kFusedBins = 512;
 
 template <int KIND, bool BATCH>
 __global__ __launch_bounds__(kGenThreads, 3) void k_generate_scatter(const PDesc *D, const double2 *pmag, const u64 *cb,
-                                                                 const u64 *wb, u64 N, const u64 *tab,
+                                                                 const u64 *wb, u64 N, const u64 *cpar, const u64 *tab,
                                                                  const u64 *pbegin, qd_rule rule, u64 W, u32 b1,
                                                                  u64 cap1, u64 *cnt1, Entry *E1, u32 *err,
                                                                  const qd_rule *rules, const u64 *iseg, u32 ninst,
                                                                  u64 fold) {
     constexpr u32 kKids = 1u << kItemBits;
     __shared__ u32 bcnt[kFusedBins];
-    __shared__ u64 bbase[kFusedBins];
+    __shared__ u64 bbase[kFusedBins];  // first reserved slot of the bucket
+    __shared__ u64 bidx[kFusedBins];   // its entry index in E1: bucket * cap1 + first slot
     __shared__ u64 swb[kGenThreads + 1];
     __shared__ u64 pfirst;
     const u32 nbin = 1u << b1;
@@ -419,7 +429,7 @@ __global__ __launch_bounds__(kGenThreads, 3) void k_generate_scatter(const PDesc
     bool spill = false;
     for (u64 w0 = u64(blockIdx.x) * kGenThreads; w0 < W; w0 += u64(gridDim.x) * kGenThreads) {
         __syncthreads();
-        if (threadIdx.x == 0) pfirst = find_parent(wb, N, w0);
+        if (threadIdx.x == 0) pfirst = cpar[w0 / kGenThreads];
         for (u32 i = threadIdx.x; i < nbin; i += blockDim.x) bcnt[i] = 0;
         __syncthreads();
         const u64 pf = pfirst;
@@ -479,20 +489,23 @@ __global__ __launch_bounds__(kGenThreads, 3) void k_generate_scatter(const PDesc
         }
         __syncthreads();
         for (u32 i = threadIdx.x; i < nbin; i += blockDim.x)
-            if (bcnt[i]) bbase[i] = atomicAdd(reinterpret_cast<unsigned long long *>(cnt1 + i), (unsigned long long)bcnt[i]);
+            if (bcnt[i]) {
+                const u64 b0 = atomicAdd(reinterpret_cast<unsigned long long *>(cnt1 + i), (unsigned long long)bcnt[i]);
+                bbase[i] = b0;
+                bidx[i] = u64(i) * cap1 + b0;
+            }
         __syncthreads();
 #pragma unroll
         for (u32 u = 0; u < kKids; ++u) {
             if (u < nk) {
-                const u32 d = br[u] >> 16;
-                const u64 slot = bbase[d] + (br[u] & 0xFFFF);
-                if (slot < cap1) {
+                const u32 d = br[u] >> 16, r = br[u] & 0xFFFF;
+                if (bbase[d] + r < cap1) {
                     Entry e;
                     e.hash = hs[u];
                     e.cand = cbase + (u64(u) << s);
                     e.re = ms[u].x;
                     e.im = ms[u].y;
-                    E1[u64(d) * cap1 + slot] = e;
+                    E1[bidx[d] + r] = e;
                 } else {
                     spill = true;
                 }
@@ -516,7 +529,8 @@ __device__ inline u64 rotr12(u64 k) { return (k >> 12) | (k << 52); }
 
 template <int KIND, bool COUNT>
 __global__ __launch_bounds__(kGenThreads) void k_generate_range(const PDesc *D, const double2 *pmag, const u64 *cb,
-                                                               const u64 *wb, u64 N, const u64 *tab, const u64 *pbegin,
+                                                               const u64 *wb, u64 N, const u64 *cpar, const u64 *tab,
+                                                               const u64 *pbegin,
                                                                qd_rule rule, u64 W, u32 rlo, u32 rhi, u64 *HK,
                                                                double2 *MK, u64 *CK, u64 *rcount, u32 b1, u64 *hist) {
     __shared__ u32 lh[kHistBins];
@@ -529,7 +543,7 @@ __global__ __launch_bounds__(kGenThreads) void k_generate_range(const PDesc *D, 
     const QcgdCoef k = coef_of(rule);
     for (u64 w0 = u64(blockIdx.x) * kGenThreads; w0 < W; w0 += u64(gridDim.x) * kGenThreads) {
         __syncthreads();
-        if (threadIdx.x == 0) pfirst = find_parent(wb, N, w0);
+        if (threadIdx.x == 0) pfirst = cpar[w0 / kGenThreads];
         __syncthreads();
         const u64 pf = pfirst;
         for (u32 q = threadIdx.x; q <= kGenThreads; q += blockDim.x) swb[q] = pf + q <= N ? wb[pf + q] : ~0ull;

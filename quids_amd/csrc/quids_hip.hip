@@ -154,7 +154,7 @@ struct qd_ctx {
     DBuf sel0, sel1;
     DBuf bitmap, wprefix, padded, chunks, scanblk;
     DBuf biglist, redolist, bigsoff, bigkey, bigcand, bigmag, bigseg;
-    DBuf pdesc, wi, wb, ebuf2, cnt2, slist;
+    DBuf pdesc, wi, wb, cpar, ebuf2, cnt2, slist;
     DBuf adjk, adjv, adjg, adjd, adjp;
     DBuf pool0, pool1, pwpre, blist2;
     // output buffers of batched chunks (objects, begin, size, mag per chunk),
@@ -169,7 +169,7 @@ struct qd_ctx {
     std::vector<DBuf *> all_named() {
         return {&nc, &cb, &pre, &tab, &kept, &keys, &ebuf0, &ebuf1, &hist1, &off1, &cur1, &off2,
                 &nhist, &counters, &sel0, &sel1, &bitmap, &wprefix, &padded, &chunks, &scanblk,
-                &biglist, &redolist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &ebuf2, &cnt2, &slist,
+                &biglist, &redolist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &cpar, &ebuf2, &cnt2, &slist,
                 &adjk, &adjv, &adjg, &adjd, &adjp, &bseg, &boseg, &brules, &bsinst, &bcount, &bsel, &btrunc,
                 &blist, &bcur, &btile, &bhist, &bkseg, &bchunk, &bcsum, &btot, &pool0, &pool1, &pwpre, &blist2};
     }
@@ -677,7 +677,7 @@ static int range_step(qd_ctx *ctx, const Parents &P, const qd_rule &rule, u64 C,
     QD_HIP(hipMemsetAsync(hist1, 0, kHistBins * sizeof(u64), s));
     prof_begin(ctx, QD_PHASE_GENERATE);
     hipLaunchKernelGGL((k_generate_range<KIND, true>), dim3(gg), dim3(kGenThreads), 0, s, ctx->pdesc.as<PDesc>(), P.mag,
-                       ctx->cb.as<u64>(), ctx->wb.as<u64>(), N, ctx->tab.as<u64>(), P.begin, rule, nitems, 0u, 0u,
+                       ctx->cb.as<u64>(), ctx->wb.as<u64>(), N, ctx->cpar.as<u64>(), ctx->tab.as<u64>(), P.begin, rule, nitems, 0u, 0u,
                        (u64 *)nullptr, (double2 *)nullptr, (u64 *)nullptr, (u64 *)nullptr, 0u, hist1);
     QD_HIP(hipGetLastError());
     prof_end(ctx, QD_PHASE_GENERATE);
@@ -734,7 +734,7 @@ static int range_step(qd_ctx *ctx, const Parents &P, const qd_rule &rule, u64 C,
         u64 *CK = reinterpret_cast<u64 *>(MK + n);
         prof_begin(ctx, QD_PHASE_GENERATE);
         hipLaunchKernelGGL((k_generate_range<KIND, false>), dim3(gg), dim3(kGenThreads), 0, s, ctx->pdesc.as<PDesc>(),
-                           P.mag, ctx->cb.as<u64>(), ctx->wb.as<u64>(), N, ctx->tab.as<u64>(), P.begin, rule, nitems,
+                           P.mag, ctx->cb.as<u64>(), ctx->wb.as<u64>(), N, ctx->cpar.as<u64>(), ctx->tab.as<u64>(), P.begin, rule, nitems,
                            cut[r], cut[r + 1], HK, MK, CK, counters + 7, b1, hist1);
         QD_HIP(hipGetLastError());
         prof_end(ctx, QD_PHASE_GENERATE);
@@ -1096,6 +1096,13 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
         QD_HIP(hipStreamSynchronize(s));
         prof_collect(ctx);
         nitems = hb[3];
+        if (nitems && !(u32(hb[2]) & ERR_WIDE)) {
+            const u64 nchunk = (nitems + kGenThreads - 1) / kGenThreads;
+            QD_TRY(ctx->cpar.ensure(nchunk * sizeof(u64)));
+            hipLaunchKernelGGL(k_chunk_parent, dim3(grid_for(nchunk)), dim3(256), 0, s, ctx->wb.as<u64>(), N, nchunk,
+                               ctx->cpar.as<u64>());
+            QD_HIP(hipGetLastError());
+        }
         if (BATCH && (u32(hb[2]) & ERR_WIDE))
             return fail(QD_ERR_UNSUPPORTED, "the batched step handles graphs of at most 64 nodes");
         if (u32(hb[2]) & ERR_WIDE) {  // a graph wider than 64 nodes: generic kernels
@@ -1171,12 +1178,12 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
         prof_begin(ctx, QD_PHASE_GENERATE);
         if (BATCH)
             hipLaunchKernelGGL((k_generate_scatter<KIND, true>), dim3(std::max(gg, 1u)), dim3(kGenThreads), 0, s,
-                               ctx->pdesc.as<PDesc>(), P.mag, ctx->cb.as<u64>(), ctx->wb.as<u64>(), N,
+                               ctx->pdesc.as<PDesc>(), P.mag, ctx->cb.as<u64>(), ctx->wb.as<u64>(), N, ctx->cpar.as<u64>(),
                                ctx->tab.as<u64>(), P.begin, rule, nitems, b1, cap1, ctx->cur1.as<u64>(), E1, errw,
                                bt->d_rules, bt->d_oseg, bt->n, kFold);
         else
             hipLaunchKernelGGL((k_generate_scatter<KIND, false>), dim3(std::max(gg, 1u)), dim3(kGenThreads), 0, s,
-                               ctx->pdesc.as<PDesc>(), P.mag, ctx->cb.as<u64>(), ctx->wb.as<u64>(), N,
+                               ctx->pdesc.as<PDesc>(), P.mag, ctx->cb.as<u64>(), ctx->wb.as<u64>(), N, ctx->cpar.as<u64>(),
                                ctx->tab.as<u64>(), P.begin, rule, nitems, b1, cap1, ctx->cur1.as<u64>(), E1, errw,
                                (const qd_rule *)nullptr, (const u64 *)nullptr, 0u, u64(0));
         QD_HIP(hipGetLastError());
@@ -1198,12 +1205,12 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
             unsigned(std::min<u64>((nitems + kGenThreads - 1) / kGenThreads, ctx->tune.generate_grid));
         if (BATCH)
             hipLaunchKernelGGL((k_generate_fast<KIND, true>), dim3(std::max(gg, 1u)), dim3(kGenThreads), 0, s,
-                               ctx->pdesc.as<PDesc>(), P.mag, ctx->cb.as<u64>(), ctx->wb.as<u64>(), N,
+                               ctx->pdesc.as<PDesc>(), P.mag, ctx->cb.as<u64>(), ctx->wb.as<u64>(), N, ctx->cpar.as<u64>(),
                                ctx->tab.as<u64>(), P.begin, rule, nitems, H0, M0, b1, hist1, bt->d_rules, bt->d_oseg,
                                bt->n, kFold);
         else
             hipLaunchKernelGGL((k_generate_fast<KIND, false>), dim3(std::max(gg, 1u)), dim3(kGenThreads), 0, s,
-                               ctx->pdesc.as<PDesc>(), P.mag, ctx->cb.as<u64>(), ctx->wb.as<u64>(), N,
+                               ctx->pdesc.as<PDesc>(), P.mag, ctx->cb.as<u64>(), ctx->wb.as<u64>(), N, ctx->cpar.as<u64>(),
                                ctx->tab.as<u64>(), P.begin, rule, nitems, H0, M0, b1, hist1,
                                (const qd_rule *)nullptr, (const u64 *)nullptr, 0u, u64(0));
     } else {
