@@ -36,10 +36,11 @@ __device__ inline u32 align_pad(u32 size, u32 align) {
 
 template <int KIND>
 __global__ __launch_bounds__(256) void k_child_size(Parents P, const u64 *cb, qd_rule rule, const u64 *orep, u64 K,
-                                                    u32 align, u32 *osize, u64 *padded) {
+                                                    u32 align, u32 *osize, u64 *padded, u64 *opar) {
     for (u64 k = u64(blockIdx.x) * blockDim.x + threadIdx.x; k < K; k += u64(gridDim.x) * blockDim.x) {
         const u64 c = orep[k];
         const u64 p = find_parent(cb, P.n, c);
+        opar[k] = p;  // the populate kernels start from it
         const u32 j = u32(c - cb[p]);
         const u8 *o = P.obj + P.begin[p];
         u32 s;
@@ -127,14 +128,15 @@ __device__ inline void populate_flip_wave(const u8 *o, u32 psz, u8 *d, u64 span,
 // erase_create / coin / hadamard: one wavefront per survivor, four per
 // block, no LDS.
 template <int KIND>
-__global__ __launch_bounds__(256) void k_populate_flip(Parents P, const u64 *cb, qd_rule rule, const u64 *orep, u64 K,
-                                                       const u64 *obegin, const u32 *osize, u8 *oobj) {
+__global__ __launch_bounds__(256) void k_populate_flip(Parents P, const u64 *cb, qd_rule rule, const u64 *orep,
+                                                       const u64 *opar, u64 K, const u64 *obegin, const u32 *osize,
+                                                       u8 *oobj) {
     const u32 lane = threadIdx.x & (kWave - 1);
     const u64 wave = (u64(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
     const u64 nwaves = u64(gridDim.x) * blockDim.x / kWave;
     for (u64 k = wave; k < K; k += nwaves) {
         const u64 c = orep[k];
-        const u64 p = find_parent(cb, P.n, c);
+        const u64 p = opar[k];
         const u32 j = u32(c - cb[p]);
         const u8 *o = P.obj + P.begin[p];
         const u32 psz = P.size[p];
@@ -155,8 +157,9 @@ __global__ __launch_bounds__(256) void k_populate_flip(Parents P, const u64 *cb,
 }
 
 // split_merge: one single-wave block per survivor (see above).
-__global__ __launch_bounds__(64) void k_populate_sm(Parents P, const u64 *cb, qd_rule rule, const u64 *orep, u64 K,
-                                                 const u64 *obegin, const u32 *osize, u8 *oobj) {
+__global__ __launch_bounds__(64) void k_populate_sm(Parents P, const u64 *cb, qd_rule rule, const u64 *orep,
+                                                 const u64 *opar, u64 K, const u64 *obegin, const u32 *osize,
+                                                 u8 *oobj) {
     __shared__ __align__(16) u8 pbuf[kPopParent];
     __shared__ __align__(16) u8 cbuf[kPopChild];
     __shared__ u8 ckind[kPopNodes], cl[kPopNodes], cr[kPopNodes];
@@ -165,7 +168,7 @@ __global__ __launch_bounds__(64) void k_populate_sm(Parents P, const u64 *cb, qd
     const u32 lane = threadIdx.x;
     for (u64 k = blockIdx.x; k < K; k += gridDim.x) {
         const u64 c = orep[k];
-        const u64 p = find_parent(cb, P.n, c);
+        const u64 p = opar[k];
         const u32 j = u32(c - cb[p]);
         const u8 *o = P.obj + P.begin[p];
         const u32 psz = P.size[p];

@@ -154,7 +154,7 @@ struct qd_ctx {
     DBuf sel0, sel1;
     DBuf bitmap, wprefix, padded, chunks, scanblk;
     DBuf biglist, redolist, bigsoff, bigkey, bigcand, bigmag, bigseg;
-    DBuf pdesc, wi, wb, cpar, ebuf2, cnt2, slist;
+    DBuf pdesc, wi, wb, cpar, opar, ebuf2, cnt2, slist;
     DBuf adjk, adjv, adjg, adjd, adjp;
     DBuf pool0, pool1, pwpre, blist2;
     // output buffers of batched chunks (objects, begin, size, mag per chunk),
@@ -169,7 +169,7 @@ struct qd_ctx {
     std::vector<DBuf *> all_named() {
         return {&nc, &cb, &pre, &tab, &kept, &keys, &ebuf0, &ebuf1, &hist1, &off1, &cur1, &off2,
                 &nhist, &counters, &sel0, &sel1, &bitmap, &wprefix, &padded, &chunks, &scanblk,
-                &biglist, &redolist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &cpar, &ebuf2, &cnt2, &slist,
+                &biglist, &redolist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &cpar, &opar, &ebuf2, &cnt2, &slist,
                 &adjk, &adjv, &adjg, &adjd, &adjp, &bseg, &boseg, &brules, &bsinst, &bcount, &bsel, &btrunc,
                 &blist, &bcur, &btile, &bhist, &bkseg, &bchunk, &bcsum, &btot, &pool0, &pool1, &pwpre, &blist2};
     }
@@ -625,9 +625,10 @@ static int populate_out(qd_ctx *ctx, const Parents &P, const qd_rule &rule, cons
                         const qd_params &prm, qd_state *out, qd_step_stats &st) {
     hipStream_t s = ctx->stream;
     u64 *hb = ctx->host;
+    QD_TRY(ctx->opar.ensure(std::max<u64>(K, 1) * sizeof(u64)));
     prof_begin(ctx, QD_PHASE_FINALIZE);
     hipLaunchKernelGGL(k_child_size<KIND>, dim3(grid_for(K)), dim3(256), 0, s, P, ctx->cb.as<u64>(), rule, orep, K,
-                       prm.align_byte_length, out->size.as<u32>(), ctx->padded.as<u64>());
+                       prm.align_byte_length, out->size.as<u32>(), ctx->padded.as<u64>(), ctx->opar.as<u64>());
     QD_HIP(hipGetLastError());
     QD_TRY(run_scan(ctx, LoadU64{ctx->padded.as<u64>()}, K, out->begin.as<u64>()));
     prof_end(ctx, QD_PHASE_FINALIZE);
@@ -637,12 +638,12 @@ static int populate_out(qd_ctx *ctx, const Parents &P, const qd_rule &rule, cons
     prof_begin(ctx, QD_PHASE_FINALIZE);
     if (KIND == QD_RULE_QCGD_SPLIT_MERGE)
         hipLaunchKernelGGL(k_populate_sm, dim3(unsigned(std::min<u64>(K, 65536))), dim3(64), 0, s, P,
-                           ctx->cb.as<u64>(), rule, orep, K, out->begin.as<u64>(), out->size.as<u32>(),
-                           out->objects.as<u8>());
+                           ctx->cb.as<u64>(), rule, orep, (const u64 *)ctx->opar.as<u64>(), K, out->begin.as<u64>(),
+                           out->size.as<u32>(), out->objects.as<u8>());
     else
         hipLaunchKernelGGL(k_populate_flip<KIND>, dim3(unsigned(std::min<u64>((K + 3) / 4, 65536))), dim3(256), 0, s,
-                           P, ctx->cb.as<u64>(), rule, orep, K, out->begin.as<u64>(), out->size.as<u32>(),
-                           out->objects.as<u8>());
+                           P, ctx->cb.as<u64>(), rule, orep, (const u64 *)ctx->opar.as<u64>(), K,
+                           out->begin.as<u64>(), out->size.as<u32>(), out->objects.as<u8>());
     QD_HIP(hipGetLastError());
     out->n = K;
     out->bytes = bytes;
