@@ -36,105 +36,130 @@ enum : u8 { PD_FS = 1, PD_LM = 2, PD_HMLZ = 4 };
 // step then runs on the generic per-byte kernels).
 constexpr u32 ERR_WIDE = 8u;
 
-// One wavefront per parent.
+// Descriptor of parent p, computed by a segment of SEG lanes (one lane per
+// node: SEG = 64, or 16 when four parents of at most 16 nodes share a
+// wavefront; the segment's lanes are active together).  have: the segment
+// has a parent.  Ballots and shuffles stay within the segment.
+template <int KIND, int SEG>
+__device__ inline void prep_parent(const Parents &P, u64 p, bool have, const u64 *keep, PDesc *D, u32 *nc, u32 *wi,
+                                   u64 *tab, u32 &local_max, u32 *err) {
+    const u32 lane = threadIdx.x & (kWave - 1);
+    const u32 sl = lane & (SEG - 1);      // node of this lane
+    const u32 sbase = lane & ~(SEG - 1);  // first lane of the segment
+    const u64 smask = SEG == kWave ? ~0ull : (((1ull << SEG) - 1) << sbase);
+    auto sballot = [&](bool x) -> u64 { return (u64(__ballot(x)) & smask) >> sbase; };
+    if (!have) return;
+    const u8 *o = P.obj + P.begin[p];
+    const u32 sz = P.size[p];
+    PDesc d{};
+    bool ok = !(keep && !((keep[p >> 6] >> (p & 63)) & 1));
+    // qcgd_valid, one node per lane
+    const u32 n = (ok && sz >= 4) ? ld16(o) : 0;
+    if (ok) {
+        bool valid = n > 0 && 4 + 4 * u64(n) <= sz;
+        if (valid) {
+            const u8 *nb = o + 2 + 2 * n;
+            const u32 m = ld16(nb + 2 * n);
+            valid = ld16(nb) == 0 && 4 + 4 * u64(n) + 16 * u64(m) == sz;
+            if (valid && n <= SEG) {
+                bool bad = false;
+                if (sl < n) {
+                    const u32 b = ld16(nb + 2 * sl), e = ld16(nb + 2 * sl + 2);
+                    if (e <= b || e > m) bad = true;
+                    else {
+                        const u32 mi = e - b;
+                        const i16 t = ldi16(o + 4 + 4 * n + 16 * size_t(b) + 2);
+                        if (t >= 0) bad = t < 2 || u32(t) >= mi;
+                        else if (t == QCGD_DOT_L || t == QCGD_DOT_R) bad = mi < 2;
+                        else bad = t != QCGD_ELEMENT;
+                    }
+                }
+                valid = sballot(bad) == 0;
+            }
+        }
+        if (!valid) {
+            if (sl == 0) atomicOr(err, ERR_OBJECT);
+            ok = false;
+        }
+    }
+    if (ok && n > 64) {
+        if (sl == 0) atomicOr(err, ERR_WIDE);
+        ok = false;
+    }
+    if (!ok) {
+        if (sl == 0) {
+            nc[p] = 0;
+            wi[p] = 0;
+            D[p] = d;
+        }
+        return;
+    }
+    Graph g(o);
+    const bool in = sl < n;
+    const bool Li = in && g.L(sl), Ri = in && g.R(sl);
+    const u64 lm = sballot(Li), rm = sballot(Ri);
+    d.lmask = lm;
+    d.rmask = rm;
+    d.n = u8(n);
+    u32 k = 0;
+    if (KIND == QD_RULE_QCGD_SPLIT_MERGE) {
+        const bool Ln = sl + 1 < n && ((lm >> (sl + 1)) & 1);
+        const bool Rn = sl + 1 < n && ((rm >> (sl + 1)) & 1);
+        const bool split = Li && Ri;
+        const bool merge = !split && sl + 1 < n && Li && Rn && !Ln;
+        d.dmask = sballot(in && (split || merge));
+        const bool fs = (lm & 1) && (rm & 1);
+        const bool lmg = !fs && n > 1 && (rm & 1) && ((lm >> (n - 1)) & 1) && !((rm >> (n - 1)) & 1);
+        d.flags = u8((fs ? PD_FS : 0) | (lmg ? PD_LM : 0));
+        if (sl == 0 && sm_first_hmlz(g)) d.flags |= PD_HMLZ;
+        d.flags = u8(__shfl(int(d.flags), int(sbase), kWave));
+        k = u32(__popcll(d.dmask)) + (lmg ? 1 : 0);
+        if (in) {
+            u64 *t = tab + P.begin[p] / 2 + 4 * sl;
+            const u32 b = g.name_begin(sl);
+            t[0] = g.sub_hash(b);
+            t[1] = split_left_hash(g, b);
+            t[2] = split_right_hash(g, b);
+            t[3] = merge_hash(g, b, g.name_begin(sl + 1 < n ? sl + 1 : 0));
+        }
+        local_max = max(local_max, 4 * sz);
+    } else {
+        const bool coin = KIND == QD_RULE_QCGD_COIN;
+        d.dmask = sballot(in && (coin ? (Li != Ri) : (Li == Ri)));
+        k = u32(__popcll(d.dmask));
+        const u64 hr = in ? g.sub_hash(g.name_begin(sl)) : 0;
+        u64 nf = 0;
+        for (u32 i = 0; i < n; ++i) nf = hash_combine(nf, __shfl(hr, int(sbase + i), kWave));
+        d.nf = nf;
+        local_max = max(local_max, sz);
+    }
+    d.k = u8(k);
+    d.nc = k >= 32 ? 0u : (1u << k);
+    if (sl == 0) {
+        const u32 t = min(k, kItemBits);
+        nc[p] = d.nc;
+        wi[p] = d.nc >> t;
+        D[p] = d;
+    }
+}
+
+// Four parents per wavefront: in 16-lane segments when none has more than
+// 16 nodes, else one after the other on the whole wavefront.
 template <int KIND>
 __global__ __launch_bounds__(256) void k_prep_fast(Parents P, const u64 *keep, PDesc *D, u32 *nc, u32 *wi, u64 *tab,
                                                    u32 *maxsz, u32 *err) {
     const u32 lane = threadIdx.x & (kWave - 1);
     const u64 nwaves = u64(gridDim.x) * (blockDim.x / kWave);
     u32 local_max = 0;
-    for (u64 p = u64(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave; p < P.n; p += nwaves) {
-        const u8 *o = P.obj + P.begin[p];
-        const u32 sz = P.size[p];
-        PDesc d{};
-        bool ok = !(keep && !((keep[p >> 6] >> (p & 63)) & 1));
-        // qcgd_valid, one node per lane
-        const u32 n = (ok && sz >= 4) ? ld16(o) : 0;
-        if (ok) {
-            bool valid = n > 0 && 4 + 4 * u64(n) <= sz;
-            if (valid) {
-                const u8 *nb = o + 2 + 2 * n;
-                const u32 m = ld16(nb + 2 * n);
-                valid = ld16(nb) == 0 && 4 + 4 * u64(n) + 16 * u64(m) == sz;
-                if (valid && n <= 64) {
-                    bool bad = false;
-                    if (lane < n) {
-                        const u32 b = ld16(nb + 2 * lane), e = ld16(nb + 2 * lane + 2);
-                        if (e <= b || e > m) bad = true;
-                        else {
-                            const u32 mi = e - b;
-                            const i16 t = ldi16(o + 4 + 4 * n + 16 * size_t(b) + 2);
-                            if (t >= 0) bad = t < 2 || u32(t) >= mi;
-                            else if (t == QCGD_DOT_L || t == QCGD_DOT_R) bad = mi < 2;
-                            else bad = t != QCGD_ELEMENT;
-                        }
-                    }
-                    valid = __ballot(bad) == 0;
-                }
-            }
-            if (!valid) {
-                if (lane == 0) atomicOr(err, ERR_OBJECT);
-                ok = false;
-            }
-        }
-        if (ok && n > 64) {
-            if (lane == 0) atomicOr(err, ERR_WIDE);
-            ok = false;
-        }
-        if (!ok) {
-            if (lane == 0) {
-                nc[p] = 0;
-                wi[p] = 0;
-                D[p] = d;
-            }
-            continue;
-        }
-        Graph g(o);
-        const bool in = lane < n;
-        const bool Li = in && g.L(lane), Ri = in && g.R(lane);
-        const u64 lm = __ballot(Li), rm = __ballot(Ri);
-        d.lmask = lm;
-        d.rmask = rm;
-        d.n = u8(n);
-        u32 k = 0;
-        if (KIND == QD_RULE_QCGD_SPLIT_MERGE) {
-            const bool Ln = lane + 1 < n && ((lm >> (lane + 1)) & 1);
-            const bool Rn = lane + 1 < n && ((rm >> (lane + 1)) & 1);
-            const bool split = Li && Ri;
-            const bool merge = !split && lane + 1 < n && Li && Rn && !Ln;
-            d.dmask = __ballot(in && (split || merge));
-            const bool fs = (lm & 1) && (rm & 1);
-            const bool lmg = !fs && n > 1 && (rm & 1) && ((lm >> (n - 1)) & 1) && !((rm >> (n - 1)) & 1);
-            d.flags = u8((fs ? PD_FS : 0) | (lmg ? PD_LM : 0));
-            if (lane == 0 && sm_first_hmlz(g)) d.flags |= PD_HMLZ;
-            d.flags = u8(__shfl(int(d.flags), 0, kWave));
-            k = u32(__popcll(d.dmask)) + (lmg ? 1 : 0);
-            if (in) {
-                u64 *t = tab + P.begin[p] / 2 + 4 * lane;
-                const u32 b = g.name_begin(lane);
-                t[0] = g.sub_hash(b);
-                t[1] = split_left_hash(g, b);
-                t[2] = split_right_hash(g, b);
-                t[3] = merge_hash(g, b, g.name_begin(lane + 1 < n ? lane + 1 : 0));
-            }
-            local_max = max(local_max, 4 * sz);
+    for (u64 p0 = 4 * (u64(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave); p0 < P.n; p0 += 4 * nwaves) {
+        u32 nq = 0;  // node count of parent p0 + lane (lanes 0..3), as far as its bytes tell
+        if (lane < 4 && p0 + lane < P.n && P.size[p0 + lane] >= 4) nq = ld16(P.obj + P.begin[p0 + lane]);
+        if (__ballot(nq > 16) == 0) {
+            const u64 p = p0 + lane / 16;
+            prep_parent<KIND, 16>(P, p, p < P.n, keep, D, nc, wi, tab, local_max, err);
         } else {
-            const bool coin = KIND == QD_RULE_QCGD_COIN;
-            d.dmask = __ballot(in && (coin ? (Li != Ri) : (Li == Ri)));
-            k = u32(__popcll(d.dmask));
-            const u64 hr = in ? g.sub_hash(g.name_begin(lane)) : 0;
-            u64 nf = 0;
-            for (u32 i = 0; i < n; ++i) nf = hash_combine(nf, __shfl(hr, int(i), kWave));
-            d.nf = nf;
-            local_max = max(local_max, sz);
-        }
-        d.k = u8(k);
-        d.nc = k >= 32 ? 0u : (1u << k);
-        if (lane == 0) {
-            const u32 t = min(k, kItemBits);
-            nc[p] = d.nc;
-            wi[p] = d.nc >> t;
-            D[p] = d;
+            for (u32 q = 0; q < 4; ++q)
+                prep_parent<KIND, kWave>(P, p0 + q, p0 + q < P.n, keep, D, nc, wi, tab, local_max, err);
         }
     }
     for (int off = kWave / 2; off > 0; off >>= 1) local_max = max(local_max, (u32)__shfl_xor(int(local_max), off, kWave));

@@ -1082,7 +1082,7 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
         QD_TRY(ctx->wi.ensure(N * sizeof(u32)));
         QD_TRY(ctx->wb.ensure((N + 1) * sizeof(u64)));
         prof_begin(ctx, QD_PHASE_PREP);
-        hipLaunchKernelGGL(k_prep_fast<KIND>, dim3(grid_for(N, 4, 65536)), dim3(256), 0, s, P, keep,
+        hipLaunchKernelGGL(k_prep_fast<KIND>, dim3(grid_for(N, 16, 65536)), dim3(256), 0, s, P, keep,
                            ctx->pdesc.as<PDesc>(), ctx->nc.as<u32>(), ctx->wi.as<u32>(), ctx->tab.as<u64>(), maxsz,
                            errw);
         QD_HIP(hipGetLastError());

@@ -1,5 +1,5 @@
-"""The autotuner: picks a tuning from its grid, sets it on the context, and
-every tuning gives bit-identical steps."""
+"""The autotuner: picks the fastest of its grid and the starting tuning, sets
+it on the context, and every tuning gives bit-identical steps."""
 import math
 
 import numpy as np
@@ -33,7 +33,8 @@ def test_autotune_sets_a_grid_point_and_keeps_results():
         assert res["tried"] == 4
         t = ctx.get_tuning()
         for k, v in res["tuning"].items():
-            assert t[k] == v and v in grid[k]
+            # a grid point, or the starting value when none was faster
+            assert t[k] == v and (v in grid[k] or v == before[k])
         got = q.it_t()
         q.simulate(st, rules[0], got, None, 5000)
         a, b = ref.download(), got.download()
