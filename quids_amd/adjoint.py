@@ -8,8 +8,9 @@ one parameter vector.  The loss is ``average_value`` of an observable of the
 final state (src/quids.hpp:208-234).
 
 The forward pass runs on the device and keeps a copy of the state every
-``checkpoint_every`` iterations.  The backward pass recomputes each segment
-from its checkpoint and runs the library's adjoint step
+``checkpoint_every`` iterations (the last segment keeps all its states).
+The backward pass recomputes each earlier segment from its checkpoint and
+runs the library's adjoint step
 (``qd_simulate_adjoint``: the transpose of the interference scatter-add as a
 hash-keyed gather, the truncation held fixed) and modifier adjoint through it
 in reverse.  Cotangent vectors live in torch device tensors; the library
@@ -144,34 +145,49 @@ def loss(program, params, initial, observable, arg=0):
 def forward_checkpointed(program, params, initial, observable, arg=0, checkpoint_every=2):
     """Forward pass keeping a copy of the state before every
     checkpoint_every-th iteration: (loss, checkpoints, cotangent of the final
-    state's magnitudes as an (N, 2) device tensor)."""
+    state's magnitudes as an (N, 2) device tensor).  The last segment keeps
+    its intermediate states (its records) instead of a copy of its first
+    state, so the backward pass does not recompute it."""
     params = np.asarray(params, np.float64)
     k = max(int(checkpoint_every), 1)
+    last = ((program.n_iter - 1) // k) * k if program.n_iter else 0
     checkpoints = {}
     cur = initial.copy()
+    tail = []
     for i in range(program.n_iter):
+        if i >= last:
+            cur = _iterate(program, params, cur, tail)
+            continue
         if i % k == 0:
             checkpoints[i] = cur.copy()
         cur = _iterate(program, params, cur)
     value, g = average_value(cur, observable, arg, with_cot=True)
-    cur.close()
+    if tail:
+        checkpoints[last] = tail  # the final state is the last record's output
+    else:
+        cur.close()
     return value, checkpoints, g
 
 
 def backward_checkpointed(program, params, checkpoints, g):
-    """Recomputes each segment from its checkpoint (consuming them) and runs
-    the adjoint steps through it in reverse, starting from the cotangent g of
-    the final state.  Returns dloss/dparams."""
+    """Runs the adjoint steps through each segment in reverse, starting from
+    the cotangent g of the final state; a segment stored as a state is first
+    recomputed from it, one stored as records is used as it is (both are
+    consumed).  Returns dloss/dparams."""
     params = np.asarray(params, np.float64)
     grads = np.zeros(max(program.num_params, len(params)), np.float64)
     starts = sorted(checkpoints)
     for si in reversed(range(len(starts))):
         i0 = starts[si]
         i1 = starts[si + 1] if si + 1 < len(starts) else program.n_iter
-        cur = checkpoints.pop(i0)
-        records = []
-        for _ in range(i0, i1):
-            cur = _iterate(program, params, cur, records)
+        saved = checkpoints.pop(i0)
+        if isinstance(saved, list):
+            records = saved
+        else:
+            records = []
+            cur = saved
+            for _ in range(i0, i1):
+                cur = _iterate(program, params, cur, records)
         for rec in reversed(records):
             if rec[0] == "rule":
                 _, op, rule, it_in, it_out = rec

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include "qd_device.hpp"
+#include "qd_k_fastgen.hpp"
 #include "qd_k_generate.hpp"
 #include "qd_rules.hpp"
 
@@ -135,6 +136,96 @@ __global__ __launch_bounds__(256) void k_adjoint(Parents P, const u64 *cb, qd_ru
             gx[p] = make_double2(axr, axi);
             for (int t = 0; t < ndir; ++t) gpart[u64(t) * P.n + p] = at[t];
         }
+    }
+}
+
+// The same gather over the QCGD fast walk: one thread per work item (up to
+// 8 sibling children sharing a walk prefix, enumerated depth first as in
+// the generate kernels), the children's rule factors f as dual numbers
+// starting from 1 (multiplied in the same order as the plain magnitudes),
+// the parent magnitude x applied at the end.  A work item writes its sums
+// conj(f) g and Re(conj(g) x df/dt) to wpart[w * kAdjPart ..]; then
+// k_adjoint_reduce folds each parent's work items in order, so the outputs
+// do not depend on scheduling.
+constexpr int kAdjPart = 2 + kMaxDir;
+
+template <int KIND>
+__global__ __launch_bounds__(kGenThreads) void k_adjoint_fast(const PDesc *D, const double2 *pmag, const u64 *wb, u64 N,
+                                                             const u64 *cpar, const u64 *tab, const u64 *pbegin,
+                                                             DualCoef dk, int ndir, const u64 *keys, const u32 *vals,
+                                                             u64 mask, u64 special, const double2 *gy, u64 W,
+                                                             double *wpart) {
+    __shared__ u64 swb[kGenThreads + 1];
+    __shared__ u64 pfirst;
+    for (u64 w0 = u64(blockIdx.x) * kGenThreads; w0 < W; w0 += u64(gridDim.x) * kGenThreads) {
+        __syncthreads();
+        if (threadIdx.x == 0) pfirst = cpar[w0 / kGenThreads];
+        __syncthreads();
+        const u64 pf = pfirst;
+        for (u32 q = threadIdx.x; q <= kGenThreads; q += blockDim.x) swb[q] = pf + q <= N ? wb[pf + q] : ~0ull;
+        __syncthreads();
+        const u64 w = w0 + threadIdx.x;
+        if (w >= W) continue;
+        u64 p;
+        if (swb[kGenThreads] > w) {
+            u32 lo = 0, hi = kGenThreads;
+            while (lo + 1 < hi) {
+                const u32 mid = (lo + hi) >> 1;
+                if (swb[mid] <= w) lo = mid;
+                else hi = mid;
+            }
+            p = pf + lo;
+        } else {
+            p = find_parent(wb, N, w);
+        }
+        const PDesc pd = D[p];
+        const u32 j0 = u32(w - wb[p]);
+        const u32 t = min(u32(pd.k), kItemBits);
+        const u32 s = pd.k - t;
+        const double2 xm = pmag[p];
+        const cplx x{xm.x, xm.y};
+        WalkT<DualC> pre{};
+        pre.mag = dual_one();
+        const u64 *ptab = tab + pbegin[p] / 2;
+        if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(pre, pd, ptab, j0, s, dk);
+        else flip_run<KIND == QD_RULE_QCGD_COIN>(pre, pd, j0, s, dk);
+        double axr = 0, axi = 0, at[kMaxDir] = {};
+        walk_item<KIND>(pre, pd, ptab, s, t, dk, [&](u32, u64 h, const DualC &f) {
+            u32 k;
+            if (!adj_lookup(keys, vals, mask, special, h, k)) return;
+            const double2 gk = gy[k];
+            // conj(f) g
+            axr += f.v.re * gk.x + f.v.im * gk.y;
+            axi += f.v.re * gk.y - f.v.im * gk.x;
+#pragma unroll
+            for (int d = 0; d < kMaxDir; ++d) {
+                if (d < ndir) {
+                    const cplx xd = cmul(x, f.t[d]);
+                    at[d] += gk.x * xd.re + gk.y * xd.im;  // Re(conj(g) x df)
+                }
+            }
+        });
+        double *o = wpart + w * kAdjPart;
+        o[0] = axr;
+        o[1] = axi;
+#pragma unroll
+        for (int d = 0; d < kMaxDir; ++d) o[2 + d] = at[d];
+    }
+}
+
+// Per parent: its work items' partial sums in order.
+__global__ __launch_bounds__(256) void k_adjoint_reduce(const u64 *wb, u64 N, const double *wpart, int ndir,
+                                                        double2 *gx, double *gpart) {
+    for (u64 p = u64(blockIdx.x) * blockDim.x + threadIdx.x; p < N; p += u64(gridDim.x) * blockDim.x) {
+        double a[kAdjPart] = {};
+        for (u64 w = wb[p]; w < wb[p + 1]; ++w) {
+#pragma unroll
+            for (int i = 0; i < kAdjPart; ++i) a[i] += wpart[w * kAdjPart + i];
+        }
+        gx[p] = make_double2(a[0], a[1]);
+#pragma unroll
+        for (int d = 0; d < kMaxDir; ++d)
+            if (d < ndir) gpart[u64(d) * N + p] = a[2 + d];
     }
 }
 

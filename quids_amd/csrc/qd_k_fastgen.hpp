@@ -166,20 +166,24 @@ __global__ __launch_bounds__(256) void k_prep_fast(Parents P, const u64 *keep, P
     if (lane == 0 && local_max) atomicMax(maxsz, local_max);
 }
 
-// Walk state of one child (or of the shared prefix of a work item).
-struct Walk {
+// Walk state of one child (or of the shared prefix of a work item).  M is
+// the magnitude type: cplx, or DualC (with derivatives, for the adjoint).
+template <typename M>
+struct WalkT {
     u64 nh, lh, rh;
-    cplx mag;
+    M mag;
     u32 i, end, c, d;
     bool started, overflow;
 };
+using Walk = WalkT<cplx>;
 
 // split_merge child walk (populate_child, qcgd.hpp:643-848) over the
 // parent's masks and root-hash table.  Consumes decisions d while d <
 // dlimit, the bit of decision d being bit (d - d0) of `bits` where d0 is
 // w.d on entry; returns when the next decision is not available or the walk
 // is complete.
-__device__ inline void sm_run(Walk &w, const PDesc &pd, const u64 *tab, u32 bits, u32 dlimit, const QcgdCoef &k) {
+template <typename W, typename KC>
+__device__ inline void sm_run(W &w, const PDesc &pd, const u64 *tab, u32 bits, u32 dlimit, const KC &k) {
     const u32 n = pd.n;
     const u32 d0 = w.d;
     auto emit = [&](u64 root, bool l, bool r) {
@@ -258,7 +262,8 @@ __device__ inline void sm_run(Walk &w, const PDesc &pd, const u64 *tab, u32 bits
     }
 }
 
-__device__ inline u64 sm_finish(Walk &w, const u64 *tab) {
+template <typename W>
+__device__ inline u64 sm_finish(W &w, const u64 *tab) {
     if (w.overflow) {  // left half of the first node goes last (qcgd.hpp:838-845)
         w.nh = hash_combine(w.nh, tab[1]);
         w.lh = hash_combine_small(w.lh, w.c);
@@ -268,8 +273,8 @@ __device__ inline u64 sm_finish(Walk &w, const u64 *tab) {
 }
 
 // erase_create / coin child walk (qcgd.hpp:487-510, 562-584).
-template <bool COIN>
-__device__ inline void flip_run(Walk &w, const PDesc &pd, u32 bits, u32 dlimit, const QcgdCoef &k) {
+template <bool COIN, typename W, typename KC>
+__device__ inline void flip_run(W &w, const PDesc &pd, u32 bits, u32 dlimit, const KC &k) {
     const u32 n = pd.n;
     const u32 d0 = w.d;
     while (w.i < n) {
@@ -302,30 +307,30 @@ __device__ inline void flip_run(Walk &w, const PDesc &pd, u32 bits, u32 dlimit, 
 // runs once per child, u a compile-time constant after unrolling.
 static_assert(kItemBits == 3, "walk_item enumerates three decision levels");
 
-template <int KIND>
-__device__ __attribute__((always_inline)) inline void walk_step(Walk &w, const PDesc &pd, const u64 *ptab, u32 bit,
-                                                                u32 dlimit, const QcgdCoef &k) {
+template <int KIND, typename W, typename KC>
+__device__ __attribute__((always_inline)) inline void walk_step(W &w, const PDesc &pd, const u64 *ptab, u32 bit,
+                                                                u32 dlimit, const KC &k) {
     if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(w, pd, ptab, bit, dlimit, k);
     else flip_run<KIND == QD_RULE_QCGD_COIN>(w, pd, bit, dlimit, k);
 }
 
-template <int KIND, typename Leaf>
-__device__ __attribute__((always_inline)) inline void walk_item(const Walk &pre, const PDesc &pd, const u64 *ptab, u32 s,
-                                                                u32 t, const QcgdCoef &k, Leaf &&leaf) {
+template <int KIND, typename W, typename KC, typename Leaf>
+__device__ __attribute__((always_inline)) inline void walk_item(const W &pre, const PDesc &pd, const u64 *ptab, u32 s,
+                                                                u32 t, const KC &k, Leaf &&leaf) {
 #pragma unroll
     for (u32 b0 = 0; b0 < 2; ++b0) {
         if (b0 && t < 1) break;
-        Walk w0 = pre;
+        W w0 = pre;
         walk_step<KIND>(w0, pd, ptab, b0, s + 1, k);
 #pragma unroll
         for (u32 b1 = 0; b1 < 2; ++b1) {
             if (b1 && t < 2) break;
-            Walk w1 = w0;
+            W w1 = w0;
             walk_step<KIND>(w1, pd, ptab, b1, s + 2, k);
 #pragma unroll
             for (u32 b2 = 0; b2 < 2; ++b2) {
                 if (b2 && t < 3) break;
-                Walk w2 = w1;
+                W w2 = w1;
                 walk_step<KIND>(w2, pd, ptab, b2, s + 3, k);
                 u64 h;
                 if (KIND == QD_RULE_QCGD_SPLIT_MERGE) h = sm_finish(w2, ptab);

@@ -155,7 +155,7 @@ struct qd_ctx {
     DBuf bitmap, wprefix, padded, chunks, scanblk;
     DBuf biglist, redolist, bigsoff, bigkey, bigcand, bigmag, bigseg;
     DBuf pdesc, wi, wb, cpar, opar, ebuf2, cnt2, slist;
-    DBuf adjk, adjv, adjg, adjd, adjp;
+    DBuf adjk, adjv, adjg, adjd, adjp, adjw;
     DBuf pool0, pool1, pwpre, blist2;
     // output buffers of batched chunks (objects, begin, size, mag per chunk),
     // kept between calls
@@ -170,7 +170,7 @@ struct qd_ctx {
         return {&nc, &cb, &pre, &tab, &kept, &keys, &ebuf0, &ebuf1, &hist1, &off1, &cur1, &off2,
                 &nhist, &counters, &sel0, &sel1, &bitmap, &wprefix, &padded, &chunks, &scanblk,
                 &biglist, &redolist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &cpar, &opar, &ebuf2, &cnt2, &slist,
-                &adjk, &adjv, &adjg, &adjd, &adjp, &bseg, &boseg, &brules, &bsinst, &bcount, &bsel, &btrunc,
+                &adjk, &adjv, &adjg, &adjd, &adjp, &adjw, &bseg, &boseg, &brules, &bsinst, &bcount, &bsel, &btrunc,
                 &blist, &bcur, &btile, &bhist, &bkseg, &bchunk, &bcsum, &btot, &pool0, &pool1, &pwpre, &blist2};
     }
 };
@@ -1336,10 +1336,42 @@ static int adjoint_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, const qd
     QD_HIP(hipMemsetAsync(counters, 0, 64 * sizeof(u64), s));
     u32 *maxsz = reinterpret_cast<u32 *>(counters + 3);
     u32 *errw = reinterpret_cast<u32 *>(counters + 4);
-    hipLaunchKernelGGL(k_prep<KIND>, dim3(grid_for(N)), dim3(256), 0, s, P, (const u64 *)nullptr, rule,
-                       ctx->nc.as<u32>(), ctx->pre.as<u64>(), ctx->tab.as<u64>(), maxsz, errw);
-    QD_HIP(hipGetLastError());
-    QD_TRY(run_scan(ctx, LoadU32{ctx->nc.as<u32>()}, N, ctx->cb.as<u64>()));
+    // QCGD graphs of at most 64 nodes: the fast walk's descriptors and work
+    // items (as in the step); else, and for hadamard, the per-child kernels
+    bool fast = KIND != QD_RULE_HADAMARD && !ctx->tune.generic_path;
+    u64 nitems = 0;
+    if (fast) {
+        QD_TRY(ctx->pdesc.ensure(N * sizeof(PDesc)));
+        QD_TRY(ctx->wi.ensure(N * sizeof(u32)));
+        QD_TRY(ctx->wb.ensure((N + 1) * sizeof(u64)));
+        hipLaunchKernelGGL(k_prep_fast<KIND>, dim3(grid_for(N, 16, 65536)), dim3(256), 0, s, P, (const u64 *)nullptr,
+                           ctx->pdesc.as<PDesc>(), ctx->nc.as<u32>(), ctx->wi.as<u32>(), ctx->tab.as<u64>(), maxsz,
+                           errw);
+        QD_HIP(hipGetLastError());
+        QD_TRY(run_scan(ctx, LoadU32{ctx->nc.as<u32>()}, N, ctx->cb.as<u64>()));
+        QD_TRY(run_scan(ctx, LoadU32{ctx->wi.as<u32>()}, N, ctx->wb.as<u64>()));
+        u64 h2[2];
+        QD_HIP(hipMemcpyAsync(h2, ctx->wb.as<u64>() + N, sizeof(u64), hipMemcpyDeviceToHost, s));
+        QD_HIP(hipMemcpyAsync(h2 + 1, counters + 4, sizeof(u64), hipMemcpyDeviceToHost, s));
+        QD_HIP(hipStreamSynchronize(s));
+        nitems = h2[0];
+        if (u32(h2[1]) & ERR_WIDE) {  // a graph wider than 64 nodes
+            fast = false;
+            QD_HIP(hipMemsetAsync(counters + 3, 0, 2 * sizeof(u64), s));
+        } else if (nitems) {
+            const u64 nchunk = (nitems + kGenThreads - 1) / kGenThreads;
+            QD_TRY(ctx->cpar.ensure(nchunk * sizeof(u64)));
+            hipLaunchKernelGGL(k_chunk_parent, dim3(grid_for(nchunk)), dim3(256), 0, s, ctx->wb.as<u64>(), N, nchunk,
+                               ctx->cpar.as<u64>());
+            QD_HIP(hipGetLastError());
+        }
+    }
+    if (!fast) {
+        hipLaunchKernelGGL(k_prep<KIND>, dim3(grid_for(N)), dim3(256), 0, s, P, (const u64 *)nullptr, rule,
+                           ctx->nc.as<u32>(), ctx->pre.as<u64>(), ctx->tab.as<u64>(), maxsz, errw);
+        QD_HIP(hipGetLastError());
+        QD_TRY(run_scan(ctx, LoadU32{ctx->nc.as<u32>()}, N, ctx->cb.as<u64>()));
+    }
 
     // g_y from the cotangent of the normalised output
     QD_TRY(ctx->adjd.ensure(K * sizeof(double)));
@@ -1370,13 +1402,27 @@ static int adjoint_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, const qd
     QD_TRY(sync_read(ctx, counters, 16 * sizeof(u64), hb));
     QD_TRY(check_err_word(u32(hb[4])));
 
-    // gather over all candidates, one wavefront per parent
+    // gather over all candidates
     const int nd = std::max(ndir, 0);
     QD_TRY(ctx->adjp.ensure(std::max<u64>(u64(nd) * N, 1) * sizeof(double)));
     const DualCoef dk = dual_coef_of(rule, dcoef, nd);
-    hipLaunchKernelGGL(k_adjoint<KIND>, dim3(grid_for(N, 4, 65536)), dim3(256), 0, s, P, ctx->cb.as<u64>(), rule, dk,
-                       nd, ctx->pre.as<u64>(), ctx->tab.as<u64>(), ctx->adjk.as<u64>(), ctx->adjv.as<u32>(), T - 1,
-                       hb[8], ctx->adjg.as<double2>(), gin, ctx->adjp.as<double>());
+    if (fast) {  // one thread per work item, then per-parent sums in order
+        QD_TRY(ctx->adjw.ensure(std::max<u64>(nitems, 1) * kAdjPart * sizeof(double)));
+        const unsigned gg =
+            unsigned(std::max<u64>(std::min<u64>((nitems + kGenThreads - 1) / kGenThreads, ctx->tune.generate_grid), 1));
+        if (nitems)
+            hipLaunchKernelGGL(k_adjoint_fast<KIND>, dim3(gg), dim3(kGenThreads), 0, s, ctx->pdesc.as<PDesc>(), P.mag,
+                               ctx->wb.as<u64>(), N, ctx->cpar.as<u64>(), ctx->tab.as<u64>(), P.begin, dk, nd,
+                               ctx->adjk.as<u64>(), ctx->adjv.as<u32>(), T - 1, hb[8], ctx->adjg.as<double2>(), nitems,
+                               ctx->adjw.as<double>());
+        QD_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_adjoint_reduce, dim3(grid_for(N)), dim3(256), 0, s, ctx->wb.as<u64>(), N,
+                           ctx->adjw.as<double>(), nd, gin, ctx->adjp.as<double>());
+    } else {  // one wavefront per parent
+        hipLaunchKernelGGL(k_adjoint<KIND>, dim3(grid_for(N, 4, 65536)), dim3(256), 0, s, P, ctx->cb.as<u64>(), rule,
+                           dk, nd, ctx->pre.as<u64>(), ctx->tab.as<u64>(), ctx->adjk.as<u64>(), ctx->adjv.as<u32>(),
+                           T - 1, hb[8], ctx->adjg.as<double2>(), gin, ctx->adjp.as<double>());
+    }
     QD_HIP(hipGetLastError());
     for (int t = 0; t < nd; ++t) QD_TRY(canonical_sum(ctx, nullptr, ctx->adjp.as<double>() + u64(t) * N, N, &grad[t]));
     QD_HIP(hipStreamSynchronize(s));
