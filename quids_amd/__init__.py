@@ -26,6 +26,11 @@ simple_truncation = True
 #: key of the probabilistic truncation's draws (simple_truncation False):
 #: Philox2x64-10 keyed by (object hash, truncation_seed)
 truncation_seed = 0
+# memory safety margin of max_num_object == 0 (src/quids.hpp:64) and the CPU
+# load-balancing granularity (src/quids.hpp:68): read by the QCGD
+# command-line flags; the second has no effect on the device step
+safety_margin = 0.2
+load_balancing_bucket_per_thread = 32
 
 
 def _params():

@@ -47,6 +47,9 @@ inline bool simple_truncation = true;
 inline uint64_t truncation_seed = 0;
 /// memory safety margin of max_num_object == 0 (src/quids.hpp:64)
 inline float safety_margin = 0.2f;
+/// CPU load-balancing granularity (src/quids.hpp:68); read by the QCGD
+/// command-line flags, no effect on the device step
+inline int load_balancing_bucket_per_thread = 32;
 
 typedef std::complex<double> mag_t;
 typedef class iteration it_t;
