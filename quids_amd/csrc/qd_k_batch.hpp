@@ -61,13 +61,23 @@ __device__ inline u32 slot_inst(const u64 *UR, u64 s, const u64 *cseg, u32 n) {
     return r == kNoRep ? kNoInst : find_seg(cseg, n, r);
 }
 
-// Per-instance unique counts.
-__global__ __launch_bounds__(256) void k_seg_count(const u64 *UR, u64 nslots, const u64 *cseg, u32 n, u64 *ucount) {
+// Per-instance unique counts; sinst[s] records each slot's instance
+// (kNoSlotInst for an empty slot) for the kernels after it.
+constexpr u16 kNoSlotInst = 0xFFFF;
+static_assert(kMaxInst < kNoSlotInst, "slot instances are 16-bit");
+__device__ inline u32 slot_inst16(const u16 *sinst, u64 s) {
+    const u16 v = sinst[s];
+    return v == kNoSlotInst ? kNoInst : u32(v);
+}
+
+__global__ __launch_bounds__(256) void k_seg_count(const u64 *UR, u64 nslots, const u64 *cseg, u32 n, u64 *ucount,
+                                                   u16 *sinst) {
     __shared__ u32 cnt[kMaxInst];
     for (u32 i = threadIdx.x; i < n; i += blockDim.x) cnt[i] = 0;
     __syncthreads();
     for (u64 s = u64(blockIdx.x) * blockDim.x + threadIdx.x; s < nslots; s += u64(gridDim.x) * blockDim.x) {
         const u32 inst = slot_inst(UR, s, cseg, n);
+        sinst[s] = inst == kNoInst ? kNoSlotInst : u16(inst);
         if (inst != kNoInst) atomicAdd(&cnt[inst], 1u);
     }
     __syncthreads();
@@ -104,9 +114,9 @@ __device__ inline bool prefix_eq(u64 hi, u64 lo, const SegSel &q) {
 // {hi, lo, slot} (read once, in slot order): each block ranks its tile's
 // slots per instance in LDS and reserves one range per instance.
 constexpr int kSegTile = 4096;
-__global__ __launch_bounds__(256) void k_seg_scatter(const u64 *UR, const u64 *cseg, u64 nslots, const u64 *UH,
-                                                     const double2 *UM, const u32 *trunc, u32 n, u64 *cursor,
-                                                     SelItem *list, SelKey sk) {
+__global__ __launch_bounds__(256) void k_seg_scatter(const u16 *sinst, u64 nslots, const u64 *UH, const double2 *UM,
+                                                     const u32 *trunc, u32 n, u64 *cursor, SelItem *list,
+                                                     SelKey sk) {
     __shared__ u32 cnt[kMaxInst];
     __shared__ u64 base[kMaxInst];
     constexpr int kPer = kSegTile / 256;
@@ -119,7 +129,7 @@ __global__ __launch_bounds__(256) void k_seg_scatter(const u64 *UR, const u64 *c
             const u64 s = t0 + threadIdx.x + u64(q) * 256;
             inst[q] = kNoInst;
             if (s < nslots) {
-                const u32 i = slot_inst(UR, s, cseg, n);
+                const u32 i = slot_inst16(sinst, s);
                 if (i != kNoInst && trunc[i]) {
                     inst[q] = i;
                     rank[q] = atomicAdd(&cnt[i], 1u);
@@ -282,15 +292,14 @@ __global__ __launch_bounds__(256) void k_seg_pick(SegSel *sel, u32 *hist, u32 n,
 
 // Marks the survivors: every unique of an instance that is not truncated,
 // and the keys >= prefix of a truncated one.  Lanes hold consecutive slots.
-__global__ __launch_bounds__(256) void k_seg_mark(const u64 *cseg, u32 n, const u64 *UH, const u64 *UR,
-                                                  const double2 *UM, u64 nslots, const SegSel *sel, u64 *bitmap,
-                                                  u64 *sbm, SelKey sk) {
+__global__ __launch_bounds__(256) void k_seg_mark(const u16 *sinst, const u64 *UH, const u64 *UR, const double2 *UM,
+                                                  u64 nslots, const SegSel *sel, u64 *bitmap, u64 *sbm, SelKey sk) {
     const u64 stride = u64(gridDim.x) * blockDim.x;
     for (u64 base = u64(blockIdx.x) * blockDim.x; base < nslots; base += stride) {
         const u64 s = base + threadIdx.x;
         bool keep = false;
         if (s < nslots) {
-            const u32 inst = slot_inst(UR, s, cseg, n);
+            const u32 inst = slot_inst16(sinst, s);
             if (inst != kNoInst) {
                 const SegSel q = sel[inst];
                 if (q.keepall) keep = true;

@@ -866,8 +866,10 @@ static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const
     QD_TRY(ctx->bcount.ensure(n * sizeof(u64)));
     const u64 *cseg = ctx->bseg.as<u64>();
     QD_HIP(hipMemsetAsync(ctx->bcount.p, 0, n * sizeof(u64), s));
+    QD_TRY(ctx->bsinst.ensure(std::max<u64>(nslots, 1) * sizeof(u16)));
+    const u16 *sinst = ctx->bsinst.as<u16>();
     hipLaunchKernelGGL(k_seg_count, dim3(grid_for(nslots, 256, 8192)), dim3(256), 0, s, UR, nslots, cseg, n,
-                       ctx->bcount.as<u64>());
+                       ctx->bcount.as<u64>(), ctx->bsinst.as<u16>());
     QD_HIP(hipGetLastError());
     bt->U.resize(n);
     QD_TRY(sync_read(ctx, ctx->bcount.p, n * sizeof(u64), bt->U.data()));
@@ -903,7 +905,7 @@ static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const
         u32 *more = reinterpret_cast<u32 *>(ctx->counters.as<u64>() + 10);
         SelItem *cur = ctx->ebuf1.as<SelItem>(), *nxt = ctx->ebuf2.as<SelItem>();
         const unsigned sg = unsigned(std::min<u64>((nslots + kSegTile - 1) / kSegTile, 16384));
-        hipLaunchKernelGGL(k_seg_scatter, dim3(std::max(sg, 1u)), dim3(256), 0, s, UR, cseg, nslots, UH, UM,
+        hipLaunchKernelGGL(k_seg_scatter, dim3(std::max(sg, 1u)), dim3(256), 0, s, sinst, nslots, UH, UM,
                            ctx->btrunc.as<u32>(), n, ctx->bcur.as<u64>(), cur, bt->sk);
         QD_HIP(hipMemsetAsync(ctx->bhist.p, 0, u64(n) * kHistBins * sizeof(u32), s));
         constexpr u64 kTileItems = 16384;
@@ -945,7 +947,7 @@ static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const
         for (u32 i = 0; i < n; ++i)
             if (sel[i].err || sel[i].active) return fail(QD_ERR_HIP, "batched select did not converge");
     }
-    hipLaunchKernelGGL(k_seg_mark, dim3(grid_for(nslots, 256, 16384)), dim3(256), 0, s, cseg, n, UH, UR, UM, nslots,
+    hipLaunchKernelGGL(k_seg_mark, dim3(grid_for(nslots, 256, 16384)), dim3(256), 0, s, sinst, UH, UR, UM, nslots,
                        (const SegSel *)dsel, bitmap, sbm, bt->sk);
     QD_HIP(hipGetLastError());
     return QD_OK;

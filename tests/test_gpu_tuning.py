@@ -118,3 +118,29 @@ def test_dedup_table_sizes_match_oracle(cap, target, copies):
     finally:
         q.tolerance = tol0
         ctx.set_tuning(dedup_cap=before["dedup_cap"], dedup_bucket_target=before["dedup_bucket_target"])
+
+
+def test_fused_generate_overflow_falls_back():
+    """6000 copies of one graph: each child hash has 6000 members, more than
+    a first-pass bucket of the fused generate holds, so the step reports the
+    unfused fallback (stats reserved bit 8) and still matches the oracle."""
+    from oracle import oracle as orc
+    rng = np.random.default_rng(6000)
+    mags = rng.normal(size=6000) + 1j * rng.normal(size=6000)
+    hs = HostState.from_objects([qcgd.seeded_graph(5, 3)] * 6000, mags=mags / np.linalg.norm(mags))
+    tol0 = q.tolerance
+    try:
+        q.tolerance = 1e-30
+        a, b = q.it_t(), q.it_t()
+        a.upload(hs)
+        rule = qcgd.erase_create(0.8, 0.3)
+        st = q.simulate(a, rule, b, None, q.UINT64_MAX)
+        ref, rst = orc.simulate(hs, rule)
+        assert st["reserved"] & 8, st
+        got = b.download()
+        assert st["num_unique"] == rst["num_unique"]
+        assert np.array_equal(got.objects, ref.objects)
+        assert np.array_equal(got.magnitude, ref.magnitude)
+        assert got.total_proba == ref.total_proba
+    finally:
+        q.tolerance = tol0
