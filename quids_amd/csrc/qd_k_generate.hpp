@@ -95,6 +95,27 @@ __global__ __launch_bounds__(256) void k_prep(Parents P, const u64 *keep, qd_rul
     if (local_max) atomicMax(maxsz, local_max);
 }
 
+// num_child of every parent alone (the same checks as k_prep, no tables):
+// the batched step's planning pass.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_num_child(Parents P, qd_rule rule, u32 *nc, u32 *err) {
+    for (u64 p = u64(blockIdx.x) * blockDim.x + threadIdx.x; p < P.n; p += u64(gridDim.x) * blockDim.x) {
+        const u8 *o = P.obj + P.begin[p];
+        const u32 sz = P.size[p];
+        u32 num = 0;
+        if (KIND == QD_RULE_HADAMARD) {
+            if (rule.bit >= sz) atomicOr(err, ERR_BIT);
+            else num = 2;
+        } else if (!qcgd_valid(o, sz)) {
+            atomicOr(err, ERR_OBJECT);
+        } else {
+            Graph g(o);
+            num = KIND == QD_RULE_QCGD_SPLIT_MERGE ? sm_num_child(g) : flip_num_child(g, KIND == QD_RULE_QCGD_COIN);
+        }
+        nc[p] = num;
+    }
+}
+
 // ===========================================================================
 // phase 3: generate
 // ===========================================================================

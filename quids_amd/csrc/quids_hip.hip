@@ -1454,15 +1454,13 @@ static int batch_plan(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
     }
     QD_TRY(ctx->nc.ensure(N * sizeof(u32)));
     QD_TRY(ctx->cb.ensure((N + 1) * sizeof(u64)));
-    QD_TRY(ctx->pre.ensure(N * sizeof(u64)));
     QD_TRY(ctx->counters.ensure(64 * sizeof(u64)));
     QD_TRY(ctx->btrunc.ensure(bt->n * sizeof(u32)));
-    if (KIND == QD_RULE_QCGD_SPLIT_MERGE) QD_TRY(ctx->tab.ensure((in_bytes / 2 + 16) * sizeof(u64)));
+    (void)in_bytes;
     u64 *counters = ctx->counters.as<u64>();
     QD_HIP(hipMemsetAsync(counters, 0, 64 * sizeof(u64), s));
     QD_HIP(hipMemsetAsync(ctx->btrunc.p, 0, bt->n * sizeof(u32), s));
-    hipLaunchKernelGGL(k_prep<KIND>, dim3(grid_for(N)), dim3(256), 0, s, P, (const u64 *)nullptr, rule,
-                       ctx->nc.as<u32>(), ctx->pre.as<u64>(), ctx->tab.as<u64>(), reinterpret_cast<u32 *>(counters + 3),
+    hipLaunchKernelGGL(k_num_child<KIND>, dim3(grid_for(N)), dim3(256), 0, s, P, rule, ctx->nc.as<u32>(),
                        reinterpret_cast<u32 *>(counters + 4));
     hipLaunchKernelGGL(k_wide_flags, dim3(grid_for(N)), dim3(256), 0, s, P.obj, P.begin, P.size, N, bt->d_oseg, bt->n,
                        ctx->btrunc.as<u32>());
