@@ -104,6 +104,13 @@ __device__ inline void seg_key(const u64 *UH, const double2 *UM, u64 slot, u32 i
     lo = ~h;
 }
 
+// Key record of a batched list: the high key word and the slot; the low
+// word (~hash) is read back from the slot when a digit reaches it.
+struct SegItem {
+    u64 hi, slot;
+};
+__device__ inline u64 seg_lo(const u64 *UH, u64 slot, u32 inst) { return ~(UH[slot] - u64(inst) * kFold); }
+
 __device__ inline bool prefix_eq(u64 hi, u64 lo, const SegSel &q) {
     if (q.pos == 0) return true;
     if (q.pos <= 64) return (hi >> (64 - q.pos)) == (q.hi >> (64 - q.pos));
@@ -111,11 +118,11 @@ __device__ inline bool prefix_eq(u64 hi, u64 lo, const SegSel &q) {
 }
 
 // Groups the uniques of truncated instances by instance as key records
-// {hi, lo, slot} (read once, in slot order): each block ranks its tile's
-// slots per instance in LDS and reserves one range per instance.
+// {hi, slot} (read once, in slot order): each block ranks its tile's slots
+// per instance in LDS and reserves one range per instance.
 constexpr int kSegTile = 4096;
 __global__ __launch_bounds__(256) void k_seg_scatter(const u16 *sinst, u64 nslots, const u64 *UH, const double2 *UM,
-                                                     const u32 *trunc, u32 n, u64 *cursor, SelItem *list,
+                                                     const u32 *trunc, u32 n, u64 *cursor, SegItem *list,
                                                      SelKey sk) {
     __shared__ u32 cnt[kMaxInst];
     __shared__ u64 base[kMaxInst];
@@ -144,8 +151,9 @@ __global__ __launch_bounds__(256) void k_seg_scatter(const u16 *sinst, u64 nslot
         for (int q = 0; q < kPer; ++q)
             if (inst[q] != kNoInst) {
                 const u64 s = t0 + threadIdx.x + u64(q) * 256;
-                SelItem it;
-                seg_key(UH, UM, s, inst[q], sk, it.hi, it.lo);
+                SegItem it;
+                u64 lo;
+                seg_key(UH, UM, s, inst[q], sk, it.hi, lo);
                 it.slot = s;
                 list[base[inst[q]] + rank[q]] = it;
             }
@@ -162,8 +170,8 @@ struct SegTile {
 // One workgroup per tile of an active instance: LDS histogram of the 12
 // key bits at the instance's position, added into its global row.  Every
 // record of the list carries the instance's current prefix.
-__global__ __launch_bounds__(256) void k_seg_hist(const SelItem *list, const SegTile *tiles, u32 ntiles,
-                                                  const SegSel *sel, u32 *hist) {
+__global__ __launch_bounds__(256) void k_seg_hist(const SegItem *list, const SegTile *tiles, u32 ntiles,
+                                                  const SegSel *sel, u32 *hist, const u64 *UH) {
     __shared__ u32 h[kHistBins];
     for (u32 t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const SegTile tl = tiles[t];
@@ -172,8 +180,9 @@ __global__ __launch_bounds__(256) void k_seg_hist(const SelItem *list, const Seg
         for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) h[i] = 0;
         __syncthreads();
         for (u64 k = tl.begin + threadIdx.x; k < tl.end; k += blockDim.x) {
-            const SelItem it = list[k];
-            atomicAdd(&h[key_digit(it.hi, it.lo, q.pos)], 1u);
+            const SegItem it = list[k];
+            const u64 lo = q.pos + 12 > 64 ? seg_lo(UH, it.slot, tl.inst) : 0;
+            atomicAdd(&h[key_digit(it.hi, lo, q.pos)], 1u);
         }
         __syncthreads();
         u32 *row = hist + u64(tl.inst) * kHistBins;
@@ -186,8 +195,8 @@ __global__ __launch_bounds__(256) void k_seg_hist(const SelItem *list, const Seg
 // After a pick: carries the records that match the instance's new prefix
 // (the tied ones) into the next list, at the instance's cursor; one
 // reservation per tile.
-__global__ __launch_bounds__(256) void k_seg_collect(const SelItem *list, const SegTile *tiles, u32 ntiles,
-                                                     const SegSel *sel, u64 *cursor, SelItem *out) {
+__global__ __launch_bounds__(256) void k_seg_collect(const SegItem *list, const SegTile *tiles, u32 ntiles,
+                                                     const SegSel *sel, u64 *cursor, SegItem *out, const u64 *UH) {
     __shared__ u64 tmp[256 / kWave + 1];
     __shared__ u64 base;
     for (u32 t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -197,10 +206,10 @@ __global__ __launch_bounds__(256) void k_seg_collect(const SelItem *list, const 
         for (u64 k0 = tl.begin; k0 < tl.end; k0 += blockDim.x) {
             const u64 k = k0 + threadIdx.x;
             bool take = false;
-            SelItem it{};
+            SegItem it{};
             if (k < tl.end) {
                 it = list[k];
-                take = prefix_eq(it.hi, it.lo, q);
+                take = prefix_eq(it.hi, q.pos > 64 ? seg_lo(UH, it.slot, tl.inst) : 0, q);
             }
             u64 total;
             const u64 off = block_excl_scan(take ? 1 : 0, tmp, total);

@@ -899,11 +899,11 @@ static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const
         // the key records go where the partition's entries were (dead by now):
         // the first list in the first pass's buffer (32 bytes per slot), the
         // next rounds' in the second pass's
-        QD_TRY(ctx->ebuf2.ensure(ntrunc * sizeof(SelItem)));
+        QD_TRY(ctx->ebuf2.ensure(ntrunc * sizeof(SegItem)));
         QD_TRY(ctx->bhist.ensure(u64(n) * kHistBins * sizeof(u32)));
         QD_TRY(ctx->counters.ensure(64 * sizeof(u64)));
         u32 *more = reinterpret_cast<u32 *>(ctx->counters.as<u64>() + 10);
-        SelItem *cur = ctx->ebuf1.as<SelItem>(), *nxt = ctx->ebuf2.as<SelItem>();
+        SegItem *cur = ctx->ebuf1.as<SegItem>(), *nxt = ctx->ebuf2.as<SegItem>();
         const unsigned sg = unsigned(std::min<u64>((nslots + kSegTile - 1) / kSegTile, 16384));
         hipLaunchKernelGGL(k_seg_scatter, dim3(std::max(sg, 1u)), dim3(256), 0, s, sinst, nslots, UH, UM,
                            ctx->btrunc.as<u32>(), n, ctx->bcur.as<u64>(), cur, bt->sk);
@@ -920,8 +920,8 @@ static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const
             QD_TRY(upload(ctx, ctx->btile, tiles.data(), tiles.size() * sizeof(SegTile)));
             const u32 nt = u32(tiles.size());
             QD_HIP(hipMemsetAsync(more, 0, sizeof(u32), s));
-            hipLaunchKernelGGL(k_seg_hist, dim3(std::min<u32>(nt, 65536)), dim3(256), 0, s, (const SelItem *)cur,
-                               ctx->btile.as<SegTile>(), nt, (const SegSel *)dsel, ctx->bhist.as<u32>());
+            hipLaunchKernelGGL(k_seg_hist, dim3(std::min<u32>(nt, 65536)), dim3(256), 0, s, (const SegItem *)cur,
+                               ctx->btile.as<SegTile>(), nt, (const SegSel *)dsel, ctx->bhist.as<u32>(), UH);
             hipLaunchKernelGGL(k_seg_pick, dim3(n), dim3(256), 0, s, dsel, ctx->bhist.as<u32>(), n, more);
             QD_HIP(hipGetLastError());
             QD_TRY(sync_read(ctx, dsel, n * sizeof(SegSel), sel.data()));
@@ -939,8 +939,8 @@ static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const
             }
             if (!any) break;
             QD_TRY(upload(ctx, ctx->bcur, off.data(), n * sizeof(u64)));
-            hipLaunchKernelGGL(k_seg_collect, dim3(std::min<u32>(nt, 65536)), dim3(256), 0, s, (const SelItem *)cur,
-                               ctx->btile.as<SegTile>(), nt, (const SegSel *)dsel, ctx->bcur.as<u64>(), nxt);
+            hipLaunchKernelGGL(k_seg_collect, dim3(std::min<u32>(nt, 65536)), dim3(256), 0, s, (const SegItem *)cur,
+                               ctx->btile.as<SegTile>(), nt, (const SegSel *)dsel, ctx->bcur.as<u64>(), nxt, UH);
             QD_HIP(hipGetLastError());
             std::swap(cur, nxt);
         }
