@@ -166,6 +166,86 @@ __global__ __launch_bounds__(256) void k_prep_fast(Parents P, const u64 *keep, P
     if (lane == 0 && local_max) atomicMax(maxsz, local_max);
 }
 
+// QCGD step / reversed_step (qcgd.hpp:443-457) on a segment of SEG lanes,
+// one lane per node (SEG = 64, or 16 when four graphs of at most 16 nodes
+// share a wavefront; the segment's lanes are active together): left[i] <-
+// left[i+1], right[i] <- right[i-1] (cyclic), or the inverse, by shuffles
+// within the segment.  The object is validated as in prep; a graph wider
+// than the segment is rotated by one lane.  Magnitudes are unchanged.
+template <int SEG>
+__device__ inline void step_object(u8 *obj, const u64 *begin, const u32 *size, u64 p, bool have, const qd_modifier &m,
+                                   u32 *err) {
+    const u32 lane = threadIdx.x & (kWave - 1);
+    const u32 sl = lane & (SEG - 1);
+    const u32 sbase = lane & ~(SEG - 1);
+    const u64 smask = SEG == kWave ? ~0ull : (((1ull << SEG) - 1) << sbase);
+    if (!have) return;
+    u8 *o = obj + begin[p];
+    const u32 sz = size[p];
+    const u32 n = sz >= 4 ? ld16(o) : 0;
+    bool valid = n > 0 && 4 + 4 * u64(n) <= sz;
+    if (valid) {
+        const u8 *nb = o + 2 + 2 * n;
+        const u32 mm = ld16(nb + 2 * n);
+        valid = ld16(nb) == 0 && 4 + 4 * u64(n) + 16 * u64(mm) == sz;
+        if (valid && n <= SEG) {
+            bool bad = false;
+            if (sl < n) {
+                const u32 b = ld16(nb + 2 * sl), e = ld16(nb + 2 * sl + 2);
+                if (e <= b || e > mm) bad = true;
+                else {
+                    const u32 mi = e - b;
+                    const i16 t = ldi16(o + 4 + 4 * n + 16 * size_t(b) + 2);
+                    if (t >= 0) bad = t < 2 || u32(t) >= mi;
+                    else if (t == QCGD_DOT_L || t == QCGD_DOT_R) bad = mi < 2;
+                    else bad = t != QCGD_ELEMENT;
+                }
+            }
+            valid = (u64(__ballot(bad)) & smask) == 0;
+        } else if (valid) {
+            valid = qcgd_valid(o, sz);  // wider than the segment: the scalar check
+        }
+    }
+    if (!valid) {
+        if (sl == 0) atomicOr(err, ERR_OBJECT);
+        return;
+    }
+    if (n > SEG) {
+        if (sl == 0) {
+            cplx c{1.0, 0.0};
+            modifier_apply(m, o, sz, c);
+        }
+        return;
+    }
+    const bool in = sl < n;
+    const int l = in ? o[2 + sl] : 0, r = in ? o[2 + n + sl] : 0;
+    const int up = int(sbase + (sl + 1 >= n ? 0 : sl + 1)), dn = int(sbase + (sl == 0 ? n - 1 : sl - 1));
+    const bool fwd = m.kind == QD_MOD_QCGD_STEP;
+    const int nl = __shfl(l, fwd ? up : dn, kWave), nr = __shfl(r, fwd ? dn : up, kWave);
+    if (in) {
+        o[2 + sl] = u8(nl);
+        o[2 + n + sl] = u8(nr);
+    }
+}
+
+// Four graphs per wavefront: in 16-lane segments when none has more than 16
+// nodes, else one after the other on the whole wavefront.
+__global__ __launch_bounds__(256) void k_step_modifier(u8 *obj, const u64 *begin, const u32 *size, u64 N, qd_modifier m,
+                                                       u32 *err) {
+    const u32 lane = threadIdx.x & (kWave - 1);
+    const u64 nwaves = u64(gridDim.x) * (blockDim.x / kWave);
+    for (u64 p0 = 4 * (u64(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave); p0 < N; p0 += 4 * nwaves) {
+        u32 nq = 0;
+        if (lane < 4 && p0 + lane < N && size[p0 + lane] >= 4) nq = ld16(obj + begin[p0 + lane]);
+        if (__ballot(nq > 16) == 0) {
+            const u64 p = p0 + lane / 16;
+            step_object<16>(obj, begin, size, p, p < N, m, err);
+        } else {
+            for (u32 q = 0; q < 4; ++q) step_object<kWave>(obj, begin, size, p0 + q, p0 + q < N, m, err);
+        }
+    }
+}
+
 // Walk state of one child (or of the shared prefix of a work item).  M is
 // the magnitude type: cplx, or DualC (with derivatives, for the adjoint).
 template <typename M>

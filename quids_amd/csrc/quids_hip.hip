@@ -1992,8 +1992,12 @@ int qd_apply_modifier(qd_ctx *c, qd_state *s, const qd_modifier *m) {
     QD_TRY(c->counters.ensure(64 * sizeof(u64)));
     u32 *errw = reinterpret_cast<u32 *>(c->counters.as<u64>() + 4);
     QD_HIP(hipMemsetAsync(errw, 0, sizeof(u32), c->stream));
-    hipLaunchKernelGGL(k_modifier, dim3(grid_for(s->n)), dim3(256), 0, c->stream, s->objects.as<u8>(),
-                       s->begin.as<u64>(), s->size.as<u32>(), s->mag.as<double2>(), s->n, *m, errw);
+    if (m->kind == QD_MOD_QCGD_STEP || m->kind == QD_MOD_QCGD_REVERSED_STEP)
+        hipLaunchKernelGGL(k_step_modifier, dim3(grid_for(s->n, 16, 65536)), dim3(256), 0, c->stream,
+                           s->objects.as<u8>(), s->begin.as<u64>(), s->size.as<u32>(), s->n, *m, errw);
+    else
+        hipLaunchKernelGGL(k_modifier, dim3(grid_for(s->n)), dim3(256), 0, c->stream, s->objects.as<u8>(),
+                           s->begin.as<u64>(), s->size.as<u32>(), s->mag.as<double2>(), s->n, *m, errw);
     QD_HIP(hipGetLastError());
     u32 w = 0;
     QD_TRY(sync_read(c, errw, sizeof(u32), &w));
