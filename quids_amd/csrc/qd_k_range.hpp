@@ -54,20 +54,6 @@ __global__ __launch_bounds__(256) void k_pool_append(const u64 *UK, const u64 *U
     }
 }
 
-// Histogram of the top 12 |m|^2 bits of the pool.
-__global__ __launch_bounds__(256) void k_pool_hist0(const double2 *Pm, u64 n, u64 *hist) {
-    __shared__ u32 h[kHistBins];
-    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) h[i] = 0;
-    __syncthreads();
-    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x) {
-        const double2 m = Pm[i];
-        atomicAdd(&h[u32(u64(__double_as_longlong(m.x * m.x + m.y * m.y)) >> 52)], 1u);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x)
-        if (h[i]) atomicAdd(reinterpret_cast<unsigned long long *>(hist + i), (unsigned long long)h[i]);
-}
-
 // Keeps the pool items whose bit is set in keep, in pool order (wprefix:
 // exclusive popcount prefix of keep).
 __global__ __launch_bounds__(256) void k_pool_compact(const u64 *Ph, const u64 *Pr, const double2 *Pm, u64 n,
