@@ -37,8 +37,8 @@ enum : u8 { PD_FS = 1, PD_LM = 2, PD_HMLZ = 4 };
 constexpr u32 ERR_WIDE = 8u;
 
 // Descriptor of parent p, computed by a segment of SEG lanes (one lane per
-// node: SEG = 64, or 16 when four parents of at most 16 nodes share a
-// wavefront; the segment's lanes are active together).  have: the segment
+// node: SEG = 64, or 32 / 16 when two / four parents share a wavefront; the
+// segment's lanes are active together).  have: the segment
 // has a parent.  Ballots and shuffles stay within the segment.
 template <int KIND, int SEG>
 __device__ inline void prep_parent(const Parents &P, u64 p, bool have, const u64 *keep, PDesc *D, u32 *nc, u32 *wi,
@@ -144,7 +144,8 @@ __device__ inline void prep_parent(const Parents &P, u64 p, bool have, const u64
 }
 
 // Four parents per wavefront: in 16-lane segments when none has more than
-// 16 nodes, else one after the other on the whole wavefront.
+// 16 nodes, two by two in 32-lane segments when none has more than 32, else
+// one after the other on the whole wavefront.
 template <int KIND>
 __global__ __launch_bounds__(256) void k_prep_fast(Parents P, const u64 *keep, PDesc *D, u32 *nc, u32 *wi, u64 *tab,
                                                    u32 *maxsz, u32 *err) {
@@ -157,6 +158,11 @@ __global__ __launch_bounds__(256) void k_prep_fast(Parents P, const u64 *keep, P
         if (__ballot(nq > 16) == 0) {
             const u64 p = p0 + lane / 16;
             prep_parent<KIND, 16>(P, p, p < P.n, keep, D, nc, wi, tab, local_max, err);
+        } else if (__ballot(nq > 32) == 0) {
+            for (u32 h = 0; h < 2; ++h) {
+                const u64 p = p0 + 2 * h + lane / 32;
+                prep_parent<KIND, 32>(P, p, p < P.n, keep, D, nc, wi, tab, local_max, err);
+            }
         } else {
             for (u32 q = 0; q < 4; ++q)
                 prep_parent<KIND, kWave>(P, p0 + q, p0 + q < P.n, keep, D, nc, wi, tab, local_max, err);
@@ -167,8 +173,8 @@ __global__ __launch_bounds__(256) void k_prep_fast(Parents P, const u64 *keep, P
 }
 
 // QCGD step / reversed_step (qcgd.hpp:443-457) on a segment of SEG lanes,
-// one lane per node (SEG = 64, or 16 when four graphs of at most 16 nodes
-// share a wavefront; the segment's lanes are active together): left[i] <-
+// one lane per node (SEG = 64, or 32 / 16 when two / four graphs share a
+// wavefront; the segment's lanes are active together): left[i] <-
 // left[i+1], right[i] <- right[i-1] (cyclic), or the inverse, by shuffles
 // within the segment.  The object is validated as in prep; a graph wider
 // than the segment is rotated by one lane.  Magnitudes are unchanged.
@@ -229,7 +235,8 @@ __device__ inline void step_object(u8 *obj, const u64 *begin, const u32 *size, u
 }
 
 // Four graphs per wavefront: in 16-lane segments when none has more than 16
-// nodes, else one after the other on the whole wavefront.
+// nodes, two by two in 32-lane segments when none has more than 32, else one
+// after the other on the whole wavefront.
 __global__ __launch_bounds__(256) void k_step_modifier(u8 *obj, const u64 *begin, const u32 *size, u64 N, qd_modifier m,
                                                        u32 *err) {
     const u32 lane = threadIdx.x & (kWave - 1);
@@ -240,6 +247,11 @@ __global__ __launch_bounds__(256) void k_step_modifier(u8 *obj, const u64 *begin
         if (__ballot(nq > 16) == 0) {
             const u64 p = p0 + lane / 16;
             step_object<16>(obj, begin, size, p, p < N, m, err);
+        } else if (__ballot(nq > 32) == 0) {
+            for (u32 h = 0; h < 2; ++h) {
+                const u64 p = p0 + 2 * h + lane / 32;
+                step_object<32>(obj, begin, size, p, p < N, m, err);
+            }
         } else {
             for (u32 q = 0; q < 4; ++q) step_object<kWave>(obj, begin, size, p0 + q, p0 + q < N, m, err);
         }
