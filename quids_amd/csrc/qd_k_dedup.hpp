@@ -20,10 +20,13 @@ namespace qd {
 // global atomics order the output.  k_dedup is the main kernel; it hands
 // the rare buckets it cannot pack to k_dedup_sort, and buckets larger than
 // the LDS capacity go to k_dedup_big.
+// With UK, every slot also gets the top 32 bits of its |m|^2 bit pattern
+// (0 for an empty slot): the select then reads 4 bytes per slot.
 constexpr int kNormBins = 2048;  // |m|^2 exponent histogram (the sign bit is 0)
 constexpr u64 kNoRep = ~0ull;
 
 __device__ inline u32 norm_top12(double nrm) { return u32(__double_as_longlong(nrm) >> 52) & 0xFFF; }
+__device__ inline u32 norm_top32(double nrm) { return u32(u64(__double_as_longlong(nrm)) >> 32); }
 
 __device__ inline bool key_gt(u64 ka, u64 ca, u64 kb, u64 cb) { return ka > kb || (ka == kb && ca > cb); }
 
@@ -68,8 +71,8 @@ __device__ inline u32 block_excl_max(u32 v, u64 *tmp) {
 // buckets are list[0 .. nbk) (those k_dedup hands back); else 0 .. nbk.
 template <int T, int CAP, int TS>
 __global__ __launch_bounds__(T) void k_dedup_sort(const Entry *E, u64 rstride, const u64 *woff, u64 nbk, u32 L,
-                                                  double tol, u64 *UH, u64 *UR, double2 *UM, u64 *counters,
-                                                  u64 *nhist, u64 *biglist, const u64 *list) {
+                                                  double tol, u64 *UH, u64 *UR, double2 *UM, u32 *UK,
+                                                  u64 *counters, u64 *nhist, u64 *biglist, const u64 *list) {
     constexpr int PER = CAP / T;
     constexpr int SPER = TS / T;
     __shared__ u64 key[CAP];
@@ -207,9 +210,11 @@ __global__ __launch_bounds__(T) void k_dedup_sort(const Entry *E, u64 rstride, c
         // the bucket's slot range
         u32 cnt = 0;
         u32 keepbits = 0;
+        u32 nk[PER];
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const u32 p = threadIdx.x * PER + k;
+            nk[k] = 0;
             if (p < n) {
                 const u32 ia = perm[p];
                 if (gstart[gslot[ia]] == p) {
@@ -218,6 +223,7 @@ __global__ __launch_bounds__(T) void k_dedup_sort(const Entry *E, u64 rstride, c
                     if (nrm > tol) {
                         keepbits |= 1u << k;
                         ++cnt;
+                        nk[k] = norm_top32(nrm);
                         atomicAdd(&nh[min(norm_top12(nrm), u32(kNormBins - 1))], 1u);
                     }
                 }
@@ -232,10 +238,14 @@ __global__ __launch_bounds__(T) void k_dedup_sort(const Entry *E, u64 rstride, c
                 UH[pos] = key[pi];
                 UR[pos] = cand[pi];
                 UM[pos] = mag[pi];
+                if (UK) UK[pos] = nk[k];
                 ++pos;
             }
         }
-        for (u64 jx = beg + total + threadIdx.x; jx < beg + n; jx += T) UR[jx] = kNoRep;
+        for (u64 jx = beg + total + threadIdx.x; jx < beg + n; jx += T) {
+            UR[jx] = kNoRep;
+            if (UK) UK[jx] = 0;
+        }
         if (threadIdx.x == 0) uniques += total;
         __syncthreads();
         // clear the table: owners of the slots this bucket used, and every
@@ -357,8 +367,8 @@ constexpr int kWideCandBits = 40;
 
 template <int T, int CAP, int TS>
 __global__ __launch_bounds__(T) void k_dedup(const Entry *E, u64 rstride, const u64 *woff, u64 nbk, u32 L, double tol,
-                                             u64 *UH, u64 *UR, double2 *UM, u64 *counters, u64 *nhist, u64 *biglist,
-                                             u64 *redolist) {
+                                             u64 *UH, u64 *UR, double2 *UM, u32 *UK, u64 *counters, u64 *nhist,
+                                             u64 *biglist, u64 *redolist) {
     constexpr int PER = CAP / T;
     constexpr u32 kNil = 0xFFFFFFFFu;  // empty member list (any value >= CAP ends a list)
     static_assert(CAP <= 4096 && TS <= 4096, "packed wide keys hold 12-bit slots and entries");
@@ -601,13 +611,16 @@ __global__ __launch_bounds__(T) void k_dedup(const Entry *E, u64 rstride, const 
             }
         }
         u32 cnt = 0, keepbits = 0;
+        u32 nk[PER];
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
+            nk[k] = 0;
             if (lead[k]) {
                 const double nrm = rs[k].x * rs[k].x + rs[k].y * rs[k].y;
                 if (nrm > tol) {
                     keepbits |= 1u << k;
                     ++cnt;
+                    nk[k] = norm_top32(nrm);
                     atomicAdd(&nh[min(norm_top12(nrm), u32(kNormBins - 1))], 1u);
                 }
             }
@@ -625,6 +638,7 @@ __global__ __launch_bounds__(T) void k_dedup(const Entry *E, u64 rstride, const 
                 UH[pos] = key[threadIdx.x + T * k];
                 UR[pos] = rc[k];
                 UM[pos] = rs[k];
+                if (UK) UK[pos] = nk[k];
                 ++pos;
             }
         }
@@ -633,7 +647,10 @@ __global__ __launch_bounds__(T) void k_dedup(const Entry *E, u64 rstride, const 
             if (lead[k]) owner[slot[k]] = gcnt[slot[k]] = 0;
         __syncthreads();
         const u32 total = ocount;  // reset after the next bucket's staging barrier
-        for (u64 jx = beg + total + threadIdx.x; jx < beg + n; jx += T) UR[jx] = kNoRep;
+        for (u64 jx = beg + total + threadIdx.x; jx < beg + n; jx += T) {
+            UR[jx] = kNoRep;
+            if (UK) UK[jx] = 0;
+        }
         if (threadIdx.x == 0) uniques += total;
     }
     __syncthreads();
@@ -649,7 +666,7 @@ __global__ __launch_bounds__(T) void k_dedup(const Entry *E, u64 rstride, const 
 __global__ __launch_bounds__(1024) void k_dedup_big(const Entry *E, u64 rstride, const u64 *boff, const u64 *biglist,
                                                     const u64 *soff, u64 *skeyg, u64 *scandg, double2 *smagg,
                                                     u32 *segg, double tol, u64 *UH, u64 *UR, double2 *UM,
-                                                    u64 *counters, u64 *nhist) {
+                                                    u32 *UK, u64 *counters, u64 *nhist) {
     __shared__ u32 maxg;
     __shared__ u64 tmp[1024 / kWave + 1];
     const u64 b = biglist[blockIdx.x];
@@ -732,13 +749,17 @@ __global__ __launch_bounds__(1024) void k_dedup_big(const Entry *E, u64 rstride,
                 UH[pos] = skey[i];
                 UR[pos] = scand[i];
                 UM[pos] = a;
+                if (UK) UK[pos] = norm_top32(nrm);
                 ++pos;
                 atomicAdd(reinterpret_cast<unsigned long long *>(nhist + min(norm_top12(nrm), u32(kNormBins - 1))),
                           1ull);
             }
         }
         atomicAdd(reinterpret_cast<unsigned long long *>(counters), (unsigned long long)(pos - beg));
-        for (u64 j = pos; j < beg + n; ++j) UR[j] = kNoRep;
+        for (u64 j = pos; j < beg + n; ++j) {
+            UR[j] = kNoRep;
+            if (UK) UK[j] = 0;
+        }
     }
 }
 

@@ -154,7 +154,7 @@ struct qd_ctx {
     DBuf sel0, sel1;
     DBuf bitmap, wprefix, padded, chunks, scanblk;
     DBuf biglist, redolist, bigsoff, bigkey, bigcand, bigmag, bigseg;
-    DBuf pdesc, wi, wb, cpar, opar, ebuf2, cnt2, slist;
+    DBuf pdesc, wi, wb, cpar, opar, ebuf2, cnt2, slist, ukey;
     DBuf adjk, adjv, adjg, adjd, adjp, adjw;
     DBuf pool0, pool1, pwpre, blist2;
     // output buffers of batched chunks (objects, begin, size, mag per chunk),
@@ -169,7 +169,7 @@ struct qd_ctx {
     std::vector<DBuf *> all_named() {
         return {&nc, &cb, &pre, &tab, &kept, &keys, &ebuf0, &ebuf1, &hist1, &off1, &cur1, &off2,
                 &nhist, &counters, &sel0, &sel1, &bitmap, &wprefix, &padded, &chunks, &scanblk,
-                &biglist, &redolist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &cpar, &opar, &ebuf2, &cnt2, &slist,
+                &biglist, &redolist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &cpar, &opar, &ebuf2, &cnt2, &slist, &ukey,
                 &adjk, &adjv, &adjg, &adjd, &adjp, &adjw, &bseg, &boseg, &brules, &bsinst, &bcount, &bsel, &btrunc,
                 &blist, &bcur, &btile, &bhist, &bkseg, &bchunk, &bcsum, &btot, &pool0, &pool1, &pwpre, &blist2};
     }
@@ -355,9 +355,11 @@ static int select_loop(qd_ctx *ctx, SelItem *S0, SelItem *S1, u64 ns, u64 r, u32
 // Marks the top r slots of nslots (empty slots have UR == kNoRep) by the
 // key (|m|^2 bits desc, hash asc): their bits in sbm (whole words written)
 // and their representatives in bitmap.  h is the histogram of the top 12
-// |m|^2 bits over the slots (it is overwritten).
-static int select_topk(qd_ctx *ctx, const u64 *UH, const u64 *UR, const double2 *UM, u64 nslots, u64 r,
-                       std::vector<u64> &h, u64 *bitmap, u64 *sbm, const SelKey &sk) {
+// |m|^2 bits over the slots (it is overwritten).  UK, when not null, holds
+// the top 32 key bits of every slot (0 when empty; plain |m|^2 keys): the
+// first two digits then read 4 bytes per slot instead of its whole record.
+static int select_topk(qd_ctx *ctx, const u64 *UH, const u64 *UR, const double2 *UM, const u32 *UK, u64 nslots,
+                       u64 r, std::vector<u64> &h, u64 *bitmap, u64 *sbm, const SelKey &sk) {
     hipStream_t s = ctx->stream;
     u64 *hist1 = ctx->hist1.as<u64>();
     u64 *counters = ctx->counters.as<u64>();
@@ -372,8 +374,14 @@ static int select_topk(qd_ctx *ctx, const u64 *UH, const u64 *UR, const double2 
     const u32 d1 = u32(d);
     QD_HIP(hipMemsetAsync(hist1, 0, kHistBins * sizeof(u64), s));
     QD_HIP(hipMemsetAsync(counters + 1, 0, sizeof(u64), s));
-    hipLaunchKernelGGL(k_sel_hist_u, dim3(grid_for(nslots, 256, 8192)), dim3(256), 0, s, UH, UR, UM, nslots, d1, hist1,
-                       sk);
+    // (an empty slot's UK of 0 ties with no nonzero d1)
+    const bool compact = UK && d1 > 0 && !sk.prob;
+    if (compact)
+        hipLaunchKernelGGL(k_sel_hist_k, dim3(grid_for(nslots / 4 + 1, 256, 8192)), dim3(256), 0, s, UK, nslots, d1,
+                           hist1);
+    else
+        hipLaunchKernelGGL(k_sel_hist_u, dim3(grid_for(nslots, 256, 8192)), dim3(256), 0, s, UH, UR, UM, nslots, d1,
+                           hist1, sk);
     QD_HIP(hipGetLastError());
     QD_TRY(sync_read(ctx, hist1, kHistBins * sizeof(u64), h.data()));
     cum = 0;
@@ -386,15 +394,19 @@ static int select_topk(qd_ctx *ctx, const u64 *UH, const u64 *UR, const double2 
     QD_TRY(ctx->sel0.ensure(std::max<u64>(ns, 1) * sizeof(SelItem)));
     QD_TRY(ctx->sel1.ensure(std::max<u64>(ns, 1) * sizeof(SelItem)));
     SelItem *S0 = ctx->sel0.as<SelItem>(), *S1 = ctx->sel1.as<SelItem>();
-    hipLaunchKernelGGL(k_sel_first, dim3(grid_for(nslots, 256, 16384)), dim3(256), 0, s, UH, UR, UM, nslots,
-                       (d1 << 12) | u32(d), S0, counters + 1, bitmap, sbm, sk);
+    if (compact)
+        hipLaunchKernelGGL(k_sel_first_k, dim3(grid_for(nslots, 256, 16384)), dim3(256), 0, s, UK, UH, UR, UM, nslots,
+                           (d1 << 12) | u32(d), S0, counters + 1, bitmap, sbm);
+    else
+        hipLaunchKernelGGL(k_sel_first, dim3(grid_for(nslots, 256, 16384)), dim3(256), 0, s, UH, UR, UM, nslots,
+                           (d1 << 12) | u32(d), S0, counters + 1, bitmap, sbm, sk);
     QD_HIP(hipGetLastError());
     return select_loop(ctx, S0, S1, ns, r, 24, bitmap, UR, sbm);
 }
 
 // Buckets beyond the LDS capacity: one workgroup each on global scratch.
 static int dedup_big_buckets(qd_ctx *ctx, const Entry *D, u64 rstride, const u64 *boff, u64 nbig, double tol, u64 *UH, u64 *UR,
-                             double2 *UM, u64 *U) {
+                             double2 *UM, u32 *UK, u64 *U) {
     hipStream_t s = ctx->stream;
     std::vector<u64> list(nbig), soff(nbig + 1, 0);
     QD_TRY(sync_read(ctx, ctx->biglist.p, nbig * sizeof(u64), list.data()));
@@ -415,7 +427,7 @@ static int dedup_big_buckets(qd_ctx *ctx, const Entry *D, u64 rstride, const u64
     u64 *counters = ctx->counters.as<u64>();
     hipLaunchKernelGGL(k_dedup_big, dim3(nbig), dim3(1024), 0, s, D, rstride, boff, ctx->biglist.as<u64>(),
                        ctx->bigsoff.as<u64>(), ctx->bigkey.as<u64>(), ctx->bigcand.as<u64>(), ctx->bigmag.as<double2>(),
-                       ctx->bigseg.as<u32>(), tol, UH, UR, UM, counters, ctx->nhist.as<u64>());
+                       ctx->bigseg.as<u32>(), tol, UH, UR, UM, UK, counters, ctx->nhist.as<u64>());
     QD_HIP(hipGetLastError());
     QD_TRY(sync_read(ctx, counters, sizeof(u64), U));
     return QD_OK;
@@ -430,9 +442,10 @@ static int state_reserve(qd_state *s, u64 n, u64 bytes) {
 }
 
 // Dedup layout for C candidates: total bucket bits L so that buckets
-// average <= 1536 entries (half the LDS capacity, so a bucket overflows only
-// under a pathological hash distribution); first pass up to 12 bits, second
-// pass the rest.
+// average at most dedup_bucket_target entries (at most half the LDS table
+// capacity, so a bucket overflows only under a pathological hash
+// distribution); the first pass takes up to first_pass_bits, the second
+// pass the rest (at most 12 bits).
 static void dedup_layout(u64 C, const qd_tuning &t, u32 &b1, u32 &b2) {
     u32 L = 0;
     while ((C >> L) > t.dedup_bucket_target && L < 24) ++L;
@@ -445,8 +458,8 @@ static void dedup_layout(u64 C, const qd_tuning &t, u32 &b1, u32 &b2) {
 // range) may take: 2^chunk_log2,
 // or what free device memory (plus the context's cached scratch) holds at
 // about 150 bytes of scratch per candidate (entries, partition buffers, dedup
-// output, bitmaps, growth slack), with room left for the states and for
-// buffers that grow one at a time.
+// output, select keys, bitmaps, growth slack), with room left for the states
+// and for buffers that grow one at a time.
 static u64 step_budget(qd_ctx *ctx) {
     if (ctx->tune.chunk_log2) return 1ull << ctx->tune.chunk_log2;
     size_t freeb = 0, totalb = 0;
@@ -481,9 +494,10 @@ static int ensure_big(DBuf &b, u64 bytes, u64 full) {
 // oversized-bucket count).
 // With in_stride != 0 the first pass is done already: E1 holds bucket d at
 // [d * in_stride, ...) with its count in cur1[d] (the fused generate).
+// With UK (n u32), the dedup also writes every slot's top 32 |m|^2 bits.
 static int partition_dedup(qd_ctx *ctx, const u64 *H0, const double2 *M0, const u64 *CI, u64 n, u32 b1, u32 b2,
                            double tol, qd_step_stats &st, u64 *&UH, u64 *&UR, double2 *&UM, u64 &U,
-                           u64 in_stride = 0) {
+                           u64 in_stride = 0, u32 *UK = nullptr) {
     hipStream_t s = ctx->stream;
     u64 *counters = ctx->counters.as<u64>();
     u32 *errw = reinterpret_cast<u32 *>(counters + 4);
@@ -580,7 +594,7 @@ static int partition_dedup(qd_ctx *ctx, const u64 *H0, const double2 *M0, const 
     };
     auto run_dedup = [&](auto kern, int threads) -> int {
         hipLaunchKernelGGL(kern, dim3(resident(kern, threads, nbk)), dim3(threads), 0, s, D, rstride, woff, nbk, L,
-                           tol, UH, UR, UM, counters, ctx->nhist.as<u64>(), ctx->biglist.as<u64>(),
+                           tol, UH, UR, UM, UK, counters, ctx->nhist.as<u64>(), ctx->biglist.as<u64>(),
                            ctx->redolist.as<u64>());
         QD_HIP(hipGetLastError());
         return QD_OK;
@@ -588,7 +602,7 @@ static int partition_dedup(qd_ctx *ctx, const u64 *H0, const double2 *M0, const 
     // buckets holding a group too large for registers: the sort-based kernel
     auto run_sort = [&](auto kern, int threads, u64 nred) -> int {
         hipLaunchKernelGGL(kern, dim3(resident(kern, threads, nred)), dim3(threads), 0, s, D, rstride, woff, nred, L,
-                           tol, UH, UR, UM, counters, ctx->nhist.as<u64>(), ctx->biglist.as<u64>(),
+                           tol, UH, UR, UM, UK, counters, ctx->nhist.as<u64>(), ctx->biglist.as<u64>(),
                            (const u64 *)ctx->redolist.as<u64>());
         QD_HIP(hipGetLastError());
         return QD_OK;
@@ -614,7 +628,7 @@ static int partition_dedup(qd_ctx *ctx, const u64 *H0, const double2 *M0, const 
     }
     QD_TRY(check_err_word(u32(hb[4])));
     U = hb[0];
-    if (hb[5]) QD_TRY(dedup_big_buckets(ctx, D, rstride, woff, hb[5], tol, UH, UR, UM, &U));
+    if (hb[5]) QD_TRY(dedup_big_buckets(ctx, D, rstride, woff, hb[5], tol, UH, UR, UM, UK, &U));
     return QD_OK;
 }
 
@@ -764,7 +778,7 @@ static int range_step(qd_ctx *ctx, const Parents &P, const qd_rule &rule, u64 C,
         QD_HIP(hipGetLastError());
         QD_TRY(sync_read(ctx, hist1, kHistBins * sizeof(u64), h.data()));
         u64 *keep = ctx->slist.as<u64>();
-        QD_TRY(select_topk(ctx, Ph, Pr, Pm, ns, max_obj, h, ctx->bitmap.as<u64>(), keep, sk));
+        QD_TRY(select_topk(ctx, Ph, Pr, Pm, nullptr, ns, max_obj, h, ctx->bitmap.as<u64>(), keep, sk));
         const u64 Wp = (ns + 63) / 64;
         QD_TRY(run_scan(ctx, LoadPopc{keep}, Wp, ctx->pwpre.as<u64>()));
         hipLaunchKernelGGL(k_pool_compact, dim3(grid_for(ns)), dim3(256), 0, s, (const u64 *)Ph, (const u64 *)Pr,
@@ -1157,10 +1171,16 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
         const double mean = double(C) / double(nbin1);
         cap1 = std::min<u64>(C, u64(mean + 8.0 * std::sqrt(mean)) + 4096);
     }
+    // compact select keys when a top-K select of plain |m|^2 keys may follow
+    u32 *UK = nullptr;
     {
         const u64 full = step_budget(ctx) * 32;
         QD_TRY(ensure_big(ctx->ebuf0, C * 32, full));
         QD_TRY(ensure_big(ctx->ebuf1, std::max(C, nbin1 * cap1) * 32, full + full / 32));
+        if (!BATCH && !sk.prob && (max_obj == 0 || C > max_obj)) {
+            QD_TRY(ensure_big(ctx->ukey, C * sizeof(u32), full / 8));
+            UK = ctx->ukey.as<u32>();
+        }
     }
     QD_TRY(ctx->hist1.ensure(kHistBins * sizeof(u64)));
     QD_TRY(ctx->off1.ensure((kHistBins + 1) * sizeof(u64)));
@@ -1228,7 +1248,7 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
     u64 *UH, *UR;
     double2 *UM;
     u64 U;
-    QD_TRY(partition_dedup(ctx, H0, M0, nullptr, C, b1, b2, prm.tolerance, st, UH, UR, UM, U, in_stride));
+    QD_TRY(partition_dedup(ctx, H0, M0, nullptr, C, b1, b2, prm.tolerance, st, UH, UR, UM, U, in_stride, UK));
     st.num_unique = U;
 
     // memory-driven cap (max_num_object == 0, quids.hpp:459-486, 510-536):
@@ -1265,7 +1285,7 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
             QD_HIP(hipGetLastError());
         }
         QD_TRY(sync_read(ctx, ctx->nhist.p, kHistBins * sizeof(u64), h.data()));
-        QD_TRY(select_topk(ctx, UH, UR, UM, C, max_obj, h, bitmap, sbm, sk));
+        QD_TRY(select_topk(ctx, UH, UR, UM, UK, C, max_obj, h, bitmap, sbm, sk));
     }
     prof_end(ctx, QD_PHASE_SELECT);
     prof_begin(ctx, QD_PHASE_ORDER);

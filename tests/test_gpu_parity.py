@@ -101,6 +101,26 @@ def test_qcgd_truncation(ctx, cap):
         assert_states_identical(g, o, "pre-truncation")
 
 
+@pytest.mark.parametrize("scale", [1.0, 3e-153, 1e-154, 1e-160])
+def test_truncation_key_ranges(ctx, scale):
+    """Top-K select whose threshold lies among normal |m|^2 (the select
+    reads the dedup's 32-bit keys), around the smallest normal and among
+    subnormal |m|^2, some of which underflow to 0 (tolerance 0; a threshold
+    with a zero exponent field takes the full-record select): identical to
+    the oracle.  About 530 uniques from 52 parents."""
+    base = qcgd_initial(7, 11)
+    rules = qcgd_iteration_rules()
+    for r in rules + rules[:2]:
+        base = orc.apply_modifier(base, r) if r.is_modifier else orc.simulate(base, r)[0]
+    hs = HostState(base.objects, base.object_begin, base.object_size, base.magnitude * scale, base.total_proba)
+    rule = qcgd.erase_create(math.pi / 3, math.pi / 5)
+    for cap in (5, 37, 200):
+        g, st, _ = gpu_step(hs, rule, max_num_object=cap, tolerance=0.0)
+        o, ost = orc.simulate(hs, rule, max_num_object=cap, tolerance=0.0)
+        assert st["num_unique"] == ost["num_unique"]
+        assert_states_identical(g, o, f"scale {scale} cap {cap}")
+
+
 def test_coin_and_tolerance(ctx):
     hs = qcgd_initial(7, 5)
     r1 = qcgd.coin(math.pi / 7, 0.3, 0.2)
