@@ -59,16 +59,17 @@ __global__ __launch_bounds__(256) void k_child_size(Parents P, const u64 *cb, qd
 //  * erase_create / coin / hadamard: the child is the parent with some
 //    particle (qubit) bytes flipped -- one coalesced copy, the flips applied
 //    in registers by the lanes whose words cover those bytes;
-//  * split_merge: the parent is staged in LDS; lane 0 lists the child's
+//  * split_merge: four survivors per wave, one 16-lane segment each: the
+//    parent is staged in LDS; the segment's first lane lists the child's
 //    nodes (kind, parent node) from the decisions (qcgd.hpp:850-1035); each
-//    lane then sizes its child node's name tree, a wave scan places them,
+//    lane then sizes a child node's name tree, a segment scan places them,
 //    and the lanes write the header and copy their name segments into an LDS
 //    child buffer, stored with coalesced 4-byte writes.
 // Objects beyond the LDS buffers (or not 4-byte aligned) are written by one
 // lane straight from global memory.
-constexpr int kPopParent = 2048;  // LDS bytes per wave: small, so ~20 waves fit a CU
-constexpr int kPopChild = 4096;
-constexpr int kPopNodes = 128;
+constexpr int kPopParent = 1024;  // LDS bytes per split_merge survivor (four per wave)
+constexpr int kPopChild = 2048;
+constexpr int kPopNodes = 64;
 
 __device__ inline u32 wave_excl_sum(u32 v, u32 &total) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -156,151 +157,177 @@ __global__ __launch_bounds__(256) void k_populate_flip(Parents P, const u64 *cb,
     }
 }
 
-// split_merge: one single-wave block per survivor (see above).
-__global__ __launch_bounds__(64) void k_populate_sm(Parents P, const u64 *cb, qd_rule rule, const u64 *orep,
-                                                 const u64 *opar, u64 K, const u64 *obegin, const u32 *osize,
-                                                 u8 *oobj) {
-    __shared__ __align__(16) u8 pbuf[kPopParent];
-    __shared__ __align__(16) u8 cbuf[kPopChild];
-    __shared__ u8 ckind[kPopNodes], cl[kPopNodes], cr[kPopNodes];
-    __shared__ u16 cnode[kPopNodes];
-    __shared__ u32 nnodes;
-    const u32 lane = threadIdx.x;
-    for (u64 k = blockIdx.x; k < K; k += gridDim.x) {
-        const u64 c = orep[k];
-        const u64 p = opar[k];
-        const u32 j = u32(c - cb[p]);
-        const u8 *o = P.obj + P.begin[p];
-        const u32 psz = P.size[p];
-        const u32 csz = osize[k];
-        const u64 ob = obegin[k], oe = obegin[k + 1];
-        const u64 span = oe - ob;  // child bytes + zero padding
-        u8 *d = oobj + ob;
-        const bool aligned = ((P.begin[p] | ob) & 3) == 0;
-        if (!(aligned && psz <= kPopParent && csz <= kPopChild)) {
-            if (lane == 0) {
-                Graph g(o);
-                const u32 s = sm_child_write(g, sm_first_hmlz(g), j, d);
-                for (u64 q = s; q < span; ++q) d[q] = 0;
-            }
-            continue;
+// split_merge: kPopSegs survivors per single-wave block, one segment of
+// kPopSeg lanes each (see above).
+constexpr int kPopSeg = 16;
+constexpr int kPopSegs = kWave / kPopSeg;
+
+// Exclusive sum over the kPopSeg lanes of this lane's segment.
+__device__ inline u32 seg_excl_sum(u32 v, u32 &total) {
+    const int sl = threadIdx.x & (kPopSeg - 1);
+    u32 inc = v;
+#pragma unroll
+    for (int d = 1; d < kPopSeg; d <<= 1) {
+        const u32 o = __shfl_up(inc, d, kPopSeg);
+        if (sl >= d) inc += o;
+    }
+    total = __shfl(inc, kPopSeg - 1, kPopSeg);
+    return inc - v;
+}
+
+__global__ __launch_bounds__(kWave) void k_populate_sm(Parents P, const u64 *cb, qd_rule rule, const u64 *orep,
+                                                    const u64 *opar, u64 K, const u64 *obegin, const u32 *osize,
+                                                    u8 *oobj) {
+    __shared__ __align__(16) u8 pbuf[kPopSegs][kPopParent];
+    __shared__ __align__(16) u8 cbuf[kPopSegs][kPopChild];
+    __shared__ u8 ckind[kPopSegs][kPopNodes], cl[kPopSegs][kPopNodes], cr[kPopSegs][kPopNodes];
+    __shared__ u16 cnode[kPopSegs][kPopNodes];
+    __shared__ u32 nnodes[kPopSegs];
+    const u32 sg = threadIdx.x / kPopSeg, sl = threadIdx.x & (kPopSeg - 1);
+    u8 *const pb = pbuf[sg];
+    u8 *const cbf = cbuf[sg];
+    for (u64 k0 = u64(blockIdx.x) * kPopSegs; k0 < K; k0 += u64(gridDim.x) * kPopSegs) {
+        // every lane of a segment holds its survivor's scalars
+        const u64 k = k0 + sg;
+        const bool valid = k < K;
+        u32 j = 0, psz = 0, csz = 0;
+        u64 ob = 0, span = 0;
+        const u8 *o = P.obj;
+        bool fast = false;
+        if (valid) {
+            const u64 c = orep[k];
+            const u64 p = opar[k];
+            j = u32(c - cb[p]);
+            o = P.obj + P.begin[p];
+            psz = P.size[p];
+            csz = osize[k];
+            ob = obegin[k];
+            span = obegin[k + 1] - ob;
+            fast = ((P.begin[p] | ob) & 3) == 0 && psz <= u32(kPopParent) && csz <= u32(kPopChild);
         }
-        for (u32 w = lane; w < psz / 4; w += 64) reinterpret_cast<u32 *>(pbuf)[w] = *reinterpret_cast<const u32 *>(o + 4 * w);
+        u8 *d = oobj + ob;
+        if (fast)
+            for (u32 w = sl; w < psz / 4; w += kPopSeg)
+                reinterpret_cast<u32 *>(pb)[w] = *reinterpret_cast<const u32 *>(o + 4 * w);
         __syncthreads();
-        Graph g(pbuf);
-        if (lane == 0) {
+        // one lane per segment lists the child's nodes
+        if (fast && sl == 0) {
+            Graph g(pb);
             u32 cnt = 0;
             cplx dm{0, 0};
             QcgdCoef kk{};
             sm_walk(g, sm_first_hmlz(g), j, kk, dm, false, [&](int kind, u32 i, bool l, bool r) {
                 if (cnt < kPopNodes) {
-                    ckind[cnt] = u8(kind);
-                    cnode[cnt] = u16(i);
-                    cl[cnt] = l;
-                    cr[cnt] = r;
+                    ckind[sg][cnt] = u8(kind);
+                    cnode[sg][cnt] = u16(i);
+                    cl[sg][cnt] = l;
+                    cr[sg][cnt] = r;
                 }
                 ++cnt;
             });
-            nnodes = cnt;
+            nnodes[sg] = cnt;
         }
         __syncthreads();
-        const u32 nc = nnodes;
-        if (nc > kPopNodes) {  // more child nodes than the list holds
-            if (lane == 0) {
-                const u32 s = sm_child_write(g, sm_first_hmlz(g), j, d);
-                for (u64 q = s; q < span; ++q) d[q] = 0;
-            }
-            __syncthreads();
-            continue;
+        const u32 nc = fast ? nnodes[sg] : 0u;
+        if (nc > u32(kPopNodes)) fast = false;  // more child nodes than the list holds
+        if (valid && !fast && sl == 0) {
+            Graph g(o);
+            const u32 sz = sm_child_write(g, sm_first_hmlz(g), j, d);
+            for (u64 q = sz; q < span; ++q) d[q] = 0;
         }
-        // name tree of every child node: [src_b, src_e) copied after an
-        // optional new root
-        u32 base = 0;
-        for (u32 c0 = 0; c0 < nc; c0 += kWave) {
-            const u32 cn = c0 + lane;
-            u32 len = 0, sb = 0, se = 0, sb2 = 0, se2 = 0;
-            bool newroot = false;
-            i16 nh = 0, nt = 0;
-            u64 nhash = 0;
-            if (cn < nc) {
-                const u32 kind = ckind[cn], i = cnode[cn];
-                const u32 b = g.name_begin(i), e = g.name_begin(i + 1), m = e - b;
-                if (kind == 0) {
-                    sb = b;
-                    se = e;
-                } else if (kind == 1 || kind == 2) {
-                    const i16 t = g.sub_type(b);
-                    if (t >= 0) {
-                        sb = kind == 1 ? b + 1 : b + u32(t);
-                        se = kind == 1 ? b + u32(t) : e;
-                    } else {
-                        newroot = true;
-                        nt = kind == 1 ? QCGD_DOT_L : QCGD_DOT_R;
-                        nh = (nt == QCGD_DOT_L && g.sub_hmlz(b) < 0) ? i16(-1) : i16(1);
-                        nhash = kind == 1 ? split_left_hash(g, b) : split_right_hash(g, b);
+        if (fast) {
+            // name tree of every child node: [src_b, src_e) copied after an
+            // optional new root
+            Graph g(pb);
+            u32 base = 0;
+            for (u32 c0 = 0; c0 < nc; c0 += kPopSeg) {
+                const u32 cn = c0 + sl;
+                u32 len = 0, sb = 0, se = 0, sb2 = 0, se2 = 0;
+                bool newroot = false;
+                i16 nh = 0, nt = 0;
+                u64 nhash = 0;
+                if (cn < nc) {
+                    const u32 kind = ckind[sg][cn], i = cnode[sg][cn];
+                    const u32 b = g.name_begin(i), e = g.name_begin(i + 1), m = e - b;
+                    if (kind == 0) {
                         sb = b;
                         se = e;
+                    } else if (kind == 1 || kind == 2) {
+                        const i16 t = g.sub_type(b);
+                        if (t >= 0) {
+                            sb = kind == 1 ? b + 1 : b + u32(t);
+                            se = kind == 1 ? b + u32(t) : e;
+                        } else {
+                            newroot = true;
+                            nt = kind == 1 ? QCGD_DOT_L : QCGD_DOT_R;
+                            nh = (nt == QCGD_DOT_L && g.sub_hmlz(b) < 0) ? i16(-1) : i16(1);
+                            nhash = kind == 1 ? split_left_hash(g, b) : split_right_hash(g, b);
+                            sb = b;
+                            se = e;
+                        }
+                    } else {
+                        const u32 jn = i + 1 < g.n ? i + 1 : 0;
+                        const u32 bb = g.name_begin(jn), eb = g.name_begin(jn + 1);
+                        if (merge_collapses(g, b, bb)) {
+                            sb = b + 1;
+                            se = e;
+                        } else {
+                            newroot = true;
+                            nt = i16(m + 1);
+                            nh = (g.sub_hmlz(b) < 0 || g.sub_hmlz(bb) < 0) ? i16(-1) : i16(1);
+                            nhash = hash_combine(g.sub_hash(b), g.sub_hash(bb));
+                            sb = b;
+                            se = e;
+                            sb2 = bb;
+                            se2 = eb;
+                        }
                     }
+                    len = u32(newroot) + (se - sb) + (se2 - sb2);
+                }
+                u32 tot;
+                const u32 off = base + seg_excl_sum(len, tot);
+                if (cn < nc) {
+                    u8 *hdr = cbf;
+                    hdr[2 + cn] = cl[sg][cn];
+                    hdr[2 + nc + cn] = cr[sg][cn];
+                    st16(hdr + 2 + 2 * nc + 2 * cn, u16(off));
+                    u8 *dst = cbf + 4 + 4 * nc + 16 * size_t(off);
+                    if (newroot) {
+                        put_sub(dst, nh, nt, nhash);
+                        dst += 16;
+                    }
+                    const u32 *src = reinterpret_cast<const u32 *>(g.sub(sb));
+                    for (u32 q = 0; q < 4 * (se - sb); ++q) reinterpret_cast<u32 *>(dst)[q] = src[q];
+                    dst += 16 * (se - sb);
+                    if (se2 > sb2) {
+                        const u32 *src2 = reinterpret_cast<const u32 *>(g.sub(sb2));
+                        for (u32 q = 0; q < 4 * (se2 - sb2); ++q) reinterpret_cast<u32 *>(dst)[q] = src2[q];
+                    }
+                }
+                base += tot;
+            }
+            if (sl == 0) {
+                st16(cbf, u16(nc));
+                st16(cbf + 2 + 2 * nc + 2 * nc, u16(base));
+            }
+        }
+        __syncthreads();
+        if (fast) {
+            for (u64 w = sl; w < (span + 3) / 4; w += kPopSeg) {
+                const u64 at = 4 * w;
+                if (at + 4 <= span) {
+                    u32 v = 0;
+                    if (at + 4 <= csz) v = reinterpret_cast<const u32 *>(cbf)[w];
+                    else
+                        for (u32 q = 0; q < 4; ++q)
+                            if (at + q < csz) v |= u32(cbf[at + q]) << (8 * q);
+                    *reinterpret_cast<u32 *>(d + at) = v;
                 } else {
-                    const u32 jn = i + 1 < g.n ? i + 1 : 0;
-                    const u32 bb = g.name_begin(jn), eb = g.name_begin(jn + 1);
-                    if (merge_collapses(g, b, bb)) {
-                        sb = b + 1;
-                        se = e;
-                    } else {
-                        newroot = true;
-                        nt = i16(m + 1);
-                        nh = (g.sub_hmlz(b) < 0 || g.sub_hmlz(bb) < 0) ? i16(-1) : i16(1);
-                        nhash = hash_combine(g.sub_hash(b), g.sub_hash(bb));
-                        sb = b;
-                        se = e;
-                        sb2 = bb;
-                        se2 = eb;
-                    }
-                }
-                len = u32(newroot) + (se - sb) + (se2 - sb2);
-            }
-            u32 tot;
-            const u32 off = base + wave_excl_sum(len, tot);
-            if (cn < nc) {
-                u8 *hdr = cbuf;
-                hdr[2 + cn] = cl[cn];
-                hdr[2 + nc + cn] = cr[cn];
-                st16(hdr + 2 + 2 * nc + 2 * cn, u16(off));
-                u8 *dst = cbuf + 4 + 4 * nc + 16 * size_t(off);
-                if (newroot) {
-                    put_sub(dst, nh, nt, nhash);
-                    dst += 16;
-                }
-                const u32 *src = reinterpret_cast<const u32 *>(g.sub(sb));
-                for (u32 q = 0; q < 4 * (se - sb); ++q) reinterpret_cast<u32 *>(dst)[q] = src[q];
-                dst += 16 * (se - sb);
-                if (se2 > sb2) {
-                    const u32 *src2 = reinterpret_cast<const u32 *>(g.sub(sb2));
-                    for (u32 q = 0; q < 4 * (se2 - sb2); ++q) reinterpret_cast<u32 *>(dst)[q] = src2[q];
+                    for (u64 q = at; q < span; ++q) d[q] = q < csz ? cbf[q] : 0;
                 }
             }
-            base += tot;
         }
-        if (lane == 0) {
-            st16(cbuf, u16(nc));
-            st16(cbuf + 2 + 2 * nc + 2 * nc, u16(base));
-        }
-        __syncthreads();
-        for (u64 w = lane; w < (span + 3) / 4; w += 64) {
-            const u64 at = 4 * w;
-            if (at + 4 <= span) {
-                u32 v = 0;
-                if (at + 4 <= csz) v = reinterpret_cast<const u32 *>(cbuf)[w];
-                else
-                    for (u32 q = 0; q < 4; ++q)
-                        if (at + q < csz) v |= u32(cbuf[at + q]) << (8 * q);
-                *reinterpret_cast<u32 *>(d + at) = v;
-            } else {
-                for (u64 q = at; q < span; ++q) d[q] = q < csz ? cbuf[q] : 0;
-            }
-        }
-        __syncthreads();
+        __syncthreads();  // the buffers are reused by the next survivors
     }
 }
 

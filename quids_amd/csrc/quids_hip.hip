@@ -651,7 +651,8 @@ static int populate_out(qd_ctx *ctx, const Parents &P, const qd_rule &rule, cons
     QD_TRY(out->objects.ensure(std::max<u64>(bytes, 1)));
     prof_begin(ctx, QD_PHASE_FINALIZE);
     if (KIND == QD_RULE_QCGD_SPLIT_MERGE)
-        hipLaunchKernelGGL(k_populate_sm, dim3(unsigned(std::min<u64>(K, 65536))), dim3(64), 0, s, P,
+        hipLaunchKernelGGL(k_populate_sm, dim3(unsigned(std::min<u64>((K + kPopSegs - 1) / kPopSegs, 65536))),
+                           dim3(kWave), 0, s, P,
                            ctx->cb.as<u64>(), rule, orep, (const u64 *)ctx->opar.as<u64>(), K, out->begin.as<u64>(),
                            out->size.as<u32>(), out->objects.as<u8>());
     else

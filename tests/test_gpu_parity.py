@@ -121,6 +121,38 @@ def test_truncation_key_ranges(ctx, scale):
         assert_states_identical(g, o, f"scale {scale} cap {cap}")
 
 
+def _sparse_graph(n, splits, merges):
+    left, right = [0] * n, [0] * n
+    for i in splits:
+        left[i] = right[i] = 1
+    for i in merges:  # node i merges with i + 1 (left[i + 1] stays 0)
+        left[i] = right[i + 1] = 1
+    return qcgd.with_particles(qcgd.make_graph(n), left, right)
+
+
+def test_populate_large_objects(ctx):
+    """Objects beyond the populate kernels' LDS buffers and node lists
+    (parents of 1.2-1.6 KB, children of up to 74 nodes, graphs of more than
+    64 nodes) mixed with small ones in one state, so that survivors written
+    by the one-lane path and by the segment path share wavefronts;
+    split_merge and coin steps identical to the oracle.  (The sparse graphs
+    have few split / merge / coin operations; erase_create would give them
+    2^n children.)"""
+    objs = [_sparse_graph(70, (3, 11, 20, 33, 47, 58, 66), (7, 25, 40)),
+            _sparse_graph(60, (2, 9, 30, 44, 51), (14, 36)),
+            qcgd.seeded_graph(12, 8), qcgd.seeded_graph(9, 2)]
+    rng = np.random.default_rng(17)
+    mags = rng.normal(size=len(objs)) + 1j * rng.normal(size=len(objs))
+    hs_g = hs_o = HostState.from_objects(objs, mags=mags / np.linalg.norm(mags))
+    rules = [qcgd.split_merge(0.6, 0.3), qcgd.coin(0.9, 0.4), qcgd.split_merge(1.1, 0.2)]
+    for k, r in enumerate(rules):
+        hs_g, _, _ = gpu_step(hs_g, r, max_num_object=600)
+        hs_o, _ = orc.simulate(hs_o, r, max_num_object=600)
+        assert_states_identical(hs_g, hs_o, f"step {k} {type(r).__name__}")
+    sizes = hs_o.object_size
+    assert int(sizes.max()) > 1024 and int(sizes.min()) < 1024 and hs_o.num_object > 100
+
+
 def test_coin_and_tolerance(ctx):
     hs = qcgd_initial(7, 5)
     r1 = qcgd.coin(math.pi / 7, 0.3, 0.2)
