@@ -12,19 +12,32 @@ namespace qd {
 // ===========================================================================
 // Survivors listed by their dedup output slot: rank = number of smaller
 // representatives (popcount prefix of the representative bitmap).
-// Survivors enumerated from the slot bitmap (one wave per 64 dedup output
-// slots); each goes to the rank of its representative among the marked
-// representatives.
+// Survivors enumerated from the slot bitmap (one lane per 64-slot word, no
+// bit set at or past U); each goes to the rank of its representative among
+// the marked representatives.
 __global__ __launch_bounds__(256) void k_place(const u64 *UR, const double2 *UM, const u64 *sbm, u64 U,
                                                const u64 *bitmap, const u64 *wprefix, u64 *orep, double2 *omag) {
-    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < U; i += u64(gridDim.x) * blockDim.x) {
-        if (!((sbm[i >> 6] >> (i & 63)) & 1)) continue;
-        const u64 rep = UR[i];
-        const u64 word = bitmap[rep >> 6];
-        const u32 bit = u32(rep & 63);
-        const u64 r = wprefix[rep >> 6] + __popcll(word & ((1ull << bit) - 1));
-        orep[r] = rep;
-        omag[r] = UM[i];
+    constexpr int kUnroll = 4;  // loads in flight per lane
+    const u64 nw = (U + 63) / 64, stride = u64(gridDim.x) * blockDim.x;
+    for (u64 w0 = u64(blockIdx.x) * blockDim.x + threadIdx.x; w0 < nw; w0 += kUnroll * stride) {
+        u64 m[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const u64 w = w0 + u * stride;
+            m[u] = w < nw ? sbm[w] : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            for (u64 mm = m[u]; mm; mm &= mm - 1) {
+                const u64 i = (w0 + u * stride) * 64 + u64(__ffsll((unsigned long long)mm) - 1);
+                const u64 rep = UR[i];
+                const u64 word = bitmap[rep >> 6];
+                const u32 bit = u32(rep & 63);
+                const u64 r = wprefix[rep >> 6] + __popcll(word & ((1ull << bit) - 1));
+                orep[r] = rep;
+                omag[r] = UM[i];
+            }
+        }
     }
 }
 

@@ -129,19 +129,28 @@ __global__ __launch_bounds__(256) void k_sel_hist_k(const u32 *UK, u64 U, u32 d1
     __shared__ u32 h[kHistBins];
     for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) h[i] = 0;
     __syncthreads();
+    constexpr int kUnroll = 4;  // loads in flight per lane
     const u64 n4 = U / 4, stride = u64(gridDim.x) * blockDim.x;
     const uint4 *U4 = reinterpret_cast<const uint4 *>(UK);
-    for (u64 q = u64(blockIdx.x) * blockDim.x + threadIdx.x; q < n4; q += stride) {
-        const uint4 v = U4[q];
-        if ((v.x >> 20) == d1) atomicAdd(&h[(v.x >> 8) & 0xFFF], 1u);
-        if ((v.y >> 20) == d1) atomicAdd(&h[(v.y >> 8) & 0xFFF], 1u);
-        if ((v.z >> 20) == d1) atomicAdd(&h[(v.z >> 8) & 0xFFF], 1u);
-        if ((v.w >> 20) == d1) atomicAdd(&h[(v.w >> 8) & 0xFFF], 1u);
-    }
-    if (blockIdx.x == 0 && threadIdx.x < U - 4 * n4) {
-        const u32 v = UK[4 * n4 + threadIdx.x];
+    auto count = [&](u32 v) {
         if ((v >> 20) == d1) atomicAdd(&h[(v >> 8) & 0xFFF], 1u);
+    };
+    for (u64 q0 = u64(blockIdx.x) * blockDim.x + threadIdx.x; q0 < n4; q0 += kUnroll * stride) {
+        uint4 v[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const u64 q = q0 + u * stride;
+            v[u] = q < n4 ? U4[q] : make_uint4(0u, 0u, 0u, 0u);
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            count(v[u].x);
+            count(v[u].y);
+            count(v[u].z);
+            count(v[u].w);
+        }
     }
+    if (blockIdx.x == 0 && threadIdx.x < U - 4 * n4) count(UK[4 * n4 + threadIdx.x]);
     __syncthreads();
     for (int i = threadIdx.x; i < kHistBins; i += blockDim.x)
         if (h[i]) atomicAdd(reinterpret_cast<unsigned long long *>(hist + i), (unsigned long long)h[i]);
@@ -150,26 +159,34 @@ __global__ __launch_bounds__(256) void k_sel_hist_k(const u32 *UK, u64 U, u32 d1
 __global__ __launch_bounds__(256) void k_sel_first_k(const u32 *UK, const u64 *UH, const u64 *UR, const double2 *UM,
                                                      u64 U, u32 prefix24, SelItem *S, u64 *scount, u64 *bitmap,
                                                      u64 *sbm) {
+    constexpr int kUnroll = 4;  // loads in flight per lane
     const u64 stride = u64(gridDim.x) * blockDim.x;
     const SelKey plain{0u, 0u, 0ull};
-    for (u64 base = u64(blockIdx.x) * blockDim.x; base < U; base += stride) {
-        const u64 i = base + threadIdx.x;
-        bool take = false, keep = false;
-        SelItem it{};
-        if (i < U) {
-            const u32 t = UK[i] >> 8;
-            if (t > prefix24) keep = true;
-            else if (t == prefix24) {
+    // uniform trip count per wave so that the ballots see every lane; each
+    // wave's lanes hold 64 consecutive slots
+    for (u64 base = u64(blockIdx.x) * blockDim.x; base < U; base += kUnroll * stride) {
+        u32 t[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const u64 i = base + u * stride + threadIdx.x;
+            t[u] = i < U ? UK[i] >> 8 : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const u64 i = base + u * stride + threadIdx.x;
+            const bool keep = i < U && t[u] > prefix24;
+            const bool take = i < U && t[u] == prefix24;
+            SelItem it{};
+            if (take) {
                 const double2 m = UM[i];
                 const u64 h = UH[i];
-                take = true;
                 it = SelItem{sel_hi(m.x, m.y, h, plain), ~h, i};
             }
+            const u64 first = i & ~u64(kWave - 1);
+            if (first < U) select_word(keep, i, first, UR, bitmap, sbm);
+            const u64 at = wave_append(take, scount);
+            if (take) S[at] = it;
         }
-        const u64 first = i & ~u64(kWave - 1);
-        if (first < U) select_word(keep, i, first, UR, bitmap, sbm);
-        const u64 at = wave_append(take, scount);
-        if (take) S[at] = it;
     }
 }
 

@@ -1306,7 +1306,7 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
     QD_TRY(ctx->keys.ensure(K * sizeof(u64)));
     u64 *orep = ctx->keys.as<u64>();
     prof_begin(ctx, QD_PHASE_ORDER);
-    hipLaunchKernelGGL(k_place, dim3(grid_for(C, 256, 16384)), dim3(256), 0, s, UR, UM, (const u64 *)sbm, C, bitmap,
+    hipLaunchKernelGGL(k_place, dim3(grid_for((C + 63) / 64, 256, 4096)), dim3(256), 0, s, UR, UM, (const u64 *)sbm, C, bitmap,
                        ctx->wprefix.as<u64>(), orep, out->mag.as<double2>());
     prof_end(ctx, QD_PHASE_ORDER);
     QD_TRY(populate_out<KIND>(ctx, P, rule, orep, K, prm, out, st));
