@@ -237,23 +237,34 @@ def batched_bench(n_inst=256, n_iter=8, log2_cap=18, seed=4242, lo_nodes=6, hi_n
         cls = qcgd.split_merge if which == 0 else qcgd.erase_create
         return [cls(float(theta[i]), float(phi[i])) for i in range(n_inst)]
 
-    # batched
+    # batched: one untimed pass first, which sizes the context's scratch
+    # (allocations of tens of GB take up to seconds; the sequential calls
+    # below find the scratch allocated too)
+    def run_batched(label, a, b):
+        stats = []
+        for it in range(n_iter):
+            apply_modifier(a, qcgd.step)
+            for which in (0, 1):
+                stats.append(simulate_batched(a, rules(which), b, cap))
+                a, b = b, a
+                log(f"{label} batched iteration {it + 1} rule {which}: "
+                    f"C={sum(x['num_candidates'] for x in stats[-1])} "
+                    f"objects={int(a.seg[-1])} chunked={bool(stats[-1][0]['reserved'] & 2)} "
+                    f"scratch={ctx.scratch_bytes() / 1e9:.1f} GB free={_dev_free_gb():.1f} GB")
+        return stats, a, b
+
+    for x in run_batched("untimed", batch_t.from_states(states), batch_t())[1:]:
+        x.close()
     a, b = batch_t.from_states(states), batch_t()
-    bstats = []
     ctx.set_profiling(True)
     ctx.reset_phase_times()
     ctx.synchronize()
     t0 = time.perf_counter()
-    for it in range(n_iter):
-        apply_modifier(a, qcgd.step)
-        for which in (0, 1):
-            bstats.append(simulate_batched(a, rules(which), b, cap))
-            a, b = b, a
-            log(f"batched iteration {it + 1} rule {which}: C={sum(x['num_candidates'] for x in bstats[-1])} "
-                f"objects={int(a.seg[-1])} chunked={bool(bstats[-1][0]['reserved'] & 2)} "
-                f"scratch={ctx.scratch_bytes() / 1e9:.1f} GB free={_dev_free_gb():.1f} GB")
+    bstats, a, b = run_batched("timed", a, b)
     ctx.synchronize()
     t_batch = time.perf_counter() - t0
+    a.close()
+    b.close()
     bphase = {k: round(v[0], 2) for k, v in ctx.phase_times().items()}
     ctx.reset_phase_times()
     C = sum(st["num_candidates"] for step in bstats for st in step)
@@ -262,8 +273,6 @@ def batched_bench(n_inst=256, n_iter=8, log2_cap=18, seed=4242, lo_nodes=6, hi_n
                        f"{n_iter} iterations, per-instance cap 2^{log2_cap}, seed {seed}",
            "candidates": C, "batched_seconds": t_batch, "batched_children_per_s": C / t_batch,
            "batched_phase_ms": bphase}
-    a.close()
-    b.close()
     if sequential:
         its = []
         for hs in states:
