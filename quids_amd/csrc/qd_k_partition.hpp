@@ -63,53 +63,35 @@ constexpr u32 ERR_SPILL = 16u;
 
 // Input bucket b holds off1[b+1] - off1[b] entries, read from in at
 // b * in_stride (fixed-capacity first pass) or at off1[b] (in_stride 0).
+// Workgroup g takes buckets g, g + gridDim.x, ... of the 2^b1.
 __global__ __launch_bounds__(1024) void k_level2_fixed(const Entry *in, u64 in_stride, const u64 *off1, u32 b1, u32 b2,
                                                        u64 cap2, Entry *out, u64 *cnt2, u32 *err) {
     __shared__ u32 ccnt[kHistBins];
     __shared__ u32 cur[kHistBins];
-    const u64 b = blockIdx.x;
-    const u64 beg = off1[b], end = off1[b + 1];
-    const u64 ibase = in_stride ? b * in_stride : beg;  // entry i of [beg, end) is in[ibase + i - beg]
-    const u32 nb2 = 1u << b2;
-    const u32 shift = 64 - b1 - b2;
-    Entry *region = out + (b << b2) * cap2;
-    for (u32 i = threadIdx.x; i < nb2; i += blockDim.x) cur[i] = 0;
     bool spill = false;
-    // entries of the chunk being split, and the next chunk's, loaded while
-    // this one is ranked and written: four 64-bit words per entry (key,
-    // candidate, magnitude bits), in plain registers
-    // (lanes past the bucket's end load its first entry and drop it)
-    const u64 nent = end - beg;
-    if (nent == 0) {  // uniform
-        for (u32 i = threadIdx.x; i < nb2; i += blockDim.x) cnt2[(b << b2) + i] = 0;
-        return;
-    }
-    const ulonglong2 *src = reinterpret_cast<const ulonglong2 *>(in + ibase);
-    ulonglong2 *dst = reinterpret_cast<ulonglong2 *>(region);
-    u64 w0[4], w1[4], w2[4], w3[4], n0[4], n1[4], n2[4], n3[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const u64 i = threadIdx.x + 1024 * k;
-        const u64 ii = i < nent ? i : 0;
-        const ulonglong2 lo = src[2 * ii], hi = src[2 * ii + 1];
-        n0[k] = lo.x;
-        n1[k] = lo.y;
-        n2[k] = hi.x;
-        n3[k] = hi.y;
-    }
-    for (u64 cbase = beg; cbase < end; cbase += 4096) {
-        for (u32 i = threadIdx.x; i < nb2; i += blockDim.x) ccnt[i] = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            w0[k] = n0[k];
-            w1[k] = n1[k];
-            w2[k] = n2[k];
-            w3[k] = n3[k];
+    for (u64 b = blockIdx.x; b < (1ull << b1); b += gridDim.x) {
+        const u64 beg = off1[b], end = off1[b + 1];
+        const u64 ibase = in_stride ? b * in_stride : beg;  // entry i of [beg, end) is in[ibase + i - beg]
+        const u32 nb2 = 1u << b2;
+        const u32 shift = 64 - b1 - b2;
+        Entry *region = out + (b << b2) * cap2;
+        for (u32 i = threadIdx.x; i < nb2; i += blockDim.x) cur[i] = 0;
+        // entries of the chunk being split, and the next chunk's, loaded while
+        // this one is ranked and written: four 64-bit words per entry (key,
+        // candidate, magnitude bits), in plain registers
+        // (lanes past the bucket's end load its first entry and drop it)
+        const u64 nent = end - beg;
+        if (nent == 0) {  // uniform
+            for (u32 i = threadIdx.x; i < nb2; i += blockDim.x) cnt2[(b << b2) + i] = 0;
+            __syncthreads();
+            continue;
         }
-        __syncthreads();
+        const ulonglong2 *src = reinterpret_cast<const ulonglong2 *>(in + ibase);
+        ulonglong2 *dst = reinterpret_cast<ulonglong2 *>(region);
+        u64 w0[4], w1[4], w2[4], w3[4], n0[4], n1[4], n2[4], n3[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const u64 i = cbase + 4096 - beg + threadIdx.x + 1024 * k;
+            const u64 i = threadIdx.x + 1024 * k;
             const u64 ii = i < nent ? i : 0;
             const ulonglong2 lo = src[2 * ii], hi = src[2 * ii + 1];
             n0[k] = lo.x;
@@ -117,41 +99,63 @@ __global__ __launch_bounds__(1024) void k_level2_fixed(const Entry *in, u64 in_s
             n2[k] = hi.x;
             n3[k] = hi.y;
         }
-        u32 dr[4] = {0, 0, 0, 0};
+        for (u64 cbase = beg; cbase < end; cbase += 4096) {
+            for (u32 i = threadIdx.x; i < nb2; i += blockDim.x) ccnt[i] = 0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const u64 i = cbase + threadIdx.x + 1024 * k;
-            if (i < end) {
-                const u32 d = u32(w0[k] >> shift) & (nb2 - 1);
-                dr[k] = (d << 16) | atomicAdd(&ccnt[d], 1u);
+            for (int k = 0; k < 4; ++k) {
+                w0[k] = n0[k];
+                w1[k] = n1[k];
+                w2[k] = n2[k];
+                w3[k] = n3[k];
             }
-        }
-        __syncthreads();
+            __syncthreads();
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const u64 i = cbase + threadIdx.x + 1024 * k;
-            if (i < end) {
-                const u32 d = dr[k] >> 16;
-                const u64 slot = u64(cur[d]) + (dr[k] & 0xFFFF);
-                if (slot < cap2) {
-                    const u64 at = u64(d) * cap2 + slot;
-                    ulonglong2 lo, hi;
-                    lo.x = w0[k];
-                    lo.y = w1[k];
-                    hi.x = w2[k];
-                    hi.y = w3[k];
-                    dst[2 * at] = lo;
-                    dst[2 * at + 1] = hi;
-                } else {
-                    spill = true;
+            for (int k = 0; k < 4; ++k) {
+                const u64 i = cbase + 4096 - beg + threadIdx.x + 1024 * k;
+                const u64 ii = i < nent ? i : 0;
+                const ulonglong2 lo = src[2 * ii], hi = src[2 * ii + 1];
+                n0[k] = lo.x;
+                n1[k] = lo.y;
+                n2[k] = hi.x;
+                n3[k] = hi.y;
+            }
+            u32 dr[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const u64 i = cbase + threadIdx.x + 1024 * k;
+                if (i < end) {
+                    const u32 d = u32(w0[k] >> shift) & (nb2 - 1);
+                    dr[k] = (d << 16) | atomicAdd(&ccnt[d], 1u);
                 }
             }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const u64 i = cbase + threadIdx.x + 1024 * k;
+                if (i < end) {
+                    const u32 d = dr[k] >> 16;
+                    const u64 slot = u64(cur[d]) + (dr[k] & 0xFFFF);
+                    if (slot < cap2) {
+                        const u64 at = u64(d) * cap2 + slot;
+                        ulonglong2 lo, hi;
+                        lo.x = w0[k];
+                        lo.y = w1[k];
+                        hi.x = w2[k];
+                        hi.y = w3[k];
+                        dst[2 * at] = lo;
+                        dst[2 * at + 1] = hi;
+                    } else {
+                        spill = true;
+                    }
+                }
+            }
+            __syncthreads();
+            for (u32 i = threadIdx.x; i < nb2; i += blockDim.x) cur[i] += ccnt[i];
         }
         __syncthreads();
-        for (u32 i = threadIdx.x; i < nb2; i += blockDim.x) cur[i] += ccnt[i];
+        for (u32 i = threadIdx.x; i < nb2; i += blockDim.x) cnt2[(b << b2) + i] = min<u64>(cur[i], cap2);
+        __syncthreads();  // cur is cleared for the next bucket
     }
-    __syncthreads();
-    for (u32 i = threadIdx.x; i < nb2; i += blockDim.x) cnt2[(b << b2) + i] = min<u64>(cur[i], cap2);
     if (spill) atomicOr(err, ERR_SPILL);
 }
 

@@ -548,7 +548,14 @@ static int partition_dedup(qd_ctx *ctx, const u64 *H0, const double2 *M0, const 
         Entry *E2 = ctx->ebuf2.as<Entry>();
         u32 ew = ERR_SPILL;
         if (fixed) {
-            hipLaunchKernelGGL(k_level2_fixed, dim3(nb1), dim3(1024), 0, s, (const Entry *)E1, in_stride,
+            // one workgroup per first-level bucket, on at most half the CUs
+            // at a time: the open lines of the sub-buckets (2^b2 per
+            // workgroup) then fit L2 better, and the pass runs faster than
+            // on every CU
+            int ncu2 = 256;
+            (void)hipDeviceGetAttribute(&ncu2, hipDeviceAttributeMultiprocessorCount, ctx->device);
+            const unsigned g2 = unsigned(std::min<u64>(nb1, std::max(ncu2 / 2, 1)));
+            hipLaunchKernelGGL(k_level2_fixed, dim3(g2), dim3(1024), 0, s, (const Entry *)E1, in_stride,
                                ctx->off1.as<u64>(), b1, b2, cap2, E2, ctx->cnt2.as<u64>(), errw);
             QD_HIP(hipGetLastError());
             QD_TRY(run_scan(ctx, LoadU64{ctx->cnt2.as<u64>()}, nb, ctx->off2.as<u64>()));

@@ -165,11 +165,14 @@ def test_coin_and_tolerance(ctx):
     assert_states_identical(g2, o2, "split_merge xi")
 
 
-def test_larger_step_two_level_partition(ctx):
-    """Small dedup buckets and a narrow first pass, so that steps of a few
-    hundred thousand candidates take two partition passes."""
+@pytest.mark.parametrize("first_pass_bits", [6, 8])
+def test_larger_step_two_level_partition(ctx, first_pass_bits):
+    """Small dedup buckets, so that steps of a few hundred thousand
+    candidates take two partition passes; with 8 first-pass bits the second
+    pass has more first-level buckets than workgroups (several per
+    workgroup)."""
     old = ctx.get_tuning()
-    ctx.set_tuning(dedup_bucket_target=64, first_pass_bits=6)
+    ctx.set_tuning(dedup_bucket_target=64, first_pass_bits=first_pass_bits)
     try:
         _two_level_chain()
     finally:
