@@ -156,6 +156,8 @@ typedef struct qd_tuning {
                                      into chunks, and a step beyond it runs in hash-range passes */
     uint32_t split_generate;      /* 1: generate, then the first radix pass as a kernel of its own
                                      (default 0: fused when the first pass has at most 2^8 buckets) */
+    uint32_t eager_select;        /* 1: launch the top-K select's digit rounds one by one (default 0:
+                                     replay them from a hipGraph the context captures once) */
 } qd_tuning;
 int qd_ctx_get_tuning(qd_ctx *ctx, qd_tuning *tuning);
 int qd_ctx_set_tuning(qd_ctx *ctx, const qd_tuning *tuning);

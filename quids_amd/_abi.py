@@ -98,6 +98,7 @@ class QdTuning(ctypes.Structure):
         ("generic_path", ctypes.c_uint32),
         ("chunk_log2", ctypes.c_uint32),
         ("split_generate", ctypes.c_uint32),
+        ("eager_select", ctypes.c_uint32),
     ]
 
     def as_dict(self):

@@ -190,20 +190,90 @@ __global__ __launch_bounds__(256) void k_sel_first_k(const u32 *UK, const u64 *U
     }
 }
 
-__global__ __launch_bounds__(256) void k_sel_hist(const SelItem *S, u64 n, u32 pos, u64 *hist) {
+// ---------------------------------------------------------------------------
+// The select's later rounds, driven from the device: each round histograms
+// one 12-bit key digit of the tied items, picks the digit that holds the
+// r-th largest key, marks the items above it and carries the items on it to
+// the next round.  Counts and decisions live in SelLoop in device memory, so
+// a fixed sequence of rounds (rounds after the last one return at once)
+// needs no host round trip and is captured once in a hipGraph.
+// ---------------------------------------------------------------------------
+struct SelLoop {
+    SelItem *items[2];  // the current and the next round's tied items
+    const u64 *UR;      // representative of each slot (null: the slot is the key's index)
+    u64 *bitmap, *sbm;
+    u64 *hist;          // kHistBins counters, zero between rounds
+    u64 r;              // keys still to select among the current items (1 <= r < ns)
+    u64 ns;             // current items
+    u64 cnt;            // items carried so far in this round
+    u64 tied;           // items on the picked digit
+    u32 cur;            // items[cur] holds the current items
+    u32 pos;            // key bit of the current digit
+    u32 d;              // the picked digit
+    u32 state;          // 0 running, 1 done, 2 no convergence (equal keys)
+};
+constexpr int kSelLoopGrid = 1024;  // blocks of the histogram and split kernels
+
+__global__ __launch_bounds__(1024) void k_sl_clear(SelLoop *L) {
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < kHistBins; i += gridDim.x * blockDim.x) L->hist[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_sl_hist(const SelLoop *L) {
+    if (L->state) return;  // uniform
     __shared__ u32 h[kHistBins];
     for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) h[i] = 0;
     __syncthreads();
+    const SelItem *S = L->items[L->cur];
+    const u64 n = L->ns;
+    const u32 pos = L->pos;
     for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
         atomicAdd(&h[key_digit(S[i].hi, S[i].lo, pos)], 1u);
     __syncthreads();
     for (int i = threadIdx.x; i < kHistBins; i += blockDim.x)
-        if (h[i]) atomicAdd(reinterpret_cast<unsigned long long *>(hist + i), (unsigned long long)h[i]);
+        if (h[i]) atomicAdd(reinterpret_cast<unsigned long long *>(L->hist + i), (unsigned long long)h[i]);
 }
 
-__global__ __launch_bounds__(256) void k_sel_split(const SelItem *S, u64 n, u32 pos, u32 d, SelItem *out,
-                                                   u64 *ocount, const u64 *UR, u64 *bitmap, u64 *sbm) {
+// One block of 1024 threads: the digit d whose keys, with those of every
+// higher digit, first reach r; then clears the histogram.
+__global__ __launch_bounds__(1024) void k_sl_pick(SelLoop *L) {
+    if (L->state) return;  // uniform
+    __shared__ u64 tmp[1024 / kWave + 1];
+    constexpr int kPer = kHistBins / 1024;
+    const u64 r = L->r;  // read by every thread before the scan's barriers, written after them
+    u64 v[kPer];
+    u64 sum = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {  // digits in descending order
+        v[k] = L->hist[kHistBins - 1 - (kPer * threadIdx.x + k)];
+        sum += v[k];
+    }
+    u64 total;
+    u64 above = block_excl_scan(sum, tmp, total);  // keys of higher digits than this thread's
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        if (above < r && above + v[k] >= r) {  // exactly one digit
+            L->d = u32(kHistBins - 1 - (kPer * threadIdx.x + k));
+            L->r = r - above;
+            L->tied = v[k];
+            L->cnt = 0;
+        }
+        above += v[k];
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) L->hist[kHistBins - 1 - (kPer * threadIdx.x + k)] = 0;
+}
+
+// Marks the items above the picked digit (and those on it when all of them
+// are taken); carries the others on it to the next round's buffer.
+__global__ __launch_bounds__(256) void k_sl_split(SelLoop *L) {
+    if (L->state) return;  // uniform
+    const SelItem *S = L->items[L->cur];
+    SelItem *out = L->items[L->cur ^ 1u];
+    const u64 n = L->ns;
+    const u32 pos = L->pos, d = L->d;
+    const bool all = L->r == L->tied;
     const u64 stride = u64(gridDim.x) * blockDim.x;
+    // uniform trip count per wave so that the appends see every lane
     for (u64 base = u64(blockIdx.x) * blockDim.x; base < n; base += stride) {
         const u64 i = base + threadIdx.x;
         bool take = false, keep = false;
@@ -211,13 +281,25 @@ __global__ __launch_bounds__(256) void k_sel_split(const SelItem *S, u64 n, u32 
         if (i < n) {
             it = S[i];
             const u32 t = key_digit(it.hi, it.lo, pos);
-            keep = t > d;
-            take = t == d;
+            keep = t > d || (all && t == d);
+            take = !all && t == d;
         }
-        select_item(keep, it.slot, UR, bitmap, sbm);
-        const u64 at = wave_append(take, ocount);
+        select_item(keep, it.slot, L->UR, L->bitmap, L->sbm);
+        const u64 at = wave_append(take, &L->cnt);
         if (take) out[at] = it;
     }
+}
+
+__global__ void k_sl_advance(SelLoop *L) {
+    if (threadIdx.x || blockIdx.x || L->state) return;
+    if (L->r == L->tied) {
+        L->state = 1;
+        return;
+    }
+    L->ns = L->tied;
+    L->cur ^= 1u;
+    L->pos += 12;
+    if (L->pos >= 128) L->state = 2;
 }
 
 __global__ __launch_bounds__(256) void k_sel_mark_items(const SelItem *S, u64 n, const u64 *UR, u64 *bitmap,

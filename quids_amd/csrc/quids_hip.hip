@@ -133,6 +133,7 @@ struct DBuf {
 };
 
 constexpr u64 kHostN = 8191;  // slot of the pinned area that stages a value for the device
+constexpr u64 kHostSelLoop = 8064;  // first of the slots that stage the select loop's state
 
 struct qd_ctx {
     int device = 0;
@@ -143,7 +144,7 @@ struct qd_ctx {
     bool pending[QD_NUM_PHASES] = {};
     double phase_ms[QD_NUM_PHASES] = {};
     // defaults: what quids_amd.tuning picked on the 12-node QCGD workload
-    qd_tuning tune = {512, 8, 256, 4096, 1024, 0, 0, 0};
+    qd_tuning tune = {512, 8, 256, 4096, 1024, 0, 0, 0, 0};
     u64 phase_calls[QD_NUM_PHASES] = {};
     u64 *host = nullptr;  // pinned readback area, 8192 u64
     // scratch of the step
@@ -154,7 +155,12 @@ struct qd_ctx {
     DBuf sel0, sel1;
     DBuf bitmap, wprefix, padded, chunks, scanblk;
     DBuf biglist, redolist, bigsoff, bigkey, bigcand, bigmag, bigseg;
-    DBuf pdesc, wi, wb, cpar, opar, ebuf2, cnt2, slist, ukey;
+    DBuf pdesc, wi, wb, cpar, opar, ebuf2, cnt2, slist, ukey, selloop;
+    // the select rounds' graphs, by round count, and the SelLoop they read
+    struct SelGraph {
+        hipGraphExec_t exec = nullptr;
+        const void *L = nullptr;
+    } selgraph[16];
     DBuf adjk, adjv, adjg, adjd, adjp, adjw;
     DBuf pool0, pool1, pwpre, blist2;
     // output buffers of batched chunks (objects, begin, size, mag per chunk),
@@ -169,7 +175,7 @@ struct qd_ctx {
     std::vector<DBuf *> all_named() {
         return {&nc, &cb, &pre, &tab, &kept, &keys, &ebuf0, &ebuf1, &hist1, &off1, &cur1, &off2,
                 &nhist, &counters, &sel0, &sel1, &bitmap, &wprefix, &padded, &chunks, &scanblk,
-                &biglist, &redolist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &cpar, &opar, &ebuf2, &cnt2, &slist, &ukey,
+                &biglist, &redolist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &cpar, &opar, &ebuf2, &cnt2, &slist, &ukey, &selloop,
                 &adjk, &adjv, &adjg, &adjd, &adjp, &adjw, &bseg, &boseg, &brules, &bsinst, &bcount, &bsel, &btrunc,
                 &blist, &bcur, &btile, &bhist, &bkseg, &bchunk, &bcsum, &btot, &pool0, &pool1, &pwpre, &blist2};
     }
@@ -314,43 +320,83 @@ __global__ __launch_bounds__(256) void k_parent_items(Parents P, qd_rule rule, S
     }
 }
 
-// Selects the r largest 128-bit keys among the ns items of S0, whose
-// leading `pos` key bits are already known to tie; marks their rep bits in
-// bitmap.  Items above the chosen digit are marked as soon as it is known;
-// the tied ones are carried to the next 12-bit digit.
-static int select_loop(qd_ctx *ctx, SelItem *S0, SelItem *S1, u64 ns, u64 r, u32 pos, u64 *bitmap,
-                       const u64 *UR, u64 *sbm) {
-    hipStream_t s = ctx->stream;
-    u64 *hist = ctx->hist1.as<u64>();
-    u64 *cnt = ctx->counters.as<u64>() + 2;
-    std::vector<u64> h(kHistBins);
-    while (r > 0 && r < ns) {
-        if (pos >= 128) return fail(QD_ERR_HIP, "select did not converge (duplicate keys)");
-        QD_HIP(hipMemsetAsync(hist, 0, kHistBins * sizeof(u64), s));
-        QD_HIP(hipMemsetAsync(cnt, 0, sizeof(u64), s));
-        hipLaunchKernelGGL(k_sel_hist, dim3(grid_for(ns, 256, 4096)), dim3(256), 0, s, S0, ns, pos, hist);
-        QD_HIP(hipGetLastError());
-        QD_TRY(sync_read(ctx, hist, kHistBins * sizeof(u64), h.data()));
-        u64 cum = 0;
-        int d = kHistBins - 1;
-        for (; d >= 0; --d) {
-            if (cum + h[d] >= r) break;
-            cum += h[d];
-        }
-        r -= cum;
-        hipLaunchKernelGGL(k_sel_split, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, s, S0, ns, pos, u32(d), S1, cnt,
-                           UR, bitmap, sbm);
-        QD_HIP(hipGetLastError());
-        ns = h[d];
-        std::swap(S0, S1);
-        pos += 12;
+// The rounds of the device-driven select loop: one clear, then per 12-bit
+// digit a histogram, the pick, the split and the advance (fixed grids;
+// sizes and decisions are read from L).
+static int launch_sel_rounds(hipStream_t s, SelLoop *L, int rounds) {
+    hipLaunchKernelGGL(k_sl_clear, dim3(4), dim3(1024), 0, s, L);
+    for (int k = 0; k < rounds; ++k) {
+        hipLaunchKernelGGL(k_sl_hist, dim3(kSelLoopGrid), dim3(256), 0, s, (const SelLoop *)L);
+        hipLaunchKernelGGL(k_sl_pick, dim3(1), dim3(1024), 0, s, L);
+        hipLaunchKernelGGL(k_sl_split, dim3(kSelLoopGrid), dim3(256), 0, s, L);
+        hipLaunchKernelGGL(k_sl_advance, dim3(1), dim3(kWave), 0, s, L);
     }
-    if (r > 0 && r == ns)
-        hipLaunchKernelGGL(k_sel_mark_items, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, s, S0, ns, UR, bitmap, sbm);
     QD_HIP(hipGetLastError());
     return QD_OK;
 }
 
+// Selects the r largest 128-bit keys among the ns items of S0, whose
+// leading `pos` key bits are already known to tie; marks their rep bits in
+// bitmap (and their slots in sbm).  Items above the chosen digit are marked
+// as soon as it is known; the tied ones are carried to the next 12-bit
+// digit.  The rounds run from the device (SelLoop); the context replays
+// them from a hipGraph captured once per round count, unless the tuning
+// asks for eager launches.
+static int select_loop(qd_ctx *ctx, SelItem *S0, SelItem *S1, u64 ns, u64 r, u32 pos, u64 *bitmap,
+                       const u64 *UR, u64 *sbm) {
+    hipStream_t s = ctx->stream;
+    if (r == 0 || r > ns) return QD_OK;
+    if (r == ns) {
+        hipLaunchKernelGGL(k_sel_mark_items, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, s, S0, ns, UR, bitmap,
+                           sbm);
+        QD_HIP(hipGetLastError());
+        return QD_OK;
+    }
+    QD_TRY(ctx->selloop.ensure(sizeof(SelLoop)));
+    SelLoop *dL = ctx->selloop.as<SelLoop>();
+    SelLoop h{};
+    h.items[0] = S0;
+    h.items[1] = S1;
+    h.UR = UR;
+    h.bitmap = bitmap;
+    h.sbm = sbm;
+    h.hist = ctx->hist1.as<u64>();
+    h.r = r;
+    h.ns = ns;
+    h.pos = pos;
+    static_assert(sizeof(SelLoop) <= 8 * (kHostN - 1 - kHostSelLoop), "staging area");
+    std::memcpy(ctx->host + kHostSelLoop, &h, sizeof h);
+    QD_HIP(hipMemcpyAsync(dL, ctx->host + kHostSelLoop, sizeof h, hipMemcpyHostToDevice, s));
+    const int rounds = int((128 - pos + 11) / 12);
+    if (ctx->tune.eager_select) {
+        QD_TRY(launch_sel_rounds(s, dL, rounds));
+    } else {
+        qd_ctx::SelGraph &g = ctx->selgraph[rounds];
+        if (g.exec && g.L != dL) {  // the state buffer moved (trimmed, then allocated again)
+            QD_HIP(hipGraphExecDestroy(g.exec));
+            g.exec = nullptr;
+        }
+        if (!g.exec) {
+            hipGraph_t graph = nullptr;
+            QD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            const int rc = launch_sel_rounds(s, dL, rounds);
+            const hipError_t e = hipStreamEndCapture(s, &graph);
+            QD_TRY(rc);
+            if (e != hipSuccess) return fail(QD_ERR_HIP, std::string("select graph capture: ") + hipGetErrorString(e));
+            const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ei != hipSuccess) {
+                g.exec = nullptr;
+                return fail(QD_ERR_HIP, std::string("select graph instantiate: ") + hipGetErrorString(ei));
+            }
+            g.L = dL;
+        }
+        QD_HIP(hipGraphLaunch(g.exec, s));
+    }
+    QD_TRY(sync_read(ctx, &dL->state, sizeof(u32), ctx->host));
+    if (*reinterpret_cast<const u32 *>(ctx->host) != 1) return fail(QD_ERR_HIP, "select did not converge (duplicate keys)");
+    return QD_OK;
+}
 
 // Marks the top r slots of nslots (empty slots have UR == kNoRep) by the
 // key (|m|^2 bits desc, hash asc): their bits in sbm (whole words written)
@@ -1716,6 +1762,8 @@ int qd_ctx_destroy(qd_ctx *c) {
     if (!c) return QD_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    for (qd_ctx::SelGraph &g : c->selgraph)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
     for (DBuf *b : c->all()) b->release();
     if (c->host) (void)hipHostFree(c->host);
     for (int i = 0; i < 2 * QD_NUM_PHASES; ++i)
@@ -1781,6 +1829,7 @@ int qd_ctx_set_tuning(qd_ctx *c, const qd_tuning *t) {
         return fail(QD_ERR_INVALID, "chunk_log2 must be 0 or in [10, 40]");
     n.chunk_log2 = t->chunk_log2;  // 0 = from free device memory
     n.split_generate = t->split_generate ? 1 : 0;
+    n.eager_select = t->eager_select ? 1 : 0;
     c->tune = n;
     return QD_OK;
 }

@@ -144,3 +144,42 @@ def test_fused_generate_overflow_falls_back():
         assert got.total_proba == ref.total_proba
     finally:
         q.tolerance = tol0
+
+
+def test_select_graph_replays_match_eager():
+    """The select's digit rounds replayed from the context's hipGraph
+    against the same rounds launched one by one (eager_select=1): a chain
+    of truncated QCGD steps, each replaying the graph between the step's own
+    kernels and host reads, gives identical states, and so does the
+    parent pre-truncation (a graph of more rounds)."""
+    ctx = q.default_context()
+    before = ctx.get_tuning()
+    rules = [qcgd.split_merge(0.7, 0.2), qcgd.erase_create(1.1, 0.5)]
+    out = {}
+    try:
+        for eager in (1, 0, 1, 0):
+            ctx.set_tuning(eager_select=eager)
+            a, b = q.it_t(), q.it_t()
+            a.upload(HostState.from_objects([qcgd.seeded_graph(8, 4)]))
+            res = []
+            for _ in range(3):
+                q.simulate(a, qcgd.step)
+                for cap in (300, 41):
+                    for r in rules:
+                        q.simulate(a, r, b, None, cap)
+                        a, b = b, a
+                        res.append(a.download())
+            big = res[-3]  # pre-truncation: a parent state larger than the cap
+            c, d = q.it_t(), q.it_t()
+            c.upload(big)
+            q.simulate(c, rules[0], d, None, max(1, big.num_object // 3))
+            res.append(d.download())
+            out.setdefault(eager, []).append(res)
+        ref = out[1][0]
+        for runs in out.values():
+            for res in runs:
+                for x, y in zip(res, ref):
+                    assert np.array_equal(x.objects, y.objects) and np.array_equal(x.magnitude, y.magnitude)
+                    assert x.total_proba == y.total_proba
+    finally:
+        ctx.set_tuning(eager_select=before["eager_select"])
