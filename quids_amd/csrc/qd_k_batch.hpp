@@ -167,25 +167,70 @@ struct SegTile {
     u32 inst, pad;
 };
 
+// The per-instance digit rounds, driven from the device: the state below
+// lives in device memory, so that a fixed sequence of rounds (rounds after
+// the last needed one find no active instance) needs no host round trip
+// and is captured once in a hipGraph.  A round is k_seg_tiles (the tiles
+// of the active instances' lists), k_seg_hist, k_seg_pick, k_seg_next (the
+// next lists' offsets) and k_seg_collect.
+constexpr u64 kSegTileItems = 16384;
+constexpr int kSegLoopGrid = 4096;  // blocks of the tile kernels
+struct SegLoop {
+    SegItem *list[2];  // a round reads list[k & 1] and writes list[(k + 1) & 1]
+    SegTile *tiles;    // at most (items / kSegTileItems + n) tiles
+    u32 *hist;         // n rows of kHistBins counters, zero between rounds
+    SegSel *sel;
+    u64 *off, *cnt;    // an active instance's current list range
+    u64 *cursor;       // its next list's cursor
+    const u64 *UH;
+    u32 n, ntiles;
+};
+
+// One block of 1024 threads: the tiles of every active instance's list.
+__global__ __launch_bounds__(1024) void k_seg_tiles(SegLoop *L) {
+    __shared__ u64 tmp[1024 / kWave + 1];
+    __shared__ u64 carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (u32 i0 = 0; i0 < L->n; i0 += blockDim.x) {
+        const u32 i = i0 + threadIdx.x;
+        u64 nt = 0, off = 0, cnt = 0;
+        if (i < L->n && L->sel[i].active) {
+            off = L->off[i];
+            cnt = L->cnt[i];
+            nt = (cnt + kSegTileItems - 1) / kSegTileItems;
+        }
+        u64 total;
+        const u64 first = carry + block_excl_scan(nt, tmp, total);
+        for (u64 b = 0; b < nt; ++b)
+            L->tiles[first + b] = SegTile{off + b * kSegTileItems, off + min(cnt, (b + 1) * kSegTileItems), i, 0};
+        __syncthreads();
+        if (threadIdx.x == 0) carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) L->ntiles = u32(carry);
+}
+
 // One workgroup per tile of an active instance: LDS histogram of the 12
 // key bits at the instance's position, added into its global row.  Every
 // record of the list carries the instance's current prefix.
-__global__ __launch_bounds__(256) void k_seg_hist(const SegItem *list, const SegTile *tiles, u32 ntiles,
-                                                  const SegSel *sel, u32 *hist, const u64 *UH) {
+__global__ __launch_bounds__(256) void k_seg_hist(const SegLoop *L, u32 parity) {
     __shared__ u32 h[kHistBins];
+    const SegItem *list = L->list[parity];
+    const u32 ntiles = L->ntiles;
     for (u32 t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const SegTile tl = tiles[t];
-        const SegSel q = sel[tl.inst];
+        const SegTile tl = L->tiles[t];
+        const SegSel q = L->sel[tl.inst];
         if (!q.active) continue;  // uniform over the block
         for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) h[i] = 0;
         __syncthreads();
         for (u64 k = tl.begin + threadIdx.x; k < tl.end; k += blockDim.x) {
             const SegItem it = list[k];
-            const u64 lo = q.pos + 12 > 64 ? seg_lo(UH, it.slot, tl.inst) : 0;
+            const u64 lo = q.pos + 12 > 64 ? seg_lo(L->UH, it.slot, tl.inst) : 0;
             atomicAdd(&h[key_digit(it.hi, lo, q.pos)], 1u);
         }
         __syncthreads();
-        u32 *row = hist + u64(tl.inst) * kHistBins;
+        u32 *row = L->hist + u64(tl.inst) * kHistBins;
         for (int i = threadIdx.x; i < kHistBins; i += blockDim.x)
             if (h[i]) atomicAdd(&row[i], h[i]);
         __syncthreads();
@@ -195,13 +240,15 @@ __global__ __launch_bounds__(256) void k_seg_hist(const SegItem *list, const Seg
 // After a pick: carries the records that match the instance's new prefix
 // (the tied ones) into the next list, at the instance's cursor; one
 // reservation per tile.
-__global__ __launch_bounds__(256) void k_seg_collect(const SegItem *list, const SegTile *tiles, u32 ntiles,
-                                                     const SegSel *sel, u64 *cursor, SegItem *out, const u64 *UH) {
+__global__ __launch_bounds__(256) void k_seg_collect(const SegLoop *L, u32 parity) {
     __shared__ u64 tmp[256 / kWave + 1];
     __shared__ u64 base;
+    const SegItem *list = L->list[parity];
+    SegItem *out = L->list[parity ^ 1u];
+    const u32 ntiles = L->ntiles;
     for (u32 t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const SegTile tl = tiles[t];
-        const SegSel q = sel[tl.inst];
+        const SegTile tl = L->tiles[t];
+        const SegSel q = L->sel[tl.inst];
         if (!q.active) continue;  // uniform over the block
         for (u64 k0 = tl.begin; k0 < tl.end; k0 += blockDim.x) {
             const u64 k = k0 + threadIdx.x;
@@ -209,12 +256,13 @@ __global__ __launch_bounds__(256) void k_seg_collect(const SegItem *list, const 
             SegItem it{};
             if (k < tl.end) {
                 it = list[k];
-                take = prefix_eq(it.hi, q.pos > 64 ? seg_lo(UH, it.slot, tl.inst) : 0, q);
+                take = prefix_eq(it.hi, q.pos > 64 ? seg_lo(L->UH, it.slot, tl.inst) : 0, q);
             }
             u64 total;
             const u64 off = block_excl_scan(take ? 1 : 0, tmp, total);
             if (threadIdx.x == 0 && total)
-                base = atomicAdd(reinterpret_cast<unsigned long long *>(cursor + tl.inst), (unsigned long long)total);
+                base = atomicAdd(reinterpret_cast<unsigned long long *>(L->cursor + tl.inst),
+                                 (unsigned long long)total);
             __syncthreads();
             if (take) out[base + off] = it;
             __syncthreads();
@@ -222,80 +270,105 @@ __global__ __launch_bounds__(256) void k_seg_collect(const SegItem *list, const 
     }
 }
 
-// One workgroup (256 threads) per instance: picks the digit holding the
-// r-th largest key from the instance's histogram row, updates the state and
-// clears the row for the next round; sets *more when the instance needs
-// another round.
-__global__ __launch_bounds__(256) void k_seg_pick(SegSel *sel, u32 *hist, u32 n, u32 *more) {
+// One workgroup (256 threads) per instance (a grid-stride loop over them):
+// picks the digit holding the r-th largest key from the instance's
+// histogram row, updates the state and clears the row for the next round.
+__global__ __launch_bounds__(256) void k_seg_pick(SegLoop *L) {
     __shared__ u64 part[256];
     __shared__ u32 found_d;
     __shared__ u64 found_above, found_cnt;
-    const u32 inst = blockIdx.x;
-    if (inst >= n) return;
-    SegSel q = sel[inst];
-    if (!q.active) return;
-    u32 *row = hist + u64(inst) * kHistBins;
-    constexpr int kPer = kHistBins / 256;  // 16 bins per thread, thread 0 holds the top ones
-    const u32 t = threadIdx.x;
-    if (t == 0) found_cnt = ~0ull;
-    u64 local = 0;
-    for (int j = 0; j < kPer; ++j) local += row[kHistBins - 1 - (t * kPer + j)];
-    part[t] = local;
-    __syncthreads();
-    // inclusive prefix over threads in order (bins descending)
-    for (u32 d = 1; d < 256; d <<= 1) {
-        const u64 v = t >= d ? part[t - d] : 0;
+    const u32 n = L->n;
+    for (u32 inst = blockIdx.x; inst < n; inst += gridDim.x) {
+        SegSel q = L->sel[inst];
+        if (!q.active) continue;  // uniform over the block
+        u32 *row = L->hist + u64(inst) * kHistBins;
+        constexpr int kPer = kHistBins / 256;  // 16 bins per thread, thread 0 holds the top ones
+        const u32 t = threadIdx.x;
+        if (t == 0) found_cnt = ~0ull;
+        u64 local = 0;
+        for (int j = 0; j < kPer; ++j) local += row[kHistBins - 1 - (t * kPer + j)];
+        part[t] = local;
         __syncthreads();
-        part[t] += v;
+        // inclusive prefix over threads in order (bins descending)
+        for (u32 d = 1; d < 256; d <<= 1) {
+            const u64 v = t >= d ? part[t - d] : 0;
+            __syncthreads();
+            part[t] += v;
+            __syncthreads();
+        }
+        const u64 before = part[t] - local;
+        if (before < q.r && q.r <= before + local) {
+            u64 cum = before;
+            for (int j = 0; j < kPer; ++j) {
+                const u32 d = kHistBins - 1 - (t * kPer + j);
+                const u64 c = row[d];
+                if (cum + c >= q.r) {
+                    found_d = d;
+                    found_above = cum;
+                    found_cnt = c;
+                    break;
+                }
+                cum += c;
+            }
+        }
         __syncthreads();
-    }
-    const u64 before = part[t] - local;
-    if (before < q.r && q.r <= before + local) {
-        u64 cum = before;
-        for (int j = 0; j < kPer; ++j) {
-            const u32 d = kHistBins - 1 - (t * kPer + j);
-            const u64 c = row[d];
-            if (cum + c >= q.r) {
-                found_d = d;
-                found_above = cum;
-                found_cnt = c;
-                break;
+        for (int i = t; i < kHistBins; i += 256) row[i] = 0;
+        if (t == 0) {
+            if (found_cnt == ~0ull) {  // fewer keys than r: inconsistent counts
+                q.active = 0;
+                q.err = 1;
+            } else {
+                // the digit covers key bits [116 - pos, 128 - pos) (bit 127 = top of hi)
+                const u64 d = found_d;
+                if (q.pos < 64) {
+                    const u32 sh = 64 - q.pos;
+                    if (sh >= 12) q.hi |= d << (sh - 12);
+                    else {
+                        q.hi |= d >> (12 - sh);
+                        q.lo |= d << (52 + sh);
+                    }
+                } else if (q.pos <= 116) {
+                    q.lo |= d << (116 - q.pos);
+                } else {
+                    q.lo |= d >> (q.pos - 116);  // pos 120: the last 8 key bits
+                }
+                q.pos = min(q.pos + 12, 128u);
+                q.r -= found_above;
+                q.tied = found_cnt;
+                if (found_cnt == q.r) q.active = 0;
+                else if (q.pos >= 128) {
+                    q.active = 0;
+                    q.err = 1;  // equal keys: cannot happen, hashes are distinct in an instance
+                }
             }
-            cum += c;
+            L->sel[inst] = q;
         }
+        __syncthreads();  // the shared results are reused by the next instance
     }
+}
+
+// One block of 1024 threads, after a pick: the instances still active get
+// their tied records' range in the next list (in instance order), and the
+// cursor of the collect.
+__global__ __launch_bounds__(1024) void k_seg_next(SegLoop *L) {
+    __shared__ u64 tmp[1024 / kWave + 1];
+    __shared__ u64 carry;
+    if (threadIdx.x == 0) carry = 0;
     __syncthreads();
-    for (int i = t; i < kHistBins; i += 256) row[i] = 0;
-    if (t == 0) {
-        if (found_cnt == ~0ull) {  // fewer keys than r: inconsistent counts
-            q.active = 0;
-            q.err = 1;
-            sel[inst] = q;
-            return;
+    for (u32 i0 = 0; i0 < L->n; i0 += blockDim.x) {
+        const u32 i = i0 + threadIdx.x;
+        u64 c = 0;
+        if (i < L->n && L->sel[i].active) c = L->sel[i].tied;
+        u64 total;
+        const u64 off = carry + block_excl_scan(c, tmp, total);
+        if (i < L->n) {
+            L->off[i] = off;
+            L->cnt[i] = c;
+            L->cursor[i] = off;
         }
-        // the digit covers key bits [116 - pos, 128 - pos) (bit 127 = top of hi)
-        const u64 d = found_d;
-        if (q.pos < 64) {
-            const u32 sh = 64 - q.pos;
-            if (sh >= 12) q.hi |= d << (sh - 12);
-            else {
-                q.hi |= d >> (12 - sh);
-                q.lo |= d << (52 + sh);
-            }
-        } else if (q.pos <= 116) {
-            q.lo |= d << (116 - q.pos);
-        } else {
-            q.lo |= d >> (q.pos - 116);  // pos 120: the last 8 key bits
-        }
-        q.pos = min(q.pos + 12, 128u);
-        q.r -= found_above;
-        q.tied = found_cnt;
-        if (found_cnt == q.r) q.active = 0;
-        else if (q.pos >= 128) {
-            q.active = 0;
-            q.err = 1;  // equal keys: cannot happen, hashes are distinct in an instance
-        } else atomicOr(more, 1u);
-        sel[inst] = q;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += total;
+        __syncthreads();
     }
 }
 

@@ -133,6 +133,7 @@ struct DBuf {
 };
 
 constexpr u64 kHostN = 8191;  // slot of the pinned area that stages a value for the device
+constexpr u64 kHostSegLoop = 8000;  // first of the slots that stage the batched select loop's state
 constexpr u64 kHostSelLoop = 8064;  // first of the slots that stage the select loop's state
 
 struct qd_ctx {
@@ -155,12 +156,13 @@ struct qd_ctx {
     DBuf sel0, sel1;
     DBuf bitmap, wprefix, padded, chunks, scanblk;
     DBuf biglist, redolist, bigsoff, bigkey, bigcand, bigmag, bigseg;
-    DBuf pdesc, wi, wb, cpar, opar, ebuf2, cnt2, slist, ukey, selloop;
-    // the select rounds' graphs, by round count, and the SelLoop they read
+    DBuf pdesc, wi, wb, cpar, opar, ebuf2, cnt2, slist, ukey, selloop, segloop, bsoff, bscnt;
+    // the select rounds' graphs (by round count, and the batched one) and
+    // the device state they read
     struct SelGraph {
         hipGraphExec_t exec = nullptr;
         const void *L = nullptr;
-    } selgraph[16];
+    } selgraph[16], seggraph;
     DBuf adjk, adjv, adjg, adjd, adjp, adjw;
     DBuf pool0, pool1, pwpre, blist2;
     // output buffers of batched chunks (objects, begin, size, mag per chunk),
@@ -175,7 +177,7 @@ struct qd_ctx {
     std::vector<DBuf *> all_named() {
         return {&nc, &cb, &pre, &tab, &kept, &keys, &ebuf0, &ebuf1, &hist1, &off1, &cur1, &off2,
                 &nhist, &counters, &sel0, &sel1, &bitmap, &wprefix, &padded, &chunks, &scanblk,
-                &biglist, &redolist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &cpar, &opar, &ebuf2, &cnt2, &slist, &ukey, &selloop,
+                &biglist, &redolist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &cpar, &opar, &ebuf2, &cnt2, &slist, &ukey, &selloop, &segloop, &bsoff, &bscnt,
                 &adjk, &adjv, &adjg, &adjd, &adjp, &adjw, &bseg, &boseg, &brules, &bsinst, &bcount, &bsel, &btrunc,
                 &blist, &bcur, &btile, &bhist, &bkseg, &bchunk, &bcsum, &btot, &pool0, &pool1, &pwpre, &blist2};
     }
@@ -320,6 +322,50 @@ __global__ __launch_bounds__(256) void k_parent_items(Parents P, qd_rule rule, S
     }
 }
 
+// Runs a fixed launch sequence on the context's stream: launched one by
+// one when the tuning asks for eager launches, else replayed from a graph
+// captured the first time (and again when the state it reads, L, moved).
+template <class Launch>
+static int run_rounds(qd_ctx *ctx, qd_ctx::SelGraph &g, const void *L, Launch &&launch) {
+    hipStream_t s = ctx->stream;
+    if (ctx->tune.eager_select) return launch(s);
+    if (g.exec && g.L != L) {
+        QD_HIP(hipGraphExecDestroy(g.exec));
+        g.exec = nullptr;
+    }
+    if (!g.exec) {
+        hipGraph_t graph = nullptr;
+        QD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = launch(s);
+        const hipError_t e = hipStreamEndCapture(s, &graph);
+        QD_TRY(rc);
+        if (e != hipSuccess) return fail(QD_ERR_HIP, std::string("select graph capture: ") + hipGetErrorString(e));
+        const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            g.exec = nullptr;
+            return fail(QD_ERR_HIP, std::string("select graph instantiate: ") + hipGetErrorString(ei));
+        }
+        g.L = L;
+    }
+    QD_HIP(hipGraphLaunch(g.exec, s));
+    return QD_OK;
+}
+
+// The batched select's digit rounds (at most 11 are needed for 128-bit
+// keys from bit 0): tiles, histogram, pick, next ranges, collect.
+static int launch_seg_rounds(hipStream_t s, SegLoop *L) {
+    for (u32 k = 0; k < 12; ++k) {
+        hipLaunchKernelGGL(k_seg_tiles, dim3(1), dim3(1024), 0, s, L);
+        hipLaunchKernelGGL(k_seg_hist, dim3(kSegLoopGrid), dim3(256), 0, s, (const SegLoop *)L, k & 1u);
+        hipLaunchKernelGGL(k_seg_pick, dim3(256), dim3(256), 0, s, L);
+        hipLaunchKernelGGL(k_seg_next, dim3(1), dim3(1024), 0, s, L);
+        hipLaunchKernelGGL(k_seg_collect, dim3(kSegLoopGrid), dim3(256), 0, s, (const SegLoop *)L, k & 1u);
+    }
+    QD_HIP(hipGetLastError());
+    return QD_OK;
+}
+
 // The rounds of the device-driven select loop: one clear, then per 12-bit
 // digit a histogram, the pick, the split and the advance (fixed grids;
 // sizes and decisions are read from L).
@@ -368,33 +414,11 @@ static int select_loop(qd_ctx *ctx, SelItem *S0, SelItem *S1, u64 ns, u64 r, u32
     std::memcpy(ctx->host + kHostSelLoop, &h, sizeof h);
     QD_HIP(hipMemcpyAsync(dL, ctx->host + kHostSelLoop, sizeof h, hipMemcpyHostToDevice, s));
     const int rounds = int((128 - pos + 11) / 12);
-    if (ctx->tune.eager_select) {
-        QD_TRY(launch_sel_rounds(s, dL, rounds));
-    } else {
-        qd_ctx::SelGraph &g = ctx->selgraph[rounds];
-        if (g.exec && g.L != dL) {  // the state buffer moved (trimmed, then allocated again)
-            QD_HIP(hipGraphExecDestroy(g.exec));
-            g.exec = nullptr;
-        }
-        if (!g.exec) {
-            hipGraph_t graph = nullptr;
-            QD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            const int rc = launch_sel_rounds(s, dL, rounds);
-            const hipError_t e = hipStreamEndCapture(s, &graph);
-            QD_TRY(rc);
-            if (e != hipSuccess) return fail(QD_ERR_HIP, std::string("select graph capture: ") + hipGetErrorString(e));
-            const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            if (ei != hipSuccess) {
-                g.exec = nullptr;
-                return fail(QD_ERR_HIP, std::string("select graph instantiate: ") + hipGetErrorString(ei));
-            }
-            g.L = dL;
-        }
-        QD_HIP(hipGraphLaunch(g.exec, s));
-    }
+    QD_TRY(run_rounds(ctx, ctx->selgraph[rounds], dL,
+                      [&](hipStream_t st) { return launch_sel_rounds(st, dL, rounds); }));
     QD_TRY(sync_read(ctx, &dL->state, sizeof(u32), ctx->host));
-    if (*reinterpret_cast<const u32 *>(ctx->host) != 1) return fail(QD_ERR_HIP, "select did not converge (duplicate keys)");
+    if (*reinterpret_cast<const u32 *>(ctx->host) != 1)
+        return fail(QD_ERR_HIP, "select did not converge (duplicate keys)");
     return QD_OK;
 }
 
@@ -969,49 +993,33 @@ static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const
         // next rounds' in the second pass's
         QD_TRY(ctx->ebuf2.ensure(ntrunc * sizeof(SegItem)));
         QD_TRY(ctx->bhist.ensure(u64(n) * kHistBins * sizeof(u32)));
-        QD_TRY(ctx->counters.ensure(64 * sizeof(u64)));
-        u32 *more = reinterpret_cast<u32 *>(ctx->counters.as<u64>() + 10);
         SegItem *cur = ctx->ebuf1.as<SegItem>(), *nxt = ctx->ebuf2.as<SegItem>();
         const unsigned sg = unsigned(std::min<u64>((nslots + kSegTile - 1) / kSegTile, 16384));
         hipLaunchKernelGGL(k_seg_scatter, dim3(std::max(sg, 1u)), dim3(256), 0, s, sinst, nslots, UH, UM,
                            ctx->btrunc.as<u32>(), n, ctx->bcur.as<u64>(), cur, bt->sk);
         QD_HIP(hipMemsetAsync(ctx->bhist.p, 0, u64(n) * kHistBins * sizeof(u32), s));
-        constexpr u64 kTileItems = 16384;
-        std::vector<SegTile> tiles;
-        for (int round = 0; round < 12; ++round) {
-            tiles.clear();
-            for (u32 i = 0; i < n; ++i)
-                if (sel[i].active)
-                    for (u64 b = 0; b < cnt[i]; b += kTileItems)
-                        tiles.push_back(SegTile{off[i] + b, off[i] + std::min(cnt[i], b + kTileItems), i, 0});
-            if (tiles.empty()) break;
-            QD_TRY(upload(ctx, ctx->btile, tiles.data(), tiles.size() * sizeof(SegTile)));
-            const u32 nt = u32(tiles.size());
-            QD_HIP(hipMemsetAsync(more, 0, sizeof(u32), s));
-            hipLaunchKernelGGL(k_seg_hist, dim3(std::min<u32>(nt, 65536)), dim3(256), 0, s, (const SegItem *)cur,
-                               ctx->btile.as<SegTile>(), nt, (const SegSel *)dsel, ctx->bhist.as<u32>(), UH);
-            hipLaunchKernelGGL(k_seg_pick, dim3(n), dim3(256), 0, s, dsel, ctx->bhist.as<u32>(), n, more);
-            QD_HIP(hipGetLastError());
-            QD_TRY(sync_read(ctx, dsel, n * sizeof(SegSel), sel.data()));
-            u64 next = 0;
-            bool any = false;
-            for (u32 i = 0; i < n; ++i) {
-                if (sel[i].active) {
-                    off[i] = next;
-                    cnt[i] = sel[i].tied;
-                    next += cnt[i];
-                    any = true;
-                } else {
-                    cnt[i] = 0;
-                }
-            }
-            if (!any) break;
-            QD_TRY(upload(ctx, ctx->bcur, off.data(), n * sizeof(u64)));
-            hipLaunchKernelGGL(k_seg_collect, dim3(std::min<u32>(nt, 65536)), dim3(256), 0, s, (const SegItem *)cur,
-                               ctx->btile.as<SegTile>(), nt, (const SegSel *)dsel, ctx->bcur.as<u64>(), nxt, UH);
-            QD_HIP(hipGetLastError());
-            std::swap(cur, nxt);
-        }
+        // the digit rounds, driven from the device (SegLoop)
+        QD_TRY(upload(ctx, ctx->bsoff, off.data(), n * sizeof(u64)));
+        QD_TRY(upload(ctx, ctx->bscnt, cnt.data(), n * sizeof(u64)));
+        QD_TRY(ctx->btile.ensure((ntrunc / kSegTileItems + n + 1) * sizeof(SegTile)));
+        QD_TRY(ctx->segloop.ensure(sizeof(SegLoop)));
+        SegLoop *dL = ctx->segloop.as<SegLoop>();
+        SegLoop h{};
+        h.list[0] = cur;
+        h.list[1] = nxt;
+        h.tiles = ctx->btile.as<SegTile>();
+        h.hist = ctx->bhist.as<u32>();
+        h.sel = dsel;
+        h.off = ctx->bsoff.as<u64>();
+        h.cnt = ctx->bscnt.as<u64>();
+        h.cursor = ctx->bcur.as<u64>();
+        h.UH = UH;
+        h.n = n;
+        static_assert(sizeof(SegLoop) <= 8 * (kHostSelLoop - kHostSegLoop), "staging area");
+        std::memcpy(ctx->host + kHostSegLoop, &h, sizeof h);
+        QD_HIP(hipMemcpyAsync(dL, ctx->host + kHostSegLoop, sizeof h, hipMemcpyHostToDevice, s));
+        QD_TRY(run_rounds(ctx, ctx->seggraph, dL, [&](hipStream_t st) { return launch_seg_rounds(st, dL); }));
+        QD_TRY(sync_read(ctx, dsel, n * sizeof(SegSel), sel.data()));
         for (u32 i = 0; i < n; ++i)
             if (sel[i].err || sel[i].active) return fail(QD_ERR_HIP, "batched select did not converge");
     }
@@ -1764,6 +1772,7 @@ int qd_ctx_destroy(qd_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     for (qd_ctx::SelGraph &g : c->selgraph)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (c->seggraph.exec) (void)hipGraphExecDestroy(c->seggraph.exec);
     for (DBuf *b : c->all()) b->release();
     if (c->host) (void)hipHostFree(c->host);
     for (int i = 0; i < 2 * QD_NUM_PHASES; ++i)

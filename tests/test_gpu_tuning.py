@@ -147,11 +147,13 @@ def test_fused_generate_overflow_falls_back():
 
 
 def test_select_graph_replays_match_eager():
-    """The select's digit rounds replayed from the context's hipGraph
+    """The select's digit rounds replayed from the context's hipGraphs
     against the same rounds launched one by one (eager_select=1): a chain
     of truncated QCGD steps, each replaying the graph between the step's own
-    kernels and host reads, gives identical states, and so does the
-    parent pre-truncation (a graph of more rounds)."""
+    kernels and host reads, gives identical states, and so do the parent
+    pre-truncation (a graph of more rounds) and truncated batched steps
+    (the per-instance rounds' graph)."""
+    from quids_amd.batch import apply_modifier, batch_t, simulate_batched
     ctx = q.default_context()
     before = ctx.get_tuning()
     rules = [qcgd.split_merge(0.7, 0.2), qcgd.erase_create(1.1, 0.5)]
@@ -174,6 +176,16 @@ def test_select_graph_replays_match_eager():
             c.upload(big)
             q.simulate(c, rules[0], d, None, max(1, big.num_object // 3))
             res.append(d.download())
+            # batched: per-instance rounds from the device
+            bt = batch_t.from_states([HostState.from_objects([qcgd.seeded_graph(n, sd)])
+                                      for n, sd in ((9, 4), (6, 1), (12, 3), (7, 7))])
+            bb = batch_t()
+            for _ in range(2):
+                apply_modifier(bt, qcgd.step)
+                for r in rules:
+                    simulate_batched(bt, [r] * 4, bb, 250)
+                    bt, bb = bb, bt
+                    res.append(bt.state.download())
             out.setdefault(eager, []).append(res)
         ref = out[1][0]
         for runs in out.values():
