@@ -359,6 +359,9 @@ def main():
     from quids_amd.rules import qcgd
     from quids_amd.state import HostState
 
+    if args.tune:
+        q.default_context().set_tuning(**{k: int(v) for k, v in (kv.split("=", 1) for kv in args.tune)})
+
     if args.workload == "adjoint":
         res = adjoint_bench(gd_steps=args.gd_steps)
         res.update({"metric": "adjoint gradient cost over one forward pass", "value": res["gradient_over_forward"],
@@ -384,8 +387,6 @@ def main():
     cap = 1 << args.log2_cap
     rules = iteration_rules()
     ctx = q.default_context()
-    if args.tune:
-        ctx.set_tuning(**{k: int(v) for k, v in (kv.split("=", 1) for kv in args.tune)})
     a, b = q.it_t(ctx=ctx), q.it_t(ctx=ctx)
     sy = q.sy_it_t(ctx=ctx)
     a.upload(HostState.from_objects([qcgd.seeded_graph(args.nodes, args.seed)]))
