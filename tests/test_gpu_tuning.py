@@ -152,7 +152,8 @@ def test_select_graph_replays_match_eager():
     of truncated QCGD steps, each replaying the graph between the step's own
     kernels and host reads, gives identical states, and so do the parent
     pre-truncation (a graph of more rounds) and truncated batched steps
-    (the per-instance rounds' graph)."""
+    (the per-instance rounds' graph), also after the context's scratch was
+    trimmed (a graph is captured again when its state moved)."""
     from quids_amd.batch import apply_modifier, batch_t, simulate_batched
     ctx = q.default_context()
     before = ctx.get_tuning()
@@ -171,6 +172,7 @@ def test_select_graph_replays_match_eager():
                         q.simulate(a, r, b, None, cap)
                         a, b = b, a
                         res.append(a.download())
+            ctx.trim()  # the scratch, the loops' device state included, is allocated again
             big = res[-3]  # pre-truncation: a parent state larger than the cap
             c, d = q.it_t(), q.it_t()
             c.upload(big)
