@@ -148,7 +148,7 @@ typedef struct qd_tuning {
     uint32_t dedup_bucket_target; /* mean candidates per final dedup bucket, 64..dedup_cap/2 (default 512) */
     uint32_t first_pass_bits;     /* max bits of the first radix pass, 1..12 (default 8) */
     uint32_t generate_block;      /* threads per generate block: 64, 128, 256, 512 or 1024 (default 256) */
-    uint32_t generate_grid;       /* max generate blocks (default 4096) */
+    uint32_t generate_grid;       /* max generate blocks (default 16384) */
     uint32_t dedup_cap;           /* LDS bucket capacity of the dedup: 256, 512, 1024 or 2048 (default 1024; larger buckets take the global-memory path) */
     uint32_t generic_path;        /* 1: byte-level generate kernels for every rule (default 0: QCGD fast path) */
     uint32_t chunk_log2;          /* at most 2^k candidates per pass, 10..40 (default 0: as many as

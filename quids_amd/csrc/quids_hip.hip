@@ -145,7 +145,7 @@ struct qd_ctx {
     bool pending[QD_NUM_PHASES] = {};
     double phase_ms[QD_NUM_PHASES] = {};
     // defaults: what quids_amd.tuning picked on the 12-node QCGD workload
-    qd_tuning tune = {512, 8, 256, 4096, 1024, 0, 0, 0, 0};
+    qd_tuning tune = {512, 8, 256, 16384, 1024, 0, 0, 0, 0};
     u64 phase_calls[QD_NUM_PHASES] = {};
     u64 *host = nullptr;  // pinned readback area, 8192 u64
     // scratch of the step

@@ -20,7 +20,7 @@ DEFAULT_GRID = {
     "dedup_cap": (512, 1024, 2048),
     "dedup_bucket_target": (256, 384, 512, 1024),
     "first_pass_bits": (8, 9),
-    "generate_grid": (2048, 4096),
+    "generate_grid": (4096, 16384),
 }
 
 _memo = {}
