@@ -28,6 +28,14 @@ struct __align__(16) Entry {
     double re, im;
 };
 
+// Raises *m to at least v.  Atomics on one address from every wavefront of
+// a large grid serialise (about 11 ns each, measured), so the atomic is
+// issued only when a plain read says it can raise the value; a stale read
+// costs one atomic more, never a wrong result.
+__device__ inline void raise_max(u32 *m, u32 v) {
+    if (v > __hip_atomic_load(m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(m, v);
+}
+
 __device__ inline u64 wave_incl_scan(u64 v) {
     const int lane = threadIdx.x & (kWave - 1);
 #pragma unroll

@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256) void k_prep_fast(Parents P, const u64 *keep, P
         }
     }
     for (int off = kWave / 2; off > 0; off >>= 1) local_max = max(local_max, (u32)__shfl_xor(int(local_max), off, kWave));
-    if (lane == 0 && local_max) atomicMax(maxsz, local_max);
+    if (lane == 0) raise_max(maxsz, local_max);
 }
 
 // QCGD step / reversed_step (qcgd.hpp:443-457) on a segment of SEG lanes,

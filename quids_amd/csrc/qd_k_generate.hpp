@@ -92,7 +92,8 @@ __global__ __launch_bounds__(256) void k_prep(Parents P, const u64 *keep, qd_rul
         nc[p] = num;
         local_max = max(local_max, ms);
     }
-    if (local_max) atomicMax(maxsz, local_max);
+    for (int off = kWave / 2; off > 0; off >>= 1) local_max = max(local_max, (u32)__shfl_xor(int(local_max), off, kWave));
+    if ((threadIdx.x & (kWave - 1)) == 0) raise_max(maxsz, local_max);
 }
 
 // num_child of every parent alone (the same checks as k_prep, no tables):
