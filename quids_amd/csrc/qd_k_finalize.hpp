@@ -98,21 +98,25 @@ __device__ inline u32 wave_excl_sum(u32 v, u32 &total) {
 
 // Copies the parent's words to the child with the flips of child_id applied
 // (erase_create / coin: both particle bytes of every flipped node; hadamard:
-// byte `bit` xor !child_id).
-template <int KIND>
-__device__ inline void populate_flip_wave(const u8 *o, u32 psz, u8 *d, u64 span, u32 j, const qd_rule &rule) {
+// byte `bit` xor !child_id), on a segment of SEG lanes (SEG = 64, or 32 when
+// two survivors share a wavefront; the segment's lanes are active together;
+// a QCGD parent has at most SEG nodes).
+template <int KIND, int SEG>
+__device__ inline void populate_flip_seg(const u8 *o, u32 psz, u8 *d, u64 span, u32 j, const qd_rule &rule) {
     const u32 lane = threadIdx.x & (kWave - 1);
-    u64 flipmask = 0;  // nodes whose particles flip (n <= 64)
+    const u32 sl = lane & (SEG - 1), sbase = lane & ~(SEG - 1);
+    const u64 smask = SEG == kWave ? ~0ull : (((1ull << SEG) - 1) << sbase);
+    u64 flipmask = 0;  // nodes whose particles flip (n <= SEG)
     u32 n = 0;
     if (KIND != QD_RULE_HADAMARD) {
         n = ld16(o);
-        const bool l = lane < n && o[2 + lane], r = lane < n && o[2 + n + lane];
-        const bool elig = lane < n && flip_eligible(KIND == QD_RULE_QCGD_COIN, l, r);
-        const u64 em = __ballot(elig);
-        const u32 rank = __popcll(em & ((1ull << lane) - 1));
-        flipmask = __ballot(elig && rank < 32 && ((j >> rank) & 1));
+        const bool l = sl < n && o[2 + sl], r = sl < n && o[2 + n + sl];
+        const bool elig = sl < n && flip_eligible(KIND == QD_RULE_QCGD_COIN, l, r);
+        const u64 em = (u64(__ballot(elig)) & smask) >> sbase;
+        const u32 rank = __popcll(em & ((1ull << sl) - 1));
+        flipmask = (u64(__ballot(elig && rank < 32 && ((j >> rank) & 1))) & smask) >> sbase;
     }
-    for (u64 w = lane; w < (span + 3) / 4; w += kWave) {
+    for (u64 w = sl; w < (span + 3) / 4; w += SEG) {
         const u32 at = u32(4 * w);
         u32 v = 0;
         if (at + 4 <= psz) v = *reinterpret_cast<const u32 *>(o + at);
@@ -139,8 +143,35 @@ __device__ inline void populate_flip_wave(const u8 *o, u32 psz, u8 *d, u64 span,
     }
 }
 
-// erase_create / coin / hadamard: one wavefront per survivor, four per
-// block, no LDS.
+// One survivor, with the lanes of its segment (SEG as above); objects that
+// are not 4-byte aligned or have more nodes than the segment are written by
+// the segment's first lane.
+template <int KIND, int SEG>
+__device__ inline void populate_flip_one(const Parents &P, const u64 *cb, const qd_rule &rule, const u64 *orep,
+                                         const u64 *opar, u64 k, const u64 *obegin, u8 *oobj) {
+    const u32 sl = threadIdx.x & (SEG - 1);
+    const u64 c = orep[k];
+    const u64 p = opar[k];
+    const u32 j = u32(c - cb[p]);
+    const u8 *o = P.obj + P.begin[p];
+    const u32 psz = P.size[p];
+    const u64 ob = obegin[k], oe = obegin[k + 1];
+    const u64 span = oe - ob;  // child bytes + zero padding
+    u8 *d = oobj + ob;
+    const bool aligned = ((P.begin[p] | ob) & 3) == 0;
+    const u32 n = KIND == QD_RULE_HADAMARD ? 0 : ld16(o);
+    if (aligned && n <= u32(SEG)) {
+        populate_flip_seg<KIND, SEG>(o, psz, d, span, j, rule);
+    } else if (sl == 0) {
+        if (KIND == QD_RULE_HADAMARD) hadamard_child_write(o, psz, rule.bit, j, d);
+        else flip_child_write(o, psz, KIND == QD_RULE_QCGD_COIN, j, d);
+        for (u64 q = psz; q < span; ++q) d[q] = 0;
+    }
+}
+
+// erase_create / coin / hadamard: two survivors per wavefront in 32-lane
+// segments when both parents have at most 32 nodes, else one after the
+// other on the whole wavefront; four wavefronts per block, no LDS.
 template <int KIND>
 __global__ __launch_bounds__(256) void k_populate_flip(Parents P, const u64 *cb, qd_rule rule, const u64 *orep,
                                                        const u64 *opar, u64 K, const u64 *obegin, const u32 *osize,
@@ -148,26 +179,17 @@ __global__ __launch_bounds__(256) void k_populate_flip(Parents P, const u64 *cb,
     const u32 lane = threadIdx.x & (kWave - 1);
     const u64 wave = (u64(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
     const u64 nwaves = u64(gridDim.x) * blockDim.x / kWave;
-    for (u64 k = wave; k < K; k += nwaves) {
-        const u64 c = orep[k];
-        const u64 p = opar[k];
-        const u32 j = u32(c - cb[p]);
-        const u8 *o = P.obj + P.begin[p];
-        const u32 psz = P.size[p];
-        const u64 ob = obegin[k], oe = obegin[k + 1];
-        const u64 span = oe - ob;  // child bytes + zero padding
-        u8 *d = oobj + ob;
-        const bool aligned = ((P.begin[p] | ob) & 3) == 0;
-        const u32 n = KIND == QD_RULE_HADAMARD ? 0 : ld16(o);
-        if (aligned && n <= 64) {
-            populate_flip_wave<KIND>(o, psz, d, span, j, rule);
-        } else if (lane == 0) {
-            if (KIND == QD_RULE_HADAMARD) hadamard_child_write(o, psz, rule.bit, j, d);
-            else flip_child_write(o, psz, KIND == QD_RULE_QCGD_COIN, j, d);
-            for (u64 q = psz; q < span; ++q) d[q] = 0;
+    for (u64 k0 = 2 * wave; k0 < K; k0 += 2 * nwaves) {
+        u32 nq = 0;  // node count of survivor k0 + lane's parent (lanes 0..1)
+        if (KIND != QD_RULE_HADAMARD && lane < 2 && k0 + lane < K) nq = ld16(P.obj + P.begin[opar[k0 + lane]]);
+        if (__ballot(nq > 32) == 0) {
+            const u64 k = k0 + lane / 32;
+            if (k < K) populate_flip_one<KIND, 32>(P, cb, rule, orep, opar, k, obegin, oobj);
+        } else {
+            for (u64 k = k0; k < min(k0 + 2, K); ++k) populate_flip_one<KIND, kWave>(P, cb, rule, orep, opar, k, obegin, oobj);
         }
-        (void)osize;
     }
+    (void)osize;
 }
 
 // split_merge: kPopSegs survivors per single-wave block, one segment of

@@ -733,7 +733,7 @@ static int populate_out(qd_ctx *ctx, const Parents &P, const qd_rule &rule, cons
                            ctx->cb.as<u64>(), rule, orep, (const u64 *)ctx->opar.as<u64>(), K, out->begin.as<u64>(),
                            out->size.as<u32>(), out->objects.as<u8>());
     else
-        hipLaunchKernelGGL(k_populate_flip<KIND>, dim3(unsigned(std::min<u64>((K + 3) / 4, 65536))), dim3(256), 0, s,
+        hipLaunchKernelGGL(k_populate_flip<KIND>, dim3(unsigned(std::min<u64>((K + 7) / 8, 65536))), dim3(256), 0, s,
                            P, ctx->cb.as<u64>(), rule, orep, (const u64 *)ctx->opar.as<u64>(), K,
                            out->begin.as<u64>(), out->size.as<u32>(), out->objects.as<u8>());
     QD_HIP(hipGetLastError());
