@@ -17,40 +17,50 @@ namespace qd {
 constexpr int kPoolPer = 16;  // slots per thread of k_pool_append
 
 // Appends the slots holding a unique (UR != kNoRep) to the pool: hash (the
-// slot key rotated back), representative, magnitude.  One reservation per
-// block tile of 256 * kPoolPer slots.
+// slot key rotated back), representative, magnitude, in slot order within a
+// block tile of 256 * kPoolPer slots, so that the uniques of one
+// wavefront's 64 slots land side by side (coalesced stores).  One
+// reservation per tile.
 __global__ __launch_bounds__(256) void k_pool_append(const u64 *UK, const u64 *UR, const double2 *UM, u64 nslots,
                                                      u64 *Ph, u64 *Pr, double2 *Pm, u64 *pcount) {
-    __shared__ u64 tmp[256 / kWave + 1];
+    constexpr int kWaves = 256 / kWave;
+    constexpr int kRuns = kPoolPer * kWaves;  // runs of 64 slots in a tile, in slot order
+    static_assert(kRuns == kWave, "one lane of the first wavefront per run");
+    __shared__ u64 roff[kRuns];  // count, then exclusive prefix, of each run's uniques
     __shared__ u64 base;
     constexpr u64 kTile = 256 * kPoolPer;
+    const u32 lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const u64 below = (1ull << lane) - 1;
     for (u64 t0 = u64(blockIdx.x) * kTile; t0 < nslots; t0 += u64(gridDim.x) * kTile) {
-        u32 cnt = 0;
+        u64 r[kPoolPer], m[kPoolPer];
 #pragma unroll
         for (int k = 0; k < kPoolPer; ++k) {
             const u64 i = t0 + threadIdx.x + 256 * k;
-            cnt += (i < nslots && UR[i] != kNoRep) ? 1u : 0u;
+            r[k] = i < nslots ? UR[i] : kNoRep;
+            m[k] = __ballot(r[k] != kNoRep);
+            if (lane == 0) roff[k * kWaves + wid] = __popcll(m[k]);
         }
-        u64 total;
-        u64 at = block_excl_scan(u64(cnt), tmp, total);
-        if (threadIdx.x == 0 && total)
-            base = atomicAdd(reinterpret_cast<unsigned long long *>(pcount), (unsigned long long)total);
         __syncthreads();
-        at += base;
+        if (wid == 0) {
+            const u64 c = roff[lane];
+            const u64 inc = wave_incl_scan(c);
+            roff[lane] = inc - c;
+            if (lane == kWave - 1)
+                base = inc ? atomicAdd(reinterpret_cast<unsigned long long *>(pcount), (unsigned long long)inc) : 0;
+        }
+        __syncthreads();
+        const u64 b = base;
 #pragma unroll
         for (int k = 0; k < kPoolPer; ++k) {
-            const u64 i = t0 + threadIdx.x + 256 * k;
-            if (i < nslots) {
-                const u64 r = UR[i];
-                if (r != kNoRep) {
-                    Ph[at] = rotr12(UK[i]);
-                    Pr[at] = r;
-                    Pm[at] = UM[i];
-                    ++at;
-                }
+            if (r[k] != kNoRep) {
+                const u64 i = t0 + threadIdx.x + 256 * k;
+                const u64 at = b + roff[k * kWaves + wid] + __popcll(m[k] & below);
+                Ph[at] = rotr12(UK[i]);
+                Pr[at] = r[k];
+                Pm[at] = UM[i];
             }
         }
-        __syncthreads();
+        __syncthreads();  // roff and base are rewritten by the next tile
     }
 }
 
