@@ -641,13 +641,18 @@ __global__ __launch_bounds__(kGenThreads, 3) void k_generate_scatter(const PDesc
 // Hash-range passes, for steps whose candidates exceed the scratch memory.
 // COUNT: histogram of the top 12 bits of every child hash into hist (4096
 // bins).  Otherwise: only the children whose top 12 hash bits lie in
-// [rlo, rhi) are kept, compacted (a block reserves one range per iteration)
+// [rlo, rhi) are kept, compacted (a block reserves one range per iteration,
+// filled in child-major order: child u of every thread, then child u + 1)
 // as key = hash rotated left by 12 (so the partition and the dedup, which
 // bucket on the top key bits, see well spread bits), magnitude and candidate
 // index, and histogrammed on the top b1 key bits for the partition.
 // ---------------------------------------------------------------------------
 __device__ inline u64 rotl12(u64 h) { return (h << 12) | (h >> 52); }
 __device__ inline u64 rotr12(u64 k) { return (k >> 12) | (k << 52); }
+
+constexpr int kRangeWaves = kGenThreads / kWave;
+constexpr int kRangeRuns = (1 << kItemBits) * kRangeWaves;  // (child, wavefront) runs of an iteration
+static_assert(kRangeRuns <= kWave, "one wavefront scans the run counts");
 
 template <int KIND, bool COUNT>
 __global__ __launch_bounds__(kGenThreads) void k_generate_range(const PDesc *D, const double2 *pmag, const u64 *cb,
@@ -658,7 +663,7 @@ __global__ __launch_bounds__(kGenThreads) void k_generate_range(const PDesc *D, 
     __shared__ u32 lh[kHistBins];
     __shared__ u64 swb[kGenThreads + 1];
     __shared__ u64 pfirst, wbase;
-    __shared__ u64 tmp[kGenThreads / kWave + 1];
+    __shared__ u64 roff[kRangeRuns];  // count, then exclusive prefix, of each run's kept children
     const u32 hb = COUNT ? 12 : b1;
     if (hb)
         for (int i = threadIdx.x; i < (1 << hb); i += blockDim.x) lh[i] = 0;
@@ -711,20 +716,33 @@ __global__ __launch_bounds__(kGenThreads) void k_generate_range(const PDesc *D, 
             });
         }
         if (!COUNT) {
-            u64 total;
-            const u64 off = block_excl_scan(u64(__popc(keep)), tmp, total);
-            if (threadIdx.x == 0 && total)
-                wbase = atomicAdd(reinterpret_cast<unsigned long long *>(rcount), (unsigned long long)total);
+            // kept children placed in (child u, wavefront, lane) order, so
+            // that the lanes of one store instruction write side by side
+            const u32 lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+            u64 mk[1 << kItemBits];
+#pragma unroll
+            for (u32 u = 0; u < (1u << kItemBits); ++u) {
+                mk[u] = __ballot((keep >> u) & 1u);
+                if (lane == 0) roff[u * kRangeWaves + wid] = __popcll(mk[u]);
+            }
             __syncthreads();
-            u64 at = wbase + off;
+            if (wid == 0) {
+                const u64 c = lane < kRangeRuns ? roff[lane] : 0;
+                const u64 inc = wave_incl_scan(c);
+                if (lane < kRangeRuns) roff[lane] = inc - c;
+                if (lane == kWave - 1)
+                    wbase = inc ? atomicAdd(reinterpret_cast<unsigned long long *>(rcount), (unsigned long long)inc) : 0;
+            }
+            __syncthreads();
+            const u64 below = (1ull << lane) - 1;
 #pragma unroll
             for (u32 u = 0; u < (1u << kItemBits); ++u) {
                 if (keep & (1u << u)) {
+                    const u64 at = wbase + roff[u * kRangeWaves + wid] + __popcll(mk[u] & below);
                     HK[at] = hs[u];
                     MK[at] = ms[u];
                     CK[at] = cbase + (u64(u) << s);
                     if (hb) atomicAdd(&lh[hs[u] >> (64 - hb)], 1u);
-                    ++at;
                 }
             }
         }
