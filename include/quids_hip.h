@@ -28,7 +28,7 @@ typedef enum qd_status {
     QD_ERR_INVALID = 1,     /* bad argument or malformed object */
     QD_ERR_OOM = 2,         /* device allocation failed */
     QD_ERR_HIP = 3,         /* a HIP runtime call failed */
-    QD_ERR_CAPACITY = 4,    /* a size limit of this build was exceeded */
+    QD_ERR_CAPACITY = 4,    /* a size limit of the library was exceeded */
     QD_ERR_UNSUPPORTED = 5  /* rule / option without a device implementation */
 } qd_status;
 
