@@ -610,7 +610,7 @@ __global__ __launch_bounds__(T) void k_dedup(const Entry *E, u64 rstride, const 
                 }
             }
         }
-        u32 cnt = 0, keepbits = 0;
+        u32 keepbits = 0;
         u32 nk[PER];
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
@@ -619,28 +619,37 @@ __global__ __launch_bounds__(T) void k_dedup(const Entry *E, u64 rstride, const 
                 const double nrm = rs[k].x * rs[k].x + rs[k].y * rs[k].y;
                 if (nrm > tol) {
                     keepbits |= 1u << k;
-                    ++cnt;
                     nk[k] = norm_top32(nrm);
                     atomicAdd(&nh[min(norm_top12(nrm), u32(kNormBins - 1))], 1u);
                 }
             }
         }
         // output places: one LDS counter, reserved once per wavefront (the
-        // order of groups within the bucket's range carries no meaning)
-        const u32 inc = u32(wave_incl_scan(cnt));
+        // order of groups within the bucket's range carries no meaning),
+        // filled entry slot by entry slot so that a store instruction's
+        // lanes write side by side
+        u64 km[PER];
+        u32 wcnt = 0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            km[k] = __ballot((keepbits >> k) & 1u);
+            wcnt += u32(__popcll(km[k]));
+        }
         u32 wbase = 0;
-        if (lane == kWave - 1 && inc) wbase = atomicAdd(&ocount, inc);
+        if (lane == kWave - 1 && wcnt) wbase = atomicAdd(&ocount, wcnt);
         wbase = __shfl(wbase, kWave - 1, kWave);
-        u64 pos = beg + wbase + inc - cnt;
+        const u64 below = (1ull << lane) - 1;
+        u32 run = 0;
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             if (keepbits & (1u << k)) {
+                const u64 pos = beg + wbase + run + u32(__popcll(km[k] & below));
                 UH[pos] = key[threadIdx.x + T * k];
                 UR[pos] = rc[k];
                 UM[pos] = rs[k];
                 if (UK) UK[pos] = nk[k];
-                ++pos;
             }
+            run += u32(__popcll(km[k]));
         }
 #pragma unroll
         for (int k = 0; k < PER; ++k)
