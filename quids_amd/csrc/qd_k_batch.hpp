@@ -73,10 +73,12 @@ __device__ inline u32 slot_inst16(const u16 *sinst, u64 s) {
 __global__ __launch_bounds__(256) void k_seg_count(const u64 *UR, u64 nslots, const u64 *cseg, u32 n, u64 *ucount,
                                                    u16 *sinst) {
     __shared__ u32 cnt[kMaxInst];
+    __shared__ u64 sseg[kMaxInst + 1];  // the instances' candidate offsets, searched per slot
     for (u32 i = threadIdx.x; i < n; i += blockDim.x) cnt[i] = 0;
+    for (u32 i = threadIdx.x; i <= n; i += blockDim.x) sseg[i] = cseg[i];
     __syncthreads();
     for (u64 s = u64(blockIdx.x) * blockDim.x + threadIdx.x; s < nslots; s += u64(gridDim.x) * blockDim.x) {
-        const u32 inst = slot_inst(UR, s, cseg, n);
+        const u32 inst = slot_inst(UR, s, sseg, n);
         sinst[s] = inst == kNoInst ? kNoSlotInst : u16(inst);
         if (inst != kNoInst) atomicAdd(&cnt[inst], 1u);
     }
