@@ -825,6 +825,10 @@ or_state *or_state_new(uint64_t n, const void *objects, const uint64_t *begin, c
 
 void or_state_free(or_state *s) { delete s; }
 
+// OpenMP threads of the calling thread's later calls (per host thread, so
+// that several host threads can each run small checks single-threaded).
+void or_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
 void or_state_info(const or_state *s, uint64_t *n, uint64_t *bytes, double *total) {
     *n = s->s.n();
     *bytes = s->s.begin.back();

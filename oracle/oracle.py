@@ -58,6 +58,7 @@ def _load():
         L.or_simulate_jvp.argtypes = [vp, ctypes.POINTER(QdRule), ctypes.POINTER(QdRule), ctypes.POINTER(vp), u64,
                                       ctypes.POINTER(QdParams), ctypes.POINTER(QdStepStats)]
         L.or_average_value_jvp.argtypes = [vp, ctypes.c_int, ctypes.c_int, pd]
+        L.or_set_threads.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
 
@@ -103,6 +104,11 @@ def simulate(hs, rule, max_num_object=(1 << 64) - 1, tolerance=1e-30, align=8, s
     res = _from_or(out)
     L.or_state_free(out)
     return res, st.as_dict()
+
+
+def set_threads(n):
+    """OpenMP threads of the calling host thread's later oracle calls."""
+    _load().or_set_threads(int(n))
 
 
 def apply_modifier(hs, mod):

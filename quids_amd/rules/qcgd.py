@@ -8,6 +8,7 @@ import ctypes
 import ctypes.util
 import math
 import struct
+import threading
 
 import numpy as np
 
@@ -146,6 +147,9 @@ def with_particles(obj, left_bits, right_bits):
 _libc = None
 
 
+_rand_lock = threading.Lock()
+
+
 def _c_rand():
     global _libc
     if _libc is None:
@@ -160,11 +164,12 @@ def seeded_graph(n, seed):
     srand(seed), then graphs::randomize draws rand() & 1 for left[0],
     right[0], left[1], ... (qcgd.hpp:114-120) with the C library's rand()."""
     libc = _c_rand()
-    libc.srand(int(seed))
     L, R = [], []
-    for _ in range(n):
-        L.append(libc.rand() & 1)
-        R.append(libc.rand() & 1)
+    with _rand_lock:  # srand / rand share one process-wide state
+        libc.srand(int(seed))
+        for _ in range(n):
+            L.append(libc.rand() & 1)
+            R.append(libc.rand() & 1)
     return with_particles(make_graph(n), L, R)
 
 
