@@ -1,14 +1,18 @@
 #!/usr/bin/env python3
 """bench.py -- throughput of the QuIDS rule-application step on one MI355X.
 
-Workload: plain QCGD, 12-node cycle graph with seeded particles (srand(seed),
-rand() & 1 in left[0], right[0], left[1], ... order), one iteration =
-step -> split_merge(theta=pi/4, phi=0) -> erase_create(theta=pi/3, phi=pi/5),
-max_num_object = 2^22, FP64 magnitudes, tolerance 1e-30.  The first
-`--warmup` iterations grow the state untimed; the next `--steps` iterations
-are timed.  A bench step is one iteration (two simulate() calls and the
-step modifier); the value is candidates (sy_it.num_object summed over the
-timed simulate() calls) per second of timed wall time.
+Workload (default): plain QCGD, 16-node cycle graph with seeded particles
+(srand(42), rand() & 1 in left[0], right[0], left[1], ... order), one
+iteration = step -> split_merge(theta=pi/4, phi=0) -> erase_create(theta=pi/3,
+phi=pi/5), max_num_object = 2^26, FP64 magnitudes, tolerance 1e-30.
+
+A bench step is one simulate() rule application (the step modifier that
+precedes a split_merge is timed with it): steps 1..4 are iterations 1-2,
+which grow the state from one graph to 2^26 objects; step 5 is iteration
+3's split_merge.  The first `--warmup` steps run untimed, the next `--steps`
+are timed; the value is candidates (sy_it.num_object summed over the timed
+simulate() calls) per second of timed wall time, with the state resident in
+HBM.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -47,40 +51,103 @@ def alg_bytes(st):
     return {"generate": gen, "dedup": dedup, "finalize": fin, "total": gen + dedup + fin}
 
 
-def cpu_baseline(nodes, seed, cap, budget_s, iterations=2):
-    """The compiled reference (oracle/_ref/ref_driver) on the same workload's
-    first `iterations` iterations (the second one is the first large step:
-    ~1.7e7 candidates truncated to 2^22), on this host's cores."""
+def bench_ops():
+    """(modifier or None, rule) of the two bench steps of an iteration."""
+    rules = iteration_rules()
+    return [(rules[0], rules[1]), (None, rules[2])]
+
+
+def step_name(k):
+    """Bench step k (1-based) as 'iteration i split_merge / erase_create'."""
+    return f"iteration {(k + 1) // 2} {'split_merge' if k % 2 else 'erase_create'}"
+
+
+def host_threads():
+    """Host threads the CPU baseline uses: the box's CPU share (OMP_NUM_THREADS
+    when the environment sets it), else the CPUs this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(nodes, seed, step, log2_cap, budget_s):
+    """The compiled reference (oracle/_ref/ref_driver) on the same workload up
+    to bench step `step` (the first timed one), on this host's cores; the
+    value is that step's candidates over its wall time.  The run uses cap
+    2^log2_cap: at the GPU's cap the reference needs about 60 bytes of host
+    memory per candidate (sy_it_t, src/quids.hpp:44), some 10^12 bytes."""
     from oracle import oracle as orc
-    from quids_amd.rules import qcgd
     from quids_amd.state import HostState
+    from quids_amd.rules import qcgd
     if not os.path.exists(orc.REF_DRIVER):
         return None
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_threads()
     hs = HostState.from_objects([qcgd.seeded_graph(nodes, seed)])
-    rules = iteration_rules()
-    ops = [orc.mod_op(rules[0]), orc.rule_op(rules[1], cap, dump=False), orc.rule_op(rules[2], cap, dump=False)]
-    ops = ops * iterations
+    cap = 1 << log2_cap
+    ops, op_of_step = [], {}
+    for k in range(1, step + 1):
+        mod, rule = bench_ops()[(k - 1) % 2]
+        if mod is not None:
+            ops.append(orc.mod_op(mod))
+        op_of_step[len(ops)] = k
+        ops.append(orc.rule_op(rule, cap, dump=False))
     try:
         _, out = orc.run_reference(1, hs, ops, threads=threads, timeout=budget_s)
     except Exception as e:  # noqa: BLE001
         log(f"cpu baseline failed: {e}")
         return None
-    C = secs = 0.0
     for line in out.splitlines():
-        f = dict(kv.split("=") for kv in line.split()[2:])
-        C += float(f["C"])
-        secs += float(f["seconds"])
-    return {"value": C / secs if secs else None, "unit": "children/s", "cores": threads, "kind": "reference",
-            "sample": f"reference simulate() on iterations 1..{iterations} of the same workload "
-                      f"({nodes} nodes, cap 2^{cap.bit_length() - 1}), "
-                      f"{int(C)} candidates in {secs:.2f} s, OMP_NUM_THREADS={threads}"}
+        w = line.split()
+        f = dict(kv.split("=") for kv in w[2:])
+        if op_of_step.get(int(w[1])) == step:
+            C, secs = float(f["C"]), float(f["seconds"])
+            return {"value": C / secs if secs else None, "unit": "children/s", "cores": threads, "kind": "reference",
+                    "sample": f"reference simulate() on bench step {step} ({step_name(step)}) of the same workload "
+                              f"at cap 2^{log2_cap} (steps 1..{step} run at that cap), {int(C)} candidates in "
+                              f"{secs:.2f} s, OMP_NUM_THREADS={threads} (os.cpu_count() = {os.cpu_count()})"}
+    return None
 
 
-# The generate phase's kernel: k_generate_scatter (generate fused with the
-# first radix pass, the default) or k_generate_fast (split_generate=1).
-KERNEL_OF_PHASE = {"generate": "k_generate_scatter", "dedup": "k_dedup<", "finalize": "k_populate"}
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_qcgd12.json")
+# Kernels of the three main phases (name prefixes in rocprofv3 traces): the
+# generate phase is k_generate_scatter (generate fused with the first radix
+# pass), k_generate_fast (split_generate=1) or, for steps beyond the pass
+# budget, k_generate_range (the counting pass and one launch per hash range).
+KERNELS_OF_PHASE = {"generate": ("k_generate_scatter", "k_generate_fast", "k_generate_range"),
+                    "dedup": ("k_dedup<", "k_dedup_sort", "k_dedup_big"),
+                    "finalize": ("k_populate",)}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_qcgd16.json")
+MARKER_BYTES = 4096  # the plain copy bench.py runs just before its timed region (a marker in traces)
+
+
+def dominant_kernel_name(phase, ctx, stats):
+    if phase == "generate":
+        names = []
+        if any(not (s["reserved"] & 4) for s in stats):
+            names.append("k_generate_fast" if ctx.get_tuning()["split_generate"] else "k_generate_scatter")
+        if any(s["reserved"] & 4 for s in stats):
+            names.append("k_generate_range")
+        return "+".join(names)
+    return {"dedup": "k_dedup", "finalize": "k_populate"}[phase]
+
+
+def traffic_per_launch(phase, kname, candidates, launches, key):
+    """HBM bytes per launch of the phase's kernels from the rocprofv3 counter
+    runs of this workload (profiles/traffic_qcgd16.json: FETCH_SIZE +
+    WRITE_SIZE over the timed dispatches, scaled by the plain-copy
+    calibration, per candidate of the timed steps), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            tr = json.load(f)
+    except (OSError, ValueError):
+        return None
+    kt = tr.get("kernels", {}).get(phase, {})
+    if tr.get("workload") != key or kt.get("kernel") != kname:
+        return None
+    return kt["hbm_bytes_per_candidate"] * candidates / launches
 
 
 def _csv(path_glob):
@@ -93,60 +160,85 @@ def _csv(path_glob):
         return list(csv.DictReader(f))
 
 
-def _counter_per_dispatch(rows, kernel, counter):
-    """{dispatch id: counter value (summed over the rows of the dispatch)}"""
+def _dispatches(rows, prefixes, counter=None):
+    """{dispatch id: (kernel name, value)}: the counter summed over the rows of
+    each dispatch of a kernel whose name starts with one of the prefixes
+    (after 'void qd::'), or the trace row's duration in ns."""
     out = {}
     for r in rows:
-        if kernel in r["Kernel_Name"] and r["Counter_Name"] == counter:
-            d = int(r["Dispatch_Id"])
-            out[d] = out.get(d, 0.0) + float(r["Counter_Value"])
-    return [out[d] for d in sorted(out)]
+        name = r["Kernel_Name"].replace("void ", "").replace("qd::", "")
+        if not any(name.startswith(p) for p in prefixes):
+            continue
+        d = int(r["Dispatch_Id"])
+        if counter is None:
+            out[d] = (name, float(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])))
+        elif r["Counter_Name"] == counter:
+            out[d] = (name, out.get(d, (name, 0.0))[1] + float(r["Counter_Value"]))
+    return out
+
+
+def _after_marker(rows, values):
+    """The dispatches after the last k_copy_probe (bench.py's marker)."""
+    marks = [int(r["Dispatch_Id"]) for r in rows if "k_copy_probe" in r["Kernel_Name"]]
+    m = max(marks) if marks else -1
+    return {d: v for d, v in values.items() if d > m}
 
 
 def rocprof_summary(args):
-    """Formats the rocprofv3 output of this script's own runs: kernel-trace
-    stats, the timed launches of the dominant kernel (the last 2*steps
-    launches: one per simulate() call), and FETCH_SIZE / WRITE_SIZE per
-    launch scaled by what the same counters read for the library's plain
-    copy (qd_copy_probe) of a known size.  Prints markdown and writes the
-    traffic figures to --traffic-out."""
+    """Formats the rocprofv3 output of this script's own runs of one command:
+    kernel-trace stats, then per phase kernel family its timed dispatches
+    (those after the marker copy that precedes the timed region) with their
+    average duration, and FETCH_SIZE / WRITE_SIZE per dispatch scaled by
+    what the same counters read for the library's plain copy
+    (qd_copy_probe) of a known size.  The traffic is also written per
+    candidate of the timed steps (--bench-json: the JSON line of the run) to
+    --traffic-out, where bench.py finds it.  Prints markdown."""
     kt, fe, wr, cf, cw = args.rocprof_summary
     stats = _csv(os.path.join(kt, "**", "*kernel_stats.csv"))
     trace = _csv(os.path.join(kt, "**", "*kernel_trace.csv"))
-    nt = 2 * args.steps
+    with open(args.bench_json) as f:
+        line = json.loads([x for x in f.read().splitlines() if x.startswith("{")][-1])
+    C = line["config"]["candidates"]
     lines = ["| kernel | calls | total ms | avg us | % |", "|---|---:|---:|---:|---:|"]
-    for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:16]:
+    for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:20]:
         name = r["Name"].split("(")[0].replace("void ", "")
         lines.append(f"| `{name}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.2f} | "
                      f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
-    calib_bytes = None
     with open(args.calib_json) as f:
-        calib_bytes = json.load(f)["copy_probe_bytes"]
-    cfetch = _counter_per_dispatch(_csv(os.path.join(cf, "**", "*counter_collection.csv")), "k_copy_probe", "FETCH_SIZE")
-    cwrite = _counter_per_dispatch(_csv(os.path.join(cw, "**", "*counter_collection.csv")), "k_copy_probe", "WRITE_SIZE")
+        calib_bytes = json.loads([x for x in f.read().splitlines() if x.startswith("{")][-1])["copy_probe_bytes"]
+    crow_f = _csv(os.path.join(cf, "**", "*counter_collection.csv"))
+    crow_w = _csv(os.path.join(cw, "**", "*counter_collection.csv"))
+    cfetch = sorted(_dispatches(crow_f, ("k_copy_probe",), "FETCH_SIZE").items())
+    cwrite = sorted(_dispatches(crow_w, ("k_copy_probe",), "WRITE_SIZE").items())
     # the copy reads and writes calib_bytes each; counters are in KiB
-    f_scale = calib_bytes / (1024.0 * cfetch[-1])
-    w_scale = calib_bytes / (1024.0 * cwrite[-1])
-    fetch = _csv(os.path.join(fe, "**", "*counter_collection.csv"))
-    write = _csv(os.path.join(wr, "**", "*counter_collection.csv"))
-    traffic = {"workload": args.workload_key, "timed_launches": nt, "fetch_scale": f_scale, "write_scale": w_scale,
+    f_scale = calib_bytes / (1024.0 * cfetch[-1][1][1])
+    w_scale = calib_bytes / (1024.0 * cwrite[-1][1][1])
+    frows = _csv(os.path.join(fe, "**", "*counter_collection.csv"))
+    wrows = _csv(os.path.join(wr, "**", "*counter_collection.csv"))
+    traffic = {"workload": args.workload_key, "candidates": C, "fetch_scale": f_scale, "write_scale": w_scale,
                "kernels": {}}
-    lines += ["", "Timed launches (the last 2 x steps of each kernel, one per simulate() call):", "",
-              "| kernel | launches | avg us (trace) | FETCH_SIZE KiB/launch | WRITE_SIZE KiB/launch | "
-              "HBM bytes/launch (scaled) |", "|---|---:|---:|---:|---:|---:|"]
-    for phase, kname in KERNEL_OF_PHASE.items():
-        durs = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) for r in trace if kname in r["Kernel_Name"]]
-        fv = _counter_per_dispatch(fetch, kname, "FETCH_SIZE")
-        wv = _counter_per_dispatch(write, kname, "WRITE_SIZE")
-        if len(durs) < nt or len(fv) < nt or len(wv) < nt:
+    lines += ["", f"Timed dispatches (after the marker copy; {C} candidates in the timed steps):", "",
+              "| phase | kernels | dispatches | avg us (trace) | FETCH_SIZE KiB/dispatch | WRITE_SIZE KiB/dispatch | "
+              "HBM bytes/dispatch (scaled) | HBM bytes/candidate |", "|---|---|---:|---:|---:|---:|---:|---:|"]
+    for phase, prefixes in KERNELS_OF_PHASE.items():
+        durs = _after_marker(trace, _dispatches(trace, prefixes))
+        fv = _after_marker(frows, _dispatches(frows, prefixes, "FETCH_SIZE"))
+        wv = _after_marker(wrows, _dispatches(wrows, prefixes, "WRITE_SIZE"))
+        if not durs or not fv or not wv:
             continue
-        durs, fv, wv = durs[-nt:], fv[-nt:], wv[-nt:]
-        avg_us = sum(durs) / nt / 1e3
-        fk, wk = sum(fv) / nt, sum(wv) / nt
+        fams = sorted({n.split("<")[0] for n, _ in durs.values()})
+        n = len(durs)
+        avg_us = sum(v for _, v in durs.values()) / n / 1e3
+        fk = sum(v for _, v in fv.values()) / len(fv)
+        wk = sum(v for _, v in wv.values()) / len(wv)
         hbm = 1024.0 * (fk * f_scale + wk * w_scale)
-        traffic["kernels"][phase] = {"kernel": kname.rstrip("<"), "avg_us": avg_us, "fetch_kib": fk,
-                                     "write_kib": wk, "hbm_bytes_per_launch": hbm}
-        lines.append(f"| `{kname.rstrip('<')}` | {nt} | {avg_us:.1f} | {fk:.0f} | {wk:.0f} | {hbm:.4g} |")
+        per_cand = hbm * len(fv) / C
+        kname = "+".join(f for f in fams if f != "k_dedup_sort" and f != "k_dedup_big")
+        traffic["kernels"][phase] = {"kernel": kname, "dispatches": n, "avg_us": avg_us, "fetch_kib": fk,
+                                     "write_kib": wk, "hbm_bytes_per_dispatch": hbm,
+                                     "hbm_bytes_per_candidate": per_cand}
+        lines.append(f"| {phase} | `{kname}` | {n} | {avg_us:.1f} | {fk:.0f} | {wk:.0f} | {hbm:.4g} | "
+                     f"{per_cand:.1f} |")
     lines += ["", f"Counter scale (copy of {calib_bytes} B read + written): FETCH_SIZE x{f_scale:.3f}, "
                   f"WRITE_SIZE x{w_scale:.3f}"]
     with open(args.traffic_out, "w") as f:
@@ -312,34 +404,38 @@ def batched_bench(n_inst=256, n_iter=8, log2_cap=18, seed=4242, lo_nodes=6, hi_n
 
 
 def workload_key(args):
-    return f"qcgd nodes={args.nodes} seed={args.seed} cap=2^{args.log2_cap} warmup={args.warmup} steps={args.steps}"
+    return f"qcgd nodes={args.nodes} seed={args.seed} cap=2^{args.log2_cap}"
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4)
-    ap.add_argument("--warmup", type=int, default=6)
-    ap.add_argument("--nodes", type=int, default=12)
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nodes", type=int, default=16)
     ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--log2-cap", type=int, default=22)
+    ap.add_argument("--log2-cap", type=int, default=26)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=120.0)
+    ap.add_argument("--cpu-log2-cap", type=int, default=20,
+                    help="cap of the CPU baseline's run (the reference needs about 60 bytes of host memory per "
+                         "candidate, so the GPU's cap does not fit a host)")
     ap.add_argument("--copy-probe", type=float, default=0,
                     help="only run the library's plain copy of this many GiB (counter calibration) and exit")
     ap.add_argument("--rocprof-summary", nargs=5, metavar=("KT", "FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE"),
                     help="summarize rocprofv3 output folders of this script's runs and exit")
     ap.add_argument("--calib-json", help="the --copy-probe JSON line of the calibration run")
+    ap.add_argument("--bench-json", help="the JSON line of the profiled bench run (--rocprof-summary)")
     ap.add_argument("--traffic-out", default=TRAFFIC_FILE)
     ap.add_argument("--workload", choices=["plain", "adjoint", "batched"], default="plain",
                     help="adjoint: the QCGD adjoint workload (with --gd-steps of gradient descent); "
                          "batched: the 256-instance batch against sequential plain calls")
-    ap.add_argument("--batch-log2-cap", type=int, default=14)
-    ap.add_argument("--batch-iters", type=int, default=4)
+    ap.add_argument("--batch-log2-cap", type=int, default=12)
+    ap.add_argument("--batch-iters", type=int, default=5)
     ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--no-sequential", action="store_true", help="batched workload without the sequential run")
     ap.add_argument("--gd-steps", type=int, default=50)
-    ap.add_argument("--no-extras", action="store_true", help="skip the adjoint summary of the plain line")
+    ap.add_argument("--no-extras", action="store_true", help="skip the adjoint and batched summaries of the plain line")
     ap.add_argument("--autotune", action="store_true",
                     help="before the timed iterations, let quids_amd.tuning pick the step's launch and bucket "
                          "parameters on the warmed-up state")
@@ -385,85 +481,92 @@ def main():
         return
 
     cap = 1 << args.log2_cap
-    rules = iteration_rules()
     ctx = q.default_context()
     a, b = q.it_t(ctx=ctx), q.it_t(ctx=ctx)
     sy = q.sy_it_t(ctx=ctx)
     a.upload(HostState.from_objects([qcgd.seeded_graph(args.nodes, args.seed)]))
+    ops = bench_ops()
 
-    def run_iteration(timed_stats=None):
+    def run_step(k, timed_stats=None):
+        """Bench step k (1-based): the modifier of its iteration if it opens
+        one, then one simulate() call."""
         nonlocal a, b
-        q.simulate(a, rules[0])
-        for r in rules[1:]:
-            in_bytes = a.info()[1] if timed_stats is not None else 0
-            st = q.simulate(a, r, b, sy, cap)
-            if timed_stats is not None:
-                st["in_bytes"] = in_bytes
-                timed_stats.append(st)
-            a, b = b, a
+        mod, rule = ops[(k - 1) % 2]
+        if mod is not None:
+            q.simulate(a, mod)
+        in_bytes = a.info()[1]
+        t = time.perf_counter()
+        st = q.simulate(a, rule, b, sy, cap)
+        st["in_bytes"] = in_bytes
+        st["seconds"] = time.perf_counter() - t  # host wall time of the call (it synchronises)
+        a, b = b, a
+        if timed_stats is not None:
+            timed_stats.append(st)
+        return st
 
-    for i in range(args.warmup):
-        t = time.time()
-        run_iteration()
-        ctx.synchronize()
-        log(f"warmup iteration {i + 1}: {a.num_object} objects, last C={sy.num_object} ({time.time() - t:.2f} s)")
+    for k in range(1, args.warmup + 1):
+        st = run_step(k)
+        log(f"warmup step {k} ({step_name(k)}): C={st['num_candidates']} U={st['num_unique']} "
+            f"objects={a.num_object} ({st['seconds']:.2f} s)")
 
     tuned = None
     if args.autotune:
         from quids_amd import tuning
-        q.simulate(a, rules[0])
-        tuned = tuning.autotune(ctx, a, rules[1:], cap, reps=1, verbose=False)
-        q.simulate(a, qcgd.reversed_step)  # back to the warmed-up state
+        tuned = tuning.autotune(ctx, a, [ops[args.warmup % 2][1]], cap, reps=1, verbose=False)
         log(f"autotune: {tuned}")
+    ctx.copy_probe(MARKER_BYTES, reps=1)  # marks the start of the timed region in kernel traces
     ctx.set_profiling(True)
     ctx.reset_phase_times()
     stats = []
     ctx.synchronize()
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        run_iteration(stats)
-        log(f"timed iteration {args.warmup + i + 1}: {a.num_object} objects, C={stats[-1]['num_candidates']}")
+    for k in range(args.warmup + 1, args.warmup + args.steps + 1):
+        st = run_step(k, stats)
+        log(f"timed step {k} ({step_name(k)}): C={st['num_candidates']} U={st['num_unique']} "
+            f"objects={a.num_object} passes={'range' if st['reserved'] & 4 else 'single'} ({st['seconds']:.2f} s)")
     ctx.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
     phases = ctx.phase_times()
     ctx.set_profiling(False)
+    a.close()
+    b.close()
 
     C = sum(s["num_candidates"] for s in stats)
     value = C / elapsed
     ab = {k: sum(alg_bytes(s)[k] for s in stats) for k in ("generate", "dedup", "finalize", "total")}
-    kernel_ms = {k: phases[k][0] for k in ("generate", "dedup", "finalize")}
+    phase_ms = {k: v[0] for k, v in phases.items()}
+    phase_calls = {k: v[1] for k, v in phases.items()}
+    kernel_ms = {k: phase_ms[k] for k in ("generate", "dedup")}  # one kernel per phase interval
     dom = max(kernel_ms, key=kernel_ms.get)
-    launches = len(stats)  # each timed simulate() launches the phase's main kernel once
+    # per launch: the phase's algorithmic bytes of the timed steps over the
+    # launches of its kernels in them (one per simulate() call, or one per
+    # hash-range pass plus the counting pass)
+    launches = max(int(phase_calls[dom]), 1)
     dom_ms = kernel_ms[dom] / launches
     per_launch = ab[dom] / launches
     achieved = per_launch / (dom_ms / 1e3) if dom_ms > 0 else None
-    kname = KERNEL_OF_PHASE[dom].rstrip("<")
-    if dom == "generate" and ctx.get_tuning()["split_generate"]:
-        kname = "k_generate_fast"
-    traffic = None
-    try:
-        with open(TRAFFIC_FILE) as f:
-            tr = json.load(f)
-        kt = tr.get("kernels", {}).get(dom, {})
-        if tr.get("workload") == workload_key(args) and kt.get("kernel") == kname:
-            traffic = kt["hbm_bytes_per_launch"]
-    except (OSError, ValueError):
-        pass
+    kname = dominant_kernel_name(dom, ctx, stats)
+    traffic = traffic_per_launch(dom, kname, C, launches, workload_key(args))
     roofline = {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9 if achieved else None,
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK if achieved else None,
                 "traffic": traffic, "alg_bytes_per_launch": per_launch, "ms_per_launch": dom_ms,
                 "launches": launches}
     step_frac = ab["total"] / elapsed / HBM_PEAK
 
-    cpu = None if args.no_cpu_baseline else cpu_baseline(args.nodes, args.seed, cap, args.cpu_budget)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.nodes, args.seed, args.warmup + 1, args.cpu_log2_cap, args.cpu_budget)
     extras = {}
     if not args.no_extras:
-        try:
-            extras["adjoint"] = adjoint_bench()
-        except Exception as e:  # noqa: BLE001
-            extras["adjoint"] = {"error": str(e)}
+        for key, fn in (("adjoint", lambda: adjoint_bench(gd_steps=args.gd_steps)),
+                        ("batched", lambda: batched_bench(n_iter=args.batch_iters, log2_cap=args.batch_log2_cap))):
+            try:
+                extras[key] = fn()
+            except Exception as e:  # noqa: BLE001
+                extras[key] = {"error": str(e)}
 
+    first, last = args.warmup + 1, args.warmup + args.steps
     out = {
         "metric": METRIC,
         "value": value,
@@ -479,11 +582,12 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"plain QCGD, {args.nodes}-node cycle graph, seed {args.seed}, "
                                f"step+split_merge(pi/4,0)+erase_create(pi/3,pi/5), max_num_object=2^{args.log2_cap}, "
-                               f"iterations {args.warmup + 1}..{args.warmup + args.steps} timed",
-                   "candidates": C, "parallelism": "single GPU"},
+                               f"simulate() calls {first}..{last} timed ({step_name(first)} .. {step_name(last)})",
+                   "candidates": C, "parallelism": "single GPU",
+                   "range_pass_steps": sum(bool(s["reserved"] & 4) for s in stats)},
         "roofline": roofline,
         "step_hbm_frac": step_frac,
-        "phase_ms": {k: round(v[0], 3) for k, v in phases.items()},
+        "phase_ms": {k: round(v, 3) for k, v in phase_ms.items()},
         "cpu_baseline": cpu,
         "tuning": ctx.get_tuning() if tuned is None else dict(tuned["tuning"], autotuned=True),
         **extras,

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define QD_ABI_VERSION 1
+#define QD_ABI_VERSION 2
 
 typedef enum qd_status {
     QD_OK = 0,
@@ -84,6 +84,9 @@ typedef struct qd_params {
                                    seed) -- reproducible, unlike the reference's rand() streams */
     uint32_t reserved;
     uint64_t seed;              /* key of the probabilistic truncation's random draws */
+    /* memory-driven truncation (max_num_object == 0), src/quids.hpp:64-66, 459-536 */
+    double safety_margin;       /* quids::safety_margin, default 0.2 (target 80 % of the memory) */
+    double equalize_factor;     /* quids::equalize_factor, default 0.25 */
 } qd_params;
 
 /* Counters of one simulate() call (reference: sy_it.num_object and
@@ -99,7 +102,14 @@ typedef struct qd_step_stats {
     double total_proba;         /* next.total_proba (before normalisation) */
     uint32_t dedup_levels;      /* radix partition passes used by the dedup */
     uint32_t reserved;          /* bit 0: the second pass fell back to exact offsets;
-                                   bit 1: a batched step ran in several chunks */
+                                   bit 1: a batched step ran in several chunks;
+                                   bit 2: the step ran in hash-range passes;
+                                   bit 3: the fused first radix pass spilled and was redone */
+    /* memory-driven truncation only (max_num_object == 0; 0 otherwise): the
+     * target_memory of the parent side and of the survivor side
+     * (src/quids.hpp:469, 520) that the binary searches kept below */
+    uint64_t mem_target_parents;
+    uint64_t mem_target_survivors;
 } qd_step_stats;
 
 typedef struct qd_ctx qd_ctx;
@@ -158,6 +168,8 @@ typedef struct qd_tuning {
                                      (default 0: fused when the first pass has at most 2^8 buckets) */
     uint32_t eager_select;        /* 1: launch the top-K select's digit rounds one by one (default 0:
                                      replay them from a hipGraph the context captures once) */
+    uint32_t free_mem_mib;        /* memory-driven truncation sees at most this much free device
+                                     memory, in MiB (default 0: what hipMemGetInfo reports) */
 } qd_tuning;
 int qd_ctx_get_tuning(qd_ctx *ctx, qd_tuning *tuning);
 int qd_ctx_set_tuning(qd_ctx *ctx, const qd_tuning *tuning);

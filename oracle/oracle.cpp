@@ -598,9 +598,14 @@ static inline u32 align_pad(u32 size, u32 align) {
 // out_tan receives the output's: the forward-mode derivative of the step
 // along D real parameters, dcoef[t] holding the derivatives of the rule's
 // coefficients along parameter t.
+// max_survivors caps the uniques as max_num_object caps the parents; the
+// reference uses one value for both unless its memory-driven searches chose
+// them (src/quids.hpp:472-485, 523-535).
 static int simulate(const State &in, const qd_rule &rule, State &out, u64 max_num_object,
                     const qd_params &prm, qd_step_stats &st, const qd_rule *dcoef = nullptr, int D = 0,
-                    const std::vector<cplx> *in_tan = nullptr, std::vector<cplx> *out_tan = nullptr) {
+                    const std::vector<cplx> *in_tan = nullptr, std::vector<cplx> *out_tan = nullptr,
+                    u64 max_survivors = ~u64(0)) {
+    if (max_survivors == ~u64(0)) max_survivors = max_num_object;
     const size_t N = in.n();
     std::memset(&st, 0, sizeof st);
     st.num_parent = N;
@@ -732,13 +737,13 @@ static int simulate(const State &in, const qd_rule &rule, State &out, u64 max_nu
     st.num_unique = uniq.size();
 
     // truncation (src/quids.hpp:866-900): top max by |m|^2 desc, hash asc
-    if (max_num_object < uniq.size()) {
-        std::nth_element(uniq.begin(), uniq.begin() + max_num_object, uniq.end(),
+    if (max_survivors < uniq.size()) {
+        std::nth_element(uniq.begin(), uniq.begin() + max_survivors, uniq.end(),
                          [&](const Uniq &a, const Uniq &b) {
                              const double na = trunc_key(a.mag, a.hash, prm), nb = trunc_key(b.mag, b.hash, prm);
                              return na != nb ? na > nb : a.hash < b.hash;
                          });
-        uniq.resize(max_num_object);
+        uniq.resize(max_survivors);
     }
     // finalize (src/quids.hpp:905-968): ascending representative order
     std::sort(uniq.begin(), uniq.end(), [](const Uniq &a, const Uniq &b) { return a.rep < b.rep; });
@@ -851,6 +856,22 @@ int or_simulate(const or_state *in, const qd_rule *rule, or_state **out, uint64_
                 const qd_params *prm, qd_step_stats *st) {
     auto *o = new or_state;
     int rc = simulate(in->s, *rule, o->s, max_num_object, *prm, *st);
+    if (rc) {
+        delete o;
+        *out = nullptr;
+        return rc;
+    }
+    *out = o;
+    return QD_OK;
+}
+
+// The step with separate truncation counts for the parents and the uniques
+// (the counts of the reference's memory-driven searches,
+// src/quids.hpp:472-485, 523-535).
+int or_simulate_caps(const or_state *in, const qd_rule *rule, or_state **out, uint64_t max_parents,
+                     uint64_t max_survivors, const qd_params *prm, qd_step_stats *st) {
+    auto *o = new or_state;
+    int rc = simulate(in->s, *rule, o->s, max_parents, *prm, *st, nullptr, 0, nullptr, nullptr, max_survivors);
     if (rc) {
         delete o;
         *out = nullptr;

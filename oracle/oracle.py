@@ -59,6 +59,8 @@ def _load():
                                       ctypes.POINTER(QdParams), ctypes.POINTER(QdStepStats)]
         L.or_average_value_jvp.argtypes = [vp, ctypes.c_int, ctypes.c_int, pd]
         L.or_set_threads.argtypes = [ctypes.c_int]
+        L.or_simulate_caps.argtypes = [vp, ctypes.POINTER(QdRule), ctypes.POINTER(vp), u64, u64,
+                                       ctypes.POINTER(QdParams), ctypes.POINTER(QdStepStats)]
         _lib = L
     return _lib
 
@@ -98,6 +100,27 @@ def simulate(hs, rule, max_num_object=(1 << 64) - 1, tolerance=1e-30, align=8, s
     d = rule.desc()
     rc = L.or_simulate(h, ctypes.byref(d), ctypes.byref(out), int(max_num_object), ctypes.byref(prm),
                        ctypes.byref(st))
+    L.or_state_free(h)
+    if rc:
+        raise RuntimeError(f"oracle simulate failed: {rc}")
+    res = _from_or(out)
+    L.or_state_free(out)
+    return res, st.as_dict()
+
+
+def simulate_caps(hs, rule, max_parents, max_survivors, tolerance=1e-30, align=8):
+    """One oracle step keeping the top max_parents parents and the top
+    max_survivors uniques (the counts a memory-driven search settles on)
+    -> (HostState, stats dict)."""
+    from quids_amd._abi import QdParams, QdStepStats
+    L = _load()
+    h = _to_or(hs)
+    out = ctypes.c_void_p()
+    st = QdStepStats()
+    prm = QdParams(tolerance, align, 1)
+    d = rule.desc()
+    rc = L.or_simulate_caps(h, ctypes.byref(d), ctypes.byref(out), int(max_parents), int(max_survivors),
+                            ctypes.byref(prm), ctypes.byref(st))
     L.or_state_free(h)
     if rc:
         raise RuntimeError(f"oracle simulate failed: {rc}")

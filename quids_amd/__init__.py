@@ -26,16 +26,18 @@ simple_truncation = True
 #: key of the probabilistic truncation's draws (simple_truncation False):
 #: Philox2x64-10 keyed by (object hash, truncation_seed)
 truncation_seed = 0
-# memory safety margin of max_num_object == 0 (src/quids.hpp:64) and the CPU
-# load-balancing granularity (src/quids.hpp:68): read by the QCGD
-# command-line flags; the second has no effect on the device step
+# memory-driven truncation (max_num_object == 0): the memory safety margin
+# and the equalization factor of the next state's memory (src/quids.hpp:64-66)
 safety_margin = 0.2
+equalize_factor = 0.25
+# the CPU load-balancing granularity (src/quids.hpp:68): read by the QCGD
+# command-line flags, no effect on the device step
 load_balancing_bucket_per_thread = 32
 
 
 def _params():
     return QdParams(float(tolerance), int(align_byte_length), 1 if simple_truncation else 0, 0,
-                    int(truncation_seed))
+                    int(truncation_seed), float(safety_margin), float(equalize_factor))
 
 
 class Context:

@@ -65,7 +65,14 @@ class QdParams(ctypes.Structure):
         ("simple_truncation", ctypes.c_uint32),
         ("reserved", ctypes.c_uint32),
         ("seed", ctypes.c_uint64),
+        ("safety_margin", ctypes.c_double),
+        ("equalize_factor", ctypes.c_double),
     ]
+
+    def __init__(self, tolerance=1e-30, align_byte_length=8, simple_truncation=1, reserved=0, seed=0,
+                 safety_margin=0.2, equalize_factor=0.25):
+        super().__init__(tolerance, align_byte_length, simple_truncation, reserved, seed, safety_margin,
+                         equalize_factor)
 
 
 class QdStepStats(ctypes.Structure):
@@ -79,12 +86,16 @@ class QdStepStats(ctypes.Structure):
         ("total_proba", ctypes.c_double),
         ("dedup_levels", ctypes.c_uint32),
         ("reserved", ctypes.c_uint32),
+        ("mem_target_parents", ctypes.c_uint64),
+        ("mem_target_survivors", ctypes.c_uint64),
     ]
 
     def as_dict(self):
         """Counters by field name; 'reserved' holds flags (bit 0: the second
         partition pass fell back to exact offsets; bit 1: a batched step ran
-        in several chunks)."""
+        in several chunks; bit 2: hash-range passes; bit 3: the fused first
+        radix pass was redone unfused); the mem_target fields are the
+        memory-driven truncation's targets (0 with an explicit cap)."""
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
@@ -99,6 +110,7 @@ class QdTuning(ctypes.Structure):
         ("chunk_log2", ctypes.c_uint32),
         ("split_generate", ctypes.c_uint32),
         ("eager_select", ctypes.c_uint32),
+        ("free_mem_mib", ctypes.c_uint32),
     ]
 
     def as_dict(self):

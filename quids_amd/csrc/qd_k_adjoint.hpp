@@ -112,7 +112,7 @@ __global__ __launch_bounds__(256) void k_adjoint(Parents P, const u64 *cb, qd_ru
                 f.v = m;
             } else if (KIND == QD_RULE_QCGD_SPLIT_MERGE) {
                 Graph g(o);
-                h = sm_child_hash(g, pre[p] != 0, tab + P.begin[p] / 2, j, dk, f);
+                h = sm_child_hash(g, pre[p] != 0, tab + sm_tab_base(P.begin[p]), j, dk, f);
             } else {
                 Graph g(o);
                 h = flip_child_hash(g, KIND == QD_RULE_QCGD_COIN, pre[p], j, dk, f);
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(kGenThreads) void k_adjoint_fast(const PDesc *D, co
         const cplx x{xm.x, xm.y};
         WalkT<DualC> pre{};
         pre.mag = dual_one();
-        const u64 *ptab = tab + pbegin[p] / 2;
+        const u64 *ptab = tab + sm_tab_base(pbegin[p]);
         if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(pre, pd, ptab, j0, s, dk);
         else flip_run<KIND == QD_RULE_QCGD_COIN>(pre, pd, j0, s, dk);
         double axr = 0, axi = 0, at[kMaxDir] = {};

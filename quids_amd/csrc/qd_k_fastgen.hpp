@@ -114,13 +114,12 @@ __device__ inline void prep_parent(const Parents &P, u64 p, bool have, const u64
         if (sl == 0 && sm_first_hmlz(g)) d.flags |= PD_HMLZ;
         d.flags = u8(__shfl(int(d.flags), int(sbase), kWave));
         k = u32(__popcll(d.dmask)) + (lmg ? 1 : 0);
-        if (in) {
-            u64 *t = tab + P.begin[p] / 2 + 4 * sl;
+        if (in) {  // the slots of sm_fill_table
+            u64 *t = tab + sm_tab_base(P.begin[p]) + kSmSlots * sl;
             const u32 b = g.name_begin(sl);
             t[0] = g.sub_hash(b);
-            t[1] = split_left_hash(g, b);
-            t[2] = split_right_hash(g, b);
-            t[3] = merge_hash(g, b, g.name_begin(sl + 1 < n ? sl + 1 : 0));
+            t[1] = split ? split_left_hash(g, b) : merge_hash(g, b, g.name_begin(sl + 1 < n ? sl + 1 : 0));
+            t[2] = split ? split_right_hash(g, b) : 0;
         }
         local_max = max(local_max, 4 * sz);
     } else {
@@ -290,7 +289,7 @@ __device__ inline void sm_run(W &w, const PDesc &pd, const u64 *tab, u32 bits, u
             const bool take = (bits >> (w.d++ - d0)) & 1;
             if (take) {
                 w.mag = cmul(w.mag, k.dc);
-                emit(tab[4 * (n - 1) + 3], true, true);
+                emit(tab[kSmSlots * (n - 1) + 1], true, true);
                 w.i = 1;
                 w.end = n - 1;
             } else {
@@ -331,24 +330,24 @@ __device__ inline void sm_run(W &w, const PDesc &pd, const u64 *tab, u32 bits, u
             if (l && r) {  // split
                 if (take) {
                     w.mag = cmul(w.mag, k.d);
-                    emit(tab[4 * i + 1], true, false);
-                    emit(tab[4 * i + 2], false, true);
+                    emit(tab[kSmSlots * i + 1], true, false);
+                    emit(tab[kSmSlots * i + 2], false, true);
                 } else {
                     w.mag = cmul(w.mag, k.dn);
-                    emit(tab[4 * i], l, r);
+                    emit(tab[kSmSlots * i], l, r);
                 }
             } else {  // merge with node i+1 (which carries no decision)
                 if (take) {
                     w.mag = cmul(w.mag, k.dc);
-                    emit(tab[4 * i + 3], true, true);
+                    emit(tab[kSmSlots * i + 1], true, true);
                     ++w.i;
                 } else {
                     w.mag = cmul(w.mag, cneg(k.dnc));
-                    emit(tab[4 * i], l, r);
+                    emit(tab[kSmSlots * i], l, r);
                 }
             }
         } else {
-            emit(tab[4 * i], l, r);
+            emit(tab[kSmSlots * i], l, r);
         }
         ++w.i;
     }
@@ -502,7 +501,7 @@ __global__ __launch_bounds__(kGenThreads) void k_generate_fast(const PDesc *D, c
         const double2 m0 = pmag[p];
         Walk pre{};
         pre.mag = cplx{m0.x, m0.y};
-        const u64 *ptab = tab + pbegin[p] / 2;
+        const u64 *ptab = tab + sm_tab_base(pbegin[p]);
         if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(pre, pd, ptab, j0, s, k);
         else flip_run<KIND == QD_RULE_QCGD_COIN>(pre, pd, j0, s, k);
         const u64 cbase = cb[p] + j0;
@@ -597,7 +596,7 @@ __global__ __launch_bounds__(kGenThreads, 3) void k_generate_scatter(const PDesc
             const double2 m0 = pmag[p];
             Walk pre{};
             pre.mag = cplx{m0.x, m0.y};
-            const u64 *ptab = tab + pbegin[p] / 2;
+            const u64 *ptab = tab + sm_tab_base(pbegin[p]);
             if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(pre, pd, ptab, j0, s, k);
             else flip_run<KIND == QD_RULE_QCGD_COIN>(pre, pd, j0, s, k);
             cbase = cb[p] + j0;
@@ -700,7 +699,7 @@ __global__ __launch_bounds__(kGenThreads) void k_generate_range(const PDesc *D, 
             const double2 m0 = pmag[p];
             Walk pre{};
             pre.mag = cplx{m0.x, m0.y};
-            const u64 *ptab = tab + pbegin[p] / 2;
+            const u64 *ptab = tab + sm_tab_base(pbegin[p]);
             if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(pre, pd, ptab, j0, s, k);
             else flip_run<KIND == QD_RULE_QCGD_COIN>(pre, pd, j0, s, k);
             cbase = cb[p] + j0;

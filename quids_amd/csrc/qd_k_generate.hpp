@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void k_prep(Parents P, const u64 *keep, qd_rul
             if (KIND == QD_RULE_QCGD_SPLIT_MERGE) {
                 num = sm_num_child(g);
                 ms = 4 * sz;
-                sm_fill_table(g, tab + P.begin[p] / 2);
+                sm_fill_table(g, tab + sm_tab_base(P.begin[p]));
                 pre[p] = sm_first_hmlz(g) ? 1 : 0;
             } else {
                 num = flip_num_child(g, KIND == QD_RULE_QCGD_COIN);
@@ -153,7 +153,7 @@ __global__ __launch_bounds__(1024) void k_generate(Parents P, const u64 *cb, qd_
             h = hadamard_child_hash(o, P.size[p], rule.bit, rule.coef, j, mag);
         } else if (KIND == QD_RULE_QCGD_SPLIT_MERGE) {
             Graph g(o);
-            h = sm_child_hash(g, pre[p] != 0, tab + P.begin[p] / 2, j, k, mag);
+            h = sm_child_hash(g, pre[p] != 0, tab + sm_tab_base(P.begin[p]), j, k, mag);
         } else {
             Graph g(o);
             h = flip_child_hash(g, KIND == QD_RULE_QCGD_COIN, pre[p], j, k, mag);

@@ -329,17 +329,27 @@ QD_HD bool sm_first_hmlz(const Graph &g) {
     return !(g.sub_type(b) >= 0 && g.sub_hmlz(b + 1) > 0);
 }
 
-// Per-parent table for split_merge children: for every node i, four u64
-// root hashes {own, split-left, split-right, merge with node (i+1) mod n}.
+// Per-parent table for split_merge children: for every node i, three u64
+// root hashes {own, split-left or merge, split-right}.  A node holding both
+// particles (a split) never merges, so its second slot holds the left half
+// of the split; any other node's holds the merge with node (i+1) mod n
+// (the wrap-around last merge of node n-1 included).
+constexpr u32 kSmSlots = 3;
+QD_HD u32 sm_slot(u32 i, int kind) { return kSmSlots * i + (kind == 0 ? 0u : kind == 2 ? 2u : 1u); }
 QD_HD void sm_fill_table(const Graph &g, u64 *tab) {
     for (u32 i = 0; i < g.n; ++i) {
         const u32 b = g.name_begin(i);
-        tab[4 * i + 0] = g.sub_hash(b);
-        tab[4 * i + 1] = split_left_hash(g, b);
-        tab[4 * i + 2] = split_right_hash(g, b);
-        tab[4 * i + 3] = merge_hash(g, b, g.name_begin(i + 1 < g.n ? i + 1 : 0));
+        const bool split = g.L(i) && g.R(i);
+        tab[kSmSlots * i + 0] = g.sub_hash(b);
+        tab[kSmSlots * i + 1] = split ? split_left_hash(g, b) : merge_hash(g, b, g.name_begin(i + 1 < g.n ? i + 1 : 0));
+        tab[kSmSlots * i + 2] = split ? split_right_hash(g, b) : 0;
     }
 }
+// First table slot of the parent at byte offset `begin`: a valid graph of n
+// nodes spans at least 4 + 20 n bytes (every node names at least one
+// 16-byte sub_node), so floor(3 begin / 20) leaves 3 n slots before the next
+// parent's; the table holds 3 / 20 of the state's bytes in u64 slots.
+QD_HD u64 sm_tab_base(u64 begin) { return begin * kSmSlots / 20; }
 
 // Walks the child node sequence of split_merge::populate_child
 // (qcgd.hpp:643-848): calls emit(kind, i, l, r) for every child node in
@@ -418,7 +428,7 @@ QD_HD u64 sm_child_hash(const Graph &g, bool first_hmlz, const u64 *tab, u32 chi
     u64 nh = 0, lh = 0, rh = 0;
     u32 c = 0;
     sm_walk(g, first_hmlz, child_id, k, mag, true, [&](int kind, u32 i, bool l, bool r) {
-        nh = hash_combine(nh, tab[4 * i + u32(kind)]);
+        nh = hash_combine(nh, tab[sm_slot(i, kind)]);
         if (l) lh = hash_combine_small(lh, c);
         if (r) rh = hash_combine_small(rh, c);
         ++c;

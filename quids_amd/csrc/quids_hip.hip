@@ -145,7 +145,7 @@ struct qd_ctx {
     bool pending[QD_NUM_PHASES] = {};
     double phase_ms[QD_NUM_PHASES] = {};
     // defaults: what quids_amd.tuning picked on the 12-node QCGD workload
-    qd_tuning tune = {512, 8, 256, 16384, 1024, 0, 0, 0, 0};
+    qd_tuning tune = {512, 8, 256, 16384, 1024, 0, 0, 0, 0, 0};
     u64 phase_calls[QD_NUM_PHASES] = {};
     u64 *host = nullptr;  // pinned readback area, 8192 u64
     // scratch of the step
@@ -165,6 +165,7 @@ struct qd_ctx {
     } selgraph[16], seggraph;
     DBuf adjk, adjv, adjg, adjd, adjp, adjw;
     DBuf pool0, pool1, pwpre, blist2;
+    DBuf mw;  // per-object memory of memory-driven truncation
     // output buffers of batched chunks (objects, begin, size, mag per chunk),
     // kept between calls
     std::vector<DBuf> parts;
@@ -179,7 +180,7 @@ struct qd_ctx {
                 &nhist, &counters, &sel0, &sel1, &bitmap, &wprefix, &padded, &chunks, &scanblk,
                 &biglist, &redolist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &cpar, &opar, &ebuf2, &cnt2, &slist, &ukey, &selloop, &segloop, &bsoff, &bscnt,
                 &adjk, &adjv, &adjg, &adjd, &adjp, &adjw, &bseg, &boseg, &brules, &bsinst, &bcount, &bsel, &btrunc,
-                &blist, &bcur, &btile, &bhist, &bkseg, &bchunk, &bcsum, &btot, &pool0, &pool1, &pwpre, &blist2};
+                &blist, &bcur, &btile, &bhist, &bkseg, &bchunk, &bcsum, &btot, &pool0, &pool1, &pwpre, &blist2, &mw};
     }
 };
 
@@ -747,6 +748,224 @@ static int populate_out(qd_ctx *ctx, const Parents &P, const qd_rule &rule, cons
 
 
 // ===========================================================================
+// memory-driven truncation (max_num_object == 0, src/quids.hpp:459-536):
+// the reference's two binary searches for how many objects (parents in
+// truncation order, then uniques in truncation order) the memory keeps,
+// with this library's footprints in the memory model
+// (src/quids.hpp:574-589, 850-861)
+// ===========================================================================
+// magnitude, offset and size of an object in a state (the reference's
+// ITERATION_MEMORY_SIZE, src/quids.hpp:43)
+constexpr u64 kMemPerObject = sizeof(double2) + sizeof(u64) + sizeof(u32);
+// step scratch per candidate in one pass (the reference's
+// SYMBOLIC_ITERATION_MEMORY_SIZE plus its hash map, src/quids.hpp:44, 577):
+// the 32-byte entry of the first radix pass, about 1.3 x 32 bytes of the
+// second, the 32-byte dedup slot (which reuses the generate buffer's
+// space), 4 bytes of select key, bitmaps, and growth slack
+constexpr u64 kMemPerCandidate = 112;
+
+// w[p] = the modelled memory of parent p: its state entry, its padded bytes
+// and its candidates' scratch (iteration::get_truncated_mem_size before the
+// upsize factor).
+__global__ __launch_bounds__(256) void k_parent_weight(Parents P, const u32 *nc, u64 *w) {
+    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < P.n; i += u64(gridDim.x) * blockDim.x)
+        w[i] = kMemPerObject + (P.begin[i + 1] - P.begin[i]) + u64(nc[i]) * kMemPerCandidate;
+}
+
+// w[i] = the modelled memory of the unique in dedup slot i (0 for an empty
+// slot): its state entry and its child's padded size
+// (symbolic_iteration::get_truncated_mem_size).
+template <int KIND>
+__global__ __launch_bounds__(256) void k_slot_weight(Parents P, const u64 *cb, qd_rule rule, const u64 *UR, u64 n,
+                                                     u32 align, u64 *w) {
+    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x) {
+        const u64 c = UR[i];
+        if (c == kNoRep) {
+            w[i] = 0;
+            continue;
+        }
+        const u64 p = find_parent(cb, P.n, c);
+        const u32 sz = rule_child_size(rule, P.obj + P.begin[p], P.size[p], u32(c - cb[p]));
+        w[i] = kMemPerObject + sz + align_pad(sz, align);
+    }
+}
+
+// *out += the sum of w[i] over i < n with bit i set in bits (every i when
+// bits is null).
+__global__ __launch_bounds__(256) void k_masked_sum(const u64 *w, const u64 *bits, u64 n, u64 *out) {
+    u64 acc = 0;
+    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+        if (!bits || ((bits[i >> 6] >> (i & 63)) & 1)) acc += w[i];
+    for (int d = kWave / 2; d > 0; d >>= 1) acc += __shfl_xor(acc, d, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0 && acc) atomicAdd(reinterpret_cast<unsigned long long *>(out), acc);
+}
+
+static u64 state_mem(const qd_state *s) { return s->objects.cap + s->begin.cap + s->size.cap + s->mag.cap; }
+
+static u64 ctx_scratch(qd_ctx *ctx) {
+    u64 t = 0;
+    for (DBuf *b : ctx->all()) t += b->cap;
+    return t;
+}
+
+// Free device memory as memory-driven truncation sees it (the reference
+// reads the host's, src/utils/memory.hpp:25-36), at most free_mem_mib.
+static int free_mem(qd_ctx *ctx, u64 &freeb) {
+    size_t f = 0, total = 0;
+    QD_HIP(hipMemGetInfo(&f, &total));
+    freeb = f;
+    if (ctx->tune.free_mem_mib) freeb = std::min<u64>(freeb, u64(ctx->tune.free_mem_mib) << 20);
+    return QD_OK;
+}
+
+// target_memory of src/quids.hpp:462-469 (parent side: avail_extra = the
+// symbolic iteration's memory) and 513-520 (survivor side: nonavail_extra =
+// it): the next state's memory is pulled towards the previous one's by
+// equalize_factor when larger; a target below zero is zero.
+static u64 target_memory(u64 next_mem, u64 prev_mem, u64 avail_extra, u64 nonavail_extra, u64 freeb,
+                         const qd_params &prm) {
+    double next = double(next_mem);
+    if (next_mem > prev_mem) next = (1 - prm.equalize_factor) * next + prm.equalize_factor * double(prev_mem);
+    const double avail = std::floor(next) + double(avail_extra) + double(freeb);
+    const double non_avail = double(prev_mem) + double(nonavail_extra);
+    const double t = (avail + non_avail) * (1 - prm.safety_margin) - non_avail;
+    return t > 0 ? u64(t) : 0;
+}
+
+// The binary search of src/quids.hpp:472-485 (and 523-535): probes mem(k),
+// the modelled memory of the first k objects in truncation order, keeps
+// ranks [begin, middle) when their memory (times 9/8, the reference's
+// upsize policy, on the parent side) is below what is left of the target,
+// and ends with the last probed count, as the reference's truncate() calls
+// leave truncated_num_object.
+template <class Mem>
+static int mem_search(u64 n, u64 target, bool upsize, Mem &&mem, u64 &kept) {
+    u64 begin = 0, end = n, mbegin = 0;
+    kept = n;
+    while (end > begin + 1) {
+        const u64 middle = (begin + end) / 2;
+        u64 mm = 0;
+        QD_TRY(mem(middle, mm));
+        kept = middle;
+        u64 used = mm - mbegin;
+        if (upsize) used += used / 8;
+        if (used < target) {
+            target -= used;
+            begin = middle;
+            mbegin = mm;
+        } else {
+            end = middle;
+        }
+    }
+    return QD_OK;
+}
+
+// Parent side (src/quids.hpp:459-486): cap = the number of parents the
+// memory keeps (UINT64_MAX when all fit).
+template <int KIND>
+static int mem_parent_cap(qd_ctx *ctx, const Parents &P, const qd_rule &rule, u64 prev_mem, u64 next_mem,
+                          const qd_params &prm, const SelKey &sk, qd_step_stats &st, u64 &cap) {
+    hipStream_t s = ctx->stream;
+    const u64 N = P.n;
+    cap = UINT64_MAX;
+    QD_TRY(ctx->nc.ensure(N * sizeof(u32)));
+    QD_TRY(ctx->mw.ensure(N * sizeof(u64)));
+    QD_TRY(ctx->counters.ensure(64 * sizeof(u64)));
+    u64 *counters = ctx->counters.as<u64>();
+    QD_HIP(hipMemsetAsync(counters, 0, 64 * sizeof(u64), s));
+    hipLaunchKernelGGL(k_num_child<KIND>, dim3(grid_for(N)), dim3(256), 0, s, P, rule, ctx->nc.as<u32>(),
+                       reinterpret_cast<u32 *>(counters + 4));
+    hipLaunchKernelGGL(k_parent_weight, dim3(grid_for(N)), dim3(256), 0, s, P, (const u32 *)ctx->nc.as<u32>(),
+                       ctx->mw.as<u64>());
+    hipLaunchKernelGGL(k_masked_sum, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, (const u64 *)ctx->mw.as<u64>(),
+                       (const u64 *)nullptr, N, counters + 8);
+    QD_HIP(hipGetLastError());
+    u64 hb[16];
+    QD_TRY(sync_read(ctx, counters, 16 * sizeof(u64), hb));
+    QD_TRY(check_err_word(u32(hb[4])));
+    u64 freeb = 0;
+    QD_TRY(free_mem(ctx, freeb));
+    const u64 target = target_memory(next_mem, prev_mem, ctx_scratch(ctx), 0, freeb, prm);
+    st.mem_target_parents = target;
+    const u64 total = hb[8];
+    if (total + total / 8 <= target) return QD_OK;
+    // top-k parents by the pre-truncation key; their memory
+    QD_TRY(ctx->sel0.ensure(N * sizeof(SelItem)));
+    QD_TRY(ctx->sel1.ensure(N * sizeof(SelItem)));
+    QD_TRY(ctx->kept.ensure(((N + 63) / 64) * sizeof(u64)));
+    QD_TRY(ctx->hist1.ensure(kHistBins * sizeof(u64)));
+    u64 *pbits = ctx->kept.as<u64>();
+    auto mem = [&](u64 k, u64 &m) -> int {
+        QD_HIP(hipMemsetAsync(pbits, 0, ((N + 63) / 64) * sizeof(u64), s));
+        hipLaunchKernelGGL(k_parent_items, dim3(grid_for(N)), dim3(256), 0, s, P, rule, sk, ctx->sel0.as<SelItem>());
+        QD_HIP(hipGetLastError());
+        QD_TRY(select_loop(ctx, ctx->sel0.as<SelItem>(), ctx->sel1.as<SelItem>(), N, k, 0, pbits, nullptr, nullptr));
+        QD_HIP(hipMemsetAsync(counters + 8, 0, sizeof(u64), s));
+        hipLaunchKernelGGL(k_masked_sum, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s,
+                           (const u64 *)ctx->mw.as<u64>(), (const u64 *)pbits, N, counters + 8);
+        QD_HIP(hipGetLastError());
+        return sync_read(ctx, counters + 8, sizeof(u64), &m);
+    };
+    u64 kept = N;
+    QD_TRY(mem_search(N, target, true, mem, kept));
+    if (kept < N) cap = kept;
+    return QD_OK;
+}
+
+// Survivor side (src/quids.hpp:510-536), after the dedup: max_obj = the
+// number of uniques the memory keeps (UINT64_MAX when all fit).  A probe
+// runs the step's own top-K select for k and sums the memory of the slots
+// it marks.
+template <int KIND>
+static int mem_survivor_cap(qd_ctx *ctx, const Parents &P, const qd_rule &rule, u64 prev_mem, qd_state *out,
+                            const qd_params &prm, const SelKey &sk, const u64 *UH, const u64 *UR, const double2 *UM,
+                            const u32 *UK, u64 C, u64 U, qd_step_stats &st, u64 &max_obj) {
+    hipStream_t s = ctx->stream;
+    max_obj = UINT64_MAX;
+    u64 freeb = 0;
+    QD_TRY(free_mem(ctx, freeb));
+    const u64 target = target_memory(state_mem(out), prev_mem, 0, ctx_scratch(ctx), freeb, prm);
+    st.mem_target_survivors = target;
+    if (U == 0) return QD_OK;
+    QD_TRY(ctx->mw.ensure(C * sizeof(u64)));
+    u64 *sum = ctx->counters.as<u64>() + 12;
+    auto masked = [&](const u64 *bits, u64 &m) -> int {
+        QD_HIP(hipMemsetAsync(sum, 0, sizeof(u64), s));
+        hipLaunchKernelGGL(k_masked_sum, dim3(grid_for(C, 256, 4096)), dim3(256), 0, s,
+                           (const u64 *)ctx->mw.as<u64>(), bits, C, sum);
+        QD_HIP(hipGetLastError());
+        return sync_read(ctx, sum, sizeof(u64), &m);
+    };
+    hipLaunchKernelGGL(k_slot_weight<KIND>, dim3(grid_for(C)), dim3(256), 0, s, P, (const u64 *)ctx->cb.as<u64>(), rule,
+                       UR, C, prm.align_byte_length, ctx->mw.as<u64>());
+    QD_HIP(hipGetLastError());
+    u64 total = 0;
+    QD_TRY(masked(nullptr, total));
+    if (total <= target) return QD_OK;
+    const u64 W = (C + 63) / 64;
+    QD_TRY(ctx->bitmap.ensure(W * sizeof(u64)));
+    QD_TRY(ctx->slist.ensure(W * sizeof(u64)));
+    u64 *bitmap = ctx->bitmap.as<u64>(), *sbm = ctx->slist.as<u64>();
+    auto mem = [&](u64 k, u64 &m) -> int {
+        QD_HIP(hipMemsetAsync(bitmap, 0, W * sizeof(u64), s));
+        std::vector<u64> h(kHistBins);
+        if (sk.prob) {
+            QD_HIP(hipMemsetAsync(ctx->nhist.p, 0, kHistBins * sizeof(u64), s));
+            hipLaunchKernelGGL(k_key_hist0, dim3(grid_for(C, 256, 4096)), dim3(256), 0, s, UH, UR, UM, C,
+                               ctx->nhist.as<u64>(), sk);
+            QD_HIP(hipGetLastError());
+        }
+        QD_TRY(sync_read(ctx, ctx->nhist.p, kHistBins * sizeof(u64), h.data()));
+        QD_TRY(select_topk(ctx, UH, UR, UM, UK, C, k, h, bitmap, sbm, sk));
+        return masked(sbm, m);
+    };
+    u64 kept = U;
+    QD_TRY(mem_search(U, target, false, mem, kept));
+    if (kept < U) max_obj = kept;
+    return QD_OK;
+}
+
+// ===========================================================================
 // hash-range passes (qd_k_range.hpp): steps whose candidates exceed the
 // scratch memory
 // ===========================================================================
@@ -1094,8 +1313,8 @@ static int batch_normalize(qd_ctx *ctx, Batch *bt, qd_state *out) {
 // One step over the parents P (of a state whose objects span in_bytes).  With
 // BATCH, the parents are the instances of bt, each with its own rule and cap.
 template <int KIND, bool BATCH>
-static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule &rule, Batch *bt, qd_state *out,
-                     qd_symbolic *sy, u64 max_obj, const qd_params &prm, qd_step_stats &st) {
+static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, u64 in_mem, const qd_rule &rule, Batch *bt,
+                     qd_state *out, qd_symbolic *sy, u64 max_obj, const qd_params &prm, qd_step_stats &st) {
     hipStream_t s = ctx->stream;
     const u64 N = P.n;
     st.num_parent = N;
@@ -1120,9 +1339,14 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
 
     // ---- pre-truncation of parents (quids.hpp:613-642): top max by |m|^2,
     // ties by ascending index, through the 128-bit select on (|m|^2, ~index).
+    // With max_num_object == 0 the cap comes from the memory model
+    // (src/quids.hpp:459-486).
     const u64 *keep = nullptr;
     u64 nkept = N;
-    if (!BATCH && max_obj != 0 && max_obj < N) {
+    u64 pcap = max_obj;
+    if (!BATCH && max_obj == 0)
+        QD_TRY(mem_parent_cap<KIND>(ctx, P, rule, in_mem, state_mem(out), prm, sk, st, pcap));
+    if (!BATCH && pcap != 0 && pcap < N) {
         QD_TRY(ctx->sel0.ensure(N * sizeof(SelItem)));
         QD_TRY(ctx->sel1.ensure(N * sizeof(SelItem)));
         QD_TRY(ctx->kept.ensure(((N + 63) / 64) * sizeof(u64)));
@@ -1133,9 +1357,9 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
         hipLaunchKernelGGL(k_parent_items, dim3(grid_for(N)), dim3(256), 0, s, P, rule, sk, ctx->sel0.as<SelItem>());
         QD_HIP(hipGetLastError());
         SelItem *S0 = ctx->sel0.as<SelItem>(), *S1 = ctx->sel1.as<SelItem>();
-        QD_TRY(select_loop(ctx, S0, S1, N, max_obj, 0, pbits, nullptr, nullptr));
+        QD_TRY(select_loop(ctx, S0, S1, N, pcap, 0, pbits, nullptr, nullptr));
         keep = pbits;
-        nkept = max_obj;
+        nkept = pcap;
     }
     st.num_parent_kept = nkept;
 
@@ -1144,7 +1368,7 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
     QD_TRY(ctx->cb.ensure((N + 1) * sizeof(u64)));
     QD_TRY(ctx->pre.ensure(N * sizeof(u64)));
     QD_TRY(ctx->counters.ensure(64 * sizeof(u64)));
-    if (KIND == QD_RULE_QCGD_SPLIT_MERGE) QD_TRY(ctx->tab.ensure((in_bytes / 2 + 16) * sizeof(u64)));
+    if (KIND == QD_RULE_QCGD_SPLIT_MERGE) QD_TRY(ctx->tab.ensure((sm_tab_base(in_bytes) + kSmSlots * 64 + 16) * sizeof(u64)));
     u64 *counters = ctx->counters.as<u64>();
     // counters: [0] unique count, [1] select count A, [2] select count B,
     // [3] low: max child size, [4] low: error word, [5] oversized buckets
@@ -1204,6 +1428,7 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
         prof_collect(ctx);
     }
     const u64 C = hb[0];
+    const u32 max_child = u32(hb[1]);  // largest child size (bytes) of the kept parents
     QD_TRY(check_err_word(u32(hb[2])));
     st.num_candidates = C;
     if (BATCH) QD_TRY(batch_cseg(ctx, bt));
@@ -1211,11 +1436,17 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
     if (!BATCH && fast) {
         const u64 budget = step_budget(ctx);
         if (C > budget) {
-            if (max_obj == 0) {  // memory-driven cap, as below
-                size_t freeb = 0, totalb = 0;
-                QD_HIP(hipMemGetInfo(&freeb, &totalb));
-                const u64 per = u64(u32(hb[1])) + 8 + sizeof(u64) + sizeof(u32) + sizeof(double2);
-                max_obj = std::max<u64>(1, u64(double(freeb) * 0.8) / per);
+            if (max_obj == 0) {
+                // memory-driven survivor cap: the uniques of a step in
+                // hash-range passes are never all held at once, so the
+                // survivor side's model takes every survivor at the largest
+                // child size (an upper bound of the reference's sum)
+                u64 freeb = 0;
+                QD_TRY(free_mem(ctx, freeb));
+                const u64 target = target_memory(state_mem(out), in_mem, 0, ctx_scratch(ctx), freeb, prm);
+                st.mem_target_survivors = target;
+                const u64 per = kMemPerObject + max_child + prm.align_byte_length;
+                max_obj = std::max<u64>(1, target / per);
             }
             return range_step<KIND>(ctx, P, rule, C, nitems, max_obj, budget, prm, out, sy, st);
         }
@@ -1313,15 +1544,9 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule 
     QD_TRY(partition_dedup(ctx, H0, M0, nullptr, C, b1, b2, prm.tolerance, st, UH, UR, UM, U, in_stride, UK));
     st.num_unique = U;
 
-    // memory-driven cap (max_num_object == 0, quids.hpp:459-486, 510-536):
-    // keep as many survivors as the free device memory holds for the next
-    // state, at the largest child size seen, with a 20% safety margin.
-    if (!BATCH && max_obj == 0) {
-        size_t freeb = 0, totalb = 0;
-        QD_HIP(hipMemGetInfo(&freeb, &totalb));
-        const u64 per = u64(u32(hb[3])) + 8 + sizeof(u64) + sizeof(u32) + sizeof(double2);
-        max_obj = std::max<u64>(1, u64(double(freeb) * 0.8) / per);
-    }
+    // memory-driven survivor cap (max_num_object == 0, src/quids.hpp:510-536)
+    if (!BATCH && max_obj == 0) QD_TRY(mem_survivor_cap<KIND>(ctx, P, rule, in_mem, out, prm, sk, UH, UR, UM, UK, C, U,
+                                                               st, max_obj));
 
     // ---- 6 select + 7 order
     const u64 W = (C + 63) / 64;
@@ -1387,7 +1612,7 @@ template <int KIND>
 static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_state *out, qd_symbolic *sy, u64 max_obj,
                          const qd_params &prm, qd_step_stats &st) {
     const Parents P{in->objects.as<u8>(), in->begin.as<u64>(), in->size.as<u32>(), in->mag.as<double2>(), in->n};
-    return step_impl<KIND, false>(ctx, P, in->bytes, rule, nullptr, out, sy, max_obj, prm, st);
+    return step_impl<KIND, false>(ctx, P, in->bytes, state_mem(in), rule, nullptr, out, sy, max_obj, prm, st);
 }
 
 // ===========================================================================
@@ -1415,7 +1640,7 @@ static int adjoint_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, const qd
     QD_TRY(ctx->cb.ensure((N + 1) * sizeof(u64)));
     QD_TRY(ctx->pre.ensure(N * sizeof(u64)));
     QD_TRY(ctx->counters.ensure(64 * sizeof(u64)));
-    if (KIND == QD_RULE_QCGD_SPLIT_MERGE) QD_TRY(ctx->tab.ensure((in->bytes / 2 + 16) * sizeof(u64)));
+    if (KIND == QD_RULE_QCGD_SPLIT_MERGE) QD_TRY(ctx->tab.ensure((sm_tab_base(in->bytes) + kSmSlots * 64 + 16) * sizeof(u64)));
     u64 *counters = ctx->counters.as<u64>();
     QD_HIP(hipMemsetAsync(counters, 0, 64 * sizeof(u64), s));
     u32 *maxsz = reinterpret_cast<u32 *>(counters + 3);
@@ -1638,12 +1863,12 @@ static int batched_kind(qd_ctx *c, qd_state *in, const u64 *in_seg, u32 n, const
         if (bt.n == 1) {
             // one instance alone: its plain step (which runs in hash-range
             // passes when it exceeds the budget by itself)
-            rc = step_impl<KIND, false>(c, Pk, in->bytes, rules[i0], nullptr, dst, nullptr, caps[i0], prm, st);
+            rc = step_impl<KIND, false>(c, Pk, in->bytes, 0, rules[i0], nullptr, dst, nullptr, caps[i0], prm, st);
             bt.U.assign(1, st.num_unique);
             bt.kseg = {0, dst->n};
             bt.total.assign(1, dst->total_proba);
         } else {
-            rc = step_impl<KIND, true>(c, Pk, in->bytes, rules[i0], &bt, dst, nullptr, UINT64_MAX, prm, st);
+            rc = step_impl<KIND, true>(c, Pk, in->bytes, 0, rules[i0], &bt, dst, nullptr, UINT64_MAX, prm, st);
         }
         if (nchunk > 1) give_back(k);
         if (rc) {
@@ -1839,6 +2064,7 @@ int qd_ctx_set_tuning(qd_ctx *c, const qd_tuning *t) {
     n.chunk_log2 = t->chunk_log2;  // 0 = from free device memory
     n.split_generate = t->split_generate ? 1 : 0;
     n.eager_select = t->eager_select ? 1 : 0;
+    n.free_mem_mib = t->free_mem_mib;  // 0 = the device's free memory
     c->tune = n;
     return QD_OK;
 }
@@ -2049,7 +2275,7 @@ int qd_simulate(qd_ctx *c, qd_state *in, const qd_rule *rule, qd_state *out, qd_
     if (!c || !in || !rule || !out) return fail(QD_ERR_INVALID, "null argument");
     if (in == out) return fail(QD_ERR_INVALID, "input and output states must differ");
     if (in->ctx != c || out->ctx != c) return fail(QD_ERR_INVALID, "state belongs to another context");
-    qd_params prm{1e-30, 8, 1, 0, 0};
+    qd_params prm{1e-30, 8, 1, 0, 0, 0.2, 0.25};
     if (params) prm = *params;
     qd_step_stats st;
     std::memset(&st, 0, sizeof st);
@@ -2119,7 +2345,7 @@ int qd_simulate_batched(qd_ctx *c, qd_state *in, const uint64_t *in_seg, uint32_
         if (in_seg[i + 1] - in_seg[i] > max_num_object[i])
             return fail(QD_ERR_UNSUPPORTED, "batched instances cannot be pre-truncated (more parents than max)");
     }
-    qd_params prm{1e-30, 8, 1, 0, 0};
+    qd_params prm{1e-30, 8, 1, 0, 0, 0.2, 0.25};
     if (params) prm = *params;
     QD_HIP(hipSetDevice(c->device));
     switch (kind) {

@@ -47,6 +47,8 @@ inline bool simple_truncation = true;
 inline uint64_t truncation_seed = 0;
 /// memory safety margin of max_num_object == 0 (src/quids.hpp:64)
 inline float safety_margin = 0.2f;
+/// memory equalization factor of the next state (src/quids.hpp:66)
+inline float equalize_factor = 0.25f;
 /// CPU load-balancing granularity (src/quids.hpp:68); read by the QCGD
 /// command-line flags, no effect on the device step
 inline int load_balancing_bucket_per_thread = 32;
@@ -79,6 +81,8 @@ inline qd_params params() {
     p.simple_truncation = simple_truncation ? 1 : 0;
     p.reserved = 0;
     p.seed = truncation_seed;
+    p.safety_margin = safety_margin;
+    p.equalize_factor = equalize_factor;
     return p;
 }
 }  // namespace detail

@@ -29,7 +29,7 @@ def test_python_binding_covers_header():
 
 def test_version_and_status_strings():
     lib = _abi.lib()
-    assert lib.qd_abi_version() == 1
+    assert lib.qd_abi_version() == 2
     assert lib.qd_status_string(0) == b"ok"
     assert lib.qd_status_string(5) == b"unsupported"
 
