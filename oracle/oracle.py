@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
 
 _lib = None
+MAX_DIRECTIONS = 8  # tangent directions of one forward-mode step (kDir in oracle.cpp)
 
 
 def _load():
@@ -54,6 +55,7 @@ def _load():
         L.or_average_value.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.or_average_value.restype = ctypes.c_double
         L.or_state_set_tangent.argtypes = [vp, ctypes.c_int, pd]
+        L.or_state_set_tangent.restype = ctypes.c_int
         L.or_state_get_tangent.argtypes = [vp, pd]
         L.or_simulate_jvp.argtypes = [vp, ctypes.POINTER(QdRule), ctypes.POINTER(QdRule), ctypes.POINTER(vp), u64,
                                       ctypes.POINTER(QdParams), ctypes.POINTER(QdStepStats)]
@@ -332,7 +334,8 @@ def _set_tan(h, tan):
     tan = np.ascontiguousarray(tan, np.complex128)
     D = tan.shape[1] if tan.ndim == 2 else 0
     flat = np.ascontiguousarray(tan.reshape(-1).view(np.float64)) if tan.size else np.zeros(1)
-    L.or_state_set_tangent(h, D, flat.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    if L.or_state_set_tangent(h, D, flat.ctypes.data_as(ctypes.POINTER(ctypes.c_double))):
+        raise ValueError(f"the oracle carries at most {MAX_DIRECTIONS} tangent directions, not {D}")
     return D
 
 

@@ -26,7 +26,7 @@ constexpr u32 kItemBits = 3;  // up to 8 children per work item
 struct __align__(16) PDesc {
     u64 lmask, rmask;  // particle bits
     u64 dmask;         // nodes that carry a decision
-    u64 nf;            // erase_create / coin: fold of the node root hashes
+    u64 nf;            // erase_create / coin: fold of the node root hashes, times kMul
     u32 nc;            // num_child, 2^k wrapped to 32 bits like the reference's uint
     u8 n, k, flags, pad;
 };
@@ -129,7 +129,7 @@ __device__ inline void prep_parent(const Parents &P, u64 p, bool have, const u64
         const u64 hr = in ? g.sub_hash(g.name_begin(sl)) : 0;
         u64 nf = 0;
         for (u32 i = 0; i < n; ++i) nf = hash_combine(nf, __shfl(hr, int(sbase + i), kWave));
-        d.nf = nf;
+        d.nf = nf * kMul;  // pre-multiplied, as the walks' folds carry it
         local_max = max(local_max, sz);
     }
     d.k = u8(k);
@@ -259,6 +259,8 @@ __global__ __launch_bounds__(256) void k_step_modifier(u8 *obj, const u64 *begin
 
 // Walk state of one child (or of the shared prefix of a work item).  M is
 // the magnitude type: cplx, or DualC (with derivatives, for the adjoint).
+// The three hash folds are carried pre-multiplied (qd_rules.hpp): nh * kMul,
+// lh * kMul^2, rh * kMul^2.
 template <typename M>
 struct WalkT {
     u64 nh, lh, rh;
@@ -278,9 +280,9 @@ __device__ inline void sm_run(W &w, const PDesc &pd, const u64 *tab, u32 bits, u
     const u32 n = pd.n;
     const u32 d0 = w.d;
     auto emit = [&](u64 root, bool l, bool r) {
-        w.nh = hash_combine(w.nh, root);
-        if (l) w.lh = hash_combine_small(w.lh, w.c);
-        if (r) w.rh = hash_combine_small(w.rh, w.c);
+        w.nh = step_pm(w.nh, root);
+        if (l) w.lh = small_step_pm(w.lh, w.c);
+        if (r) w.rh = small_step_pm(w.rh, w.c);
         ++w.c;
     };
     if (!w.started) {
@@ -356,11 +358,11 @@ __device__ inline void sm_run(W &w, const PDesc &pd, const u64 *tab, u32 bits, u
 template <typename W>
 __device__ inline u64 sm_finish(W &w, const u64 *tab) {
     if (w.overflow) {  // left half of the first node goes last (qcgd.hpp:838-845)
-        w.nh = hash_combine(w.nh, tab[1]);
-        w.lh = hash_combine_small(w.lh, w.c);
+        w.nh = step_pm(w.nh, tab[1]);
+        w.lh = small_step_pm(w.lh, w.c);
         ++w.c;
     }
-    return hash_combine(hash_combine(w.nh, w.lh), w.rh);
+    return finish_pm(w.nh, w.lh, w.rh);
 }
 
 // erase_create / coin child walk (qcgd.hpp:487-510, 562-584).
@@ -382,8 +384,8 @@ __device__ inline void flip_run(W &w, const PDesc &pd, u32 bits, u32 dlimit, con
                 w.mag = cmul(w.mag, l ? cneg(k.dnc) : k.dn);
             }
         }
-        if (l) w.lh = hash_combine_small(w.lh, i);
-        if (r) w.rh = hash_combine_small(w.rh, i);
+        if (l) w.lh = small_step_pm(w.lh, i);
+        if (r) w.rh = small_step_pm(w.rh, i);
         ++w.i;
     }
 }
@@ -425,7 +427,7 @@ __device__ __attribute__((always_inline)) inline void walk_item(const W &pre, co
                 walk_step<KIND>(w2, pd, ptab, b2, s + 3, k);
                 u64 h;
                 if (KIND == QD_RULE_QCGD_SPLIT_MERGE) h = sm_finish(w2, ptab);
-                else h = hash_combine(hash_combine(pd.nf, w2.lh), w2.rh);
+                else h = finish_pm(pd.nf, w2.lh, w2.rh);
                 leaf(b0 | (b1 << 1) | (b2 << 2), h, w2.mag);
             }
         }

@@ -97,6 +97,35 @@ QD_HD u64 hash_combine_small(u64 seed, u64 v) {
     return seed;
 }
 
+// Folds carried in pre-multiplied form (the fast walks' child hashes).
+// hash_combine starts by multiplying its seed by kMul, hash_combine_small by
+// kMul^2; a fold can instead carry u = seed * kMul (or seed * kMul^2) and
+// end each step with one multiplication by a combined constant, which drops
+// one of hash_combine's three 64-bit multiplications and one of
+// hash_combine_small's two.  The seed comes back as u times the inverse of
+// the constant mod 2^64 (kMul is odd), once at the end.  The values are
+// those of the plain folds, bit for bit: only the association of the
+// multiplications mod 2^64 changes.
+constexpr u64 kAdd = 0xe6546b64ULL;
+constexpr u64 kMul3 = kMul2 * kMul;
+constexpr u64 inverse_mod_2_64(u64 a) {  // a odd: Newton's iteration doubles the correct low bits
+    u64 x = a;                          // a * a == 1 mod 8
+    for (int i = 0; i < 5; ++i) x *= 2 - a * x;
+    return x;
+}
+constexpr u64 kInvMul = inverse_mod_2_64(kMul);
+constexpr u64 kInvMul2 = inverse_mod_2_64(kMul2);
+static_assert(kMul * kInvMul == 1 && kMul2 * kInvMul2 == 1, "inverses mod 2^64");
+// u = s * kMul^2  ->  hash_combine_small(s, v) * kMul^2
+QD_HD u64 small_step_pm(u64 u, u64 v) { return (u ^ v) * kMul3 + kAdd * kMul2; }
+// u = s * kMul  ->  hash_combine(s, v) * kMul
+QD_HD u64 step_pm(u64 u, u64 v) { return (((u ^ (v >> 47)) * kMul) ^ v) * (kMul * kMul) + kAdd * kMul; }
+// u = s * kMul  ->  hash_combine(s, v)
+QD_HD u64 last_step_pm(u64 u, u64 v) { return (((u ^ (v >> 47)) * kMul) ^ v) * kMul + kAdd; }
+// The child hash hash_combine(hash_combine(s, lh), rh) (qcgd.hpp:143-145)
+// from the pre-multiplied s * kMul, lh * kMul^2 and rh * kMul^2.
+QD_HD u64 finish_pm(u64 su, u64 lu, u64 ru) { return last_step_pm(step_pm(su, lu * kInvMul2), ru * kInvMul2); }
+
 // Default rule hasher: std::hash<std::string_view> of libstdc++ on LP64,
 // which is MurmurHash64A (Austin Appleby's published algorithm) with seed
 // 0xc70f6907 and the trailing bytes loaded little-endian.  Checked against
