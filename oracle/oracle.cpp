@@ -598,14 +598,15 @@ static inline u32 align_pad(u32 size, u32 align) {
 // out_tan receives the output's: the forward-mode derivative of the step
 // along D real parameters, dcoef[t] holding the derivatives of the rule's
 // coefficients along parameter t.
-// max_survivors caps the uniques as max_num_object caps the parents; the
-// reference uses one value for both unless its memory-driven searches chose
-// them (src/quids.hpp:472-485, 523-535).
+// max_survivors caps the uniques as max_num_object caps the parents
+// (has_survivor_cap false: the same value); the reference uses one value for
+// both unless its memory-driven searches chose them (src/quids.hpp:472-485,
+// 523-535).
 static int simulate(const State &in, const qd_rule &rule, State &out, u64 max_num_object,
                     const qd_params &prm, qd_step_stats &st, const qd_rule *dcoef = nullptr, int D = 0,
                     const std::vector<cplx> *in_tan = nullptr, std::vector<cplx> *out_tan = nullptr,
-                    u64 max_survivors = ~u64(0)) {
-    if (max_survivors == ~u64(0)) max_survivors = max_num_object;
+                    bool has_survivor_cap = false, u64 max_survivors = 0, bool normalize = true) {
+    if (!has_survivor_cap) max_survivors = max_num_object;
     const size_t N = in.n();
     std::memset(&st, 0, sizeof st);
     st.num_parent = N;
@@ -795,7 +796,7 @@ static int simulate(const State &in, const qd_rule &rule, State &out, u64 max_nu
             }
         }
     }
-    if (K > 0 && f != 1)
+    if (K > 0 && f != 1 && normalize)
         for (auto &m : out.mag) m = {m.re / f, m.im / f};
     if (K == 0) out.total_proba = 0;
     st.total_proba = out.total_proba;
@@ -867,11 +868,13 @@ int or_simulate(const or_state *in, const qd_rule *rule, or_state **out, uint64_
 
 // The step with separate truncation counts for the parents and the uniques
 // (the counts of the reference's memory-driven searches,
-// src/quids.hpp:472-485, 523-535).
+// src/quids.hpp:472-485, 523-535); with normalize 0 the magnitudes are left
+// as the interference summed them (the values the truncation orders).
 int or_simulate_caps(const or_state *in, const qd_rule *rule, or_state **out, uint64_t max_parents,
-                     uint64_t max_survivors, const qd_params *prm, qd_step_stats *st) {
+                     uint64_t max_survivors, const qd_params *prm, qd_step_stats *st, int normalize) {
     auto *o = new or_state;
-    int rc = simulate(in->s, *rule, o->s, max_parents, *prm, *st, nullptr, 0, nullptr, nullptr, max_survivors);
+    int rc = simulate(in->s, *rule, o->s, max_parents, *prm, *st, nullptr, 0, nullptr, nullptr, true, max_survivors,
+                      normalize != 0);
     if (rc) {
         delete o;
         *out = nullptr;

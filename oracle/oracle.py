@@ -62,7 +62,7 @@ def _load():
         L.or_average_value_jvp.argtypes = [vp, ctypes.c_int, ctypes.c_int, pd]
         L.or_set_threads.argtypes = [ctypes.c_int]
         L.or_simulate_caps.argtypes = [vp, ctypes.POINTER(QdRule), ctypes.POINTER(vp), u64, u64,
-                                       ctypes.POINTER(QdParams), ctypes.POINTER(QdStepStats)]
+                                       ctypes.POINTER(QdParams), ctypes.POINTER(QdStepStats), ctypes.c_int]
         _lib = L
     return _lib
 
@@ -110,10 +110,11 @@ def simulate(hs, rule, max_num_object=(1 << 64) - 1, tolerance=1e-30, align=8, s
     return res, st.as_dict()
 
 
-def simulate_caps(hs, rule, max_parents, max_survivors, tolerance=1e-30, align=8):
+def simulate_caps(hs, rule, max_parents, max_survivors, tolerance=1e-30, align=8, normalize=True):
     """One oracle step keeping the top max_parents parents and the top
     max_survivors uniques (the counts a memory-driven search settles on)
-    -> (HostState, stats dict)."""
+    -> (HostState, stats dict).  normalize=False leaves the magnitudes as
+    the interference summed them (the values the truncation orders)."""
     from quids_amd._abi import QdParams, QdStepStats
     L = _load()
     h = _to_or(hs)
@@ -122,7 +123,7 @@ def simulate_caps(hs, rule, max_parents, max_survivors, tolerance=1e-30, align=8
     prm = QdParams(tolerance, align, 1)
     d = rule.desc()
     rc = L.or_simulate_caps(h, ctypes.byref(d), ctypes.byref(out), int(max_parents), int(max_survivors),
-                            ctypes.byref(prm), ctypes.byref(st))
+                            ctypes.byref(prm), ctypes.byref(st), 1 if normalize else 0)
     L.or_state_free(h)
     if rc:
         raise RuntimeError(f"oracle simulate failed: {rc}")

@@ -755,4 +755,115 @@ __global__ __launch_bounds__(kGenThreads) void k_generate_range(const PDesc *D, 
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// A hash-range pass fused with its first radix pass: the children whose top
+// 12 hash bits lie in [rlo, rhi) are ranked per bucket of the top b1 bits of
+// their key (the hash rotated left by 12) in LDS, as k_generate_scatter
+// ranks all children, and written as 32-byte entries into fixed-capacity
+// buckets of cap1 entries.  The buckets are split over up to four
+// allocations (EParts); cnt1[d] counts bucket d's children, also the ones a
+// full bucket drops (then ERR_SPILL1 is raised and the pass is redone with
+// larger buckets).
+// ---------------------------------------------------------------------------
+struct EParts {
+    Entry *p[4];
+    u32 pshift;  // bucket d lives in part d >> pshift
+};
+__device__ inline Entry *ebucket(const EParts &E, u32 d, u64 cap1) {
+    return E.p[d >> E.pshift] + u64(d & ((1u << E.pshift) - 1)) * cap1;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kGenThreads, 3) void k_generate_range_scatter(const PDesc *D, const double2 *pmag,
+                                                                       const u64 *cb, const u64 *wb, u64 N,
+                                                                       const u64 *cpar, const u64 *tab,
+                                                                       const u64 *pbegin, qd_rule rule, u64 W,
+                                                                       u32 rlo, u32 rhi, u32 b1, u64 cap1,
+                                                                       u64 *cnt1, EParts E, u32 *err) {
+    constexpr u32 kKids = 1u << kItemBits;
+    __shared__ u32 bcnt[kFusedBins];
+    __shared__ u64 bbase[kFusedBins];  // first reserved slot of the bucket
+    __shared__ u64 swb[kGenThreads + 1];
+    __shared__ u64 pfirst;
+    const u32 nbin = 1u << b1;
+    const QcgdCoef k = coef_of(rule);
+    bool spill = false;
+    for (u64 w0 = u64(blockIdx.x) * kGenThreads; w0 < W; w0 += u64(gridDim.x) * kGenThreads) {
+        __syncthreads();
+        if (threadIdx.x == 0) pfirst = cpar[w0 / kGenThreads];
+        for (u32 i = threadIdx.x; i < nbin; i += blockDim.x) bcnt[i] = 0;
+        __syncthreads();
+        const u64 pf = pfirst;
+        for (u32 q = threadIdx.x; q <= kGenThreads; q += blockDim.x) swb[q] = pf + q <= N ? wb[pf + q] : ~0ull;
+        __syncthreads();
+        const u64 w = w0 + threadIdx.x;
+        // kept children (bit u), indexed by the unrolled child counter only
+        u64 hs[kKids];
+        double2 ms[kKids];
+        u32 br[kKids];
+        u32 keep = 0, s = 0;
+        u64 cbase = 0;
+        if (w < W) {
+            u64 p;
+            if (swb[kGenThreads] > w) {
+                u32 lo = 0, hi = kGenThreads;
+                while (lo + 1 < hi) {
+                    const u32 mid = (lo + hi) >> 1;
+                    if (swb[mid] <= w) lo = mid;
+                    else hi = mid;
+                }
+                p = pf + lo;
+            } else {
+                p = find_parent(wb, N, w);
+            }
+            const PDesc pd = D[p];
+            const u32 j0 = u32(w - wb[p]);
+            const u32 t = min(u32(pd.k), kItemBits);
+            s = pd.k - t;
+            const double2 m0 = pmag[p];
+            Walk pre{};
+            pre.mag = cplx{m0.x, m0.y};
+            const u64 *ptab = tab + sm_tab_base(pbegin[p]);
+            if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(pre, pd, ptab, j0, s, k);
+            else flip_run<KIND == QD_RULE_QCGD_COIN>(pre, pd, j0, s, k);
+            cbase = cb[p] + j0;
+            walk_item<KIND>(pre, pd, ptab, s, t, k, [&](u32 u, u64 h, const cplx &m) {
+                const u32 top = u32(h >> 52);
+                if (top >= rlo && top < rhi) {
+                    const u64 key = rotl12(h);
+                    const u32 d = b1 ? u32(key >> (64 - b1)) : 0u;
+                    hs[u] = key;
+                    ms[u] = make_double2(m.re, m.im);
+                    br[u] = (d << 16) | atomicAdd(&bcnt[d], 1u);
+                    keep |= 1u << u;
+                }
+            });
+        }
+        __syncthreads();
+        for (u32 i = threadIdx.x; i < nbin; i += blockDim.x)
+            if (bcnt[i])
+                bbase[i] = atomicAdd(reinterpret_cast<unsigned long long *>(cnt1 + i), (unsigned long long)bcnt[i]);
+        __syncthreads();
+#pragma unroll
+        for (u32 u = 0; u < kKids; ++u) {
+            if (keep & (1u << u)) {
+                const u32 d = br[u] >> 16;
+                const u64 at = bbase[d] + (br[u] & 0xFFFF);
+                if (at < cap1) {
+                    Entry e;
+                    e.hash = hs[u];
+                    e.cand = cbase + (u64(u) << s);
+                    e.re = ms[u].x;
+                    e.im = ms[u].y;
+                    ebucket(E, d, cap1)[at] = e;
+                } else {
+                    spill = true;
+                }
+            }
+        }
+    }
+    if (spill) atomicOr(err, ERR_SPILL1);
+}
+
 }  // namespace qd

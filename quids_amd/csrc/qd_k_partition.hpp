@@ -11,6 +11,7 @@ namespace qd {
 // phase 4: radix partition
 // ===========================================================================
 constexpr int kTile1 = 4096;  // entries per workgroup in the first pass
+constexpr int kLevel2Bins = 8192;  // up to 13 bits in the second pass
 
 // First pass: from candidate order (index = candidate, hash and magnitude
 // as written by generate) to 2^b1 buckets at exact offsets.
@@ -63,13 +64,14 @@ constexpr u32 ERR_SPILL = 16u;
 
 // Input bucket b holds off1[b+1] - off1[b] entries, read from in at
 // b * in_stride (fixed-capacity first pass) or at off1[b] (in_stride 0).
-// Workgroup g takes buckets g, g + gridDim.x, ... of the 2^b1.
-__global__ __launch_bounds__(1024) void k_level2_fixed(const Entry *in, u64 in_stride, const u64 *off1, u32 b1, u32 b2,
-                                                       u64 cap2, Entry *out, u64 *cnt2, u32 *err) {
-    __shared__ u32 ccnt[kHistBins];
-    __shared__ u32 cur[kHistBins];
+// Workgroup g takes buckets g, g + gridDim.x, ... of the nbk1 (2^b1, or a
+// group of them: the caller offsets in and off1 to the group's first).
+__global__ __launch_bounds__(1024) void k_level2_fixed(const Entry *in, u64 in_stride, const u64 *off1, u64 nbk1,
+                                                       u32 b1, u32 b2, u64 cap2, Entry *out, u64 *cnt2, u32 *err) {
+    __shared__ u32 ccnt[kLevel2Bins];
+    __shared__ u32 cur[kLevel2Bins];
     bool spill = false;
-    for (u64 b = blockIdx.x; b < (1ull << b1); b += gridDim.x) {
+    for (u64 b = blockIdx.x; b < nbk1; b += gridDim.x) {
         const u64 beg = off1[b], end = off1[b + 1];
         const u64 ibase = in_stride ? b * in_stride : beg;  // entry i of [beg, end) is in[ibase + i - beg]
         const u32 nb2 = 1u << b2;
@@ -160,12 +162,14 @@ __global__ __launch_bounds__(1024) void k_level2_fixed(const Entry *in, u64 in_s
 }
 
 // The same split with exact offsets (a counting pass first): used when a
-// sub-bucket spills its fixed capacity.
-__global__ __launch_bounds__(1024) void k_level2_exact(const Entry *in, u64 in_stride, const u64 *off1, u32 b1, u32 b2,
-                                                       Entry *out, u64 *off2) {
-    __shared__ u32 cnt[kHistBins];
-    __shared__ u32 ccnt[kHistBins];
-    __shared__ u64 cur[kHistBins];
+// sub-bucket spills its fixed capacity.  One workgroup per input bucket;
+// positions in out and off2 are off1's minus obase (the offset of the
+// group's first entry).
+__global__ __launch_bounds__(1024) void k_level2_exact(const Entry *in, u64 in_stride, const u64 *off1, u64 obase,
+                                                       u32 b1, u32 b2, Entry *out, u64 *off2) {
+    __shared__ u32 cnt[kLevel2Bins];
+    __shared__ u32 ccnt[kLevel2Bins];
+    __shared__ u64 cur[kLevel2Bins];
     __shared__ u64 tmp[1024 / kWave + 1];
     const u64 b = blockIdx.x;
     const u64 beg = off1[b], end = off1[b + 1];
@@ -177,20 +181,21 @@ __global__ __launch_bounds__(1024) void k_level2_exact(const Entry *in, u64 in_s
     for (u64 i = beg + threadIdx.x; i < end; i += blockDim.x) atomicAdd(&cnt[u32(in[ibase + (i - beg)].hash >> shift) & (nb2 - 1)], 1u);
     __syncthreads();
     {
-        u32 v[4];
+        constexpr int kPer = kLevel2Bins / 1024;
+        u32 v[kPer];
         u64 s = 0;
-        for (int k = 0; k < 4; ++k) {
-            const u32 d = threadIdx.x * 4 + k;
+        for (int k = 0; k < kPer; ++k) {
+            const u32 d = threadIdx.x * kPer + k;
             v[k] = d < nb2 ? cnt[d] : 0;
             s += v[k];
         }
         u64 total;
         u64 run = block_excl_scan(s, tmp, total);
-        for (int k = 0; k < 4; ++k) {
-            const u32 d = threadIdx.x * 4 + k;
+        for (int k = 0; k < kPer; ++k) {
+            const u32 d = threadIdx.x * kPer + k;
             if (d < nb2) {
-                cur[d] = beg + run;
-                off2[(b << b2) + d] = beg + run;
+                cur[d] = beg - obase + run;
+                off2[(b << b2) + d] = beg - obase + run;
             }
             run += v[k];
         }

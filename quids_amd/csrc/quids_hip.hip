@@ -166,6 +166,7 @@ struct qd_ctx {
     DBuf adjk, adjv, adjg, adjd, adjp, adjw;
     DBuf pool0, pool1, pwpre, blist2;
     DBuf mw;  // per-object memory of memory-driven truncation
+    DBuf rpart[4], pk0, pk1;  // hash-range passes: first-pass buckets, pool select keys
     // output buffers of batched chunks (objects, begin, size, mag per chunk),
     // kept between calls
     std::vector<DBuf> parts;
@@ -180,7 +181,8 @@ struct qd_ctx {
                 &nhist, &counters, &sel0, &sel1, &bitmap, &wprefix, &padded, &chunks, &scanblk,
                 &biglist, &redolist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &cpar, &opar, &ebuf2, &cnt2, &slist, &ukey, &selloop, &segloop, &bsoff, &bscnt,
                 &adjk, &adjv, &adjg, &adjd, &adjp, &adjw, &bseg, &boseg, &brules, &bsinst, &bcount, &bsel, &btrunc,
-                &blist, &bcur, &btile, &bhist, &bkseg, &bchunk, &bcsum, &btot, &pool0, &pool1, &pwpre, &blist2, &mw};
+                &blist, &bcur, &btile, &bhist, &bkseg, &bchunk, &bcsum, &btot, &pool0, &pool1, &pwpre, &blist2, &mw, &rpart[0], &rpart[1], &rpart[2],
+                &rpart[3], &pk0, &pk1};
     }
 };
 
@@ -429,8 +431,9 @@ static int select_loop(qd_ctx *ctx, SelItem *S0, SelItem *S1, u64 ns, u64 r, u32
 // |m|^2 bits over the slots (it is overwritten).  UK, when not null, holds
 // the top 32 key bits of every slot (0 when empty; plain |m|^2 keys): the
 // first two digits then read 4 bytes per slot instead of its whole record.
+// cut24, when not null, receives the top 24 key bits of the r-th largest key.
 static int select_topk(qd_ctx *ctx, const u64 *UH, const u64 *UR, const double2 *UM, const u32 *UK, u64 nslots,
-                       u64 r, std::vector<u64> &h, u64 *bitmap, u64 *sbm, const SelKey &sk) {
+                       u64 r, std::vector<u64> &h, u64 *bitmap, u64 *sbm, const SelKey &sk, u32 *cut24 = nullptr) {
     hipStream_t s = ctx->stream;
     u64 *hist1 = ctx->hist1.as<u64>();
     u64 *counters = ctx->counters.as<u64>();
@@ -472,6 +475,7 @@ static int select_topk(qd_ctx *ctx, const u64 *UH, const u64 *UR, const double2 
         hipLaunchKernelGGL(k_sel_first, dim3(grid_for(nslots, 256, 16384)), dim3(256), 0, s, UH, UR, UM, nslots,
                            (d1 << 12) | u32(d), S0, counters + 1, bitmap, sbm, sk);
     QD_HIP(hipGetLastError());
+    if (cut24) *cut24 = (d1 << 12) | u32(d);
     return select_loop(ctx, S0, S1, ns, r, 24, bitmap, UR, sbm);
 }
 
@@ -556,6 +560,69 @@ static int ensure_big(DBuf &b, u64 bytes, u64 full) {
     return b.ensure(bytes);
 }
 
+// The dedup of nbk buckets (D, rstride, woff as in k_dedup) holding n
+// entries, with key bits L above the buckets' LDS home slots and tables of
+// dcap entries: the uniques go to the n slots UH / UR / UM (and UK), U
+// counts them.  Counters 0 and 4..6 of the context must be zero.
+static int dedup_phase(qd_ctx *ctx, const Entry *D, u64 rstride, const u64 *woff, u64 nbk, u32 L, double tol, u32 dcap,
+                       u64 n, u64 *UH, u64 *UR, double2 *UM, u32 *UK, u64 &U) {
+    hipStream_t s = ctx->stream;
+    u64 *counters = ctx->counters.as<u64>();
+    u64 *hb = ctx->host;
+    prof_begin(ctx, QD_PHASE_DEDUP);
+    QD_TRY(ctx->biglist.ensure((n / dcap + 2) * sizeof(u64)));
+    QD_TRY(ctx->redolist.ensure((nbk + 1) * sizeof(u64)));
+    int ncu = 256;
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    // persistent grids of as many blocks as stay resident (LDS bound)
+    auto resident = [&](auto kern, int threads, u64 nwork) -> unsigned {
+        int per = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, threads, 0) != hipSuccess || per < 1) {
+            (void)hipGetLastError();
+            per = 1;
+        }
+        return unsigned(std::max<u64>(std::min<u64>(nwork, u64(ncu) * u64(per)), 1));
+    };
+    auto run_dedup = [&](auto kern, int threads) -> int {
+        hipLaunchKernelGGL(kern, dim3(resident(kern, threads, nbk)), dim3(threads), 0, s, D, rstride, woff, nbk, L,
+                           tol, UH, UR, UM, UK, counters, ctx->nhist.as<u64>(), ctx->biglist.as<u64>(),
+                           ctx->redolist.as<u64>());
+        QD_HIP(hipGetLastError());
+        return QD_OK;
+    };
+    // buckets holding a group too large for registers: the sort-based kernel
+    auto run_sort = [&](auto kern, int threads, u64 nred) -> int {
+        hipLaunchKernelGGL(kern, dim3(resident(kern, threads, nred)), dim3(threads), 0, s, D, rstride, woff, nred, L,
+                           tol, UH, UR, UM, UK, counters, ctx->nhist.as<u64>(), ctx->biglist.as<u64>(),
+                           (const u64 *)ctx->redolist.as<u64>());
+        QD_HIP(hipGetLastError());
+        return QD_OK;
+    };
+    if (dcap == 256) QD_TRY(run_dedup(k_dedup<128, 256, 512>, 128));
+    else if (dcap == 512) QD_TRY(run_dedup(k_dedup<256, 512, 1024>, 256));
+    else if (dcap == 1024) QD_TRY(run_dedup(k_dedup<512, 1024, 2048>, 512));
+    else QD_TRY(run_dedup(k_dedup<1024, 2048, 4096>, 1024));
+    prof_end(ctx, QD_PHASE_DEDUP);
+    QD_HIP(hipMemcpyAsync(hb, counters, 7 * sizeof(u64), hipMemcpyDeviceToHost, s));
+    QD_HIP(hipStreamSynchronize(s));
+    prof_collect(ctx);
+    if (hb[6]) {
+        prof_begin(ctx, QD_PHASE_DEDUP);
+        if (dcap == 256) QD_TRY(run_sort(k_dedup_sort<128, 256, 512>, 128, hb[6]));
+        else if (dcap == 512) QD_TRY(run_sort(k_dedup_sort<256, 512, 1024>, 256, hb[6]));
+        else if (dcap == 1024) QD_TRY(run_sort(k_dedup_sort<512, 1024, 2048>, 512, hb[6]));
+        else QD_TRY(run_sort(k_dedup_sort<1024, 2048, 4096>, 1024, hb[6]));
+        prof_end(ctx, QD_PHASE_DEDUP);
+        QD_HIP(hipMemcpyAsync(hb, counters, 7 * sizeof(u64), hipMemcpyDeviceToHost, s));
+        QD_HIP(hipStreamSynchronize(s));
+        prof_collect(ctx);
+    }
+    QD_TRY(check_err_word(u32(hb[4])));
+    U = hb[0];
+    if (hb[5]) QD_TRY(dedup_big_buckets(ctx, D, rstride, woff, hb[5], tol, UH, UR, UM, UK, &U));
+    return QD_OK;
+}
+
 // Partition (one or two radix passes on the top key bits) and dedup of n
 // candidates with keys H0 and magnitudes M0 (CI: candidate index of each
 // position, null when the position is the candidate index).  hist1 holds the
@@ -627,7 +694,7 @@ static int partition_dedup(qd_ctx *ctx, const u64 *H0, const double2 *M0, const 
             (void)hipDeviceGetAttribute(&ncu2, hipDeviceAttributeMultiprocessorCount, ctx->device);
             const unsigned g2 = unsigned(std::min<u64>(nb1, std::max(ncu2 / 2, 1)));
             hipLaunchKernelGGL(k_level2_fixed, dim3(g2), dim3(1024), 0, s, (const Entry *)E1, in_stride,
-                               ctx->off1.as<u64>(), b1, b2, cap2, E2, ctx->cnt2.as<u64>(), errw);
+                               ctx->off1.as<u64>(), nb1, b1, b2, cap2, E2, ctx->cnt2.as<u64>(), errw);
             QD_HIP(hipGetLastError());
             QD_TRY(run_scan(ctx, LoadU64{ctx->cnt2.as<u64>()}, nb, ctx->off2.as<u64>()));
             QD_TRY(sync_read(ctx, errw, sizeof(u32), &ew));
@@ -639,7 +706,7 @@ static int partition_dedup(qd_ctx *ctx, const u64 *H0, const double2 *M0, const 
             QD_HIP(hipMemcpyAsync(ctx->off2.as<u64>() + nb, ctx->host + kHostN, sizeof(u64), hipMemcpyHostToDevice,
                                   s));
             hipLaunchKernelGGL(k_level2_exact, dim3(nb1), dim3(1024), 0, s, (const Entry *)E1, in_stride,
-                               ctx->off1.as<u64>(), b1, b2, E2, ctx->off2.as<u64>());
+                               ctx->off1.as<u64>(), u64(0), b1, b2, E2, ctx->off2.as<u64>());
             QD_HIP(hipGetLastError());
             rstride = 0;
             if (fixed) st.reserved |= 1;  // spill fallback used
@@ -655,59 +722,7 @@ static int partition_dedup(qd_ctx *ctx, const u64 *H0, const double2 *M0, const 
     UH = static_cast<u64 *>(ubuf);
     UR = UH + n;
     UM = reinterpret_cast<double2 *>(UR + n);
-    prof_begin(ctx, QD_PHASE_DEDUP);
-    const u32 dcap = ctx->tune.dedup_cap;
-    QD_TRY(ctx->biglist.ensure((n / dcap + 2) * sizeof(u64)));
-    QD_TRY(ctx->redolist.ensure((nbk + 1) * sizeof(u64)));
-    int ncu = 256;
-    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device);
-    // persistent grids of as many blocks as stay resident (LDS bound)
-    auto resident = [&](auto kern, int threads, u64 nwork) -> unsigned {
-        int per = 1;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, threads, 0) != hipSuccess || per < 1) {
-            (void)hipGetLastError();
-            per = 1;
-        }
-        return unsigned(std::max<u64>(std::min<u64>(nwork, u64(ncu) * u64(per)), 1));
-    };
-    auto run_dedup = [&](auto kern, int threads) -> int {
-        hipLaunchKernelGGL(kern, dim3(resident(kern, threads, nbk)), dim3(threads), 0, s, D, rstride, woff, nbk, L,
-                           tol, UH, UR, UM, UK, counters, ctx->nhist.as<u64>(), ctx->biglist.as<u64>(),
-                           ctx->redolist.as<u64>());
-        QD_HIP(hipGetLastError());
-        return QD_OK;
-    };
-    // buckets holding a group too large for registers: the sort-based kernel
-    auto run_sort = [&](auto kern, int threads, u64 nred) -> int {
-        hipLaunchKernelGGL(kern, dim3(resident(kern, threads, nred)), dim3(threads), 0, s, D, rstride, woff, nred, L,
-                           tol, UH, UR, UM, UK, counters, ctx->nhist.as<u64>(), ctx->biglist.as<u64>(),
-                           (const u64 *)ctx->redolist.as<u64>());
-        QD_HIP(hipGetLastError());
-        return QD_OK;
-    };
-    if (dcap == 256) QD_TRY(run_dedup(k_dedup<128, 256, 512>, 128));
-    else if (dcap == 512) QD_TRY(run_dedup(k_dedup<256, 512, 1024>, 256));
-    else if (dcap == 1024) QD_TRY(run_dedup(k_dedup<512, 1024, 2048>, 512));
-    else QD_TRY(run_dedup(k_dedup<1024, 2048, 4096>, 1024));
-    prof_end(ctx, QD_PHASE_DEDUP);
-    QD_HIP(hipMemcpyAsync(hb, counters, 7 * sizeof(u64), hipMemcpyDeviceToHost, s));
-    QD_HIP(hipStreamSynchronize(s));
-    prof_collect(ctx);
-    if (hb[6]) {
-        prof_begin(ctx, QD_PHASE_DEDUP);
-        if (dcap == 256) QD_TRY(run_sort(k_dedup_sort<128, 256, 512>, 128, hb[6]));
-        else if (dcap == 512) QD_TRY(run_sort(k_dedup_sort<256, 512, 1024>, 256, hb[6]));
-        else if (dcap == 1024) QD_TRY(run_sort(k_dedup_sort<512, 1024, 2048>, 512, hb[6]));
-        else QD_TRY(run_sort(k_dedup_sort<1024, 2048, 4096>, 1024, hb[6]));
-        prof_end(ctx, QD_PHASE_DEDUP);
-        QD_HIP(hipMemcpyAsync(hb, counters, 7 * sizeof(u64), hipMemcpyDeviceToHost, s));
-        QD_HIP(hipStreamSynchronize(s));
-        prof_collect(ctx);
-    }
-    QD_TRY(check_err_word(u32(hb[4])));
-    U = hb[0];
-    if (hb[5]) QD_TRY(dedup_big_buckets(ctx, D, rstride, woff, hb[5], tol, UH, UR, UM, UK, &U));
-    return QD_OK;
+    return dedup_phase(ctx, D, rstride, woff, nbk, L, tol, ctx->tune.dedup_cap, n, UH, UR, UM, UK, U);
 }
 
 // Regenerates the K survivors (candidate indices orep, ascending; their
@@ -969,21 +984,46 @@ static int mem_survivor_cap(qd_ctx *ctx, const Parents &P, const qd_rule &rule, 
 // hash-range passes (qd_k_range.hpp): steps whose candidates exceed the
 // scratch memory
 // ===========================================================================
+// dst[i] = src[i] + add (mod 2^64) for i < n.
+__global__ __launch_bounds__(256) void k_add_offset(const u64 *src, u64 n, u64 add, u64 *dst) {
+    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
+        dst[i] = src[i] + add;
+}
+
+// Per-candidate bytes of a hash-range pass: the first radix pass's entry
+// (32 bytes plus its buckets' slack), and a quarter (one group of
+// first-level buckets at a time) of the second pass's entry at its
+// sub-buckets' slack and of the group's 32-bit select keys and pool
+// appends.
+constexpr u32 kRangeGroups = 4;
+constexpr double kRangeBytesPerCand = 32.6 + (32.0 * 1.35 + 4.0 + 36.0) / kRangeGroups;
+
 template <int KIND>
-static int range_step(qd_ctx *ctx, const Parents &P, const qd_rule &rule, u64 C, u64 nitems, u64 max_obj, u64 budget,
+static int range_step(qd_ctx *ctx, const Parents &P, const qd_rule &rule, u64 C, u64 nitems, u64 max_obj,
                       const qd_params &prm, qd_state *out, qd_symbolic *sy, qd_step_stats &st) {
     hipStream_t s = ctx->stream;
     const u64 N = P.n;
     const SelKey sk{prm.simple_truncation ? 0u : 1u, 0u, prm.seed};
     u64 *counters = ctx->counters.as<u64>();
+    u32 *errw = reinterpret_cast<u32 *>(counters + 4);
     u64 *hb = ctx->host;
+    // the single pass's entry buffers are not used here: give their memory
+    // to the passes
+    ctx->ebuf0.release();
+    ctx->ebuf1.release();
+    ctx->ukey.release();
+    if (KIND != QD_RULE_QCGD_SPLIT_MERGE) ctx->tab.release();  // split_merge's name-root table
     QD_TRY(ctx->hist1.ensure(kHistBins * sizeof(u64)));
-    QD_TRY(ctx->off1.ensure((kHistBins + 1) * sizeof(u64)));
+    QD_TRY(ctx->off1.ensure((kFusedBins + 1) * sizeof(u64)));
     QD_TRY(ctx->cur1.ensure(kHistBins * sizeof(u64)));
     QD_TRY(ctx->nhist.ensure(kHistBins * sizeof(u64)));
     u64 *hist1 = ctx->hist1.as<u64>();
     const unsigned gg = unsigned(std::max<u64>(
         std::min<u64>((nitems + kGenThreads - 1) / kGenThreads, ctx->tune.generate_grid), 1));
+    const bool capped = max_obj < C;
+    const bool ukeys = !sk.prob;  // 32-bit |m|^2 select keys (plain truncation keys only)
+    const u64 W = (C + 63) / 64;
+
     // 1. how the candidates spread over the 4096 values of the top 12 hash bits
     std::vector<u64> hc(kHistBins);
     QD_HIP(hipMemsetAsync(hist1, 0, kHistBins * sizeof(u64), s));
@@ -994,11 +1034,34 @@ static int range_step(qd_ctx *ctx, const Parents &P, const qd_rule &rule, u64 C,
     QD_HIP(hipGetLastError());
     prof_end(ctx, QD_PHASE_GENERATE);
     QD_TRY(sync_read(ctx, hist1, kHistBins * sizeof(u64), hc.data()));
-    // 2. contiguous ranges of bins within the budget
+
+    // 2. the pass size: what the free memory (and the buffers this path
+    // reuses) holds after the bitmaps and the pool, within the second radix
+    // pass's reach (2^22 buckets of at most 1024 entries) and the four
+    // first-pass allocations; then contiguous bin ranges within it
+    u64 budget;
+    if (ctx->tune.chunk_log2) {
+        budget = 1ull << ctx->tune.chunk_log2;
+    } else {
+        size_t freeb = 0, totalb = 0;
+        QD_HIP(hipMemGetInfo(&freeb, &totalb));
+        u64 reuse = ctx->ebuf2.cap + ctx->pool0.cap + ctx->pool1.cap + ctx->pk0.cap + ctx->pk1.cap + ctx->bitmap.cap +
+                    ctx->wprefix.cap;
+        for (DBuf &b : ctx->rpart) reuse += b.cap;
+        const double fixed = 16.0 * double(W) + (capped ? 2.0 * 36.0 * double(max_obj) : 36.0 * double(C)) + double(2ull << 30);
+        const double room = 0.85 * (double(freeb) + double(reuse)) - fixed;
+        budget = room > 0 ? u64(room / kRangeBytesPerCand) : 0;
+        budget = std::min<u64>(budget, u64(4.0 * double(kBigAlloc) / 33.0));
+    }
+    budget = std::max<u64>(std::min<u64>(budget, 1ull << 32), 1ull << 10);
     std::vector<u32> cut{0};
     std::vector<u64> rn;
     u64 acc = 0, nmax = 0;
     for (u32 d = 0; d < u32(kHistBins); ++d) {
+        if (hc[d] > budget)
+            return fail(QD_ERR_CAPACITY, "hash-range pass: " + std::to_string(hc[d]) +
+                                             " candidates share 12 leading hash bits, beyond the pass budget of " +
+                                             std::to_string(budget));
         if (acc && acc + hc[d] > budget) {
             cut.push_back(d);
             rn.push_back(acc);
@@ -1010,82 +1073,203 @@ static int range_step(qd_ctx *ctx, const Parents &P, const qd_rule &rule, u64 C,
     cut.push_back(kHistBins);
     rn.push_back(acc);
     nmax = std::max(nmax, acc);
-    // 3. scratch of one range, the pool and the candidate bitmap
-    const u64 pcap = std::max<u64>(max_obj == UINT64_MAX ? C : std::min<u64>(C, max_obj + nmax), 1);
-    QD_TRY(ctx->ebuf0.ensure(nmax * 32));
-    QD_TRY(ctx->ebuf1.ensure(nmax * 32));
+
+    // 3. layout of a pass of nmax candidates: b1 first-pass bits (at most 9,
+    // fused), b2 second-pass bits (at most 13), buckets of at most 1024
+    // entries on average
+    u32 tgt = ctx->tune.dedup_bucket_target, dcap = ctx->tune.dedup_cap;
+    while ((nmax >> 22) > tgt && tgt < 1024) tgt *= 2;
+    dcap = std::max(dcap, 2 * tgt);
+    u32 L = 0;
+    while ((nmax >> L) > tgt && L < 22) ++L;
+    const u32 b1 = std::min<u32>(L, 9), b2 = L - b1;
+    const u64 nb1 = 1ull << b1;
+    const u32 nparts = u32(std::min<u64>(nb1, 4));
+    const u32 pshift = b1 - (nparts == 4 ? 2 : nparts == 2 ? 1 : 0);
+    const u64 perpart = nb1 / nparts;
+    const double mean1 = double(nmax) / double(nb1);
+    u64 cap1 = u64(mean1 + 8.0 * std::sqrt(mean1)) + 4096;
+    const u32 ngroups = u32(std::min<u64>(nb1, kRangeGroups));
+    const u64 gbk = nb1 / ngroups;  // first-level buckets per group (a group lies in one part)
+    const u64 gcap = gbk * cap1;    // entries of a group's first-pass region
+    const u64 nb2g = gbk << b2;     // sub-buckets of a group
+    const double mean2 = double(gcap) / double(std::max<u64>(nb2g, 1));
+    const u64 cap2 = (u64(mean2 + 6.0 * std::sqrt(mean2)) + 128 + 63) / 64 * 64;
+    const u64 pcap = std::max<u64>(capped ? max_obj + gcap : C, 1);
+    auto alloc_parts = [&](u64 c1) -> int {
+        for (u32 q = 0; q < 4; ++q) {
+            if (q < nparts) QD_TRY(ctx->rpart[q].ensure(perpart * c1 * sizeof(Entry)));
+            else ctx->rpart[q].release();
+        }
+        return QD_OK;
+    };
+    QD_TRY(alloc_parts(cap1));
+    QD_TRY(ctx->ebuf2.ensure(std::max<u64>(nb2g * cap2, gcap) * sizeof(Entry)));
+    QD_TRY(ctx->cnt2.ensure(nb2g * sizeof(u64)));
+    QD_TRY(ctx->off2.ensure((nb2g + 1) * sizeof(u64)));
+    QD_TRY(ctx->ukey.ensure(gcap * sizeof(u32)));
     QD_TRY(ctx->pool0.ensure(pcap * 32));
-    QD_TRY(ctx->pool1.ensure(pcap * 32));
-    const u64 W = (C + 63) / 64;
+    QD_TRY(ctx->pool1.ensure((capped ? max_obj + 1 : 1) * 32));
+    if (ukeys) {
+        QD_TRY(ctx->pk0.ensure(pcap * sizeof(u32)));
+        QD_TRY(ctx->pk1.ensure((capped ? max_obj + 1 : 1) * sizeof(u32)));
+    }
     QD_TRY(ctx->bitmap.ensure(W * sizeof(u64)));
     QD_TRY(ctx->wprefix.ensure((W + 1) * sizeof(u64)));
     QD_TRY(ctx->slist.ensure(((pcap + 63) / 64) * sizeof(u64)));
     QD_TRY(ctx->pwpre.ensure(((pcap + 63) / 64 + 1) * sizeof(u64)));
-    auto pool = [&](DBuf &b, u64 *&h, u64 *&r, double2 *&m) {
-        h = b.as<u64>();
-        r = h + pcap;
-        m = reinterpret_cast<double2 *>(r + pcap);
-    };
-    u64 *Ph, *Pr, *Qh, *Qr;
-    double2 *Pm, *Qm;
-    pool(ctx->pool0, Ph, Pr, Pm);
-    pool(ctx->pool1, Qh, Qr, Qm);
+    // pool: hash, representative, magnitude (and 32-bit key) per unique
+    u64 *Ph = ctx->pool0.as<u64>(), *Pr = Ph + pcap;
+    double2 *Pm = reinterpret_cast<double2 *>(Pr + pcap);
+    const u64 qcap = capped ? max_obj + 1 : 1;
+    u64 *Qh = ctx->pool1.as<u64>(), *Qr = Qh + qcap;
+    double2 *Qm = reinterpret_cast<double2 *>(Qr + qcap);
+    u32 *Pk = ukeys ? ctx->pk0.as<u32>() : nullptr, *Qk = ukeys ? ctx->pk1.as<u32>() : nullptr;
     u64 ns = 0, Utot = 0;
-    std::vector<u64> h(kHistBins);
-    for (size_t r = 0; r + 1 < cut.size(); ++r) {
-        const u64 n = rn[r];
-        if (n == 0) continue;
-        u32 b1, b2;
-        dedup_layout(n, ctx->tune, b1, b2);
-        st.dedup_levels = (b1 ? 1 : 0) + (b2 ? 1 : 0);
-        QD_HIP(hipMemsetAsync(counters, 0, 16 * sizeof(u64), s));
-        QD_HIP(hipMemsetAsync(hist1, 0, kHistBins * sizeof(u64), s));
-        QD_HIP(hipMemsetAsync(ctx->nhist.p, 0, kHistBins * sizeof(u64), s));
-        u64 *HK = ctx->ebuf0.as<u64>();
-        double2 *MK = reinterpret_cast<double2 *>(HK + n);
-        u64 *CK = reinterpret_cast<u64 *>(MK + n);
-        prof_begin(ctx, QD_PHASE_GENERATE);
-        hipLaunchKernelGGL((k_generate_range<KIND, false>), dim3(gg), dim3(kGenThreads), 0, s, ctx->pdesc.as<PDesc>(),
-                           P.mag, ctx->cb.as<u64>(), ctx->wb.as<u64>(), N, ctx->cpar.as<u64>(), ctx->tab.as<u64>(), P.begin, rule, nitems,
-                           cut[r], cut[r + 1], HK, MK, CK, counters + 7, b1, hist1);
-        QD_HIP(hipGetLastError());
-        prof_end(ctx, QD_PHASE_GENERATE);
-        u64 *UH, *UR;
-        double2 *UM;
-        u64 U;
-        QD_TRY(partition_dedup(ctx, HK, MK, CK, n, b1, b2, prm.tolerance, st, UH, UR, UM, U));
-        u64 kept = 0;
-        QD_TRY(sync_read(ctx, counters + 7, sizeof(u64), &kept));
-        if (kept != n) return fail(QD_ERR_HIP, "hash-range pass kept a different candidate count than it counted");
-        Utot += U;
-        if (!U) continue;
-        // append the range's uniques to the pool
-        hb[kHostN - 1] = ns;
-        QD_HIP(hipMemcpyAsync(counters + 8, hb + kHostN - 1, sizeof(u64), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_pool_append, dim3(unsigned(std::min<u64>((n + 256 * kPoolPer - 1) / (256 * kPoolPer), 16384))),
-                           dim3(256), 0, s, UH, UR, UM, n, Ph, Pr, Pm, counters + 8);
-        QD_HIP(hipGetLastError());
-        ns += U;
-        if (ns <= max_obj) continue;
-        // keep the best max_obj of the pool
+    u32 t24 = 0;  // uniques below this 24-bit |m|^2 prefix cannot enter the pool's top max_obj
+    u64 *pcount = counters + 8;
+
+    // keeps the best max_obj of the pool (ns entries) in the pool
+    auto trim_pool = [&]() -> int {
         prof_begin(ctx, QD_PHASE_SELECT);
-        QD_HIP(hipMemsetAsync(hist1, 0, kHistBins * sizeof(u64), s));
-        hipLaunchKernelGGL(k_key_hist0, dim3(grid_for(ns, 256, 4096)), dim3(256), 0, s, (const u64 *)Ph, (const u64 *)Pr,
-                           (const double2 *)Pm, ns, hist1, sk);
+        QD_HIP(hipMemsetAsync(ctx->nhist.p, 0, kHistBins * sizeof(u64), s));
+        if (ukeys)
+            hipLaunchKernelGGL(k_ukey_hist0, dim3(grid_for(ns, 256, 4096)), dim3(256), 0, s, (const u32 *)Pk, ns,
+                               ctx->nhist.as<u64>());
+        else
+            hipLaunchKernelGGL(k_key_hist0, dim3(grid_for(ns, 256, 4096)), dim3(256), 0, s, (const u64 *)Ph,
+                               (const u64 *)Pr, (const double2 *)Pm, ns, ctx->nhist.as<u64>(), sk);
         QD_HIP(hipGetLastError());
-        QD_TRY(sync_read(ctx, hist1, kHistBins * sizeof(u64), h.data()));
+        std::vector<u64> h(kHistBins);
+        QD_TRY(sync_read(ctx, ctx->nhist.p, kHistBins * sizeof(u64), h.data()));
         u64 *keep = ctx->slist.as<u64>();
-        QD_TRY(select_topk(ctx, Ph, Pr, Pm, nullptr, ns, max_obj, h, ctx->bitmap.as<u64>(), keep, sk));
+        u32 c24 = 0;
+        QD_TRY(select_topk(ctx, Ph, Pr, Pm, Pk, ns, max_obj, h, ctx->bitmap.as<u64>(), keep, sk, &c24));
         const u64 Wp = (ns + 63) / 64;
         QD_TRY(run_scan(ctx, LoadPopc{keep}, Wp, ctx->pwpre.as<u64>()));
         hipLaunchKernelGGL(k_pool_compact, dim3(grid_for(ns)), dim3(256), 0, s, (const u64 *)Ph, (const u64 *)Pr,
-                           (const double2 *)Pm, ns, (const u64 *)keep, (const u64 *)ctx->pwpre.as<u64>(), Qh, Qr, Qm);
-        QD_HIP(hipGetLastError());
+                           (const double2 *)Pm, (const u32 *)Pk, ns, (const u64 *)keep, (const u64 *)ctx->pwpre.as<u64>(),
+                           Qh, Qr, Qm, Qk);
+        // back to the pool's first max_obj places
+        QD_HIP(hipMemcpyAsync(Ph, Qh, max_obj * sizeof(u64), hipMemcpyDeviceToDevice, s));
+        QD_HIP(hipMemcpyAsync(Pr, Qr, max_obj * sizeof(u64), hipMemcpyDeviceToDevice, s));
+        QD_HIP(hipMemcpyAsync(Pm, Qm, max_obj * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        if (ukeys) QD_HIP(hipMemcpyAsync(Pk, Qk, max_obj * sizeof(u32), hipMemcpyDeviceToDevice, s));
         prof_end(ctx, QD_PHASE_SELECT);
-        std::swap(Ph, Qh);
-        std::swap(Pr, Qr);
-        std::swap(Pm, Qm);
         ns = max_obj;
+        if (ukeys && !sk.prob) t24 = std::max(t24, c24);
+        return QD_OK;
+    };
+
+    std::vector<u64> c1(nb1);
+    for (size_t r = 0; r + 1 < cut.size(); ++r) {
+        const u64 n = rn[r];
+        if (n == 0) continue;
+        // a. the range's children into the first pass's buckets
+        u64 cap1r = cap1;
+        for (int attempt = 0;; ++attempt) {
+            EParts E{};
+            for (u32 q = 0; q < 4; ++q) E.p[q] = q < nparts ? ctx->rpart[q].as<Entry>() : nullptr;
+            E.pshift = pshift;
+            QD_HIP(hipMemsetAsync(ctx->cur1.p, 0, nb1 * sizeof(u64), s));
+            QD_HIP(hipMemsetAsync(errw, 0, sizeof(u32), s));
+            prof_begin(ctx, QD_PHASE_GENERATE);
+            hipLaunchKernelGGL(k_generate_range_scatter<KIND>, dim3(gg), dim3(kGenThreads), 0, s,
+                               ctx->pdesc.as<PDesc>(), P.mag, ctx->cb.as<u64>(), ctx->wb.as<u64>(), N,
+                               ctx->cpar.as<u64>(), ctx->tab.as<u64>(), P.begin, rule, nitems, cut[r], cut[r + 1], b1,
+                               cap1r, ctx->cur1.as<u64>(), E, errw);
+            QD_HIP(hipGetLastError());
+            prof_end(ctx, QD_PHASE_GENERATE);
+            u32 ew = 0;
+            QD_TRY(sync_read(ctx, errw, sizeof(u32), &ew));
+            QD_TRY(sync_read(ctx, ctx->cur1.p, nb1 * sizeof(u64), c1.data()));
+            u64 kept = 0, cmax = 0;
+            for (u64 d = 0; d < nb1; ++d) {
+                kept += c1[d];
+                cmax = std::max(cmax, c1[d]);
+            }
+            if (kept != n) return fail(QD_ERR_HIP, "hash-range pass kept a different candidate count than it counted");
+            if (!(ew & ERR_SPILL1)) break;
+            // a bucket passed its capacity: redo the range with buckets as
+            // large as the largest count, if memory allows
+            st.reserved |= 8;
+            if (attempt) return fail(QD_ERR_HIP, "hash-range pass: bucket overflow persisted");
+            cap1r = (cmax + 63) / 64 * 64;
+            if (cap1r * nb1 > 2 * cap1 * nb1)
+                return fail(QD_ERR_CAPACITY, "hash-range pass: hash keys too skewed for the first radix pass");
+            QD_TRY(alloc_parts(cap1r));
+        }
+        QD_TRY(run_scan(ctx, LoadU64{ctx->cur1.as<u64>()}, nb1, ctx->off1.as<u64>()));
+        std::vector<u64> o1(nb1 + 1, 0);
+        for (u64 d = 0; d < nb1; ++d) o1[d + 1] = o1[d] + c1[d];
+        for (u32 g = 0; g < ngroups; ++g)  // the buffers of a group were sized for gcap entries
+            if (o1[u64(g + 1) * gbk] - o1[u64(g) * gbk] > gcap)
+                return fail(QD_ERR_CAPACITY, "hash-range pass: hash keys too skewed for the first radix pass");
+        st.dedup_levels = (b1 ? 1 : 0) + (b2 ? 1 : 0);
+        // b. per group of first-level buckets: second pass, dedup, pool
+        for (u32 g = 0; g < ngroups; ++g) {
+            const u64 g0 = u64(g) * gbk;
+            const u64 ng = o1[g0 + gbk] - o1[g0];
+            if (ng == 0) continue;
+            Entry *Eg = ctx->rpart[g0 >> pshift].as<Entry>() + (g0 & ((1ull << pshift) - 1)) * cap1r;
+            Entry *E2 = ctx->ebuf2.as<Entry>();
+            QD_HIP(hipMemsetAsync(counters, 0, 8 * sizeof(u64), s));
+            const u64 *woff;
+            u64 rstride;
+            prof_begin(ctx, QD_PHASE_PARTITION);
+            if (b2) {
+                int ncu2 = 256;
+                (void)hipDeviceGetAttribute(&ncu2, hipDeviceAttributeMultiprocessorCount, ctx->device);
+                const unsigned g2 = unsigned(std::min<u64>(gbk, std::max(ncu2 / 2, 1)));
+                hipLaunchKernelGGL(k_level2_fixed, dim3(g2), dim3(1024), 0, s, (const Entry *)Eg, cap1r,
+                                   (const u64 *)ctx->off1.as<u64>() + g0, gbk, b1, b2, cap2, E2, ctx->cnt2.as<u64>(),
+                                   errw);
+                QD_HIP(hipGetLastError());
+                QD_TRY(run_scan(ctx, LoadU64{ctx->cnt2.as<u64>()}, nb2g, ctx->off2.as<u64>()));
+                u32 ew = 0;
+                QD_TRY(sync_read(ctx, errw, sizeof(u32), &ew));
+                rstride = cap2;
+                if (ew & ERR_SPILL) {  // a sub-bucket passed its capacity: exact offsets
+                    QD_HIP(hipMemsetAsync(errw, 0, sizeof(u32), s));
+                    hb[kHostN] = ng;
+                    QD_HIP(hipMemcpyAsync(ctx->off2.as<u64>() + nb2g, hb + kHostN, sizeof(u64),
+                                          hipMemcpyHostToDevice, s));
+                    hipLaunchKernelGGL(k_level2_exact, dim3(unsigned(gbk)), dim3(1024), 0, s, (const Entry *)Eg, cap1r,
+                                       (const u64 *)ctx->off1.as<u64>() + g0, o1[g0], b1, b2, E2, ctx->off2.as<u64>());
+                    QD_HIP(hipGetLastError());
+                    rstride = 0;
+                    st.reserved |= 1;
+                }
+                woff = ctx->off2.as<u64>();
+            } else {
+                // one level: the group's first-pass buckets are the dedup's
+                QD_HIP(hipMemcpyAsync(E2, Eg, gcap * sizeof(Entry), hipMemcpyDeviceToDevice, s));
+                hipLaunchKernelGGL(k_add_offset, dim3(grid_for(gbk + 1)), dim3(256), 0, s,
+                                   (const u64 *)ctx->off1.as<u64>() + g0, gbk + 1, u64(0) - o1[g0], ctx->off2.as<u64>());
+                QD_HIP(hipGetLastError());
+                woff = ctx->off2.as<u64>();
+                rstride = cap1r;
+            }
+            prof_end(ctx, QD_PHASE_PARTITION);
+            // the uniques go to the group's first-pass region (read by now)
+            u64 *UH = reinterpret_cast<u64 *>(Eg), *UR = UH + ng;
+            double2 *UM = reinterpret_cast<double2 *>(UR + ng);
+            u32 *UK = ukeys ? ctx->ukey.as<u32>() : nullptr;
+            u64 U = 0;
+            QD_TRY(dedup_phase(ctx, E2, rstride, woff, b2 ? nb2g : gbk, b1 + b2, prm.tolerance, dcap, ng, UH, UR, UM,
+                               UK, U));
+            Utot += U;
+            if (!U) continue;
+            hb[kHostN - 1] = ns;
+            QD_HIP(hipMemcpyAsync(pcount, hb + kHostN - 1, sizeof(u64), hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_pool_append,
+                               dim3(unsigned(std::min<u64>((ng + 256 * kPoolPer - 1) / (256 * kPoolPer), 16384))),
+                               dim3(256), 0, s, (const u64 *)UH, (const u64 *)UR, (const double2 *)UM, (const u32 *)UK,
+                               t24, ng, Ph, Pr, Pm, Pk, pcount);
+            QD_HIP(hipGetLastError());
+            QD_TRY(sync_read(ctx, pcount, sizeof(u64), &ns));
+            if (capped && ns > max_obj) QD_TRY(trim_pool());
+        }
     }
     st.num_unique = Utot;
     const u64 K = ns;
@@ -1448,7 +1632,7 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, u64 in_mem, co
                 const u64 per = kMemPerObject + max_child + prm.align_byte_length;
                 max_obj = std::max<u64>(1, target / per);
             }
-            return range_step<KIND>(ctx, P, rule, C, nitems, max_obj, budget, prm, out, sy, st);
+            return range_step<KIND>(ctx, P, rule, C, nitems, max_obj, prm, out, sy, st);
         }
     }
 
@@ -1741,10 +1925,6 @@ static int adjoint_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, const qd
 // ===========================================================================
 // batched step over instance chunks
 // ===========================================================================
-__global__ __launch_bounds__(256) void k_add_offset(const u64 *src, u64 n, u64 add, u64 *dst) {
-    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
-        dst[i] = src[i] + add;
-}
 
 // Candidates per instance of the whole batch (prep + scan only): cseg;
 // wide[i] = 1 for the instances holding graphs beyond the batched kernels'

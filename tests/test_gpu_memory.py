@@ -68,8 +68,9 @@ def expected_parent_cap(hs, rule, target):
 
 
 def expected_survivor_cap(uniq, rule, target, align=8):
-    """uniq: every unique of the step (the oracle's output without a
-    survivor cap), in truncation order |m|^2 desc, hash asc."""
+    """uniq: every unique of the step with its summed (not normalised)
+    magnitude (the oracle's output without a survivor cap); ordered here by
+    the truncation key, |m|^2 desc then hash asc."""
     keyed = []
     for i in range(uniq.num_object):
         o, m = uniq.get_object(i)
@@ -106,7 +107,7 @@ def test_memory_driven_truncation_matches_reference_search(ctx, free_mib):
                 assert Tp > 0 or st["num_parent_kept"] <= 1, what
                 k1 = expected_parent_cap(ref, r, Tp)
                 assert st["num_parent_kept"] == min(k1, ref.num_object), (what, st["num_parent_kept"], k1)
-                uniq, ust = orc.simulate_caps(ref, r, k1, U64MAX)
+                uniq, ust = orc.simulate_caps(ref, r, k1, U64MAX, normalize=False)
                 assert ust["num_unique"] == st["num_unique"], what
                 K = expected_survivor_cap(uniq, r, Ts)
                 assert st["num_survivors"] == K, (what, st["num_survivors"], K)
