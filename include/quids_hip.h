@@ -255,6 +255,7 @@ int qd_normalize(qd_ctx *ctx, qd_state *state, double *total_proba);
 int qd_average_value(qd_ctx *ctx, qd_state *state, int observable, int arg, double *value,
                      double *cot);
 
+
 /* ---- adjoint of the step (no counterpart in the reference: gradients of a
  *      loss of the final state with respect to the rules' real parameters,
  *      for fitting QCGD angles).  `out` must be what qd_simulate returned for

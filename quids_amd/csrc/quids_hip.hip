@@ -269,17 +269,27 @@ static int check_err_word(u32 w) {
     return QD_OK;
 }
 
-// canonical sum of |m|^2 (mag != null) or of vals
-static int canonical_sum(qd_ctx *ctx, const double2 *mag, const double *vals, u64 n, double *result) {
+// Doubles of the canonical sum's partial results for n values: two
+// ping-pong halves of the first level's chunk sums.
+static constexpr u64 sum_scratch_half(u64 n) {
+    return (n + kSumChunk - 1) / kSumChunk + ((n + kSumChunk - 1) / kSumChunk + kSumChunk - 1) / kSumChunk + 8;
+}
+
+// canonical sum of |m|^2 (mag != null) or of vals, the partial results in
+// buf (2 * sum_scratch_half(n) doubles; null: the context's buffer)
+static int canonical_sum(qd_ctx *ctx, const double2 *mag, const double *vals, u64 n, double *result,
+                         double *buf = nullptr) {
     if (n == 0) {
         *result = 0.0;
         return QD_OK;
     }
     u64 nchunk = (n + kSumChunk - 1) / kSumChunk;
-    // two ping-pong halves of the chunk buffer
-    u64 cap = nchunk + (nchunk + kSumChunk - 1) / kSumChunk + 8;
-    QD_TRY(ctx->chunks.ensure(2 * cap * sizeof(double)));
-    double *a = ctx->chunks.as<double>(), *b = a + cap;
+    const u64 cap = sum_scratch_half(n);
+    if (!buf) {
+        QD_TRY(ctx->chunks.ensure(2 * cap * sizeof(double)));
+        buf = ctx->chunks.as<double>();
+    }
+    double *a = buf, *b = a + cap;
     prof_begin(ctx, QD_PHASE_NORMALIZE);
     hipLaunchKernelGGL(k_sum_chunks, dim3(nchunk), dim3(kSumThreads), 0, ctx->stream, mag, vals, n, a);
     u64 m = nchunk;
@@ -999,8 +1009,8 @@ constexpr u32 kRangeGroups = 4;
 constexpr double kRangeBytesPerCand = 32.6 + (32.0 * 1.35 + 4.0 + 36.0) / kRangeGroups;
 
 template <int KIND>
-static int range_step(qd_ctx *ctx, const Parents &P, const qd_rule &rule, u64 C, u64 nitems, u64 max_obj,
-                      const qd_params &prm, qd_state *out, qd_symbolic *sy, qd_step_stats &st) {
+static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule &rule, u64 C, u64 nitems,
+                      u64 max_obj, const qd_params &prm, qd_state *out, qd_symbolic *sy, qd_step_stats &st) {
     hipStream_t s = ctx->stream;
     const u64 N = P.n;
     const SelKey sk{prm.simple_truncation ? 0u : 1u, 0u, prm.seed};
@@ -1048,7 +1058,12 @@ static int range_step(qd_ctx *ctx, const Parents &P, const qd_rule &rule, u64 C,
         u64 reuse = ctx->ebuf2.cap + ctx->pool0.cap + ctx->pool1.cap + ctx->pk0.cap + ctx->pk1.cap + ctx->bitmap.cap +
                     ctx->wprefix.cap;
         for (DBuf &b : ctx->rpart) reuse += b.cap;
-        const double fixed = 16.0 * double(W) + (capped ? 2.0 * 36.0 * double(max_obj) : 36.0 * double(C)) + double(2ull << 30);
+        // bitmaps over the candidates, the pool's kept part, and the next
+        // state with its finalize scratch (the objects at 1.5 x the parents'
+        // mean size)
+        const double K = double(capped ? max_obj : C);
+        const double next = 1.5 * double(in_bytes) * std::min(1.0, K / double(std::max<u64>(N, 1))) + 60.0 * K;
+        const double fixed = 16.0 * double(W) + (capped ? 2.0 * 36.0 * K : 36.0 * K) + next + double(2ull << 30);
         const double room = 0.85 * (double(freeb) + double(reuse)) - fixed;
         budget = room > 0 ? u64(room / kRangeBytesPerCand) : 0;
         budget = std::min<u64>(budget, u64(4.0 * double(kBigAlloc) / 33.0));
@@ -1632,7 +1647,7 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, u64 in_mem, co
                 const u64 per = kMemPerObject + max_child + prm.align_byte_length;
                 max_obj = std::max<u64>(1, target / per);
             }
-            return range_step<KIND>(ctx, P, rule, C, nitems, max_obj, prm, out, sy, st);
+            return range_step<KIND>(ctx, P, in_bytes, rule, C, nitems, max_obj, prm, out, sy, st);
         }
     }
 
@@ -1792,9 +1807,25 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, u64 in_mem, co
     return QD_OK;
 }
 
+// The context's scratch holds nothing live between calls.  When it holds
+// more than a quarter of the device while less than half is free, it is
+// released, so that the coming step's buffers (sized for its own
+// candidates) can be mapped; otherwise it is kept, and chained steps of
+// similar size map nothing.
+static int relieve_memory(qd_ctx *ctx) {
+    size_t freeb = 0, totalb = 0;
+    QD_HIP(hipMemGetInfo(&freeb, &totalb));
+    if (ctx_scratch(ctx) > totalb / 4 && freeb < totalb / 2) {
+        QD_HIP(hipStreamSynchronize(ctx->stream));
+        for (DBuf *b : ctx->all()) b->release();
+    }
+    return QD_OK;
+}
+
 template <int KIND>
 static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_state *out, qd_symbolic *sy, u64 max_obj,
                          const qd_params &prm, qd_step_stats &st) {
+    QD_TRY(relieve_memory(ctx));
     const Parents P{in->objects.as<u8>(), in->begin.as<u64>(), in->size.as<u32>(), in->mag.as<double2>(), in->n};
     return step_impl<KIND, false>(ctx, P, in->bytes, state_mem(in), rule, nullptr, out, sy, max_obj, prm, st);
 }
@@ -1965,6 +1996,7 @@ static int batched_kind(qd_ctx *c, qd_state *in, const u64 *in_seg, u32 n, const
                         u64 *out_seg, const u64 *caps, const qd_params &prm, qd_step_stats *stats) {
     hipStream_t s = c->stream;
     const Parents P{in->objects.as<u8>(), in->begin.as<u64>(), in->size.as<u32>(), in->mag.as<double2>(), in->n};
+    QD_TRY(relieve_memory(c));
     QD_TRY(upload(c, c->boseg, in_seg, (n + 1) * sizeof(u64)));
     QD_TRY(upload(c, c->brules, rules, n * sizeof(qd_rule)));
     Batch plan;
