@@ -254,7 +254,15 @@ int qd_normalize(qd_ctx *ctx, qd_state *state, double *total_proba);
  *      the adjoint starts from. */
 int qd_average_value(qd_ctx *ctx, qd_state *state, int observable, int arg, double *value,
                      double *cot);
-
+/* The same without any device allocation (no counterpart in the reference:
+ * for callers that cannot allocate, such as code under HIP graph capture or
+ * tight serving and training loops).  Its scratch is a static device array
+ * of 32 MiB that each GPU holds until the process ends; states of more than
+ * 2^22 objects are rejected with QD_ERR_CAPACITY.  Calls that overlap on one
+ * GPU share that array and race.  Results are bit-identical to
+ * qd_average_value. */
+int qd_average_value_noalloc(qd_ctx *ctx, qd_state *state, int observable, int arg, double *value,
+                             double *cot);
 
 /* ---- adjoint of the step (no counterpart in the reference: gradients of a
  *      loss of the final state with respect to the rules' real parameters,

@@ -162,6 +162,7 @@ SIGNATURES = {
     "qd_simulate_batched": [_vp, _vp, _pu64, _u32, ctypes.POINTER(QdRule), _vp, _pu64, _pu64, ctypes.POINTER(QdParams),
                             ctypes.POINTER(QdStepStats)],
     "qd_average_value": [_vp, _vp, ctypes.c_int, ctypes.c_int, _pd, _vp],
+    "qd_average_value_noalloc": [_vp, _vp, ctypes.c_int, ctypes.c_int, _pd, _vp],
     "qd_simulate_adjoint": [_vp, _vp, ctypes.POINTER(QdRule), ctypes.POINTER(QdRule), ctypes.c_int, _vp, _vp, _vp,
                             _pd, _u64],
     "qd_modifier_adjoint": [_vp, _vp, ctypes.POINTER(QdModifier), _vp],

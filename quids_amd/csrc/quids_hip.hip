@@ -132,6 +132,17 @@ struct DBuf {
     }
 };
 
+// Device scratch of qd_average_value_noalloc, a file-scope array that every
+// GPU the library runs on holds from module load until the process ends:
+// the per-object observable values of states of up to kNoAllocMaxObjects
+// objects, then the canonical sum's partial results.  Host code reaches it
+// only through the address hipGetSymbolAddress returns for the device
+// (qd_ctx::noalloc), never through the array's name.  Calls that overlap on
+// one GPU share it and race.
+constexpr u64 kNoAllocMaxObjects = 1ull << 22;
+constexpr u64 kNoAllocDoubles = kNoAllocMaxObjects + 2 * ((kNoAllocMaxObjects + 2047) / 2048 + 16);
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
 constexpr u64 kHostN = 8191;  // slot of the pinned area that stages a value for the device
 constexpr u64 kHostSegLoop = 8000;  // first of the slots that stage the batched select loop's state
 constexpr u64 kHostSelLoop = 8064;  // first of the slots that stage the select loop's state
@@ -139,6 +150,7 @@ constexpr u64 kHostSelLoop = 8064;  // first of the slots that stage the select 
 struct qd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    double *noalloc = nullptr;  // this device's copy of `scratch` (hipGetSymbolAddress)
     // phase timing
     bool profiling = false;
     hipEvent_t ev[2 * QD_NUM_PHASES] = {};
@@ -2199,6 +2211,15 @@ int qd_ctx_create(qd_ctx **out, int device) {
         delete c;
         return fail(QD_ERR_HIP, std::string("hipHostMalloc: ") + hipGetErrorString(e));
     }
+    // the device address of the no-allocation scratch on this device (the
+    // device is current)
+    e = hipGetSymbolAddress(reinterpret_cast<void **>(&c->noalloc), HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+        (void)hipHostFree(c->host);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return fail(QD_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
     *out = c;
     return QD_OK;
 }
@@ -2632,6 +2653,26 @@ int qd_average_value(qd_ctx *c, qd_state *s, int observable, int arg, double *va
                        c->adjd.as<double>(), reinterpret_cast<double2 *>(cot));
     QD_HIP(hipGetLastError());
     return canonical_sum(c, nullptr, c->adjd.as<double>(), s->n, value);
+}
+
+int qd_average_value_noalloc(qd_ctx *c, qd_state *s, int observable, int arg, double *value, double *cot) {
+    if (!c || !s || !value) return fail(QD_ERR_INVALID, "null argument");
+    if (observable < 0 || observable > 2) return fail(QD_ERR_INVALID, "unknown observable");
+    if (s->ctx != c) return fail(QD_ERR_INVALID, "state belongs to another context");
+    if (s->n > kNoAllocMaxObjects)
+        return fail(QD_ERR_CAPACITY, "qd_average_value_noalloc: " + std::to_string(s->n) +
+                                         " objects, beyond its static scratch of " +
+                                         std::to_string(kNoAllocMaxObjects) + " (use qd_average_value)");
+    static_assert(kNoAllocDoubles >= kNoAllocMaxObjects + 2 * sum_scratch_half(kNoAllocMaxObjects), "scratch size");
+    QD_HIP(hipSetDevice(c->device));
+    *value = 0.0;
+    if (s->n == 0) return QD_OK;
+    double *vals = c->noalloc;  // per-object values, then the sum's partials
+    hipLaunchKernelGGL(k_observable, dim3(grid_for(s->n)), dim3(256), 0, c->stream, s->objects.as<u8>(),
+                       s->begin.as<u64>(), s->size.as<u32>(), s->mag.as<double2>(), s->n, observable, arg, vals,
+                       reinterpret_cast<double2 *>(cot));
+    QD_HIP(hipGetLastError());
+    return canonical_sum(c, nullptr, vals, s->n, value, vals + kNoAllocMaxObjects);
 }
 
 int qd_rule_num_child(const qd_rule *r, const void *parent, uint32_t size, uint32_t *num_child,
