@@ -148,6 +148,18 @@ typedef enum qd_phase {
     QD_NUM_PHASES = 9
 } qd_phase;
 int qd_ctx_set_profiling(qd_ctx *ctx, int enable);
+
+/* Phase hook of qd_simulate (the reference's mid_step_function,
+ * src/quids.hpp:90, 448): called on the calling thread with the reference's
+ * phase names in its order -- "num_child", "truncate_symbolic - prepare",
+ * "truncate_symbolic", "prepare_index", "symbolic_iteration",
+ * "compute_collisions - prepare" / "- insert" / "- finalize",
+ * "truncate - prepare", "truncate", "prepare_final", "final", "normalize",
+ * "end" (src/quids.hpp:454-1016) -- each once per call, after the device has
+ * finished the work of the phases before it (so a hook costs a stream
+ * synchronisation per phase).  NULL removes the hook. */
+typedef void (*qd_phase_hook)(const char *phase, void *user);
+int qd_ctx_set_phase_hook(qd_ctx *ctx, qd_phase_hook hook, void *user);
 int qd_ctx_reset_phase_times(qd_ctx *ctx);
 /* ms[QD_NUM_PHASES] accumulated GPU milliseconds, calls[QD_NUM_PHASES]. */
 int qd_ctx_phase_times(qd_ctx *ctx, double *ms, uint64_t *calls);

@@ -322,13 +322,17 @@ def simulate(it, rule, next_iteration=None, symbolic_iteration=None, max_num_obj
     prm = _params()
     st = QdStepStats()
     sy = symbolic_iteration.handle if symbolic_iteration is not None else None
-    if mid_step_function is not None:
-        mid_step_function("num_child")
-    check(lib().qd_simulate(it.ctx.handle, it.handle, ctypes.byref(r), next_iteration.handle, sy,
-                            int(max_num_object), ctypes.byref(prm), ctypes.byref(st)), "qd_simulate")
+    hook = None
+    if mid_step_function is not None:  # called by the library between its phases, with the reference's names
+        hook = _abi.PHASE_HOOK(lambda name, _user: mid_step_function(name.decode()))
+        check(lib().qd_ctx_set_phase_hook(it.ctx.handle, hook, None), "qd_ctx_set_phase_hook")
+    try:
+        check(lib().qd_simulate(it.ctx.handle, it.handle, ctypes.byref(r), next_iteration.handle, sy,
+                                int(max_num_object), ctypes.byref(prm), ctypes.byref(st)), "qd_simulate")
+    finally:
+        if hook is not None:
+            lib().qd_ctx_set_phase_hook(it.ctx.handle, _abi.PHASE_HOOK(), None)  # a null hook
     next_iteration._device_changed()
-    if mid_step_function is not None:
-        mid_step_function("end")
     d = st.as_dict()
     if symbolic_iteration is not None:
         symbolic_iteration.last_stats = d

@@ -117,6 +117,8 @@ class QdTuning(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
+PHASE_HOOK = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_void_p)
+
 _vp = ctypes.c_void_p
 _u32 = ctypes.c_uint32
 _u64 = ctypes.c_uint64
@@ -163,6 +165,7 @@ SIGNATURES = {
                             ctypes.POINTER(QdStepStats)],
     "qd_average_value": [_vp, _vp, ctypes.c_int, ctypes.c_int, _pd, _vp],
     "qd_average_value_noalloc": [_vp, _vp, ctypes.c_int, ctypes.c_int, _pd, _vp],
+    "qd_ctx_set_phase_hook": [_vp, PHASE_HOOK, _vp],
     "qd_simulate_adjoint": [_vp, _vp, ctypes.POINTER(QdRule), ctypes.POINTER(QdRule), ctypes.c_int, _vp, _vp, _vp,
                             _pd, _u64],
     "qd_modifier_adjoint": [_vp, _vp, ctypes.POINTER(QdModifier), _vp],

@@ -640,34 +640,25 @@ __global__ __launch_bounds__(kGenThreads, 3) void k_generate_scatter(const PDesc
 
 // ---------------------------------------------------------------------------
 // Hash-range passes, for steps whose candidates exceed the scratch memory.
-// COUNT: histogram of the top 12 bits of every child hash into hist (4096
-// bins).  Otherwise: only the children whose top 12 hash bits lie in
-// [rlo, rhi) are kept, compacted (a block reserves one range per iteration,
-// filled in child-major order: child u of every thread, then child u + 1)
-// as key = hash rotated left by 12 (so the partition and the dedup, which
-// bucket on the top key bits, see well spread bits), magnitude and candidate
-// index, and histogrammed on the top b1 key bits for the partition.
+// The counting pass histograms the top 12 bits of every child hash into
+// hist (4096 bins) and records, per work item, which of 2^rbits equal slices
+// of the hash space (its top rbits bits) its children fall in (imask, one
+// bit per slice).  A range pass then walks only the work items whose mask
+// meets the pass's slices, so that across the passes a work item is walked
+// at most once per distinct slice of its children (at most 8 times), not
+// once per pass.
 // ---------------------------------------------------------------------------
 __device__ inline u64 rotl12(u64 h) { return (h << 12) | (h >> 52); }
 __device__ inline u64 rotr12(u64 k) { return (k >> 12) | (k << 52); }
 
-constexpr int kRangeWaves = kGenThreads / kWave;
-constexpr int kRangeRuns = (1 << kItemBits) * kRangeWaves;  // (child, wavefront) runs of an iteration
-static_assert(kRangeRuns <= kWave, "one wavefront scans the run counts");
-
-template <int KIND, bool COUNT>
-__global__ __launch_bounds__(kGenThreads) void k_generate_range(const PDesc *D, const double2 *pmag, const u64 *cb,
-                                                               const u64 *wb, u64 N, const u64 *cpar, const u64 *tab,
-                                                               const u64 *pbegin,
-                                                               qd_rule rule, u64 W, u32 rlo, u32 rhi, u64 *HK,
-                                                               double2 *MK, u64 *CK, u64 *rcount, u32 b1, u64 *hist) {
+template <int KIND>
+__global__ __launch_bounds__(kGenThreads) void k_range_count(const PDesc *D, const double2 *pmag, const u64 *wb, u64 N,
+                                                            const u64 *cpar, const u64 *tab, const u64 *pbegin,
+                                                            qd_rule rule, u64 W, u32 rbits, u32 *imask, u64 *hist) {
     __shared__ u32 lh[kHistBins];
     __shared__ u64 swb[kGenThreads + 1];
-    __shared__ u64 pfirst, wbase;
-    __shared__ u64 roff[kRangeRuns];  // count, then exclusive prefix, of each run's kept children
-    const u32 hb = COUNT ? 12 : b1;
-    if (hb)
-        for (int i = threadIdx.x; i < (1 << hb); i += blockDim.x) lh[i] = 0;
+    __shared__ u64 pfirst;
+    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) lh[i] = 0;
     const QcgdCoef k = coef_of(rule);
     for (u64 w0 = u64(blockIdx.x) * kGenThreads; w0 < W; w0 += u64(gridDim.x) * kGenThreads) {
         __syncthreads();
@@ -677,88 +668,46 @@ __global__ __launch_bounds__(kGenThreads) void k_generate_range(const PDesc *D, 
         for (u32 q = threadIdx.x; q <= kGenThreads; q += blockDim.x) swb[q] = pf + q <= N ? wb[pf + q] : ~0ull;
         __syncthreads();
         const u64 w = w0 + threadIdx.x;
-        u64 hs[1 << kItemBits];
-        double2 ms[1 << kItemBits];
-        u32 keep = 0, s = 0;  // children kept (bit u), indexed by the unrolled child counter only
-        u64 cbase = 0;
-        if (w < W) {
-            u64 p;
-            if (swb[kGenThreads] > w) {
-                u32 lo = 0, hi = kGenThreads;
-                while (lo + 1 < hi) {
-                    const u32 mid = (lo + hi) >> 1;
-                    if (swb[mid] <= w) lo = mid;
-                    else hi = mid;
-                }
-                p = pf + lo;
-            } else {
-                p = find_parent(wb, N, w);
+        if (w >= W) continue;
+        u64 p;
+        if (swb[kGenThreads] > w) {
+            u32 lo = 0, hi = kGenThreads;
+            while (lo + 1 < hi) {
+                const u32 mid = (lo + hi) >> 1;
+                if (swb[mid] <= w) lo = mid;
+                else hi = mid;
             }
-            const PDesc pd = D[p];
-            const u32 j0 = u32(w - wb[p]);
-            const u32 t = min(u32(pd.k), kItemBits);
-            s = pd.k - t;
-            const double2 m0 = pmag[p];
-            Walk pre{};
-            pre.mag = cplx{m0.x, m0.y};
-            const u64 *ptab = tab + sm_tab_base(pbegin[p]);
-            if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(pre, pd, ptab, j0, s, k);
-            else flip_run<KIND == QD_RULE_QCGD_COIN>(pre, pd, j0, s, k);
-            cbase = cb[p] + j0;
-            walk_item<KIND>(pre, pd, ptab, s, t, k, [&](u32 u, u64 h, const cplx &m) {
-                const u32 top = u32(h >> 52);
-                if (COUNT) {
-                    atomicAdd(&lh[top], 1u);
-                } else if (top >= rlo && top < rhi) {
-                    hs[u] = rotl12(h);
-                    ms[u] = make_double2(m.re, m.im);
-                    keep |= 1u << u;
-                }
-            });
+            p = pf + lo;
+        } else {
+            p = find_parent(wb, N, w);
         }
-        if (!COUNT) {
-            // kept children placed in (child u, wavefront, lane) order, so
-            // that the lanes of one store instruction write side by side
-            const u32 lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-            u64 mk[1 << kItemBits];
-#pragma unroll
-            for (u32 u = 0; u < (1u << kItemBits); ++u) {
-                mk[u] = __ballot((keep >> u) & 1u);
-                if (lane == 0) roff[u * kRangeWaves + wid] = __popcll(mk[u]);
-            }
-            __syncthreads();
-            if (wid == 0) {
-                const u64 c = lane < kRangeRuns ? roff[lane] : 0;
-                const u64 inc = wave_incl_scan(c);
-                if (lane < kRangeRuns) roff[lane] = inc - c;
-                if (lane == kWave - 1)
-                    wbase = inc ? atomicAdd(reinterpret_cast<unsigned long long *>(rcount), (unsigned long long)inc) : 0;
-            }
-            __syncthreads();
-            const u64 below = (1ull << lane) - 1;
-#pragma unroll
-            for (u32 u = 0; u < (1u << kItemBits); ++u) {
-                if (keep & (1u << u)) {
-                    const u64 at = wbase + roff[u * kRangeWaves + wid] + __popcll(mk[u] & below);
-                    HK[at] = hs[u];
-                    MK[at] = ms[u];
-                    CK[at] = cbase + (u64(u) << s);
-                    if (hb) atomicAdd(&lh[hs[u] >> (64 - hb)], 1u);
-                }
-            }
-        }
+        const PDesc pd = D[p];
+        const u32 j0 = u32(w - wb[p]);
+        const u32 t = min(u32(pd.k), kItemBits);
+        const u32 s = pd.k - t;
+        const double2 m0 = pmag[p];
+        Walk pre{};
+        pre.mag = cplx{m0.x, m0.y};
+        const u64 *ptab = tab + sm_tab_base(pbegin[p]);
+        if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(pre, pd, ptab, j0, s, k);
+        else flip_run<KIND == QD_RULE_QCGD_COIN>(pre, pd, j0, s, k);
+        u32 mask = 0;
+        walk_item<KIND>(pre, pd, ptab, s, t, k, [&](u32, u64 h, const cplx &) {
+            atomicAdd(&lh[u32(h >> 52)], 1u);
+            mask |= 1u << (rbits ? u32(h >> (64 - rbits)) : 0u);
+        });
+        imask[w] = mask;
     }
-    if (hb) {
-        __syncthreads();
-        for (int i = threadIdx.x; i < (1 << hb); i += blockDim.x)
-            if (lh[i]) atomicAdd(reinterpret_cast<unsigned long long *>(hist + i), (unsigned long long)lh[i]);
-    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x)
+        if (lh[i]) atomicAdd(reinterpret_cast<unsigned long long *>(hist + i), (unsigned long long)lh[i]);
 }
 
-
 // ---------------------------------------------------------------------------
-// A hash-range pass fused with its first radix pass: the children whose top
-// 12 hash bits lie in [rlo, rhi) are ranked per bucket of the top b1 bits of
+// A hash-range pass fused with its first radix pass.  A block takes tiles of
+// kRangeTile work items, keeps those whose mask meets pmask (compacted in
+// LDS), and walks them a block-width at a time: the children whose top 12
+// hash bits lie in [rlo, rhi) are ranked per bucket of the top b1 bits of
 // their key (the hash rotated left by 12) in LDS, as k_generate_scatter
 // ranks all children, and written as 32-byte entries into fixed-capacity
 // buckets of cap1 entries.  The buckets are split over up to four
@@ -774,93 +723,131 @@ __device__ inline Entry *ebucket(const EParts &E, u32 d, u64 cap1) {
     return E.p[d >> E.pshift] + u64(d & ((1u << E.pshift) - 1)) * cap1;
 }
 
+constexpr u32 kRangeBins = 2048;  // first-pass buckets of a range pass, at most
+constexpr u32 kRangeTile = 2048;  // work items per tile
+static_assert(kRangeTile % kGenThreads == 0, "tiles of whole chunks");
+
 template <int KIND>
 __global__ __launch_bounds__(kGenThreads, 3) void k_generate_range_scatter(const PDesc *D, const double2 *pmag,
                                                                        const u64 *cb, const u64 *wb, u64 N,
                                                                        const u64 *cpar, const u64 *tab,
                                                                        const u64 *pbegin, qd_rule rule, u64 W,
-                                                                       u32 rlo, u32 rhi, u32 b1, u64 cap1,
-                                                                       u64 *cnt1, EParts E, u32 *err) {
+                                                                       const u32 *imask, u32 pmask, u32 rlo,
+                                                                       u32 rhi, u32 b1, u64 cap1, u64 *cnt1,
+                                                                       EParts E, u32 *err) {
     constexpr u32 kKids = 1u << kItemBits;
-    __shared__ u32 bcnt[kFusedBins];
-    __shared__ u64 bbase[kFusedBins];  // first reserved slot of the bucket
-    __shared__ u64 swb[kGenThreads + 1];
+    constexpr u32 kPer = kRangeTile / kGenThreads;
+    constexpr u32 kWaves = kGenThreads / kWave;
+    __shared__ u32 bcnt[kRangeBins];
+    __shared__ u64 bbase[kRangeBins];  // first reserved slot of the bucket
+    __shared__ u64 swb[kRangeTile + 1];
+    __shared__ u16 tl[kRangeTile];     // the tile's walked items (offsets in the tile)
+    __shared__ u32 woff[kWaves * kPer + 1];
     __shared__ u64 pfirst;
     const u32 nbin = 1u << b1;
+    const u32 lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     const QcgdCoef k = coef_of(rule);
     bool spill = false;
-    for (u64 w0 = u64(blockIdx.x) * kGenThreads; w0 < W; w0 += u64(gridDim.x) * kGenThreads) {
+    for (u64 t0 = u64(blockIdx.x) * kRangeTile; t0 < W; t0 += u64(gridDim.x) * kRangeTile) {
         __syncthreads();
-        if (threadIdx.x == 0) pfirst = cpar[w0 / kGenThreads];
-        for (u32 i = threadIdx.x; i < nbin; i += blockDim.x) bcnt[i] = 0;
-        __syncthreads();
-        const u64 pf = pfirst;
-        for (u32 q = threadIdx.x; q <= kGenThreads; q += blockDim.x) swb[q] = pf + q <= N ? wb[pf + q] : ~0ull;
-        __syncthreads();
-        const u64 w = w0 + threadIdx.x;
-        // kept children (bit u), indexed by the unrolled child counter only
-        u64 hs[kKids];
-        double2 ms[kKids];
-        u32 br[kKids];
-        u32 keep = 0, s = 0;
-        u64 cbase = 0;
-        if (w < W) {
-            u64 p;
-            if (swb[kGenThreads] > w) {
-                u32 lo = 0, hi = kGenThreads;
-                while (lo + 1 < hi) {
-                    const u32 mid = (lo + hi) >> 1;
-                    if (swb[mid] <= w) lo = mid;
-                    else hi = mid;
-                }
-                p = pf + lo;
-            } else {
-                p = find_parent(wb, N, w);
-            }
-            const PDesc pd = D[p];
-            const u32 j0 = u32(w - wb[p]);
-            const u32 t = min(u32(pd.k), kItemBits);
-            s = pd.k - t;
-            const double2 m0 = pmag[p];
-            Walk pre{};
-            pre.mag = cplx{m0.x, m0.y};
-            const u64 *ptab = tab + sm_tab_base(pbegin[p]);
-            if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(pre, pd, ptab, j0, s, k);
-            else flip_run<KIND == QD_RULE_QCGD_COIN>(pre, pd, j0, s, k);
-            cbase = cb[p] + j0;
-            walk_item<KIND>(pre, pd, ptab, s, t, k, [&](u32 u, u64 h, const cplx &m) {
-                const u32 top = u32(h >> 52);
-                if (top >= rlo && top < rhi) {
-                    const u64 key = rotl12(h);
-                    const u32 d = b1 ? u32(key >> (64 - b1)) : 0u;
-                    hs[u] = key;
-                    ms[u] = make_double2(m.re, m.im);
-                    br[u] = (d << 16) | atomicAdd(&bcnt[d], 1u);
-                    keep |= 1u << u;
-                }
-            });
+        if (threadIdx.x == 0) pfirst = cpar[t0 / kGenThreads];
+        // the tile's items whose children meet the pass's slices, compacted
+        // in tile order: per (round j, wavefront) run, a ballot count
+        u64 mk[kPer];
+#pragma unroll
+        for (u32 j = 0; j < kPer; ++j) {
+            const u64 w = t0 + j * kGenThreads + threadIdx.x;
+            mk[j] = __ballot(w < W && (imask[w] & pmask));
+            if (lane == 0) woff[j * kWaves + wid] = u32(__popcll(mk[j]));
         }
         __syncthreads();
-        for (u32 i = threadIdx.x; i < nbin; i += blockDim.x)
-            if (bcnt[i])
-                bbase[i] = atomicAdd(reinterpret_cast<unsigned long long *>(cnt1 + i), (unsigned long long)bcnt[i]);
+        if (threadIdx.x == 0) {
+            u32 acc = 0;
+            for (u32 q = 0; q < kWaves * kPer; ++q) {
+                const u32 c = woff[q];
+                woff[q] = acc;
+                acc += c;
+            }
+            woff[kWaves * kPer] = acc;
+        }
         __syncthreads();
+        const u64 pf = pfirst;
+        for (u32 q = threadIdx.x; q <= kRangeTile; q += blockDim.x) swb[q] = pf + q <= N ? wb[pf + q] : ~0ull;
 #pragma unroll
-        for (u32 u = 0; u < kKids; ++u) {
-            if (keep & (1u << u)) {
-                const u32 d = br[u] >> 16;
-                const u64 at = bbase[d] + (br[u] & 0xFFFF);
-                if (at < cap1) {
-                    Entry e;
-                    e.hash = hs[u];
-                    e.cand = cbase + (u64(u) << s);
-                    e.re = ms[u].x;
-                    e.im = ms[u].y;
-                    ebucket(E, d, cap1)[at] = e;
+        for (u32 j = 0; j < kPer; ++j)
+            if ((mk[j] >> lane) & 1)
+                tl[woff[j * kWaves + wid] + __popcll(mk[j] & ((1ull << lane) - 1))] = u16(j * kGenThreads + threadIdx.x);
+        __syncthreads();
+        const u32 nt = woff[kWaves * kPer];
+        for (u32 r0 = 0; r0 < nt; r0 += kGenThreads) {
+            for (u32 i = threadIdx.x; i < nbin; i += blockDim.x) bcnt[i] = 0;
+            __syncthreads();
+            // kept children (bit u), indexed by the unrolled child counter only
+            u64 hs[kKids];
+            double2 ms[kKids];
+            u32 br[kKids];
+            u32 keep = 0, s = 0;
+            u64 cbase = 0;
+            if (r0 + threadIdx.x < nt) {
+                const u64 w = t0 + tl[r0 + threadIdx.x];
+                u64 p;
+                if (swb[kRangeTile] > w) {
+                    u32 lo = 0, hi = kRangeTile;
+                    while (lo + 1 < hi) {
+                        const u32 mid = (lo + hi) >> 1;
+                        if (swb[mid] <= w) lo = mid;
+                        else hi = mid;
+                    }
+                    p = pf + lo;
                 } else {
-                    spill = true;
+                    p = find_parent(wb, N, w);
+                }
+                const PDesc pd = D[p];
+                const u32 j0 = u32(w - wb[p]);
+                const u32 t = min(u32(pd.k), kItemBits);
+                s = pd.k - t;
+                const double2 m0 = pmag[p];
+                Walk pre{};
+                pre.mag = cplx{m0.x, m0.y};
+                const u64 *ptab = tab + sm_tab_base(pbegin[p]);
+                if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(pre, pd, ptab, j0, s, k);
+                else flip_run<KIND == QD_RULE_QCGD_COIN>(pre, pd, j0, s, k);
+                cbase = cb[p] + j0;
+                walk_item<KIND>(pre, pd, ptab, s, t, k, [&](u32 u, u64 h, const cplx &m) {
+                    const u32 top = u32(h >> 52);
+                    if (top >= rlo && top < rhi) {
+                        const u64 key = rotl12(h);
+                        const u32 d = b1 ? u32(key >> (64 - b1)) : 0u;
+                        hs[u] = key;
+                        ms[u] = make_double2(m.re, m.im);
+                        br[u] = (d << 16) | atomicAdd(&bcnt[d], 1u);
+                        keep |= 1u << u;
+                    }
+                });
+            }
+            __syncthreads();
+            for (u32 i = threadIdx.x; i < nbin; i += blockDim.x)
+                if (bcnt[i])
+                    bbase[i] = atomicAdd(reinterpret_cast<unsigned long long *>(cnt1 + i), (unsigned long long)bcnt[i]);
+            __syncthreads();
+#pragma unroll
+            for (u32 u = 0; u < kKids; ++u) {
+                if (keep & (1u << u)) {
+                    const u32 d = br[u] >> 16;
+                    const u64 at = bbase[d] + (br[u] & 0xFFFF);
+                    if (at < cap1) {
+                        Entry e;
+                        e.hash = hs[u];
+                        e.cand = cbase + (u64(u) << s);
+                        e.re = ms[u].x;
+                        e.im = ms[u].y;
+                        ebucket(E, d, cap1)[at] = e;
+                    } else {
+                        spill = true;
+                    }
                 }
             }
+            __syncthreads();  // bcnt / bbase are rewritten by the next round
         }
     }
     if (spill) atomicOr(err, ERR_SPILL1);

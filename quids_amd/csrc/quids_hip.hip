@@ -150,6 +150,9 @@ constexpr u64 kHostSelLoop = 8064;  // first of the slots that stage the select 
 struct qd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    qd_phase_hook hook = nullptr;  // simulate()'s phase hook and its argument
+    void *hook_user = nullptr;
+    int next_phase = 0;            // the next of kPhaseNames the current call reports
     double *noalloc = nullptr;  // this device's copy of `scratch` (hipGetSymbolAddress)
     // phase timing
     bool profiling = false;
@@ -178,7 +181,7 @@ struct qd_ctx {
     DBuf adjk, adjv, adjg, adjd, adjp, adjw;
     DBuf pool0, pool1, pwpre, blist2;
     DBuf mw;  // per-object memory of memory-driven truncation
-    DBuf rpart[4], pk0, pk1;  // hash-range passes: first-pass buckets, pool select keys
+    DBuf rpart[4], pk0, pk1, imask;  // hash-range passes: first-pass buckets, pool select keys, slice masks
     // output buffers of batched chunks (objects, begin, size, mag per chunk),
     // kept between calls
     std::vector<DBuf> parts;
@@ -194,7 +197,7 @@ struct qd_ctx {
                 &biglist, &redolist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &cpar, &opar, &ebuf2, &cnt2, &slist, &ukey, &selloop, &segloop, &bsoff, &bscnt,
                 &adjk, &adjv, &adjg, &adjd, &adjp, &adjw, &bseg, &boseg, &brules, &bsinst, &bcount, &bsel, &btrunc,
                 &blist, &bcur, &btile, &bhist, &bkseg, &bchunk, &bcsum, &btot, &pool0, &pool1, &pwpre, &blist2, &mw, &rpart[0], &rpart[1], &rpart[2],
-                &rpart[3], &pk0, &pk1};
+                &rpart[3], &pk0, &pk1, &imask};
     }
 };
 
@@ -236,7 +239,37 @@ static inline unsigned grid_for(u64 n, unsigned block = 256, unsigned cap = 6553
     return unsigned(std::max<u64>(1, std::min<u64>(g, cap)));
 }
 
+// The reference's mid_step_function names in the order one simulate() call
+// reports them (src/quids.hpp:454-1016); every call reports all of them.
+static const char *const kPhaseNames[] = {"num_child",
+                                          "truncate_symbolic - prepare",
+                                          "truncate_symbolic",
+                                          "prepare_index",
+                                          "symbolic_iteration",
+                                          "compute_collisions - prepare",
+                                          "compute_collisions - insert",
+                                          "compute_collisions - finalize",
+                                          "truncate - prepare",
+                                          "truncate",
+                                          "prepare_final",
+                                          "final",
+                                          "normalize",
+                                          "end"};
+enum : int {
+    PH_NUM_CHILD, PH_TRUNC_SYM_PREP, PH_TRUNC_SYM, PH_PREP_INDEX, PH_SYMBOLIC, PH_COLL_PREP, PH_COLL_INSERT,
+    PH_COLL_FINAL, PH_TRUNC_PREP, PH_TRUNC, PH_PREP_FINAL, PH_FINAL, PH_NORMALIZE, PH_END
+};
 static void prof_collect(qd_ctx *ctx);
+// Reports the phase names up to `upto` not reported yet in this call, once
+// the device has finished the work enqueued before (the reference calls its
+// hook between phases, with the earlier phases done).
+static void phase_mark(qd_ctx *ctx, int upto) {
+    if (!ctx->hook || ctx->next_phase > upto) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    prof_collect(ctx);
+    for (; ctx->next_phase <= upto; ++ctx->next_phase) ctx->hook(kPhaseNames[ctx->next_phase], ctx->hook_user);
+}
+
 static int sync_read(qd_ctx *ctx, const void *dev, size_t bytes, void *host) {
     QD_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
     QD_HIP(hipStreamSynchronize(ctx->stream));
@@ -741,6 +774,7 @@ static int partition_dedup(qd_ctx *ctx, const u64 *H0, const double2 *M0, const 
     const u32 L = b1 + b2;
 
     // ---- 5 dedup
+    phase_mark(ctx, PH_COLL_INSERT);
     UH = static_cast<u64 *>(ubuf);
     UR = UH + n;
     UM = reinterpret_cast<double2 *>(UR + n);
@@ -1013,11 +1047,11 @@ __global__ __launch_bounds__(256) void k_add_offset(const u64 *src, u64 n, u64 a
 }
 
 // Per-candidate bytes of a hash-range pass: the first radix pass's entry
-// (32 bytes plus its buckets' slack), and a quarter (one group of
+// (32 bytes plus its buckets' slack), and an eighth (one group of
 // first-level buckets at a time) of the second pass's entry at its
 // sub-buckets' slack and of the group's 32-bit select keys and pool
 // appends.
-constexpr u32 kRangeGroups = 4;
+constexpr u32 kRangeGroups = 8;
 constexpr double kRangeBytesPerCand = 32.6 + (32.0 * 1.35 + 4.0 + 36.0) / kRangeGroups;
 
 template <int KIND>
@@ -1036,8 +1070,8 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
     ctx->ukey.release();
     if (KIND != QD_RULE_QCGD_SPLIT_MERGE) ctx->tab.release();  // split_merge's name-root table
     QD_TRY(ctx->hist1.ensure(kHistBins * sizeof(u64)));
-    QD_TRY(ctx->off1.ensure((kFusedBins + 1) * sizeof(u64)));
-    QD_TRY(ctx->cur1.ensure(kHistBins * sizeof(u64)));
+    QD_TRY(ctx->off1.ensure((kRangeBins + 1) * sizeof(u64)));
+    QD_TRY(ctx->cur1.ensure(kRangeBins * sizeof(u64)));
     QD_TRY(ctx->nhist.ensure(kHistBins * sizeof(u64)));
     u64 *hist1 = ctx->hist1.as<u64>();
     const unsigned gg = unsigned(std::max<u64>(
@@ -1046,21 +1080,10 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
     const bool ukeys = !sk.prob;  // 32-bit |m|^2 select keys (plain truncation keys only)
     const u64 W = (C + 63) / 64;
 
-    // 1. how the candidates spread over the 4096 values of the top 12 hash bits
-    std::vector<u64> hc(kHistBins);
-    QD_HIP(hipMemsetAsync(hist1, 0, kHistBins * sizeof(u64), s));
-    prof_begin(ctx, QD_PHASE_GENERATE);
-    hipLaunchKernelGGL((k_generate_range<KIND, true>), dim3(gg), dim3(kGenThreads), 0, s, ctx->pdesc.as<PDesc>(), P.mag,
-                       ctx->cb.as<u64>(), ctx->wb.as<u64>(), N, ctx->cpar.as<u64>(), ctx->tab.as<u64>(), P.begin, rule, nitems, 0u, 0u,
-                       (u64 *)nullptr, (double2 *)nullptr, (u64 *)nullptr, (u64 *)nullptr, 0u, hist1);
-    QD_HIP(hipGetLastError());
-    prof_end(ctx, QD_PHASE_GENERATE);
-    QD_TRY(sync_read(ctx, hist1, kHistBins * sizeof(u64), hc.data()));
-
-    // 2. the pass size: what the free memory (and the buffers this path
-    // reuses) holds after the bitmaps and the pool, within the second radix
-    // pass's reach (2^22 buckets of at most 1024 entries) and the four
-    // first-pass allocations; then contiguous bin ranges within it
+    // 1. the pass size: what the free memory (and the buffers this path
+    // reuses) holds after the work items' slice masks, the bitmaps, the pool
+    // and the next state, within the second radix pass's reach (2^22 buckets
+    // of at most 1024 entries) and the four first-pass allocations
     u64 budget;
     if (ctx->tune.chunk_log2) {
         budget = 1ull << ctx->tune.chunk_log2;
@@ -1068,22 +1091,46 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
         size_t freeb = 0, totalb = 0;
         QD_HIP(hipMemGetInfo(&freeb, &totalb));
         u64 reuse = ctx->ebuf2.cap + ctx->pool0.cap + ctx->pool1.cap + ctx->pk0.cap + ctx->pk1.cap + ctx->bitmap.cap +
-                    ctx->wprefix.cap;
+                    ctx->wprefix.cap + ctx->imask.cap;
         for (DBuf &b : ctx->rpart) reuse += b.cap;
-        // bitmaps over the candidates, the pool's kept part, and the next
-        // state with its finalize scratch (the objects at 1.5 x the parents'
-        // mean size)
+        // the next state with its finalize scratch (objects at 1.5 x the
+        // parents' mean size), beyond what the output state holds already
         const double K = double(capped ? max_obj : C);
-        const double next = 1.5 * double(in_bytes) * std::min(1.0, K / double(std::max<u64>(N, 1))) + 60.0 * K;
-        const double fixed = 16.0 * double(W) + (capped ? 2.0 * 36.0 * K : 36.0 * K) + next + double(2ull << 30);
+        const double next = std::max(0.0, 1.5 * double(in_bytes) * std::min(1.0, K / double(std::max<u64>(N, 1))) +
+                                              60.0 * K - double(state_mem(out)));
+        const double fixed = 16.0 * double(W) + 4.0 * double(nitems) + (capped ? 3.0 * 36.0 * K : 36.0 * K) + next +
+                             double(2ull << 30);
         const double room = 0.85 * (double(freeb) + double(reuse)) - fixed;
         budget = room > 0 ? u64(room / kRangeBytesPerCand) : 0;
         budget = std::min<u64>(budget, u64(4.0 * double(kBigAlloc) / 33.0));
     }
     budget = std::max<u64>(std::min<u64>(budget, 1ull << 32), 1ull << 10);
-    std::vector<u32> cut{0};
+    // slices of the hash space recorded per work item: 2^rbits, enough for
+    // passes of about the budget (at most 32: one u32 mask per work item)
+    u32 rbits = 0;
+    while (rbits < 5 && (budget << rbits) < C + C / 16) ++rbits;
+    QD_TRY(ctx->imask.ensure(std::max<u64>(nitems, 1) * sizeof(u32)));
+
+    // 2. how the candidates spread over the 4096 values of the top 12 hash
+    // bits, and each work item's slices
+    phase_mark(ctx, PH_SYMBOLIC);
+    std::vector<u64> hc(kHistBins);
+    QD_HIP(hipMemsetAsync(hist1, 0, kHistBins * sizeof(u64), s));
+    prof_begin(ctx, QD_PHASE_GENERATE);
+    hipLaunchKernelGGL(k_range_count<KIND>, dim3(gg), dim3(kGenThreads), 0, s, (const PDesc *)ctx->pdesc.as<PDesc>(),
+                       P.mag, (const u64 *)ctx->wb.as<u64>(), N, (const u64 *)ctx->cpar.as<u64>(),
+                       (const u64 *)ctx->tab.as<u64>(), P.begin, rule, nitems, rbits, ctx->imask.as<u32>(), hist1);
+    QD_HIP(hipGetLastError());
+    prof_end(ctx, QD_PHASE_GENERATE);
+    QD_TRY(sync_read(ctx, hist1, kHistBins * sizeof(u64), hc.data()));
+
+    // 3. passes: contiguous bin ranges within the budget; a pass walks the
+    // work items of the slices its bins lie in
+    std::vector<u32> cut{0}, pmask;
     std::vector<u64> rn;
+    const u32 sshift = 12 - rbits;  // bin -> slice
     u64 acc = 0, nmax = 0;
+    u32 pm = 0;
     for (u32 d = 0; d < u32(kHistBins); ++d) {
         if (hc[d] > budget)
             return fail(QD_ERR_CAPACITY, "hash-range pass: " + std::to_string(hc[d]) +
@@ -1092,24 +1139,28 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
         if (acc && acc + hc[d] > budget) {
             cut.push_back(d);
             rn.push_back(acc);
+            pmask.push_back(pm);
             nmax = std::max(nmax, acc);
             acc = 0;
+            pm = 0;
         }
         acc += hc[d];
+        if (hc[d]) pm |= 1u << (d >> sshift);
     }
     cut.push_back(kHistBins);
     rn.push_back(acc);
+    pmask.push_back(pm);
     nmax = std::max(nmax, acc);
 
-    // 3. layout of a pass of nmax candidates: b1 first-pass bits (at most 9,
-    // fused), b2 second-pass bits (at most 13), buckets of at most 1024
+    // 4. layout of a pass of nmax candidates: b1 first-pass bits (at most
+    // 11, fused), b2 second-pass bits (at most 13), buckets of at most 1024
     // entries on average
     u32 tgt = ctx->tune.dedup_bucket_target, dcap = ctx->tune.dedup_cap;
     while ((nmax >> 22) > tgt && tgt < 1024) tgt *= 2;
     dcap = std::max(dcap, 2 * tgt);
     u32 L = 0;
     while ((nmax >> L) > tgt && L < 22) ++L;
-    const u32 b1 = std::min<u32>(L, 9), b2 = L - b1;
+    const u32 b1 = std::min<u32>(L, 11), b2 = L - b1;
     const u64 nb1 = 1ull << b1;
     const u32 nparts = u32(std::min<u64>(nb1, 4));
     const u32 pshift = b1 - (nparts == 4 ? 2 : nparts == 2 ? 1 : 0);
@@ -1122,7 +1173,7 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
     const u64 nb2g = gbk << b2;     // sub-buckets of a group
     const double mean2 = double(gcap) / double(std::max<u64>(nb2g, 1));
     const u64 cap2 = (u64(mean2 + 6.0 * std::sqrt(mean2)) + 128 + 63) / 64 * 64;
-    const u64 pcap = std::max<u64>(capped ? max_obj + gcap : C, 1);
+    const u64 pcap = std::max<u64>(capped ? 2 * max_obj + gcap : C, 1);  // trimmed to max_obj beyond 2 max_obj
     auto alloc_parts = [&](u64 c1) -> int {
         for (u32 q = 0; q < 4; ++q) {
             if (q < nparts) QD_TRY(ctx->rpart[q].ensure(perpart * c1 * sizeof(Entry)));
@@ -1189,6 +1240,7 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
     };
 
     std::vector<u64> c1(nb1);
+    phase_mark(ctx, PH_COLL_INSERT);
     for (size_t r = 0; r + 1 < cut.size(); ++r) {
         const u64 n = rn[r];
         if (n == 0) continue;
@@ -1201,10 +1253,14 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
             QD_HIP(hipMemsetAsync(ctx->cur1.p, 0, nb1 * sizeof(u64), s));
             QD_HIP(hipMemsetAsync(errw, 0, sizeof(u32), s));
             prof_begin(ctx, QD_PHASE_GENERATE);
-            hipLaunchKernelGGL(k_generate_range_scatter<KIND>, dim3(gg), dim3(kGenThreads), 0, s,
-                               ctx->pdesc.as<PDesc>(), P.mag, ctx->cb.as<u64>(), ctx->wb.as<u64>(), N,
-                               ctx->cpar.as<u64>(), ctx->tab.as<u64>(), P.begin, rule, nitems, cut[r], cut[r + 1], b1,
-                               cap1r, ctx->cur1.as<u64>(), E, errw);
+            const unsigned gt = unsigned(std::max<u64>(
+                std::min<u64>((nitems + kRangeTile - 1) / kRangeTile, ctx->tune.generate_grid), 1));
+            hipLaunchKernelGGL(k_generate_range_scatter<KIND>, dim3(gt), dim3(kGenThreads), 0, s,
+                               (const PDesc *)ctx->pdesc.as<PDesc>(), P.mag, (const u64 *)ctx->cb.as<u64>(),
+                               (const u64 *)ctx->wb.as<u64>(), N, (const u64 *)ctx->cpar.as<u64>(),
+                               (const u64 *)ctx->tab.as<u64>(), P.begin, rule, nitems,
+                               (const u32 *)ctx->imask.as<u32>(), pmask[r], cut[r], cut[r + 1], b1, cap1r,
+                               ctx->cur1.as<u64>(), E, errw);
             QD_HIP(hipGetLastError());
             prof_end(ctx, QD_PHASE_GENERATE);
             u32 ew = 0;
@@ -1295,9 +1351,11 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
                                t24, ng, Ph, Pr, Pm, Pk, pcount);
             QD_HIP(hipGetLastError());
             QD_TRY(sync_read(ctx, pcount, sizeof(u64), &ns));
-            if (capped && ns > max_obj) QD_TRY(trim_pool());
+            if (capped && ns > 2 * max_obj) QD_TRY(trim_pool());
         }
     }
+    phase_mark(ctx, PH_TRUNC);
+    if (capped && ns > max_obj) QD_TRY(trim_pool());
     st.num_unique = Utot;
     const u64 K = ns;
     st.num_survivors = K;
@@ -1316,6 +1374,7 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
         return QD_OK;
     }
     // order: survivors in ascending representative order
+    phase_mark(ctx, PH_PREP_FINAL);
     u64 *bitmap = ctx->bitmap.as<u64>();
     prof_begin(ctx, QD_PHASE_ORDER);
     QD_HIP(hipMemsetAsync(bitmap, 0, W * sizeof(u64), s));
@@ -1331,7 +1390,9 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
                        (const u64 *)bitmap, (const u64 *)ctx->wprefix.as<u64>(), orep, out->mag.as<double2>());
     QD_HIP(hipGetLastError());
     prof_end(ctx, QD_PHASE_ORDER);
+    phase_mark(ctx, PH_FINAL);
     QD_TRY(populate_out<KIND>(ctx, P, rule, orep, K, prm, out, st));
+    phase_mark(ctx, PH_NORMALIZE);
     QD_TRY(normalize_state(ctx, out));
     prof_collect(ctx);
     st.total_proba = out->total_proba;
@@ -1552,11 +1613,13 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, u64 in_mem, co
     // ties by ascending index, through the 128-bit select on (|m|^2, ~index).
     // With max_num_object == 0 the cap comes from the memory model
     // (src/quids.hpp:459-486).
+    phase_mark(ctx, PH_NUM_CHILD);
     const u64 *keep = nullptr;
     u64 nkept = N;
     u64 pcap = max_obj;
     if (!BATCH && max_obj == 0)
         QD_TRY(mem_parent_cap<KIND>(ctx, P, rule, in_mem, state_mem(out), prm, sk, st, pcap));
+    phase_mark(ctx, PH_TRUNC_SYM);
     if (!BATCH && pcap != 0 && pcap < N) {
         QD_TRY(ctx->sel0.ensure(N * sizeof(SelItem)));
         QD_TRY(ctx->sel1.ensure(N * sizeof(SelItem)));
@@ -1573,6 +1636,7 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, u64 in_mem, co
         nkept = pcap;
     }
     st.num_parent_kept = nkept;
+    phase_mark(ctx, PH_PREP_INDEX);
 
     // ---- 1 prep
     QD_TRY(ctx->nc.ensure(N * sizeof(u32)));
@@ -1696,6 +1760,7 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, u64 in_mem, co
     u64 *hist1 = ctx->hist1.as<u64>();
     QD_HIP(hipMemsetAsync(hist1, 0, kHistBins * sizeof(u64), s));
     QD_HIP(hipMemsetAsync(ctx->nhist.p, 0, kHistBins * sizeof(u64), s));
+    phase_mark(ctx, PH_SYMBOLIC);
     u64 in_stride = 0;
     if (fused) {
         const unsigned gg =
@@ -1752,7 +1817,9 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, u64 in_mem, co
     u64 *UH, *UR;
     double2 *UM;
     u64 U;
+    phase_mark(ctx, PH_COLL_PREP);
     QD_TRY(partition_dedup(ctx, H0, M0, nullptr, C, b1, b2, prm.tolerance, st, UH, UR, UM, U, in_stride, UK));
+    phase_mark(ctx, PH_COLL_FINAL);
     st.num_unique = U;
 
     // memory-driven survivor cap (max_num_object == 0, src/quids.hpp:510-536)
@@ -1760,6 +1827,7 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, u64 in_mem, co
                                                                st, max_obj));
 
     // ---- 6 select + 7 order
+    phase_mark(ctx, PH_TRUNC);
     const u64 W = (C + 63) / 64;
     QD_TRY(ctx->bitmap.ensure(W * sizeof(u64)));
     QD_TRY(ctx->wprefix.ensure((W + 1) * sizeof(u64)));
@@ -1786,6 +1854,7 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, u64 in_mem, co
         QD_TRY(select_topk(ctx, UH, UR, UM, UK, C, max_obj, h, bitmap, sbm, sk));
     }
     prof_end(ctx, QD_PHASE_SELECT);
+    phase_mark(ctx, PH_PREP_FINAL);
     prof_begin(ctx, QD_PHASE_ORDER);
     QD_TRY(run_scan(ctx, LoadPopc{bitmap}, W, ctx->wprefix.as<u64>()));
     prof_end(ctx, QD_PHASE_ORDER);
@@ -1806,8 +1875,10 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, u64 in_mem, co
     hipLaunchKernelGGL(k_place, dim3(grid_for((C + 63) / 64, 256, 4096)), dim3(256), 0, s, UR, UM, (const u64 *)sbm, C, bitmap,
                        ctx->wprefix.as<u64>(), orep, out->mag.as<double2>());
     prof_end(ctx, QD_PHASE_ORDER);
+    phase_mark(ctx, PH_FINAL);
     QD_TRY(populate_out<KIND>(ctx, P, rule, orep, K, prm, out, st));
     // ---- 9 normalize
+    phase_mark(ctx, PH_NORMALIZE);
     if (BATCH) QD_TRY(batch_normalize(ctx, bt, out));
     else QD_TRY(normalize_state(ctx, out));
     prof_collect(ctx);
@@ -2513,6 +2584,7 @@ int qd_simulate(qd_ctx *c, qd_state *in, const qd_rule *rule, qd_state *out, qd_
     qd_step_stats st;
     std::memset(&st, 0, sizeof st);
     QD_HIP(hipSetDevice(c->device));
+    c->next_phase = 0;
     int rc;
     switch (rule->kind) {
     case QD_RULE_HADAMARD: rc = simulate_kind<QD_RULE_HADAMARD>(c, in, *rule, out, sy, max_num_object, prm, st); break;
@@ -2525,8 +2597,16 @@ int qd_simulate(qd_ctx *c, qd_state *in, const qd_rule *rule, qd_state *out, qd_
         break;
     default: return fail(QD_ERR_UNSUPPORTED, "rule kind has no device implementation");
     }
+    if (rc == QD_OK) phase_mark(c, PH_END);
     if (stats) *stats = st;
     return rc;
+}
+
+int qd_ctx_set_phase_hook(qd_ctx *c, qd_phase_hook hook, void *user) {
+    if (!c) return fail(QD_ERR_INVALID, "null ctx");
+    c->hook = hook;
+    c->hook_user = hook ? user : nullptr;
+    return QD_OK;
 }
 
 int qd_apply_modifier(qd_ctx *c, qd_state *s, const qd_modifier *m) {
@@ -2581,6 +2661,7 @@ int qd_simulate_batched(qd_ctx *c, qd_state *in, const uint64_t *in_seg, uint32_
     qd_params prm{1e-30, 8, 1, 0, 0, 0.2, 0.25};
     if (params) prm = *params;
     QD_HIP(hipSetDevice(c->device));
+    c->next_phase = 0;
     switch (kind) {
     case QD_RULE_QCGD_ERASE_CREATE:
         return batched_kind<QD_RULE_QCGD_ERASE_CREATE>(c, in, in_seg, n, rules, out, out_seg, max_num_object, prm, stats);

@@ -324,14 +324,16 @@ inline void simulate(it_t &iteration, rule_t const *rule, it_t &next_iteration, 
     if (!rule || !rule->device_rule(d)) throw std::runtime_error("simulate: rule has no device implementation");
     iteration.push();
     qd_params p = detail::params();
-    mid_step_function("num_child");
-    detail::check(qd_simulate(detail::context(), iteration.state_, &d, next_iteration.state_, symbolic_iteration.sy_,
-                              uint64_t(max_num_object), &p, &symbolic_iteration.stats),
-                  "qd_simulate");
+    // the library calls the hook between its phases, with the reference's names
+    auto trampoline = [](const char *phase, void *user) { (*static_cast<debug_t *>(user))(phase); };
+    detail::check(qd_ctx_set_phase_hook(detail::context(), trampoline, &mid_step_function), "qd_ctx_set_phase_hook");
+    const int rc = qd_simulate(detail::context(), iteration.state_, &d, next_iteration.state_, symbolic_iteration.sy_,
+                               uint64_t(max_num_object), &p, &symbolic_iteration.stats);
+    qd_ctx_set_phase_hook(detail::context(), nullptr, nullptr);
+    detail::check(rc, "qd_simulate");
     symbolic_iteration.num_object = symbolic_iteration.stats.num_candidates;
     symbolic_iteration.num_object_after_interferences = symbolic_iteration.stats.num_unique;
     next_iteration.device_changed();
-    mid_step_function("end");
 }
 
 }  // namespace quids
