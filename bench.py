@@ -417,7 +417,7 @@ def main():
     ap.add_argument("--log2-cap", type=int, default=26)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=120.0)
-    ap.add_argument("--cpu-log2-cap", type=int, default=20,
+    ap.add_argument("--cpu-log2-cap", type=int, default=18,
                     help="cap of the CPU baseline's run (the reference needs about 60 bytes of host memory per "
                          "candidate, so the GPU's cap does not fit a host)")
     ap.add_argument("--copy-probe", type=float, default=0,

@@ -273,10 +273,11 @@ using Walk = WalkT<cplx>;
 // split_merge child walk (populate_child, qcgd.hpp:643-848) over the
 // parent's masks and root-hash table.  Consumes decisions d while d <
 // dlimit, the bit of decision d being bit (d - d0) of `bits` where d0 is
-// w.d on entry; returns when the next decision is not available or the walk
-// is complete.
-template <typename W, typename KC>
-__device__ inline void sm_run(W &w, const PDesc &pd, const u64 *tab, u32 bits, u32 dlimit, const KC &k) {
+// w.d on entry (`bits` a u32, or a u64 for a walk over more than 32
+// decisions at once); returns when the next decision is not available or
+// the walk is complete.
+template <typename W, typename KC, typename BT>
+__device__ inline void sm_run(W &w, const PDesc &pd, const u64 *tab, BT bits, u32 dlimit, const KC &k) {
     const u32 n = pd.n;
     const u32 d0 = w.d;
     auto emit = [&](u64 root, bool l, bool r) {
@@ -366,8 +367,8 @@ __device__ inline u64 sm_finish(W &w, const u64 *tab) {
 }
 
 // erase_create / coin child walk (qcgd.hpp:487-510, 562-584).
-template <bool COIN, typename W, typename KC>
-__device__ inline void flip_run(W &w, const PDesc &pd, u32 bits, u32 dlimit, const KC &k) {
+template <bool COIN, typename W, typename KC, typename BT>
+__device__ inline void flip_run(W &w, const PDesc &pd, BT bits, u32 dlimit, const KC &k) {
     const u32 n = pd.n;
     const u32 d0 = w.d;
     while (w.i < n) {
@@ -640,26 +641,33 @@ __global__ __launch_bounds__(kGenThreads, 3) void k_generate_scatter(const PDesc
 
 // ---------------------------------------------------------------------------
 // Hash-range passes, for steps whose candidates exceed the scratch memory.
-// The counting pass histograms the top 12 bits of every child hash into
-// hist (4096 bins) and records, per work item, which of 2^rbits equal slices
-// of the hash space (its top rbits bits) its children fall in (imask, one
-// bit per slice).  A range pass then walks only the work items whose mask
-// meets the pass's slices, so that across the passes a work item is walked
-// at most once per distinct slice of its children (at most 8 times), not
-// once per pass.
+// The counting pass walks every work item once: it histograms the top 12
+// bits of every child hash into hist (4096 bins) and tags each child with
+// its slice of the hash space, slice = bin * S >> 12 for S = 2^T - 1 equal
+// slices (T <= 8); the tag S marks a child the work item does not have.
+// The tags are stored as T bit planes of one byte per work item: bit u of
+// plane b's byte for item w is bit b of child u's tag.  A range pass then
+// lists, from the planes alone, exactly the children of the slices it
+// covers and walks each of them on its own lane, so that across the passes
+// every child is walked once (once per pass of its slice when a slice is
+// split over passes), not once per work item that has a child in the pass.
 // ---------------------------------------------------------------------------
 __device__ inline u64 rotl12(u64 h) { return (h << 12) | (h >> 52); }
 __device__ inline u64 rotr12(u64 k) { return (k >> 12) | (k << 52); }
 
+constexpr u32 kMaxTagBits = 8;
+
 template <int KIND>
 __global__ __launch_bounds__(kGenThreads) void k_range_count(const PDesc *D, const double2 *pmag, const u64 *wb, u64 N,
                                                             const u64 *cpar, const u64 *tab, const u64 *pbegin,
-                                                            qd_rule rule, u64 W, u32 rbits, u32 *imask, u64 *hist) {
+                                                            qd_rule rule, u64 W, u32 T, u8 *planes, u64 pstride,
+                                                            u64 *hist) {
     __shared__ u32 lh[kHistBins];
     __shared__ u64 swb[kGenThreads + 1];
     __shared__ u64 pfirst;
     for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) lh[i] = 0;
     const QcgdCoef k = coef_of(rule);
+    const u32 S = (1u << T) - 1;
     for (u64 w0 = u64(blockIdx.x) * kGenThreads; w0 < W; w0 += u64(gridDim.x) * kGenThreads) {
         __syncthreads();
         if (threadIdx.x == 0) pfirst = cpar[w0 / kGenThreads];
@@ -691,111 +699,146 @@ __global__ __launch_bounds__(kGenThreads) void k_range_count(const PDesc *D, con
         const u64 *ptab = tab + sm_tab_base(pbegin[p]);
         if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(pre, pd, ptab, j0, s, k);
         else flip_run<KIND == QD_RULE_QCGD_COIN>(pre, pd, j0, s, k);
-        u32 mask = 0;
-        walk_item<KIND>(pre, pd, ptab, s, t, k, [&](u32, u64 h, const cplx &) {
-            atomicAdd(&lh[u32(h >> 52)], 1u);
-            mask |= 1u << (rbits ? u32(h >> (64 - rbits)) : 0u);
+        u32 tg[1u << kItemBits];
+#pragma unroll
+        for (u32 u = 0; u < (1u << kItemBits); ++u) tg[u] = S;
+        walk_item<KIND>(pre, pd, ptab, s, t, k, [&](u32 u, u64 h, const cplx &) {
+            const u32 bin = u32(h >> 52);
+            atomicAdd(&lh[bin], 1u);
+            tg[u] = (bin * S) >> 12;
         });
-        imask[w] = mask;
+#pragma unroll
+        for (u32 b = 0; b < kMaxTagBits; ++b) {
+            if (b >= T) break;
+            u32 byte = 0;
+#pragma unroll
+            for (u32 u = 0; u < (1u << kItemBits); ++u) byte |= ((tg[u] >> b) & 1u) << u;
+            planes[u64(b) * pstride + w] = u8(byte);
+        }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kHistBins; i += blockDim.x)
         if (lh[i]) atomicAdd(reinterpret_cast<unsigned long long *>(hist + i), (unsigned long long)lh[i]);
 }
 
+// Bit-sliced x < c over 64 tags at once: x[b] holds bit b of each tag.
+__device__ inline u64 tags_below(const u64 *x, u32 T, u32 c) {
+    u64 lt = 0, eq = ~0ull;
+#pragma unroll
+    for (int b = int(kMaxTagBits) - 1; b >= 0; --b) {
+        if (u32(b) >= T) continue;
+        if ((c >> b) & 1u) {
+            lt |= eq & ~x[b];
+            eq &= x[b];
+        } else {
+            eq &= ~x[b];
+        }
+    }
+    return lt;
+}
+
 // ---------------------------------------------------------------------------
-// A hash-range pass fused with its first radix pass.  A block takes tiles of
-// kRangeTile work items, keeps those whose mask meets pmask (compacted in
-// LDS), and walks them a block-width at a time: the children whose top 12
-// hash bits lie in [rlo, rhi) are ranked per bucket of the top b1 bits of
-// their key (the hash rotated left by 12) in LDS, as k_generate_scatter
-// ranks all children, and written as 32-byte entries into fixed-capacity
-// buckets of cap1 entries.  The buckets are split over up to four
-// allocations (EParts); cnt1[d] counts bucket d's children, also the ones a
-// full bucket drops (then ERR_SPILL1 is raised and the pass is redone with
-// larger buckets).
+// A hash-range pass, fused with its first radix pass, in two walks without
+// global atomics.  Block b of a fixed grid takes the tiles b, b + grid, ...
+// of kRangeTile work items.  For each tile it lists the children whose tag
+// lies in the pass's slices [slo, shi) (a thread reads the planes of 16
+// consecutive items; the list is in tile order) and walks each listed child
+// on its own lane.  In the first walk it counts the children whose top 12
+// hash bits lie in [rlo, rhi) per bucket of the top b1 bits of their key
+// (the hash rotated left by 12) and writes its counts as row b of bmat.  A
+// column scan turns bmat into every block's first slot per bucket; the
+// second walk visits the same children again and writes each as a 32-byte
+// entry at its block's next slot of its bucket.  The buckets are exactly
+// sized and lie back to back, split over up to four allocations by bucket
+// (EParts: bucket d in part d >> pshift, at its offset within the part).
 // ---------------------------------------------------------------------------
 struct EParts {
     Entry *p[4];
     u32 pshift;  // bucket d lives in part d >> pshift
 };
-__device__ inline Entry *ebucket(const EParts &E, u32 d, u64 cap1) {
-    return E.p[d >> E.pshift] + u64(d & ((1u << E.pshift) - 1)) * cap1;
-}
 
-constexpr u32 kRangeBins = 2048;  // first-pass buckets of a range pass, at most
-constexpr u32 kRangeTile = 2048;  // work items per tile
-static_assert(kRangeTile % kGenThreads == 0, "tiles of whole chunks");
+constexpr u32 kRangeBins = 1024;  // first-pass buckets of a range pass, at most
+constexpr u32 kRangeTile = 4096;  // work items per tile
+constexpr u32 kRangePer = kRangeTile / kGenThreads;  // consecutive items a thread lists (16)
+constexpr u32 kRangeList = 4096;  // listed children walked per round of a tile
+constexpr u32 kRangeWin = 512;    // parents of the LDS window of wb (beyond: a global search)
+constexpr u32 kRangeGrid = 2048;  // blocks of the two walks (rows of bmat)
+static_assert(kRangePer % 8 == 0 && kRangeTile % kGenThreads == 0, "tiles of whole chunks, 8-item plane words");
+static_assert(kRangeTile * (1u << kItemBits) <= 65536, "list entries are 16-bit");
 
-template <int KIND>
-__global__ __launch_bounds__(kGenThreads, 3) void k_generate_range_scatter(const PDesc *D, const double2 *pmag,
-                                                                       const u64 *cb, const u64 *wb, u64 N,
-                                                                       const u64 *cpar, const u64 *tab,
-                                                                       const u64 *pbegin, qd_rule rule, u64 W,
-                                                                       const u32 *imask, u32 pmask, u32 rlo,
-                                                                       u32 rhi, u32 b1, u64 cap1, u64 *cnt1,
-                                                                       EParts E, u32 *err) {
-    constexpr u32 kKids = 1u << kItemBits;
-    constexpr u32 kPer = kRangeTile / kGenThreads;
-    constexpr u32 kWaves = kGenThreads / kWave;
-    __shared__ u32 bcnt[kRangeBins];
-    __shared__ u64 bbase[kRangeBins];  // first reserved slot of the bucket
-    __shared__ u64 swb[kRangeTile + 1];
-    __shared__ u16 tl[kRangeTile];     // the tile's walked items (offsets in the tile)
-    __shared__ u32 woff[kWaves * kPer + 1];
-    __shared__ u64 pfirst;
-    const u32 nbin = 1u << b1;
-    const u32 lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-    const QcgdCoef k = coef_of(rule);
-    bool spill = false;
+struct RangeSel {
+    const u8 *planes;  // T planes of pstride bytes (pstride a multiple of kRangeTile)
+    u64 pstride;
+    u32 T, slo, shi;   // tag bits; the pass's slices
+    u32 rlo, rhi;      // its range of the top 12 hash bits
+};
+
+struct RangeLds {
+    u64 swb[kRangeWin + 1];
+    u16 list[kRangeList];
+    u64 tmp[kGenThreads / kWave + 1];
+    u64 pfirst;
+};
+
+// The walks' shared structure: leaf(key, magnitude, candidate) for every
+// child of the pass in the range, in a fixed order per block.
+template <int KIND, class Leaf>
+__device__ __attribute__((always_inline)) inline void range_walks(const PDesc *D, const double2 *pmag, const u64 *cb,
+                                                                  const u64 *wb, u64 N, const u64 *cpar,
+                                                                  const u64 *tab, const u64 *pbegin,
+                                                                  const QcgdCoef &k, u64 W, const RangeSel &rs,
+                                                                  RangeLds &L, Leaf &&leaf) {
     for (u64 t0 = u64(blockIdx.x) * kRangeTile; t0 < W; t0 += u64(gridDim.x) * kRangeTile) {
         __syncthreads();
-        if (threadIdx.x == 0) pfirst = cpar[t0 / kGenThreads];
-        // the tile's items whose children meet the pass's slices, compacted
-        // in tile order: per (round j, wavefront) run, a ballot count
-        u64 mk[kPer];
+        if (threadIdx.x == 0) L.pfirst = cpar[t0 / kGenThreads];
+        // this thread's items t0 + kRangePer * tid + [0, kRangePer): 64 child
+        // bits per word of 8 items (bit 8i + u: item i, child u)
+        u64 in[kRangePer / 8];
+        u32 cnt = 0;
 #pragma unroll
-        for (u32 j = 0; j < kPer; ++j) {
-            const u64 w = t0 + j * kGenThreads + threadIdx.x;
-            mk[j] = __ballot(w < W && (imask[w] & pmask));
-            if (lane == 0) woff[j * kWaves + wid] = u32(__popcll(mk[j]));
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            u32 acc = 0;
-            for (u32 q = 0; q < kWaves * kPer; ++q) {
-                const u32 c = woff[q];
-                woff[q] = acc;
-                acc += c;
+        for (u32 q = 0; q < kRangePer / 8; ++q) {
+            const u64 w8 = t0 + u64(threadIdx.x) * kRangePer + 8 * q;
+            in[q] = 0;
+            if (w8 < W) {
+                u64 x[kMaxTagBits];
+#pragma unroll
+                for (u32 b = 0; b < kMaxTagBits; ++b)
+                    x[b] = b < rs.T ? *reinterpret_cast<const u64 *>(rs.planes + u64(b) * rs.pstride + w8) : 0ull;
+                in[q] = ~tags_below(x, rs.T, rs.slo) & tags_below(x, rs.T, rs.shi);
+                if (W - w8 < 8) in[q] &= (1ull << (8 * (W - w8))) - 1;
             }
-            woff[kWaves * kPer] = acc;
+            cnt += u32(__popcll(in[q]));
         }
-        __syncthreads();
-        const u64 pf = pfirst;
-        for (u32 q = threadIdx.x; q <= kRangeTile; q += blockDim.x) swb[q] = pf + q <= N ? wb[pf + q] : ~0ull;
+        u64 total;
+        const u32 first = u32(block_excl_scan(cnt, L.tmp, total));
+        const u64 pf = L.pfirst;
+        for (u32 q = threadIdx.x; q <= kRangeWin; q += blockDim.x) L.swb[q] = pf + q <= N ? wb[pf + q] : ~0ull;
+        for (u32 base = 0; base < u32(total); base += kRangeList) {
+            if (base) __syncthreads();  // the previous round's list is walked
+            u32 o = first;
 #pragma unroll
-        for (u32 j = 0; j < kPer; ++j)
-            if ((mk[j] >> lane) & 1)
-                tl[woff[j * kWaves + wid] + __popcll(mk[j] & ((1ull << lane) - 1))] = u16(j * kGenThreads + threadIdx.x);
-        __syncthreads();
-        const u32 nt = woff[kWaves * kPer];
-        for (u32 r0 = 0; r0 < nt; r0 += kGenThreads) {
-            for (u32 i = threadIdx.x; i < nbin; i += blockDim.x) bcnt[i] = 0;
+            for (u32 q = 0; q < kRangePer / 8; ++q) {
+                u64 m = in[q];
+                while (m) {
+                    const u32 bit = u32(__builtin_ctzll(m));
+                    m &= m - 1;
+                    if (o >= base && o < base + kRangeList)
+                        L.list[o - base] = u16((threadIdx.x * kRangePer + 8 * q) * 8 + bit);
+                    ++o;
+                }
+            }
             __syncthreads();
-            // kept children (bit u), indexed by the unrolled child counter only
-            u64 hs[kKids];
-            double2 ms[kKids];
-            u32 br[kKids];
-            u32 keep = 0, s = 0;
-            u64 cbase = 0;
-            if (r0 + threadIdx.x < nt) {
-                const u64 w = t0 + tl[r0 + threadIdx.x];
+            const u32 nl = min(u32(total) - base, kRangeList);
+            for (u32 i = threadIdx.x; i < nl; i += kGenThreads) {
+                const u32 e = L.list[i];
+                const u64 w = t0 + (e >> 3);
+                const u32 u = e & 7u;
                 u64 p;
-                if (swb[kRangeTile] > w) {
-                    u32 lo = 0, hi = kRangeTile;
+                if (L.swb[kRangeWin] > w) {
+                    u32 lo = 0, hi = kRangeWin;
                     while (lo + 1 < hi) {
                         const u32 mid = (lo + hi) >> 1;
-                        if (swb[mid] <= w) lo = mid;
+                        if (L.swb[mid] <= w) lo = mid;
                         else hi = mid;
                     }
                     p = pf + lo;
@@ -803,54 +846,98 @@ __global__ __launch_bounds__(kGenThreads, 3) void k_generate_range_scatter(const
                     p = find_parent(wb, N, w);
                 }
                 const PDesc pd = D[p];
-                const u32 j0 = u32(w - wb[p]);
                 const u32 t = min(u32(pd.k), kItemBits);
-                s = pd.k - t;
+                if (u >> t) continue;  // tagged as absent: never listed
+                const u32 s = pd.k - t;
+                const u32 j0 = u32(w - wb[p]);
                 const double2 m0 = pmag[p];
-                Walk pre{};
-                pre.mag = cplx{m0.x, m0.y};
+                Walk wk{};
+                wk.mag = cplx{m0.x, m0.y};
                 const u64 *ptab = tab + sm_tab_base(pbegin[p]);
-                if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(pre, pd, ptab, j0, s, k);
-                else flip_run<KIND == QD_RULE_QCGD_COIN>(pre, pd, j0, s, k);
-                cbase = cb[p] + j0;
-                walk_item<KIND>(pre, pd, ptab, s, t, k, [&](u32 u, u64 h, const cplx &m) {
-                    const u32 top = u32(h >> 52);
-                    if (top >= rlo && top < rhi) {
-                        const u64 key = rotl12(h);
-                        const u32 d = b1 ? u32(key >> (64 - b1)) : 0u;
-                        hs[u] = key;
-                        ms[u] = make_double2(m.re, m.im);
-                        br[u] = (d << 16) | atomicAdd(&bcnt[d], 1u);
-                        keep |= 1u << u;
-                    }
-                });
+                // the child's own branch of walk_item's tree in one walk over
+                // all its decisions (the same folds): one loop over the nodes,
+                // so that lanes of different parents wait for each other
+                // once, not once per segment
+                const u64 bits = u64(j0) | (u64(u) << s);
+                if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(wk, pd, ptab, bits, s + t, k);
+                else flip_run<KIND == QD_RULE_QCGD_COIN>(wk, pd, bits, s + t, k);
+                u64 h;
+                if (KIND == QD_RULE_QCGD_SPLIT_MERGE) h = sm_finish(wk, ptab);
+                else h = finish_pm(pd.nf, wk.lh, wk.rh);
+                const u32 top = u32(h >> 52);
+                if (top >= rs.rlo && top < rs.rhi) leaf(rotl12(h), wk.mag, cb[p] + j0 + (u64(u) << s));
             }
-            __syncthreads();
-            for (u32 i = threadIdx.x; i < nbin; i += blockDim.x)
-                if (bcnt[i])
-                    bbase[i] = atomicAdd(reinterpret_cast<unsigned long long *>(cnt1 + i), (unsigned long long)bcnt[i]);
-            __syncthreads();
-#pragma unroll
-            for (u32 u = 0; u < kKids; ++u) {
-                if (keep & (1u << u)) {
-                    const u32 d = br[u] >> 16;
-                    const u64 at = bbase[d] + (br[u] & 0xFFFF);
-                    if (at < cap1) {
-                        Entry e;
-                        e.hash = hs[u];
-                        e.cand = cbase + (u64(u) << s);
-                        e.re = ms[u].x;
-                        e.im = ms[u].y;
-                        ebucket(E, d, cap1)[at] = e;
-                    } else {
-                        spill = true;
-                    }
-                }
-            }
-            __syncthreads();  // bcnt / bbase are rewritten by the next round
         }
     }
-    if (spill) atomicOr(err, ERR_SPILL1);
+}
+
+// First walk: per-block counts of the range's children per bucket, as row
+// blockIdx.x of bmat (nbin columns).
+template <int KIND>
+__global__ __launch_bounds__(kGenThreads) void k_range_bincount(const PDesc *D, const double2 *pmag, const u64 *cb,
+                                                                const u64 *wb, u64 N, const u64 *cpar,
+                                                                const u64 *tab, const u64 *pbegin, qd_rule rule,
+                                                                u64 W, RangeSel rs, u32 b1, u32 *bmat) {
+    __shared__ u32 bcnt[kRangeBins];
+    __shared__ RangeLds L;
+    const u32 nbin = 1u << b1;
+    for (u32 i = threadIdx.x; i < nbin; i += blockDim.x) bcnt[i] = 0;
+    const QcgdCoef k = coef_of(rule);
+    range_walks<KIND>(D, pmag, cb, wb, N, cpar, tab, pbegin, k, W, rs, L,
+                      [&](u64 key, const cplx &, u64) { atomicAdd(&bcnt[b1 ? u32(key >> (64 - b1)) : 0u], 1u); });
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < nbin; i += blockDim.x) bmat[u64(blockIdx.x) * nbin + i] = bcnt[i];
+}
+
+// Column scan of bmat (rows = the walks' blocks): every block's first slot
+// in bucket d, counted from the bucket's start; total[d] = the bucket's
+// size.  One workgroup per bucket.
+__global__ __launch_bounds__(256) void k_range_colscan(u32 *bmat, u32 rows, u32 nbin, u64 *total) {
+    __shared__ u64 tmp[256 / kWave + 1];
+    const u32 d = blockIdx.x;
+    constexpr u32 kPer = kRangeGrid / 256;
+    u32 v[kPer];
+    u64 sum = 0;
+#pragma unroll
+    for (u32 q = 0; q < kPer; ++q) {
+        const u32 r = threadIdx.x * kPer + q;
+        v[q] = r < rows ? bmat[u64(r) * nbin + d] : 0u;
+        sum += v[q];
+    }
+    u64 tot;
+    u64 run = block_excl_scan(sum, tmp, tot);
+#pragma unroll
+    for (u32 q = 0; q < kPer; ++q) {
+        const u32 r = threadIdx.x * kPer + q;
+        if (r < rows) bmat[u64(r) * nbin + d] = u32(run);
+        run += v[q];
+    }
+    if (threadIdx.x == 0) total[d] = tot;
+}
+
+// Second walk: the same children, each written at its block's next slot of
+// its bucket (boff[d]: the bucket's first entry within its part).
+template <int KIND>
+__global__ __launch_bounds__(kGenThreads) void k_range_binwrite(const PDesc *D, const double2 *pmag, const u64 *cb,
+                                                                const u64 *wb, u64 N, const u64 *cpar,
+                                                                const u64 *tab, const u64 *pbegin, qd_rule rule,
+                                                                u64 W, RangeSel rs, u32 b1, const u32 *bmat,
+                                                                const u64 *boff, EParts E) {
+    __shared__ u64 cur[kRangeBins];
+    __shared__ RangeLds L;
+    const u32 nbin = 1u << b1;
+    for (u32 i = threadIdx.x; i < nbin; i += blockDim.x) cur[i] = boff[i] + bmat[u64(blockIdx.x) * nbin + i];
+    const QcgdCoef k = coef_of(rule);
+    range_walks<KIND>(D, pmag, cb, wb, N, cpar, tab, pbegin, k, W, rs, L, [&](u64 key, const cplx &m, u64 cand) {
+        const u32 d = b1 ? u32(key >> (64 - b1)) : 0u;
+        const u64 at = atomicAdd(reinterpret_cast<unsigned long long *>(&cur[d]), 1ull);
+        Entry e;
+        e.hash = key;
+        e.cand = cand;
+        e.re = m.re;
+        e.im = m.im;
+        E.p[d >> E.pshift][at] = e;
+    });
 }
 
 }  // namespace qd

@@ -35,6 +35,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -181,7 +182,8 @@ struct qd_ctx {
     DBuf adjk, adjv, adjg, adjd, adjp, adjw;
     DBuf pool0, pool1, pwpre, blist2;
     DBuf mw;  // per-object memory of memory-driven truncation
-    DBuf rpart[4], pk0, pk1, imask;  // hash-range passes: first-pass buckets, pool select keys, slice masks
+    DBuf rpart[4], pk0, pk1, tags, bmat, goff;  // hash-range passes: first-pass buckets, pool select keys,
+                                                // children's slice tags, per-block bucket counts, group offsets
     // output buffers of batched chunks (objects, begin, size, mag per chunk),
     // kept between calls
     std::vector<DBuf> parts;
@@ -197,7 +199,7 @@ struct qd_ctx {
                 &biglist, &redolist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &cpar, &opar, &ebuf2, &cnt2, &slist, &ukey, &selloop, &segloop, &bsoff, &bscnt,
                 &adjk, &adjv, &adjg, &adjd, &adjp, &adjw, &bseg, &boseg, &brules, &bsinst, &bcount, &bsel, &btrunc,
                 &blist, &bcur, &btile, &bhist, &bkseg, &bchunk, &bcsum, &btot, &pool0, &pool1, &pwpre, &blist2, &mw, &rpart[0], &rpart[1], &rpart[2],
-                &rpart[3], &pk0, &pk1, &imask};
+                &rpart[3], &pk0, &pk1, &tags, &bmat, &goff};
     }
 };
 
@@ -1040,6 +1042,14 @@ static int mem_survivor_cap(qd_ctx *ctx, const Parents &P, const qd_rule &rule, 
 // hash-range passes (qd_k_range.hpp): steps whose candidates exceed the
 // scratch memory
 // ===========================================================================
+// Copies host bytes into a device buffer (grown as needed).
+static int upload(qd_ctx *ctx, DBuf &b, const void *src, size_t bytes) {
+    QD_TRY(b.ensure(std::max<size_t>(bytes, 8)));
+    if (bytes) QD_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    QD_HIP(hipStreamSynchronize(ctx->stream));  // the host source may go away
+    return QD_OK;
+}
+
 // dst[i] = src[i] + add (mod 2^64) for i < n.
 __global__ __launch_bounds__(256) void k_add_offset(const u64 *src, u64 n, u64 add, u64 *dst) {
     for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += u64(gridDim.x) * blockDim.x)
@@ -1047,12 +1057,12 @@ __global__ __launch_bounds__(256) void k_add_offset(const u64 *src, u64 n, u64 a
 }
 
 // Per-candidate bytes of a hash-range pass: the first radix pass's entry
-// (32 bytes plus its buckets' slack), and an eighth (one group of
+// (32 bytes, buckets exactly sized), and an eighth (one group of
 // first-level buckets at a time) of the second pass's entry at its
 // sub-buckets' slack and of the group's 32-bit select keys and pool
 // appends.
 constexpr u32 kRangeGroups = 8;
-constexpr double kRangeBytesPerCand = 32.6 + (32.0 * 1.35 + 4.0 + 36.0) / kRangeGroups;
+constexpr double kRangeBytesPerCand = 32.0 + (32.0 * 2.05 + 4.0 + 36.0) / kRangeGroups;
 
 template <int KIND>
 static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule &rule, u64 C, u64 nitems,
@@ -1063,14 +1073,18 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
     u64 *counters = ctx->counters.as<u64>();
     u32 *errw = reinterpret_cast<u32 *>(counters + 4);
     u64 *hb = ctx->host;
-    // the single pass's entry buffers are not used here: give their memory
-    // to the passes
-    ctx->ebuf0.release();
-    ctx->ebuf1.release();
-    ctx->ukey.release();
+    // what only the single-pass path and the adjoint use is not needed here:
+    // give its memory to the passes (the helpers re-allocate what they use,
+    // at this path's sizes).  A batched step that runs one instance here
+    // holds its plan and earlier chunks' outputs in the b* buffers and
+    // parts: those stay.
+    for (DBuf *b : {&ctx->ebuf0, &ctx->ebuf1, &ctx->ukey, &ctx->kept, &ctx->keys, &ctx->sel0, &ctx->sel1, &ctx->padded,
+                    &ctx->chunks, &ctx->biglist, &ctx->redolist, &ctx->bigsoff, &ctx->bigkey, &ctx->bigcand,
+                    &ctx->bigmag, &ctx->bigseg, &ctx->opar, &ctx->adjk, &ctx->adjv, &ctx->adjg, &ctx->adjd,
+                    &ctx->adjp, &ctx->adjw})
+        b->release();
     if (KIND != QD_RULE_QCGD_SPLIT_MERGE) ctx->tab.release();  // split_merge's name-root table
     QD_TRY(ctx->hist1.ensure(kHistBins * sizeof(u64)));
-    QD_TRY(ctx->off1.ensure((kRangeBins + 1) * sizeof(u64)));
     QD_TRY(ctx->cur1.ensure(kRangeBins * sizeof(u64)));
     QD_TRY(ctx->nhist.ensure(kHistBins * sizeof(u64)));
     u64 *hist1 = ctx->hist1.as<u64>();
@@ -1079,109 +1093,159 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
     const bool capped = max_obj < C;
     const bool ukeys = !sk.prob;  // 32-bit |m|^2 select keys (plain truncation keys only)
     const u64 W = (C + 63) / 64;
+    const u64 pstride = (nitems + kRangeTile - 1) / kRangeTile * kRangeTile;  // bytes of a tag plane
 
     // 1. the pass size: what the free memory (and the buffers this path
-    // reuses) holds after the work items' slice masks, the bitmaps, the pool
-    // and the next state, within the second radix pass's reach (2^22 buckets
-    // of at most 1024 entries) and the four first-pass allocations
+    // reuses) holds after the children's tags, the bitmaps, the pool and the
+    // next state, within the second radix pass's reach (2^22 buckets of at
+    // most 1024 entries) and the four first-pass allocations; then the tag
+    // width: slices of at most about the pass size
     u64 budget;
+    u32 T = 4;
+    auto tag_bits = [&](u64 b) {
+        u32 t = 4;
+        while (t < kMaxTagBits && double((1u << t) - 1) * double(b) < 1.1 * double(C)) ++t;
+        return t;
+    };
+    double log_free = 0, log_reuse = 0, log_fixed = 0;  // the plan's inputs, for QD_LOG_RANGE
     if (ctx->tune.chunk_log2) {
         budget = 1ull << ctx->tune.chunk_log2;
+        T = tag_bits(budget);
     } else {
         size_t freeb = 0, totalb = 0;
         QD_HIP(hipMemGetInfo(&freeb, &totalb));
         u64 reuse = ctx->ebuf2.cap + ctx->pool0.cap + ctx->pool1.cap + ctx->pk0.cap + ctx->pk1.cap + ctx->bitmap.cap +
-                    ctx->wprefix.cap + ctx->imask.cap;
+                    ctx->wprefix.cap + ctx->tags.cap + ctx->bmat.cap + ctx->cnt2.cap + ctx->off2.cap + ctx->slist.cap +
+                    ctx->pwpre.cap;
         for (DBuf &b : ctx->rpart) reuse += b.cap;
         // the next state with its finalize scratch (objects at 1.5 x the
         // parents' mean size), beyond what the output state holds already
         const double K = double(capped ? max_obj : C);
         const double next = std::max(0.0, 1.5 * double(in_bytes) * std::min(1.0, K / double(std::max<u64>(N, 1))) +
                                               60.0 * K - double(state_mem(out)));
-        const double fixed = 16.0 * double(W) + 4.0 * double(nitems) + (capped ? 3.0 * 36.0 * K : 36.0 * K) + next +
-                             double(2ull << 30);
+        const double fixed = 16.0 * double(W) + (capped ? 3.0 * 36.0 * K : 36.0 * K) + next + double(2ull << 30);
         const double room = 0.85 * (double(freeb) + double(reuse)) - fixed;
-        budget = room > 0 ? u64(room / kRangeBytesPerCand) : 0;
-        budget = std::min<u64>(budget, u64(4.0 * double(kBigAlloc) / 33.0));
+        auto fit = [&](u32 t) {
+            const double r = room - double(t) * double(pstride);
+            return r > 0 ? u64(std::min(r / kRangeBytesPerCand, 4.0 * double(kBigAlloc) / 33.0)) : 0ull;
+        };
+        T = tag_bits(fit(5));
+        budget = fit(T);
+        log_free = double(freeb);
+        log_reuse = double(reuse);
+        log_fixed = fixed;
     }
     budget = std::max<u64>(std::min<u64>(budget, 1ull << 32), 1ull << 10);
-    // slices of the hash space recorded per work item: 2^rbits, enough for
-    // passes of about the budget (at most 32: one u32 mask per work item)
-    u32 rbits = 0;
-    while (rbits < 5 && (budget << rbits) < C + C / 16) ++rbits;
-    QD_TRY(ctx->imask.ensure(std::max<u64>(nitems, 1) * sizeof(u32)));
+    const u32 S = (1u << T) - 1;  // slices; tag S: no such child
+    QD_TRY(ctx->tags.ensure(std::max<u64>(u64(T) * pstride, 8)));
 
     // 2. how the candidates spread over the 4096 values of the top 12 hash
-    // bits, and each work item's slices
+    // bits, and each child's slice
     phase_mark(ctx, PH_SYMBOLIC);
     std::vector<u64> hc(kHistBins);
     QD_HIP(hipMemsetAsync(hist1, 0, kHistBins * sizeof(u64), s));
     prof_begin(ctx, QD_PHASE_GENERATE);
     hipLaunchKernelGGL(k_range_count<KIND>, dim3(gg), dim3(kGenThreads), 0, s, (const PDesc *)ctx->pdesc.as<PDesc>(),
                        P.mag, (const u64 *)ctx->wb.as<u64>(), N, (const u64 *)ctx->cpar.as<u64>(),
-                       (const u64 *)ctx->tab.as<u64>(), P.begin, rule, nitems, rbits, ctx->imask.as<u32>(), hist1);
+                       (const u64 *)ctx->tab.as<u64>(), P.begin, rule, nitems, T, ctx->tags.as<u8>(), pstride, hist1);
     QD_HIP(hipGetLastError());
     prof_end(ctx, QD_PHASE_GENERATE);
     QD_TRY(sync_read(ctx, hist1, kHistBins * sizeof(u64), hc.data()));
 
-    // 3. passes: contiguous bin ranges within the budget; a pass walks the
-    // work items of the slices its bins lie in
-    std::vector<u32> cut{0}, pmask;
-    std::vector<u64> rn;
-    const u32 sshift = 12 - rbits;  // bin -> slice
-    u64 acc = 0, nmax = 0;
-    u32 pm = 0;
-    for (u32 d = 0; d < u32(kHistBins); ++d) {
-        if (hc[d] > budget)
-            return fail(QD_ERR_CAPACITY, "hash-range pass: " + std::to_string(hc[d]) +
-                                             " candidates share 12 leading hash bits, beyond the pass budget of " +
-                                             std::to_string(budget));
-        if (acc && acc + hc[d] > budget) {
-            cut.push_back(d);
-            rn.push_back(acc);
-            pmask.push_back(pm);
-            nmax = std::max(nmax, acc);
-            acc = 0;
-            pm = 0;
-        }
-        acc += hc[d];
-        if (hc[d]) pm |= 1u << (d >> sshift);
+    // 3. passes: runs of whole slices within the budget; a slice beyond the
+    // budget alone is split into runs of bins (its passes list all of the
+    // slice's children and keep those of their bins)
+    struct Pass {
+        u32 rlo, rhi, slo, shi;
+        u64 n;
+    };
+    std::vector<Pass> passes;
+    std::vector<u64> sc(S, 0);
+    std::vector<u32> sb(S + 1, kHistBins);  // first bin of each slice
+    for (u32 d = kHistBins; d-- > 0;) {
+        const u32 sl = (d * S) >> 12;
+        sc[sl] += hc[d];
+        sb[sl] = d;
     }
-    cut.push_back(kHistBins);
-    rn.push_back(acc);
-    pmask.push_back(pm);
-    nmax = std::max(nmax, acc);
+    u64 nmax = 0;
+    {
+        Pass cur{0, 0, 0, 0, 0};
+        auto flush = [&]() {
+            if (cur.n) {
+                passes.push_back(cur);
+                nmax = std::max(nmax, cur.n);
+            }
+        };
+        for (u32 sl = 0; sl < S; ++sl) {
+            if (sc[sl] <= budget) {
+                if (cur.n && cur.n + sc[sl] > budget) {
+                    flush();
+                    cur = Pass{sb[sl], sb[sl], sl, sl, 0};
+                }
+                if (!cur.n) cur = Pass{sb[sl], sb[sl], sl, sl, 0};
+                cur.n += sc[sl];
+                cur.rhi = sb[sl + 1];
+                cur.shi = sl + 1;
+                continue;
+            }
+            flush();
+            cur = Pass{0, 0, 0, 0, 0};
+            Pass q{sb[sl], sb[sl], sl, sl + 1, 0};
+            for (u32 d = sb[sl]; d < sb[sl + 1]; ++d) {
+                if (hc[d] > budget)
+                    return fail(QD_ERR_CAPACITY, "hash-range pass: " + std::to_string(hc[d]) +
+                                                     " candidates share 12 leading hash bits, beyond the pass budget "
+                                                     "of " + std::to_string(budget));
+                if (q.n && q.n + hc[d] > budget) {
+                    passes.push_back(q);
+                    nmax = std::max(nmax, q.n);
+                    q = Pass{d, d, sl, sl + 1, 0};
+                }
+                q.n += hc[d];
+                q.rhi = d + 1;
+            }
+            if (q.n) {
+                passes.push_back(q);
+                nmax = std::max(nmax, q.n);
+            }
+        }
+        flush();
+    }
+    static const bool log_plan = std::getenv("QD_LOG_RANGE") != nullptr;
+    if (log_plan)
+        std::fprintf(stderr,
+                     "[quids_hip] hash-range step: C=%llu work items=%llu free=%.1f GB reused=%.1f GB fixed=%.1f GB "
+                     "pass budget=%llu tag bits=%u passes=%zu largest=%llu\n",
+                     (unsigned long long)C, (unsigned long long)nitems, log_free / 1e9, log_reuse / 1e9,
+                     log_fixed / 1e9, (unsigned long long)budget, T, passes.size(), (unsigned long long)nmax);
 
     // 4. layout of a pass of nmax candidates: b1 first-pass bits (at most
-    // 11, fused), b2 second-pass bits (at most 13), buckets of at most 1024
+    // 10, fused), b2 second-pass bits (at most 12), buckets of at most 1024
     // entries on average
     u32 tgt = ctx->tune.dedup_bucket_target, dcap = ctx->tune.dedup_cap;
     while ((nmax >> 22) > tgt && tgt < 1024) tgt *= 2;
     dcap = std::max(dcap, 2 * tgt);
     u32 L = 0;
     while ((nmax >> L) > tgt && L < 22) ++L;
-    const u32 b1 = std::min<u32>(L, 11), b2 = L - b1;
+    const u32 b1 = std::min<u32>(L, 10), b2 = L - b1;
     const u64 nb1 = 1ull << b1;
     const u32 nparts = u32(std::min<u64>(nb1, 4));
     const u32 pshift = b1 - (nparts == 4 ? 2 : nparts == 2 ? 1 : 0);
     const u64 perpart = nb1 / nparts;
-    const double mean1 = double(nmax) / double(nb1);
-    u64 cap1 = u64(mean1 + 8.0 * std::sqrt(mean1)) + 4096;
     const u32 ngroups = u32(std::min<u64>(nb1, kRangeGroups));
     const u64 gbk = nb1 / ngroups;  // first-level buckets per group (a group lies in one part)
-    const u64 gcap = gbk * cap1;    // entries of a group's first-pass region
+    const double gmean = double(nmax) / double(ngroups);
+    const u64 gcap = u64(gmean * 1.02 + 8.0 * std::sqrt(gmean)) + 4096;  // entries of a group, at most
     const u64 nb2g = gbk << b2;     // sub-buckets of a group
     const double mean2 = double(gcap) / double(std::max<u64>(nb2g, 1));
-    const u64 cap2 = (u64(mean2 + 6.0 * std::sqrt(mean2)) + 128 + 63) / 64 * 64;
+    // candidates sharing a hash fall in one sub-bucket, so the counts spread
+    // more than the Poisson counts of distinct hashes: a wide margin (a
+    // spill takes the exact path)
+    const u64 cap2 = (u64(mean2 + 16.0 * std::sqrt(mean2)) + 256 + 63) / 64 * 64;
     const u64 pcap = std::max<u64>(capped ? 2 * max_obj + gcap : C, 1);  // trimmed to max_obj beyond 2 max_obj
-    auto alloc_parts = [&](u64 c1) -> int {
-        for (u32 q = 0; q < 4; ++q) {
-            if (q < nparts) QD_TRY(ctx->rpart[q].ensure(perpart * c1 * sizeof(Entry)));
-            else ctx->rpart[q].release();
-        }
-        return QD_OK;
-    };
-    QD_TRY(alloc_parts(cap1));
+    const unsigned gr = unsigned(std::max<u64>(std::min<u64>((nitems + kRangeTile - 1) / kRangeTile, kRangeGrid), 1));
+    QD_TRY(ctx->bmat.ensure(u64(gr) * nb1 * sizeof(u32)));
+    QD_TRY(ctx->off1.ensure((nb1 + 1) * sizeof(u64)));
     QD_TRY(ctx->ebuf2.ensure(std::max<u64>(nb2g * cap2, gcap) * sizeof(Entry)));
     QD_TRY(ctx->cnt2.ensure(nb2g * sizeof(u64)));
     QD_TRY(ctx->off2.ensure((nb2g + 1) * sizeof(u64)));
@@ -1239,74 +1303,79 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
         return QD_OK;
     };
 
-    std::vector<u64> c1(nb1);
+    std::vector<u64> c1(nb1), op(nb1 + 1), gofs(u64(ngroups) * (gbk + 1));
+    std::vector<u64> psize(nparts);
     phase_mark(ctx, PH_COLL_INSERT);
-    for (size_t r = 0; r + 1 < cut.size(); ++r) {
-        const u64 n = rn[r];
-        if (n == 0) continue;
-        // a. the range's children into the first pass's buckets
-        u64 cap1r = cap1;
-        for (int attempt = 0;; ++attempt) {
-            EParts E{};
-            for (u32 q = 0; q < 4; ++q) E.p[q] = q < nparts ? ctx->rpart[q].as<Entry>() : nullptr;
-            E.pshift = pshift;
-            QD_HIP(hipMemsetAsync(ctx->cur1.p, 0, nb1 * sizeof(u64), s));
-            QD_HIP(hipMemsetAsync(errw, 0, sizeof(u32), s));
-            prof_begin(ctx, QD_PHASE_GENERATE);
-            const unsigned gt = unsigned(std::max<u64>(
-                std::min<u64>((nitems + kRangeTile - 1) / kRangeTile, ctx->tune.generate_grid), 1));
-            hipLaunchKernelGGL(k_generate_range_scatter<KIND>, dim3(gt), dim3(kGenThreads), 0, s,
-                               (const PDesc *)ctx->pdesc.as<PDesc>(), P.mag, (const u64 *)ctx->cb.as<u64>(),
-                               (const u64 *)ctx->wb.as<u64>(), N, (const u64 *)ctx->cpar.as<u64>(),
-                               (const u64 *)ctx->tab.as<u64>(), P.begin, rule, nitems,
-                               (const u32 *)ctx->imask.as<u32>(), pmask[r], cut[r], cut[r + 1], b1, cap1r,
-                               ctx->cur1.as<u64>(), E, errw);
-            QD_HIP(hipGetLastError());
-            prof_end(ctx, QD_PHASE_GENERATE);
-            u32 ew = 0;
-            QD_TRY(sync_read(ctx, errw, sizeof(u32), &ew));
-            QD_TRY(sync_read(ctx, ctx->cur1.p, nb1 * sizeof(u64), c1.data()));
-            u64 kept = 0, cmax = 0;
-            for (u64 d = 0; d < nb1; ++d) {
-                kept += c1[d];
-                cmax = std::max(cmax, c1[d]);
+    for (const Pass &pass : passes) {
+        const u64 n = pass.n;
+        const RangeSel rs{ctx->tags.as<u8>(), pstride, T, pass.slo, pass.shi, pass.rlo, pass.rhi};
+        // a. the range's children into the first pass's buckets: counts per
+        // (block, bucket), their column scan, the buckets' offsets within
+        // their parts, then the writes
+        prof_begin(ctx, QD_PHASE_GENERATE);
+        hipLaunchKernelGGL(k_range_bincount<KIND>, dim3(gr), dim3(kGenThreads), 0, s,
+                           (const PDesc *)ctx->pdesc.as<PDesc>(), P.mag, (const u64 *)ctx->cb.as<u64>(),
+                           (const u64 *)ctx->wb.as<u64>(), N, (const u64 *)ctx->cpar.as<u64>(),
+                           (const u64 *)ctx->tab.as<u64>(), P.begin, rule, nitems, rs, b1, ctx->bmat.as<u32>());
+        hipLaunchKernelGGL(k_range_colscan, dim3(unsigned(nb1)), dim3(256), 0, s, ctx->bmat.as<u32>(), u32(gr), u32(nb1),
+                           ctx->cur1.as<u64>());
+        QD_HIP(hipGetLastError());
+        prof_end(ctx, QD_PHASE_GENERATE);
+        QD_TRY(sync_read(ctx, ctx->cur1.p, nb1 * sizeof(u64), c1.data()));
+        u64 kept = 0;
+        for (u32 q = 0; q < nparts; ++q) {
+            u64 acc2 = 0;
+            for (u64 d = q * perpart; d < (q + 1) * perpart; ++d) {
+                op[d] = acc2;  // bucket d's first entry within its part
+                acc2 += c1[d];
             }
-            if (kept != n) return fail(QD_ERR_HIP, "hash-range pass kept a different candidate count than it counted");
-            if (!(ew & ERR_SPILL1)) break;
-            // a bucket passed its capacity: redo the range with buckets as
-            // large as the largest count, if memory allows
-            st.reserved |= 8;
-            if (attempt) return fail(QD_ERR_HIP, "hash-range pass: bucket overflow persisted");
-            cap1r = (cmax + 63) / 64 * 64;
-            if (cap1r * nb1 > 2 * cap1 * nb1)
-                return fail(QD_ERR_CAPACITY, "hash-range pass: hash keys too skewed for the first radix pass");
-            QD_TRY(alloc_parts(cap1r));
+            psize[q] = acc2;
+            kept += acc2;
         }
-        QD_TRY(run_scan(ctx, LoadU64{ctx->cur1.as<u64>()}, nb1, ctx->off1.as<u64>()));
-        std::vector<u64> o1(nb1 + 1, 0);
-        for (u64 d = 0; d < nb1; ++d) o1[d + 1] = o1[d] + c1[d];
-        for (u32 g = 0; g < ngroups; ++g)  // the buffers of a group were sized for gcap entries
-            if (o1[u64(g + 1) * gbk] - o1[u64(g) * gbk] > gcap)
+        if (kept != n) return fail(QD_ERR_HIP, "hash-range pass kept a different candidate count than it counted");
+        for (u32 g = 0; g < ngroups; ++g) {
+            const u64 g0 = u64(g) * gbk;
+            u64 *o = gofs.data() + u64(g) * (gbk + 1);
+            for (u64 b = 0; b < gbk; ++b) o[b] = op[g0 + b];
+            o[gbk] = op[g0 + gbk - 1] + c1[g0 + gbk - 1];
+            if (o[gbk] - o[0] > gcap)
                 return fail(QD_ERR_CAPACITY, "hash-range pass: hash keys too skewed for the first radix pass");
+        }
+        for (u32 q = 0; q < nparts; ++q) QD_TRY(ctx->rpart[q].ensure(std::max<u64>(psize[q], 1) * sizeof(Entry)));
+        for (u32 q = nparts; q < 4; ++q) ctx->rpart[q].release();
+        QD_TRY(upload(ctx, ctx->off1, op.data(), nb1 * sizeof(u64)));
+        QD_TRY(upload(ctx, ctx->goff, gofs.data(), gofs.size() * sizeof(u64)));
+        EParts E{};
+        for (u32 q = 0; q < 4; ++q) E.p[q] = q < nparts ? ctx->rpart[q].as<Entry>() : nullptr;
+        E.pshift = pshift;
+        prof_begin(ctx, QD_PHASE_GENERATE);
+        hipLaunchKernelGGL(k_range_binwrite<KIND>, dim3(gr), dim3(kGenThreads), 0, s,
+                           (const PDesc *)ctx->pdesc.as<PDesc>(), P.mag, (const u64 *)ctx->cb.as<u64>(),
+                           (const u64 *)ctx->wb.as<u64>(), N, (const u64 *)ctx->cpar.as<u64>(),
+                           (const u64 *)ctx->tab.as<u64>(), P.begin, rule, nitems, rs, b1, (const u32 *)ctx->bmat.as<u32>(),
+                           (const u64 *)ctx->off1.as<u64>(), E);
+        QD_HIP(hipGetLastError());
+        prof_end(ctx, QD_PHASE_GENERATE);
         st.dedup_levels = (b1 ? 1 : 0) + (b2 ? 1 : 0);
         // b. per group of first-level buckets: second pass, dedup, pool
         for (u32 g = 0; g < ngroups; ++g) {
             const u64 g0 = u64(g) * gbk;
-            const u64 ng = o1[g0 + gbk] - o1[g0];
+            const u64 *o = gofs.data() + u64(g) * (gbk + 1);
+            const u64 ng = o[gbk] - o[0];
             if (ng == 0) continue;
-            Entry *Eg = ctx->rpart[g0 >> pshift].as<Entry>() + (g0 & ((1ull << pshift) - 1)) * cap1r;
+            Entry *Ep = ctx->rpart[g0 >> pshift].as<Entry>();  // the group's part
+            Entry *Eg = Ep + o[0];                              // the group's entries
+            const u64 *go = ctx->goff.as<u64>() + u64(g) * (gbk + 1);  // its buckets' offsets within the part
             Entry *E2 = ctx->ebuf2.as<Entry>();
             QD_HIP(hipMemsetAsync(counters, 0, 8 * sizeof(u64), s));
-            const u64 *woff;
-            u64 rstride;
+            u64 rstride = 0;
             prof_begin(ctx, QD_PHASE_PARTITION);
             if (b2) {
                 int ncu2 = 256;
                 (void)hipDeviceGetAttribute(&ncu2, hipDeviceAttributeMultiprocessorCount, ctx->device);
                 const unsigned g2 = unsigned(std::min<u64>(gbk, std::max(ncu2 / 2, 1)));
-                hipLaunchKernelGGL(k_level2_fixed, dim3(g2), dim3(1024), 0, s, (const Entry *)Eg, cap1r,
-                                   (const u64 *)ctx->off1.as<u64>() + g0, gbk, b1, b2, cap2, E2, ctx->cnt2.as<u64>(),
-                                   errw);
+                hipLaunchKernelGGL(k_level2_fixed, dim3(g2), dim3(1024), 0, s, (const Entry *)Ep, u64(0), go, gbk, b1,
+                                   b2, cap2, E2, ctx->cnt2.as<u64>(), errw);
                 QD_HIP(hipGetLastError());
                 QD_TRY(run_scan(ctx, LoadU64{ctx->cnt2.as<u64>()}, nb2g, ctx->off2.as<u64>()));
                 u32 ew = 0;
@@ -1317,21 +1386,18 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
                     hb[kHostN] = ng;
                     QD_HIP(hipMemcpyAsync(ctx->off2.as<u64>() + nb2g, hb + kHostN, sizeof(u64),
                                           hipMemcpyHostToDevice, s));
-                    hipLaunchKernelGGL(k_level2_exact, dim3(unsigned(gbk)), dim3(1024), 0, s, (const Entry *)Eg, cap1r,
-                                       (const u64 *)ctx->off1.as<u64>() + g0, o1[g0], b1, b2, E2, ctx->off2.as<u64>());
+                    hipLaunchKernelGGL(k_level2_exact, dim3(unsigned(gbk)), dim3(1024), 0, s, (const Entry *)Ep, u64(0),
+                                       go, o[0], b1, b2, E2, ctx->off2.as<u64>());
                     QD_HIP(hipGetLastError());
                     rstride = 0;
                     st.reserved |= 1;
                 }
-                woff = ctx->off2.as<u64>();
             } else {
                 // one level: the group's first-pass buckets are the dedup's
-                QD_HIP(hipMemcpyAsync(E2, Eg, gcap * sizeof(Entry), hipMemcpyDeviceToDevice, s));
-                hipLaunchKernelGGL(k_add_offset, dim3(grid_for(gbk + 1)), dim3(256), 0, s,
-                                   (const u64 *)ctx->off1.as<u64>() + g0, gbk + 1, u64(0) - o1[g0], ctx->off2.as<u64>());
+                QD_HIP(hipMemcpyAsync(E2, Eg, ng * sizeof(Entry), hipMemcpyDeviceToDevice, s));
+                hipLaunchKernelGGL(k_add_offset, dim3(grid_for(gbk + 1)), dim3(256), 0, s, go, gbk + 1, u64(0) - o[0],
+                                   ctx->off2.as<u64>());
                 QD_HIP(hipGetLastError());
-                woff = ctx->off2.as<u64>();
-                rstride = cap1r;
             }
             prof_end(ctx, QD_PHASE_PARTITION);
             // the uniques go to the group's first-pass region (read by now)
@@ -1339,10 +1405,11 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
             double2 *UM = reinterpret_cast<double2 *>(UR + ng);
             u32 *UK = ukeys ? ctx->ukey.as<u32>() : nullptr;
             u64 U = 0;
-            QD_TRY(dedup_phase(ctx, E2, rstride, woff, b2 ? nb2g : gbk, b1 + b2, prm.tolerance, dcap, ng, UH, UR, UM,
-                               UK, U));
+            QD_TRY(dedup_phase(ctx, E2, rstride, ctx->off2.as<u64>(), b2 ? nb2g : gbk, b1 + b2, prm.tolerance, dcap,
+                               ng, UH, UR, UM, UK, U));
             Utot += U;
             if (!U) continue;
+            if (capped && ns + ng > pcap) QD_TRY(trim_pool());
             hb[kHostN - 1] = ns;
             QD_HIP(hipMemcpyAsync(pcount, hb + kHostN - 1, sizeof(u64), hipMemcpyHostToDevice, s));
             hipLaunchKernelGGL(k_pool_append,
@@ -1431,12 +1498,6 @@ static int batch_cseg(qd_ctx *ctx, Batch *bt) {
     return sync_read(ctx, ctx->bseg.p, (bt->n + 1) * sizeof(u64), bt->cseg.data());
 }
 
-static int upload(qd_ctx *ctx, DBuf &b, const void *src, size_t bytes) {
-    QD_TRY(b.ensure(std::max<size_t>(bytes, 8)));
-    if (bytes) QD_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
-    QD_HIP(hipStreamSynchronize(ctx->stream));  // the host source may go away
-    return QD_OK;
-}
 
 // Per-instance top-K over the nslots dedup output slots; marks survivors.
 // The keys of truncated instances are grouped by instance once; each radix

@@ -40,12 +40,22 @@ def _chain(hs, n_iter, cap, chunk_log2):
         ctx.set_tuning(chunk_log2=0)
 
 
-@pytest.mark.parametrize("nodes,seed,n_iter,cap", [(6, 3, 3, 3000), (7, 5, 3, q.UINT64_MAX), (8, 42, 3, 700)])
-def test_range_passes_match_single_pass(nodes, seed, n_iter, cap):
-    print(f"range nodes={nodes} seed={seed} iters={n_iter} cap={cap}", flush=True)
+# The pass budget (2^chunk candidates) against a step's C candidates sets how
+# the passes cover the 2^T - 1 hash slices the children are tagged with
+# (T <= 8): several slices per pass, one, or (C > 255 * 2^chunk) a slice
+# split over several passes that each keep the children of their bins.
+@pytest.mark.parametrize("nodes,seed,n_iter,cap,chunk,min_c", [
+    (6, 3, 3, 3000, 10, 0),
+    (7, 5, 3, q.UINT64_MAX, 10, 0),
+    (7, 5, 3, q.UINT64_MAX, 14, 0),
+    (8, 42, 3, 700, 10, 0),
+    (9, 2, 3, q.UINT64_MAX, 12, 255 * 4096 + 1),
+])
+def test_range_passes_match_single_pass(nodes, seed, n_iter, cap, chunk, min_c):
+    print(f"range nodes={nodes} seed={seed} iters={n_iter} cap={cap} chunk=2^{chunk}", flush=True)
     hs = HostState.from_objects([qcgd.seeded_graph(nodes, seed)])
     one = _chain(hs, n_iter, cap, 0)
-    many = _chain(hs, n_iter, cap, 10)
+    many = _chain(hs, n_iter, cap, chunk)
     used = False
     for i, ((x, sx), (y, sy)) in enumerate(zip(one, many)):
         _same(x, y, f"step {i}")
@@ -53,6 +63,8 @@ def test_range_passes_match_single_pass(nodes, seed, n_iter, cap):
             assert sx[k] == sy[k], (i, k)
         used |= bool(sy["reserved"] & 4)
     assert used, "no step ran in hash-range passes"
+    print("candidates per step:", [sx["num_candidates"] for _, sx in one], flush=True)
+    assert max(sx["num_candidates"] for _, sx in one) >= min_c
 
 
 def test_range_passes_match_oracle():
