@@ -115,8 +115,11 @@ def cpu_baseline(nodes, seed, step, log2_cap, budget_s):
 # Kernels of the three main phases (name prefixes in rocprofv3 traces): the
 # generate phase is k_generate_scatter (generate fused with the first radix
 # pass), k_generate_fast (split_generate=1) or, for steps beyond the pass
-# budget, k_generate_range (the counting pass and one launch per hash range).
-KERNELS_OF_PHASE = {"generate": ("k_generate_scatter", "k_generate_fast", "k_generate_range"),
+# budget, k_range_count (the counting pass) and one k_range_walkwrite per
+# hash-range pass (k_range_bincount + k_range_binwrite when a pass takes two
+# walks).
+KERNELS_OF_PHASE = {"generate": ("k_generate_scatter", "k_generate_fast", "k_range_count", "k_range_walkwrite",
+                                 "k_range_bincount", "k_range_binwrite"),
                     "dedup": ("k_dedup<", "k_dedup_sort", "k_dedup_big"),
                     "finalize": ("k_populate",)}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_qcgd16.json")
@@ -129,8 +132,10 @@ def dominant_kernel_name(phase, ctx, stats):
         if any(not (s["reserved"] & 4) for s in stats):
             names.append("k_generate_fast" if ctx.get_tuning()["split_generate"] else "k_generate_scatter")
         if any(s["reserved"] & 4 for s in stats):
-            names.append("k_generate_range")
-        return "+".join(names)
+            names += ["k_range_count", "k_range_walkwrite"]
+            if any((s["reserved"] & 12) == 12 for s in stats):
+                names += ["k_range_bincount", "k_range_binwrite"]
+        return "+".join(sorted(names))
     return {"dedup": "k_dedup", "finalize": "k_populate"}[phase]
 
 

@@ -104,7 +104,8 @@ typedef struct qd_step_stats {
     uint32_t reserved;          /* bit 0: the second pass fell back to exact offsets;
                                    bit 1: a batched step ran in several chunks;
                                    bit 2: the step ran in hash-range passes;
-                                   bit 3: the fused first radix pass spilled and was redone */
+                                   bit 3: the fused first radix pass spilled and was redone, or
+                                   a hash-range pass took two walks */
     /* memory-driven truncation only (max_num_object == 0; 0 otherwise): the
      * target_memory of the parent side and of the survivor side
      * (src/quids.hpp:469, 520) that the binary searches kept below */
@@ -177,7 +178,9 @@ typedef struct qd_tuning {
                                      free device memory holds): a batched step splits its instances
                                      into chunks, and a step beyond it runs in hash-range passes */
     uint32_t split_generate;      /* 1: generate, then the first radix pass as a kernel of its own
-                                     (default 0: fused when the first pass has at most 2^8 buckets) */
+                                     (default 0: fused when the first pass has at most 2^8 buckets);
+                                     in hash-range passes, count and write in two walks per pass
+                                     (default 0: one walk into per-block regions) */
     uint32_t eager_select;        /* 1: launch the top-K select's digit rounds one by one (default 0:
                                      replay them from a hipGraph the context captures once) */
     uint32_t free_mem_mib;        /* memory-driven truncation sees at most this much free device

@@ -940,4 +940,114 @@ __global__ __launch_bounds__(kGenThreads) void k_range_binwrite(const PDesc *D, 
     });
 }
 
+// ---------------------------------------------------------------------------
+// A hash-range pass in one walk.  Each child of the pass in the range is
+// written once, into its block's region of its group (the top gbits bits of
+// its key): group q's regions lie at E.g[q], block b's at b * capr, and
+// rcount[q * gridDim.x + b] receives the region's count.  Places within a
+// region come from wavefront-aggregated LDS counters, one per group.  A
+// region that would pass capr raises ERR_SPILL1 (the caller then walks the
+// pass again the two-walk way).  Each group's regions are then merged into
+// its first-level buckets with exact offsets (k_region_hist,
+// k_range_colscan, k_excl_small, k_region_place).
+// ---------------------------------------------------------------------------
+constexpr u32 kRangeGroupsMax = 8;
+
+struct GroupBases {
+    Entry *g[kRangeGroupsMax];
+};
+
+template <int KIND>
+__global__ __launch_bounds__(kGenThreads) void k_range_walkwrite(const PDesc *D, const double2 *pmag, const u64 *cb,
+                                                                 const u64 *wb, u64 N, const u64 *cpar,
+                                                                 const u64 *tab, const u64 *pbegin, qd_rule rule,
+                                                                 u64 W, RangeSel rs, u32 gbits, GroupBases E,
+                                                                 u64 capr, u32 *rcount, u32 *err) {
+    __shared__ u32 lcnt[kRangeGroupsMax];
+    __shared__ RangeLds L;
+    if (threadIdx.x < kRangeGroupsMax) lcnt[threadIdx.x] = 0;
+    const QcgdCoef k = coef_of(rule);
+    const u32 lane = threadIdx.x & (kWave - 1);
+    const u32 ng = 1u << gbits;
+    bool spill = false;
+    range_walks<KIND>(D, pmag, cb, wb, N, cpar, tab, pbegin, k, W, rs, L, [&](u64 key, const cplx &m, u64 cand) {
+        const u32 g = gbits ? u32(key >> (64 - gbits)) : 0u;
+        u32 pos = 0;
+        for (u32 q = 0; q < ng; ++q) {
+            const u64 mk = __ballot(g == q);
+            if (!mk) continue;
+            const u32 leader = u32(__ffsll((unsigned long long)mk) - 1);
+            u32 base = 0;
+            if (lane == leader) base = atomicAdd(&lcnt[q], u32(__popcll(mk)));
+            base = __shfl(base, int(leader), kWave);
+            if (g == q) pos = base + u32(__popcll(mk & ((1ull << lane) - 1)));
+        }
+        if (pos < capr) {
+            Entry e;
+            e.hash = key;
+            e.cand = cand;
+            e.re = m.re;
+            e.im = m.im;
+            E.g[g][u64(blockIdx.x) * capr + pos] = e;
+        } else {
+            spill = true;
+        }
+    });
+    __syncthreads();
+    if (threadIdx.x < ng) rcount[threadIdx.x * gridDim.x + blockIdx.x] = min(lcnt[threadIdx.x], u32(capr));
+    if (spill) atomicOr(err, ERR_SPILL1);
+}
+
+// Digit counts of each region (nreg regions of capr entries, rcount[r]
+// used): row r of bm, nbin columns; the digit is (key >> shift) % nbin.
+__global__ __launch_bounds__(256) void k_region_hist(const Entry *in, u64 capr, const u32 *rcount, u32 nreg,
+                                                     u32 shift, u32 nbin, u32 *bm) {
+    __shared__ u32 h[kRangeBins];
+    for (u32 r = blockIdx.x; r < nreg; r += gridDim.x) {
+        __syncthreads();
+        for (u32 i = threadIdx.x; i < nbin; i += blockDim.x) h[i] = 0;
+        __syncthreads();
+        const u32 n = rcount[r];
+        const Entry *src = in + u64(r) * capr;
+        for (u32 i = threadIdx.x; i < n; i += blockDim.x)
+            atomicAdd(&h[nbin > 1 ? u32(src[i].hash >> shift) & (nbin - 1) : 0u], 1u);
+        __syncthreads();
+        for (u32 i = threadIdx.x; i < nbin; i += blockDim.x) bm[u64(r) * nbin + i] = h[i];
+    }
+}
+
+// off[i] = total[0] + ... + total[i - 1] for i <= n (n small).
+__global__ void k_excl_small(const u64 *total, u32 n, u64 *off) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        u64 acc = 0;
+        for (u32 i = 0; i < n; ++i) {
+            off[i] = acc;
+            acc += total[i];
+        }
+        off[n] = acc;
+    }
+}
+
+// Each region's entries at its running places in the buckets: bucket d
+// starts at boff[d], region r's first place in it is bm[r][d] after the
+// column scan.
+__global__ __launch_bounds__(256) void k_region_place(const Entry *in, u64 capr, const u32 *rcount, u32 nreg,
+                                                      u32 shift, u32 nbin, const u32 *bm, const u64 *boff,
+                                                      Entry *out) {
+    __shared__ u64 cur[kRangeBins];
+    for (u32 r = blockIdx.x; r < nreg; r += gridDim.x) {
+        __syncthreads();
+        for (u32 i = threadIdx.x; i < nbin; i += blockDim.x) cur[i] = boff[i] + bm[u64(r) * nbin + i];
+        __syncthreads();
+        const u32 n = rcount[r];
+        const Entry *src = in + u64(r) * capr;
+        for (u32 i = threadIdx.x; i < n; i += blockDim.x) {
+            const Entry e = src[i];
+            const u32 d = nbin > 1 ? u32(e.hash >> shift) & (nbin - 1) : 0u;
+            const u64 at = atomicAdd(reinterpret_cast<unsigned long long *>(&cur[d]), 1ull);
+            out[at] = e;
+        }
+    }
+}
+
 }  // namespace qd

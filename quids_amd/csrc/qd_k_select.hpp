@@ -39,7 +39,7 @@ __device__ inline u64 wave_append(bool pred, u64 *counter) {
 }
 
 __device__ inline void mark(u64 *bitmap, u64 rep) {
-    atomicOr(reinterpret_cast<unsigned long long *>(bitmap + (rep >> 6)), 1ull << (rep & 63));
+    if (bitmap) atomicOr(reinterpret_cast<unsigned long long *>(bitmap + (rep >> 6)), 1ull << (rep & 63));
 }
 
 // Marks a selected item: bit `key` of the bitmap (the representative, or

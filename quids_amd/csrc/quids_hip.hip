@@ -183,7 +183,8 @@ struct qd_ctx {
     DBuf pool0, pool1, pwpre, blist2;
     DBuf mw;  // per-object memory of memory-driven truncation
     DBuf rpart[4], pk0, pk1, tags, bmat, goff;  // hash-range passes: first-pass buckets, pool select keys,
-                                                // children's slice tags, per-block bucket counts, group offsets
+                                                // children's slice tags, per-block bucket counts, group offsets,
+    DBuf rcnt, abuf;                            // region counts, a group's merged first-level buckets
     // output buffers of batched chunks (objects, begin, size, mag per chunk),
     // kept between calls
     std::vector<DBuf> parts;
@@ -199,7 +200,7 @@ struct qd_ctx {
                 &biglist, &redolist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &cpar, &opar, &ebuf2, &cnt2, &slist, &ukey, &selloop, &segloop, &bsoff, &bscnt,
                 &adjk, &adjv, &adjg, &adjd, &adjp, &adjw, &bseg, &boseg, &brules, &bsinst, &bcount, &bsel, &btrunc,
                 &blist, &bcur, &btile, &bhist, &bkseg, &bchunk, &bcsum, &btot, &pool0, &pool1, &pwpre, &blist2, &mw, &rpart[0], &rpart[1], &rpart[2],
-                &rpart[3], &pk0, &pk1, &tags, &bmat, &goff};
+                &rpart[3], &pk0, &pk1, &tags, &bmat, &goff, &rcnt, &abuf};
     }
 };
 
@@ -1056,13 +1057,13 @@ __global__ __launch_bounds__(256) void k_add_offset(const u64 *src, u64 n, u64 a
         dst[i] = src[i] + add;
 }
 
-// Per-candidate bytes of a hash-range pass: the first radix pass's entry
-// (32 bytes, buckets exactly sized), and an eighth (one group of
-// first-level buckets at a time) of the second pass's entry at its
+// Per-candidate bytes of a hash-range pass: the walk's 32-byte entry in
+// its region (3 % capacity slack), and an eighth (one group at a time) of
+// the group's merged first-level entry, of the second pass's entry at its
 // sub-buckets' slack and of the group's 32-bit select keys and pool
 // appends.
 constexpr u32 kRangeGroups = 8;
-constexpr double kRangeBytesPerCand = 32.0 + (32.0 * 2.05 + 4.0 + 36.0) / kRangeGroups;
+constexpr double kRangeBytesPerCand = 32.0 * 1.03 + (32.0 + 32.0 * 2.05 + 4.0 + 36.0) / kRangeGroups;
 
 template <int KIND>
 static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule &rule, u64 C, u64 nitems,
@@ -1083,6 +1084,11 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
                     &ctx->bigmag, &ctx->bigseg, &ctx->opar, &ctx->adjk, &ctx->adjv, &ctx->adjg, &ctx->adjd,
                     &ctx->adjp, &ctx->adjw})
         b->release();
+    // nor the output state's buffers, which hold nothing of this step yet:
+    // the finalize allocates them at the step's size
+    for (DBuf *b : {&out->objects, &out->begin, &out->size, &out->mag}) b->release();
+    out->n = 0;
+    out->bytes = 0;
     if (KIND != QD_RULE_QCGD_SPLIT_MERGE) ctx->tab.release();  // split_merge's name-root table
     QD_TRY(ctx->hist1.ensure(kHistBins * sizeof(u64)));
     QD_TRY(ctx->cur1.ensure(kRangeBins * sizeof(u64)));
@@ -1104,7 +1110,7 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
     u32 T = 4;
     auto tag_bits = [&](u64 b) {
         u32 t = 4;
-        while (t < kMaxTagBits && double((1u << t) - 1) * double(b) < 1.1 * double(C)) ++t;
+        while (t < kMaxTagBits && double((1u << t) - 1) * double(b) < 1.03 * double(C)) ++t;
         return t;
     };
     double log_free = 0, log_reuse = 0, log_fixed = 0;  // the plan's inputs, for QD_LOG_RANGE
@@ -1123,7 +1129,7 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
         const double K = double(capped ? max_obj : C);
         const double next = std::max(0.0, 1.5 * double(in_bytes) * std::min(1.0, K / double(std::max<u64>(N, 1))) +
                                               60.0 * K - double(state_mem(out)));
-        const double fixed = 16.0 * double(W) + (capped ? 3.0 * 36.0 * K : 36.0 * K) + next + double(2ull << 30);
+        const double fixed = (capped ? 3.0 * 36.0 * K : 36.0 * K) + next + double(2ull << 30);
         const double room = 0.85 * (double(freeb) + double(reuse)) - fixed;
         auto fit = [&](u32 t) {
             const double r = room - double(t) * double(pstride);
@@ -1256,8 +1262,6 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
         QD_TRY(ctx->pk0.ensure(pcap * sizeof(u32)));
         QD_TRY(ctx->pk1.ensure((capped ? max_obj + 1 : 1) * sizeof(u32)));
     }
-    QD_TRY(ctx->bitmap.ensure(W * sizeof(u64)));
-    QD_TRY(ctx->wprefix.ensure((W + 1) * sizeof(u64)));
     QD_TRY(ctx->slist.ensure(((pcap + 63) / 64) * sizeof(u64)));
     QD_TRY(ctx->pwpre.ensure(((pcap + 63) / 64 + 1) * sizeof(u64)));
     // pool: hash, representative, magnitude (and 32-bit key) per unique
@@ -1286,7 +1290,7 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
         QD_TRY(sync_read(ctx, ctx->nhist.p, kHistBins * sizeof(u64), h.data()));
         u64 *keep = ctx->slist.as<u64>();
         u32 c24 = 0;
-        QD_TRY(select_topk(ctx, Ph, Pr, Pm, Pk, ns, max_obj, h, ctx->bitmap.as<u64>(), keep, sk, &c24));
+        QD_TRY(select_topk(ctx, Ph, Pr, Pm, Pk, ns, max_obj, h, nullptr, keep, sk, &c24));
         const u64 Wp = (ns + 63) / 64;
         QD_TRY(run_scan(ctx, LoadPopc{keep}, Wp, ctx->pwpre.as<u64>()));
         hipLaunchKernelGGL(k_pool_compact, dim3(grid_for(ns)), dim3(256), 0, s, (const u64 *)Ph, (const u64 *)Pr,
@@ -1305,103 +1309,187 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
 
     std::vector<u64> c1(nb1), op(nb1 + 1), gofs(u64(ngroups) * (gbk + 1));
     std::vector<u64> psize(nparts);
+    // the single walk's regions: group q's lie in part q / gpp at
+    // (q % gpp) * gr * capr, block b's at b * capr within them
+    const u32 gbits = ngroups >= 8 ? 3 : ngroups >= 4 ? 2 : ngroups >= 2 ? 1 : 0;
+    const double rmean = gmean / double(gr);
+    const u64 capr = u64(rmean * 1.03 + 8.0 * std::sqrt(rmean)) + 256;
+    const u64 gbytes = u64(gr) * capr * sizeof(Entry);
+    u32 rparts = 1;
+    while (rparts < ngroups && u64(ngroups / rparts) * gbytes > kBigAlloc) rparts *= 2;
+    if (rparts > 4) return fail(QD_ERR_CAPACITY, "hash-range pass: regions beyond four allocations");
+    const u32 gpp = ngroups / rparts;  // groups per part
+    QD_TRY(ctx->rcnt.ensure(u64(ngroups) * gr * sizeof(u32)));
+    QD_TRY(ctx->abuf.ensure(gcap * sizeof(Entry)));
+    std::vector<u32> rc(u64(ngroups) * gr);
+    bool l2_exact = false;  // second pass with exact offsets (after a spill)
     phase_mark(ctx, PH_COLL_INSERT);
     for (const Pass &pass : passes) {
         const u64 n = pass.n;
         const RangeSel rs{ctx->tags.as<u8>(), pstride, T, pass.slo, pass.shi, pass.rlo, pass.rhi};
-        // a. the range's children into the first pass's buckets: counts per
-        // (block, bucket), their column scan, the buckets' offsets within
-        // their parts, then the writes
-        prof_begin(ctx, QD_PHASE_GENERATE);
-        hipLaunchKernelGGL(k_range_bincount<KIND>, dim3(gr), dim3(kGenThreads), 0, s,
-                           (const PDesc *)ctx->pdesc.as<PDesc>(), P.mag, (const u64 *)ctx->cb.as<u64>(),
-                           (const u64 *)ctx->wb.as<u64>(), N, (const u64 *)ctx->cpar.as<u64>(),
-                           (const u64 *)ctx->tab.as<u64>(), P.begin, rule, nitems, rs, b1, ctx->bmat.as<u32>());
-        hipLaunchKernelGGL(k_range_colscan, dim3(unsigned(nb1)), dim3(256), 0, s, ctx->bmat.as<u32>(), u32(gr), u32(nb1),
-                           ctx->cur1.as<u64>());
-        QD_HIP(hipGetLastError());
-        prof_end(ctx, QD_PHASE_GENERATE);
-        QD_TRY(sync_read(ctx, ctx->cur1.p, nb1 * sizeof(u64), c1.data()));
-        u64 kept = 0;
-        for (u32 q = 0; q < nparts; ++q) {
-            u64 acc2 = 0;
-            for (u64 d = q * perpart; d < (q + 1) * perpart; ++d) {
-                op[d] = acc2;  // bucket d's first entry within its part
-                acc2 += c1[d];
+        // a. one walk: every child of the range into its block's region of
+        // its group
+        // (qd_tuning.split_generate: always the two-walk way below)
+        bool single = !ctx->tune.split_generate;
+        GroupBases GB{};
+        if (single) {
+            for (u32 q = 0; q < rparts; ++q) QD_TRY(ensure_big(ctx->rpart[q], u64(gpp) * gbytes, u64(gpp) * gbytes));
+            for (u32 g = 0; g < ngroups; ++g) GB.g[g] = ctx->rpart[g / gpp].as<Entry>() + u64(g % gpp) * gr * capr;
+            QD_HIP(hipMemsetAsync(errw, 0, sizeof(u32), s));
+            prof_begin(ctx, QD_PHASE_GENERATE);
+            hipLaunchKernelGGL(k_range_walkwrite<KIND>, dim3(gr), dim3(kGenThreads), 0, s,
+                               (const PDesc *)ctx->pdesc.as<PDesc>(), P.mag, (const u64 *)ctx->cb.as<u64>(),
+                               (const u64 *)ctx->wb.as<u64>(), N, (const u64 *)ctx->cpar.as<u64>(),
+                               (const u64 *)ctx->tab.as<u64>(), P.begin, rule, nitems, rs, gbits, GB, capr,
+                               ctx->rcnt.as<u32>(), errw);
+            QD_HIP(hipGetLastError());
+            prof_end(ctx, QD_PHASE_GENERATE);
+            u32 ew = 0;
+            QD_TRY(sync_read(ctx, errw, sizeof(u32), &ew));
+            single = !(ew & ERR_SPILL1);
+        }
+        if (single) {
+            QD_TRY(sync_read(ctx, ctx->rcnt.p, rc.size() * sizeof(u32), rc.data()));
+            u64 kept = 0;
+            for (u32 v : rc) kept += v;
+            if (kept != n) return fail(QD_ERR_HIP, "hash-range pass kept a different candidate count than it counted");
+        } else {
+            // a region passed its capacity: the two-walk way, counts per
+            // (block, bucket), their column scan, the buckets' offsets within
+            // their parts, then the writes
+            QD_HIP(hipMemsetAsync(errw, 0, sizeof(u32), s));
+            st.reserved |= 8;
+            prof_begin(ctx, QD_PHASE_GENERATE);
+            hipLaunchKernelGGL(k_range_bincount<KIND>, dim3(gr), dim3(kGenThreads), 0, s,
+                               (const PDesc *)ctx->pdesc.as<PDesc>(), P.mag, (const u64 *)ctx->cb.as<u64>(),
+                               (const u64 *)ctx->wb.as<u64>(), N, (const u64 *)ctx->cpar.as<u64>(),
+                               (const u64 *)ctx->tab.as<u64>(), P.begin, rule, nitems, rs, b1, ctx->bmat.as<u32>());
+            hipLaunchKernelGGL(k_range_colscan, dim3(unsigned(nb1)), dim3(256), 0, s, ctx->bmat.as<u32>(), u32(gr),
+                               u32(nb1), ctx->cur1.as<u64>());
+            QD_HIP(hipGetLastError());
+            prof_end(ctx, QD_PHASE_GENERATE);
+            QD_TRY(sync_read(ctx, ctx->cur1.p, nb1 * sizeof(u64), c1.data()));
+            u64 kept = 0;
+            for (u32 q = 0; q < nparts; ++q) {
+                u64 acc2 = 0;
+                for (u64 d = q * perpart; d < (q + 1) * perpart; ++d) {
+                    op[d] = acc2;  // bucket d's first entry within its part
+                    acc2 += c1[d];
+                }
+                psize[q] = acc2;
+                kept += acc2;
             }
-            psize[q] = acc2;
-            kept += acc2;
+            if (kept != n)
+                return fail(QD_ERR_HIP, "hash-range pass kept a different candidate count than it counted");
+            for (u32 g = 0; g < ngroups; ++g) {
+                const u64 g0 = u64(g) * gbk;
+                u64 *o = gofs.data() + u64(g) * (gbk + 1);
+                for (u64 b = 0; b < gbk; ++b) o[b] = op[g0 + b];
+                o[gbk] = op[g0 + gbk - 1] + c1[g0 + gbk - 1];
+                if (o[gbk] - o[0] > gcap)
+                    return fail(QD_ERR_CAPACITY, "hash-range pass: hash keys too skewed for the first radix pass");
+            }
+            for (u32 q = 0; q < nparts; ++q) QD_TRY(ctx->rpart[q].ensure(std::max<u64>(psize[q], 1) * sizeof(Entry)));
+            QD_TRY(upload(ctx, ctx->off1, op.data(), nb1 * sizeof(u64)));
+            QD_TRY(upload(ctx, ctx->goff, gofs.data(), gofs.size() * sizeof(u64)));
+            EParts E{};
+            for (u32 q = 0; q < 4; ++q) E.p[q] = q < nparts ? ctx->rpart[q].as<Entry>() : nullptr;
+            E.pshift = pshift;
+            prof_begin(ctx, QD_PHASE_GENERATE);
+            hipLaunchKernelGGL(k_range_binwrite<KIND>, dim3(gr), dim3(kGenThreads), 0, s,
+                               (const PDesc *)ctx->pdesc.as<PDesc>(), P.mag, (const u64 *)ctx->cb.as<u64>(),
+                               (const u64 *)ctx->wb.as<u64>(), N, (const u64 *)ctx->cpar.as<u64>(),
+                               (const u64 *)ctx->tab.as<u64>(), P.begin, rule, nitems, rs, b1,
+                               (const u32 *)ctx->bmat.as<u32>(), (const u64 *)ctx->off1.as<u64>(), E);
+            QD_HIP(hipGetLastError());
+            prof_end(ctx, QD_PHASE_GENERATE);
         }
-        if (kept != n) return fail(QD_ERR_HIP, "hash-range pass kept a different candidate count than it counted");
-        for (u32 g = 0; g < ngroups; ++g) {
-            const u64 g0 = u64(g) * gbk;
-            u64 *o = gofs.data() + u64(g) * (gbk + 1);
-            for (u64 b = 0; b < gbk; ++b) o[b] = op[g0 + b];
-            o[gbk] = op[g0 + gbk - 1] + c1[g0 + gbk - 1];
-            if (o[gbk] - o[0] > gcap)
-                return fail(QD_ERR_CAPACITY, "hash-range pass: hash keys too skewed for the first radix pass");
-        }
-        for (u32 q = 0; q < nparts; ++q) QD_TRY(ctx->rpart[q].ensure(std::max<u64>(psize[q], 1) * sizeof(Entry)));
-        for (u32 q = nparts; q < 4; ++q) ctx->rpart[q].release();
-        QD_TRY(upload(ctx, ctx->off1, op.data(), nb1 * sizeof(u64)));
-        QD_TRY(upload(ctx, ctx->goff, gofs.data(), gofs.size() * sizeof(u64)));
-        EParts E{};
-        for (u32 q = 0; q < 4; ++q) E.p[q] = q < nparts ? ctx->rpart[q].as<Entry>() : nullptr;
-        E.pshift = pshift;
-        prof_begin(ctx, QD_PHASE_GENERATE);
-        hipLaunchKernelGGL(k_range_binwrite<KIND>, dim3(gr), dim3(kGenThreads), 0, s,
-                           (const PDesc *)ctx->pdesc.as<PDesc>(), P.mag, (const u64 *)ctx->cb.as<u64>(),
-                           (const u64 *)ctx->wb.as<u64>(), N, (const u64 *)ctx->cpar.as<u64>(),
-                           (const u64 *)ctx->tab.as<u64>(), P.begin, rule, nitems, rs, b1, (const u32 *)ctx->bmat.as<u32>(),
-                           (const u64 *)ctx->off1.as<u64>(), E);
-        QD_HIP(hipGetLastError());
-        prof_end(ctx, QD_PHASE_GENERATE);
         st.dedup_levels = (b1 ? 1 : 0) + (b2 ? 1 : 0);
-        // b. per group of first-level buckets: second pass, dedup, pool
+        // b. per group of first-level buckets: (single walk) its regions
+        // merged into its buckets, then the second pass, dedup, pool
         for (u32 g = 0; g < ngroups; ++g) {
-            const u64 g0 = u64(g) * gbk;
-            const u64 *o = gofs.data() + u64(g) * (gbk + 1);
-            const u64 ng = o[gbk] - o[0];
-            if (ng == 0) continue;
-            Entry *Ep = ctx->rpart[g0 >> pshift].as<Entry>();  // the group's part
-            Entry *Eg = Ep + o[0];                              // the group's entries
-            const u64 *go = ctx->goff.as<u64>() + u64(g) * (gbk + 1);  // its buckets' offsets within the part
+            const Entry *gin;  // the group's buckets: at go[b] .. go[b + 1] of gin
+            const u64 *go;
+            u64 obase, ng;
+            Entry *ub;  // where the group's uniques go (read by then): ng * 32 bytes
+            if (single) {
+                ng = 0;
+                for (u32 b = 0; b < gr; ++b) ng += rc[u64(g) * gr + b];
+                if (ng == 0) continue;
+                if (ng > gcap)
+                    return fail(QD_ERR_CAPACITY, "hash-range pass: hash keys too skewed for the first radix pass");
+                const u32 *rcg = ctx->rcnt.as<u32>() + u64(g) * gr;
+                prof_begin(ctx, QD_PHASE_PARTITION);
+                hipLaunchKernelGGL(k_region_hist, dim3(gr), dim3(256), 0, s, (const Entry *)GB.g[g], capr, rcg, u32(gr),
+                                   64 - b1, u32(gbk), ctx->bmat.as<u32>());
+                hipLaunchKernelGGL(k_range_colscan, dim3(unsigned(gbk)), dim3(256), 0, s, ctx->bmat.as<u32>(), u32(gr),
+                                   u32(gbk), ctx->cur1.as<u64>());
+                hipLaunchKernelGGL(k_excl_small, dim3(1), dim3(64), 0, s, (const u64 *)ctx->cur1.as<u64>(), u32(gbk),
+                                   ctx->off1.as<u64>());
+                hipLaunchKernelGGL(k_region_place, dim3(gr), dim3(256), 0, s, (const Entry *)GB.g[g], capr, rcg, u32(gr),
+                                   64 - b1, u32(gbk), (const u32 *)ctx->bmat.as<u32>(), (const u64 *)ctx->off1.as<u64>(),
+                                   ctx->abuf.as<Entry>());
+                QD_HIP(hipGetLastError());
+                prof_end(ctx, QD_PHASE_PARTITION);
+                gin = ctx->abuf.as<Entry>();
+                go = ctx->off1.as<u64>();
+                obase = 0;
+                ub = GB.g[g];
+            } else {
+                const u64 g0 = u64(g) * gbk;
+                const u64 *o = gofs.data() + u64(g) * (gbk + 1);
+                ng = o[gbk] - o[0];
+                if (ng == 0) continue;
+                gin = ctx->rpart[g0 >> pshift].as<Entry>();  // the group's part
+                go = ctx->goff.as<u64>() + u64(g) * (gbk + 1);  // its buckets' offsets within the part
+                obase = o[0];
+                ub = ctx->rpart[g0 >> pshift].as<Entry>() + o[0];
+            }
             Entry *E2 = ctx->ebuf2.as<Entry>();
             QD_HIP(hipMemsetAsync(counters, 0, 8 * sizeof(u64), s));
             u64 rstride = 0;
             prof_begin(ctx, QD_PHASE_PARTITION);
             if (b2) {
-                int ncu2 = 256;
-                (void)hipDeviceGetAttribute(&ncu2, hipDeviceAttributeMultiprocessorCount, ctx->device);
-                const unsigned g2 = unsigned(std::min<u64>(gbk, std::max(ncu2 / 2, 1)));
-                hipLaunchKernelGGL(k_level2_fixed, dim3(g2), dim3(1024), 0, s, (const Entry *)Ep, u64(0), go, gbk, b1,
-                                   b2, cap2, E2, ctx->cnt2.as<u64>(), errw);
-                QD_HIP(hipGetLastError());
-                QD_TRY(run_scan(ctx, LoadU64{ctx->cnt2.as<u64>()}, nb2g, ctx->off2.as<u64>()));
-                u32 ew = 0;
-                QD_TRY(sync_read(ctx, errw, sizeof(u32), &ew));
-                rstride = cap2;
-                if (ew & ERR_SPILL) {  // a sub-bucket passed its capacity: exact offsets
-                    QD_HIP(hipMemsetAsync(errw, 0, sizeof(u32), s));
+                bool exact = l2_exact;
+                if (!exact) {
+                    int ncu2 = 256;
+                    (void)hipDeviceGetAttribute(&ncu2, hipDeviceAttributeMultiprocessorCount, ctx->device);
+                    const unsigned g2 = unsigned(std::min<u64>(gbk, std::max(ncu2 / 2, 1)));
+                    hipLaunchKernelGGL(k_level2_fixed, dim3(g2), dim3(1024), 0, s, gin, u64(0), go, gbk, b1, b2, cap2,
+                                       E2, ctx->cnt2.as<u64>(), errw);
+                    QD_HIP(hipGetLastError());
+                    QD_TRY(run_scan(ctx, LoadU64{ctx->cnt2.as<u64>()}, nb2g, ctx->off2.as<u64>()));
+                    u32 ew2 = 0;
+                    QD_TRY(sync_read(ctx, errw, sizeof(u32), &ew2));
+                    rstride = cap2;
+                    if (ew2 & ERR_SPILL) {
+                        QD_HIP(hipMemsetAsync(errw, 0, sizeof(u32), s));
+                        exact = true;
+                        // a rule whose hashes repeat enough to spill once
+                        // (split_merge) spills in most groups: exact offsets
+                        // from here on
+                        l2_exact = true;
+                    }
+                }
+                if (exact) {  // a sub-bucket passed its capacity: exact offsets
                     hb[kHostN] = ng;
                     QD_HIP(hipMemcpyAsync(ctx->off2.as<u64>() + nb2g, hb + kHostN, sizeof(u64),
                                           hipMemcpyHostToDevice, s));
-                    hipLaunchKernelGGL(k_level2_exact, dim3(unsigned(gbk)), dim3(1024), 0, s, (const Entry *)Ep, u64(0),
-                                       go, o[0], b1, b2, E2, ctx->off2.as<u64>());
+                    hipLaunchKernelGGL(k_level2_exact, dim3(unsigned(gbk)), dim3(1024), 0, s, gin, u64(0), go, obase, b1,
+                                       b2, E2, ctx->off2.as<u64>());
                     QD_HIP(hipGetLastError());
                     rstride = 0;
                     st.reserved |= 1;
                 }
             } else {
                 // one level: the group's first-pass buckets are the dedup's
-                QD_HIP(hipMemcpyAsync(E2, Eg, ng * sizeof(Entry), hipMemcpyDeviceToDevice, s));
-                hipLaunchKernelGGL(k_add_offset, dim3(grid_for(gbk + 1)), dim3(256), 0, s, go, gbk + 1, u64(0) - o[0],
+                QD_HIP(hipMemcpyAsync(E2, gin + obase, ng * sizeof(Entry), hipMemcpyDeviceToDevice, s));
+                hipLaunchKernelGGL(k_add_offset, dim3(grid_for(gbk + 1)), dim3(256), 0, s, go, gbk + 1, u64(0) - obase,
                                    ctx->off2.as<u64>());
                 QD_HIP(hipGetLastError());
             }
             prof_end(ctx, QD_PHASE_PARTITION);
-            // the uniques go to the group's first-pass region (read by now)
-            u64 *UH = reinterpret_cast<u64 *>(Eg), *UR = UH + ng;
+            u64 *UH = reinterpret_cast<u64 *>(ub), *UR = UH + ng;
             double2 *UM = reinterpret_cast<double2 *>(UR + ng);
             u32 *UK = ukeys ? ctx->ukey.as<u32>() : nullptr;
             u64 U = 0;
@@ -1440,8 +1528,15 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
         st.total_proba = 0;
         return QD_OK;
     }
-    // order: survivors in ascending representative order
+    // order: survivors in ascending representative order, through a bitmap
+    // over the candidates (allocated now that the passes' buffers are free)
     phase_mark(ctx, PH_PREP_FINAL);
+    QD_HIP(hipStreamSynchronize(s));
+    for (DBuf &b : ctx->rpart) b.release();
+    ctx->ebuf2.release();
+    ctx->tags.release();
+    QD_TRY(ctx->bitmap.ensure(W * sizeof(u64)));
+    QD_TRY(ctx->wprefix.ensure((W + 1) * sizeof(u64)));
     u64 *bitmap = ctx->bitmap.as<u64>();
     prof_begin(ctx, QD_PHASE_ORDER);
     QD_HIP(hipMemsetAsync(bitmap, 0, W * sizeof(u64), s));

@@ -22,9 +22,9 @@ def _same(a, b, what):
     assert a.total_proba == b.total_proba, what
 
 
-def _chain(hs, n_iter, cap, chunk_log2):
+def _chain(hs, n_iter, cap, chunk_log2, two_walk=0):
     ctx = q.default_context()
-    ctx.set_tuning(chunk_log2=chunk_log2)
+    ctx.set_tuning(chunk_log2=chunk_log2, split_generate=two_walk)
     try:
         a, b = q.it_t(), q.it_t()
         a.upload(hs)
@@ -37,7 +37,7 @@ def _chain(hs, n_iter, cap, chunk_log2):
                 out.append((a.download(), st))
         return out
     finally:
-        ctx.set_tuning(chunk_log2=0)
+        ctx.set_tuning(chunk_log2=0, split_generate=0)
 
 
 # The pass budget (2^chunk candidates) against a step's C candidates sets how
@@ -51,11 +51,14 @@ def _chain(hs, n_iter, cap, chunk_log2):
     (8, 42, 3, 700, 10, 0),
     (9, 2, 3, q.UINT64_MAX, 12, 255 * 4096 + 1),
 ])
-def test_range_passes_match_single_pass(nodes, seed, n_iter, cap, chunk, min_c):
-    print(f"range nodes={nodes} seed={seed} iters={n_iter} cap={cap} chunk=2^{chunk}", flush=True)
+@pytest.mark.parametrize("two_walk", [0, 1])
+def test_range_passes_match_single_pass(nodes, seed, n_iter, cap, chunk, min_c, two_walk):
+    """two_walk=1: each pass counts its children per bucket in one walk and
+    writes them in a second (the path a region overflow takes)."""
+    print(f"range nodes={nodes} seed={seed} iters={n_iter} cap={cap} chunk=2^{chunk} two_walk={two_walk}", flush=True)
     hs = HostState.from_objects([qcgd.seeded_graph(nodes, seed)])
     one = _chain(hs, n_iter, cap, 0)
-    many = _chain(hs, n_iter, cap, chunk)
+    many = _chain(hs, n_iter, cap, chunk, two_walk)
     used = False
     for i, ((x, sx), (y, sy)) in enumerate(zip(one, many)):
         _same(x, y, f"step {i}")
@@ -63,6 +66,7 @@ def test_range_passes_match_single_pass(nodes, seed, n_iter, cap, chunk, min_c):
             assert sx[k] == sy[k], (i, k)
         used |= bool(sy["reserved"] & 4)
     assert used, "no step ran in hash-range passes"
+    assert all(not (sy["reserved"] & 8) for _, sy in many) or two_walk, "a region overflowed"
     print("candidates per step:", [sx["num_candidates"] for _, sx in one], flush=True)
     assert max(sx["num_candidates"] for _, sx in one) >= min_c
 
