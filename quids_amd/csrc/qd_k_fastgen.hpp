@@ -114,7 +114,7 @@ __device__ inline void prep_parent(const Parents &P, u64 p, bool have, const u64
         if (sl == 0 && sm_first_hmlz(g)) d.flags |= PD_HMLZ;
         d.flags = u8(__shfl(int(d.flags), int(sbase), kWave));
         k = u32(__popcll(d.dmask)) + (lmg ? 1 : 0);
-        if (in) {  // the slots of sm_fill_table
+        if (in && tab) {  // the slots of sm_fill_table (tab null: filled by a later launch)
             u64 *t = tab + sm_tab_base(P.begin[p]) + kSmSlots * sl;
             const u32 b = g.name_begin(sl);
             t[0] = g.sub_hash(b);

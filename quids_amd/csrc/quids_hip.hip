@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -86,14 +87,27 @@ static int fail(int code, const std::string &msg) {
 // ===========================================================================
 constexpr size_t kBigAlloc = 48ull << 30;
 
+static double dbuf_ms();
+static void dbuf_log(const char *what, size_t bytes, double ms, const char *name = nullptr);
+
 struct DBuf {
     void *p = nullptr;
     size_t cap = 0;
+    bool tight = false;  // grows to the size asked (+1/8), not by 1.5x: the states' buffers
+    bool view = false;   // p points into another buffer (the hash-range slab): never freed here
+    const char *name = nullptr;  // for the QD_LOG_RANGE allocation log
     int ensure(size_t bytes) {
         if (bytes <= cap && p) return QD_OK;
+        if (view) {  // a view that must grow becomes a buffer of its own
+            p = nullptr;
+            cap = 0;
+            view = false;
+        }
         const size_t old = cap;
         if (p) {
+            const double t0 = dbuf_ms();
             (void)hipFree(p);
+            dbuf_log("hipFree", cap, dbuf_ms() - t0, name);
             p = nullptr;
             cap = 0;
         }
@@ -102,13 +116,17 @@ struct DBuf {
         // kBigAlloc unless asked for it: on MI355X, allocations of 64 GB and
         // more take seconds to map and to free, smaller ones about 1 ms
         size_t want = std::max<size_t>(bytes + bytes / 8, 256);
-        if (old) want = std::max(want, old + old / 2);
+        if (old && !tight) want = std::max(want, old + old / 2);
         want = std::min(want, std::max(bytes, kBigAlloc));
+        double t0 = dbuf_ms();
         hipError_t e = hipMalloc(&p, want);
+        dbuf_log(e == hipSuccess ? "hipMalloc" : "hipMalloc (failed)", want, dbuf_ms() - t0, name);
         if (e != hipSuccess) {
             (void)hipGetLastError();
             p = nullptr;
+            t0 = dbuf_ms();
             e = hipMalloc(&p, std::max<size_t>(bytes, 256));
+            dbuf_log(e == hipSuccess ? "hipMalloc" : "hipMalloc (failed)", bytes, dbuf_ms() - t0, name);
             if (e != hipSuccess) {
                 (void)hipGetLastError();
                 p = nullptr;
@@ -123,15 +141,55 @@ struct DBuf {
         return QD_OK;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && !view) {
+            const double t0 = dbuf_ms();
+            (void)hipFree(p);
+            dbuf_log("hipFree", cap, dbuf_ms() - t0, name);
+        }
         p = nullptr;
         cap = 0;
+        view = false;
+    }
+    // releases, then allocates exactly `bytes`
+    int alloc_exact(size_t bytes) {
+        release();
+        const double t0 = dbuf_ms();
+        const hipError_t e = hipMalloc(&p, bytes);
+        dbuf_log(e == hipSuccess ? "hipMalloc" : "hipMalloc (failed)", bytes, dbuf_ms() - t0, name);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            size_t freeb = 0, totalb = 0;
+            (void)hipMemGetInfo(&freeb, &totalb);
+            return fail(QD_ERR_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed (device free " +
+                                        std::to_string(freeb) + " of " + std::to_string(totalb) + " bytes)");
+        }
+        cap = bytes;
+        return QD_OK;
+    }
+    // points this buffer at bytes of `base` (releasing what it owned)
+    void set_view(void *base, size_t bytes) {
+        release();
+        p = base;
+        cap = bytes;
+        view = true;
     }
     template <class T>
     T *as() const {
         return static_cast<T *>(p);
     }
 };
+
+static double dbuf_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// QD_LOG_RANGE: allocations and frees of 1 GB or more, with their host time.
+static void dbuf_log(const char *what, size_t bytes, double ms, const char *name) {
+    static const bool on = std::getenv("QD_LOG_RANGE") != nullptr;
+    if (on && (bytes >= (1ull << 30) || ms > 10.0))
+        std::fprintf(stderr, "[quids_hip] %s %s %.2f GB: %.1f ms\n", what, name ? name : "-", bytes / 1e9, ms);
+}
 
 // Device scratch of qd_average_value_noalloc, a file-scope array that every
 // GPU the library runs on holds from module load until the process ends:
@@ -185,6 +243,7 @@ struct qd_ctx {
     DBuf rpart[4], pk0, pk1, tags, bmat, goff;  // hash-range passes: first-pass buckets, pool select keys,
                                                 // children's slice tags, per-block bucket counts, group offsets,
     DBuf rcnt, abuf;                            // region counts, a group's merged first-level buckets
+    DBuf slab;  // one allocation kept across hash-range steps, carved into their big buffers (views)
     // output buffers of batched chunks (objects, begin, size, mag per chunk),
     // kept between calls
     std::vector<DBuf> parts;
@@ -200,13 +259,26 @@ struct qd_ctx {
                 &biglist, &redolist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &cpar, &opar, &ebuf2, &cnt2, &slist, &ukey, &selloop, &segloop, &bsoff, &bscnt,
                 &adjk, &adjv, &adjg, &adjd, &adjp, &adjw, &bseg, &boseg, &brules, &bsinst, &bcount, &bsel, &btrunc,
                 &blist, &bcur, &btile, &bhist, &bkseg, &bchunk, &bcsum, &btot, &pool0, &pool1, &pwpre, &blist2, &mw, &rpart[0], &rpart[1], &rpart[2],
-                &rpart[3], &pk0, &pk1, &tags, &bmat, &goff, &rcnt, &abuf};
+                &rpart[3], &pk0, &pk1, &tags, &bmat, &goff, &rcnt, &abuf, &slab};
+    }
+    void set_names() {
+        static const char *const names[] = {
+            "nc", "cb", "pre", "tab", "kept", "keys", "ebuf0", "ebuf1", "hist1", "off1", "cur1", "off2", "nhist",
+            "counters", "sel0", "sel1", "bitmap", "wprefix", "padded", "chunks", "scanblk", "biglist", "redolist",
+            "bigsoff", "bigkey", "bigcand", "bigmag", "bigseg", "pdesc", "wi", "wb", "cpar", "opar", "ebuf2", "cnt2",
+            "slist", "ukey", "selloop", "segloop", "bsoff", "bscnt", "adjk", "adjv", "adjg", "adjd", "adjp", "adjw",
+            "bseg", "boseg", "brules", "bsinst", "bcount", "bsel", "btrunc", "blist", "bcur", "btile", "bhist",
+            "bkseg", "bchunk", "bcsum", "btot", "pool0", "pool1", "pwpre", "blist2", "mw", "rpart0", "rpart1",
+            "rpart2", "rpart3", "pk0", "pk1", "tags", "bmat", "goff", "rcnt", "abuf", "slab"};
+        std::vector<DBuf *> v = all_named();
+        static_assert(sizeof(names) / sizeof(names[0]) == 79, "one name per buffer");
+        for (size_t i = 0; i < v.size() && i < 79; ++i) v[i]->name = names[i];
     }
 };
 
 struct qd_state {
     qd_ctx *ctx = nullptr;
-    DBuf objects, begin, size, mag;
+    DBuf objects{nullptr, 0, true}, begin{nullptr, 0, true}, size{nullptr, 0, true}, mag{nullptr, 0, true};
     u64 n = 0;
     u64 bytes = 0;
     double total_proba = 1;
@@ -601,7 +673,8 @@ static u64 step_budget(qd_ctx *ctx) {
         return 1ull << 28;
     }
     u64 cached = 0;
-    for (DBuf *b : ctx->all()) cached += b->cap;
+    for (DBuf *b : ctx->all())
+        if (!b->view) cached += b->cap;
     // the largest buffer (the second partition pass, about 1.7 x 32 bytes per
     // candidate) stays within kBigAlloc
     const u64 by_alloc = u64(double(kBigAlloc) / (32.0 * 1.8));
@@ -615,6 +688,40 @@ static u64 step_budget(qd_ctx *ctx) {
 static int ensure_big(DBuf &b, u64 bytes, u64 full) {
     if (bytes <= b.cap && b.p) return QD_OK;
     if (full > bytes && bytes >= full / 8) bytes = full;
+    return b.ensure(bytes);
+}
+
+// Host wall clock in ms, for the QD_LOG_RANGE diagnostics.
+static double wall_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// The hash-range passes' own buffers.  They are kept from one such step to
+// the next (mapping and unmapping buffers of tens of GB costs about a
+// second per step), and released when a step runs in one pass or an
+// allocation does not fit beside them.
+static std::vector<DBuf *> range_cache(qd_ctx *ctx) {
+    return {&ctx->slab,  &ctx->rpart[0], &ctx->rpart[1], &ctx->rpart[2], &ctx->rpart[3], &ctx->tags, &ctx->abuf,
+            &ctx->rcnt, &ctx->pool0,    &ctx->pool1,    &ctx->pk0,      &ctx->pk1,      &ctx->goff};
+}
+
+static void release_range_cache(qd_ctx *ctx) {
+    bool any = false;
+    for (DBuf *b : range_cache(ctx)) any |= b->p != nullptr;
+    if (!any) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (DBuf *b : ctx->all())
+        if (b->view) b->release();  // views into the slab
+    for (DBuf *b : range_cache(ctx)) b->release();
+}
+
+// DBuf::ensure that, when the device is out of memory, releases the range
+// passes' cached buffers and tries once more.
+static int ensure_relieved(qd_ctx *ctx, DBuf &b, size_t bytes) {
+    if (bytes <= b.cap && b.p) return QD_OK;
+    const int rc = b.ensure(bytes);
+    if (rc != QD_ERR_OOM) return rc;
+    release_range_cache(ctx);
     return b.ensure(bytes);
 }
 
@@ -800,7 +907,9 @@ static int populate_out(qd_ctx *ctx, const Parents &P, const qd_rule &rule, cons
     prof_end(ctx, QD_PHASE_FINALIZE);
     QD_TRY(sync_read(ctx, out->begin.as<u64>() + K, sizeof(u64), hb));
     const u64 bytes = hb[0];
-    QD_TRY(out->objects.ensure(std::max<u64>(bytes, 1)));
+    // (after a step in hash-range passes the slab is dead here: its room is
+    // the state's if the state needs it)
+    QD_TRY(ensure_relieved(ctx, out->objects, std::max<u64>(bytes, 1)));
     prof_begin(ctx, QD_PHASE_FINALIZE);
     if (KIND == QD_RULE_QCGD_SPLIT_MERGE)
         hipLaunchKernelGGL(k_populate_sm, dim3(unsigned(std::min<u64>((K + kPopSegs - 1) / kPopSegs, 65536))),
@@ -878,7 +987,8 @@ static u64 state_mem(const qd_state *s) { return s->objects.cap + s->begin.cap +
 
 static u64 ctx_scratch(qd_ctx *ctx) {
     u64 t = 0;
-    for (DBuf *b : ctx->all()) t += b->cap;
+    for (DBuf *b : ctx->all())
+        if (!b->view) t += b->cap;
     return t;
 }
 
@@ -1067,13 +1177,15 @@ constexpr double kRangeBytesPerCand = 32.0 * 1.03 + (32.0 + 32.0 * 2.05 + 4.0 + 
 
 template <int KIND>
 static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule &rule, u64 C, u64 nitems,
-                      u64 max_obj, const qd_params &prm, qd_state *out, qd_symbolic *sy, qd_step_stats &st) {
+                      u64 max_obj, const qd_params &prm, qd_state *out, qd_symbolic *sy, qd_step_stats &st,
+                      const u64 *keep, bool fill_tab) {
     hipStream_t s = ctx->stream;
     const u64 N = P.n;
     const SelKey sk{prm.simple_truncation ? 0u : 1u, 0u, prm.seed};
     u64 *counters = ctx->counters.as<u64>();
     u32 *errw = reinterpret_cast<u32 *>(counters + 4);
     u64 *hb = ctx->host;
+    double tl[8] = {wall_ms()};  // QD_LOG_RANGE: entry, buffers released, tags, counted, first pass, passes, end
     // what only the single-pass path and the adjoint use is not needed here:
     // give its memory to the passes (the helpers re-allocate what they use,
     // at this path's sizes).  A batched step that runs one instance here
@@ -1084,16 +1196,12 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
                     &ctx->bigmag, &ctx->bigseg, &ctx->opar, &ctx->adjk, &ctx->adjv, &ctx->adjg, &ctx->adjd,
                     &ctx->adjp, &ctx->adjw})
         b->release();
-    // nor the output state's buffers, which hold nothing of this step yet:
-    // the finalize allocates them at the step's size
-    for (DBuf *b : {&out->objects, &out->begin, &out->size, &out->mag}) b->release();
-    out->n = 0;
-    out->bytes = 0;
     if (KIND != QD_RULE_QCGD_SPLIT_MERGE) ctx->tab.release();  // split_merge's name-root table
     QD_TRY(ctx->hist1.ensure(kHistBins * sizeof(u64)));
     QD_TRY(ctx->cur1.ensure(kRangeBins * sizeof(u64)));
     QD_TRY(ctx->nhist.ensure(kHistBins * sizeof(u64)));
     u64 *hist1 = ctx->hist1.as<u64>();
+    tl[1] = wall_ms();
     const unsigned gg = unsigned(std::max<u64>(
         std::min<u64>((nitems + kGenThreads - 1) / kGenThreads, ctx->tune.generate_grid), 1));
     const bool capped = max_obj < C;
@@ -1101,11 +1209,14 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
     const u64 W = (C + 63) / 64;
     const u64 pstride = (nitems + kRangeTile - 1) / kRangeTile * kRangeTile;  // bytes of a tag plane
 
-    // 1. the pass size: what the free memory (and the buffers this path
-    // reuses) holds after the children's tags, the bitmaps, the pool and the
-    // next state, within the second radix pass's reach (2^22 buckets of at
-    // most 1024 entries) and the four first-pass allocations; then the tag
-    // width: slices of at most about the pass size
+    // 1. memory: one slab, kept from one hash-range step to the next (a
+    // buffer of tens of GB that was freed costs about 25 ms per GB to map
+    // again), carved into this step's name-root table (split_merge, when the
+    // prep left it to this step), the children's tags, the pool and the
+    // passes' buffers.  Its size: what the free memory and the buffers it
+    // replaces hold, after room for the next state; the pass size: what it
+    // holds after the table, the tags and the pool; then the tag width:
+    // slices of at most about the pass size.
     u64 budget;
     u32 T = 4;
     auto tag_bits = [&](u64 b) {
@@ -1113,37 +1224,85 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
         while (t < kMaxTagBits && double((1u << t) - 1) * double(b) < 1.03 * double(C)) ++t;
         return t;
     };
+    const u64 tab_bytes = fill_tab ? (sm_tab_base(in_bytes) + kSmSlots * 64 + 16) * sizeof(u64) : 0;
+    const double Kp = double(capped ? max_obj : C);
+    const double pool_bytes = capped ? 3.0 * 36.0 * Kp : 36.0 * Kp;
+    constexpr double kSlabSlack = 1.03;  // the passes' layout margins over budget x bytes per candidate
+    std::vector<DBuf *> slabbed = {&ctx->tags, &ctx->rpart[0], &ctx->abuf, &ctx->ebuf2, &ctx->pool0,
+                                   &ctx->pool1, &ctx->pk0, &ctx->pk1, &ctx->ukey};
+    if (fill_tab) slabbed.push_back(&ctx->tab);
+    u64 slab_want = ctx->slab.cap;
     double log_free = 0, log_reuse = 0, log_fixed = 0;  // the plan's inputs, for QD_LOG_RANGE
     if (ctx->tune.chunk_log2) {
         budget = 1ull << ctx->tune.chunk_log2;
         T = tag_bits(budget);
+        const double need = double(tab_bytes) + double(T) * double(pstride) + pool_bytes +
+                            kSlabSlack * kRangeBytesPerCand * double(budget) + double(64ull << 20);
+        slab_want = std::max<u64>(slab_want, u64(need));
     } else {
         size_t freeb = 0, totalb = 0;
         QD_HIP(hipMemGetInfo(&freeb, &totalb));
-        u64 reuse = ctx->ebuf2.cap + ctx->pool0.cap + ctx->pool1.cap + ctx->pk0.cap + ctx->pk1.cap + ctx->bitmap.cap +
-                    ctx->wprefix.cap + ctx->tags.cap + ctx->bmat.cap + ctx->cnt2.cap + ctx->off2.cap + ctx->slist.cap +
-                    ctx->pwpre.cap;
-        for (DBuf &b : ctx->rpart) reuse += b.cap;
-        // the next state with its finalize scratch (objects at 1.5 x the
+        u64 reuse = 0;  // owned buffers the slab replaces
+        for (DBuf *b : slabbed)
+            if (!b->view) reuse += b->cap;
+        for (DBuf &b : ctx->rpart)
+            if (!b.view) reuse += b.cap;
+        // the next state with its finalize scratch (Kp objects at 1.5 x the
         // parents' mean size), beyond what the output state holds already
-        const double K = double(capped ? max_obj : C);
-        const double next = std::max(0.0, 1.5 * double(in_bytes) * std::min(1.0, K / double(std::max<u64>(N, 1))) +
-                                              60.0 * K - double(state_mem(out)));
-        const double fixed = (capped ? 3.0 * 36.0 * K : 36.0 * K) + next + double(2ull << 30);
-        const double room = 0.85 * (double(freeb) + double(reuse)) - fixed;
+        const double next = std::max(0.0, 1.5 * double(in_bytes) / double(std::max<u64>(N, 1)) * Kp + 60.0 * Kp -
+                                              double(state_mem(out)));
+        const double fixed = next + double(2ull << 30);
+        const double avail = 0.92 * (double(freeb) + double(ctx->slab.cap) + double(reuse)) - fixed;
+        // keep the slab while it is within 80-115 % of what would be made now
+        // (the output state's buffers, counted in `next`, change size from
+        // one step to the next while the states grow; `next` is an upper
+        // estimate, and should the next state not fit beside the slab, the
+        // finalize releases it, dead by then)
+        const double cap = double(ctx->slab.cap);
+        if (!(cap >= 0.8 * avail && cap <= 1.15 * avail)) slab_want = avail > 0 ? u64(avail) : 0;
+        const double room = double(slab_want) - double(tab_bytes) - pool_bytes;
         auto fit = [&](u32 t) {
             const double r = room - double(t) * double(pstride);
-            return r > 0 ? u64(std::min(r / kRangeBytesPerCand, 4.0 * double(kBigAlloc) / 33.0)) : 0ull;
+            return r > 0 ? u64(r / (kSlabSlack * kRangeBytesPerCand)) : 0ull;
         };
         T = tag_bits(fit(5));
         budget = fit(T);
         log_free = double(freeb);
-        log_reuse = double(reuse);
+        log_reuse = double(reuse) + cap;
         log_fixed = fixed;
     }
     budget = std::max<u64>(std::min<u64>(budget, 1ull << 32), 1ull << 10);
     const u32 S = (1u << T) - 1;  // slices; tag S: no such child
-    QD_TRY(ctx->tags.ensure(std::max<u64>(u64(T) * pstride, 8)));
+    if (slab_want != ctx->slab.cap) {  // (re)make the slab: nothing in it is live
+        QD_HIP(hipStreamSynchronize(s));
+        for (DBuf *b : ctx->all())
+            if (b->view) b->release();
+        for (DBuf *b : slabbed) b->release();
+        for (DBuf &b : ctx->rpart) b.release();
+        QD_TRY(ctx->slab.alloc_exact(std::max<u64>(slab_want, 1ull << 20)));
+    }
+    u8 *slab = ctx->slab.as<u8>();
+    u64 soff = 0;
+    // b at the slab's next bytes, or (past its end) a buffer of its own
+    auto carve = [&](DBuf &b, u64 bytes) -> int {
+        bytes = (std::max<u64>(bytes, 256) + 255) / 256 * 256;
+        if (soff + bytes <= ctx->slab.cap) {
+            b.set_view(slab + soff, bytes);
+            soff += bytes;
+            return QD_OK;
+        }
+        if (b.view) b.release();
+        return b.ensure(bytes);
+    };
+    if (fill_tab) {
+        QD_TRY(carve(ctx->tab, tab_bytes));
+        hipLaunchKernelGGL(k_prep_fast<KIND>, dim3(grid_for(N, 16, 65536)), dim3(256), 0, s, P, keep,
+                           ctx->pdesc.as<PDesc>(), ctx->nc.as<u32>(), ctx->wi.as<u32>(), ctx->tab.as<u64>(),
+                           reinterpret_cast<u32 *>(counters + 3), errw);
+        QD_HIP(hipGetLastError());
+    }
+    QD_TRY(carve(ctx->tags, std::max<u64>(u64(T) * pstride, 8)));
+    tl[2] = wall_ms();
 
     // 2. how the candidates spread over the 4096 values of the top 12 hash
     // bits, and each child's slice
@@ -1157,6 +1316,7 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
     QD_HIP(hipGetLastError());
     prof_end(ctx, QD_PHASE_GENERATE);
     QD_TRY(sync_read(ctx, hist1, kHistBins * sizeof(u64), hc.data()));
+    tl[3] = wall_ms();
 
     // 3. passes: runs of whole slices within the budget; a slice beyond the
     // budget alone is split into runs of bins (its passes list all of the
@@ -1235,9 +1395,9 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
     while ((nmax >> L) > tgt && L < 22) ++L;
     const u32 b1 = std::min<u32>(L, 10), b2 = L - b1;
     const u64 nb1 = 1ull << b1;
-    const u32 nparts = u32(std::min<u64>(nb1, 4));
-    const u32 pshift = b1 - (nparts == 4 ? 2 : nparts == 2 ? 1 : 0);
-    const u64 perpart = nb1 / nparts;
+    // the two-walk way's first-level buckets: all in the regions' memory
+    const u32 nparts = 1, pshift = b1;
+    const u64 perpart = nb1;
     const u32 ngroups = u32(std::min<u64>(nb1, kRangeGroups));
     const u64 gbk = nb1 / ngroups;  // first-level buckets per group (a group lies in one part)
     const double gmean = double(nmax) / double(ngroups);
@@ -1252,16 +1412,16 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
     const unsigned gr = unsigned(std::max<u64>(std::min<u64>((nitems + kRangeTile - 1) / kRangeTile, kRangeGrid), 1));
     QD_TRY(ctx->bmat.ensure(u64(gr) * nb1 * sizeof(u32)));
     QD_TRY(ctx->off1.ensure((nb1 + 1) * sizeof(u64)));
-    QD_TRY(ctx->ebuf2.ensure(std::max<u64>(nb2g * cap2, gcap) * sizeof(Entry)));
     QD_TRY(ctx->cnt2.ensure(nb2g * sizeof(u64)));
     QD_TRY(ctx->off2.ensure((nb2g + 1) * sizeof(u64)));
-    QD_TRY(ctx->ukey.ensure(gcap * sizeof(u32)));
-    QD_TRY(ctx->pool0.ensure(pcap * 32));
-    QD_TRY(ctx->pool1.ensure((capped ? max_obj + 1 : 1) * 32));
+    QD_TRY(carve(ctx->pool0, pcap * 32));
+    QD_TRY(carve(ctx->pool1, (capped ? max_obj + 1 : 1) * 32));
     if (ukeys) {
-        QD_TRY(ctx->pk0.ensure(pcap * sizeof(u32)));
-        QD_TRY(ctx->pk1.ensure((capped ? max_obj + 1 : 1) * sizeof(u32)));
+        QD_TRY(carve(ctx->pk0, pcap * sizeof(u32)));
+        QD_TRY(carve(ctx->pk1, (capped ? max_obj + 1 : 1) * sizeof(u32)));
     }
+    QD_TRY(carve(ctx->ebuf2, std::max<u64>(nb2g * cap2, gcap) * sizeof(Entry)));
+    QD_TRY(carve(ctx->ukey, gcap * sizeof(u32)));
     QD_TRY(ctx->slist.ensure(((pcap + 63) / 64) * sizeof(u64)));
     QD_TRY(ctx->pwpre.ensure(((pcap + 63) / 64 + 1) * sizeof(u64)));
     // pool: hash, representative, magnitude (and 32-bit key) per unique
@@ -1315,14 +1475,14 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
     const double rmean = gmean / double(gr);
     const u64 capr = u64(rmean * 1.03 + 8.0 * std::sqrt(rmean)) + 256;
     const u64 gbytes = u64(gr) * capr * sizeof(Entry);
-    u32 rparts = 1;
-    while (rparts < ngroups && u64(ngroups / rparts) * gbytes > kBigAlloc) rparts *= 2;
-    if (rparts > 4) return fail(QD_ERR_CAPACITY, "hash-range pass: regions beyond four allocations");
-    const u32 gpp = ngroups / rparts;  // groups per part
     QD_TRY(ctx->rcnt.ensure(u64(ngroups) * gr * sizeof(u32)));
-    QD_TRY(ctx->abuf.ensure(gcap * sizeof(Entry)));
+    QD_TRY(carve(ctx->abuf, gcap * sizeof(Entry)));
+    // the regions, group after group (also the two-walk way's buckets, and
+    // the order phase's bitmaps)
+    QD_TRY(carve(ctx->rpart[0], std::max<u64>(u64(ngroups) * gbytes, nmax * sizeof(Entry))));
     std::vector<u32> rc(u64(ngroups) * gr);
     bool l2_exact = false;  // second pass with exact offsets (after a spill)
+    tl[4] = wall_ms();
     phase_mark(ctx, PH_COLL_INSERT);
     for (const Pass &pass : passes) {
         const u64 n = pass.n;
@@ -1333,8 +1493,7 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
         bool single = !ctx->tune.split_generate;
         GroupBases GB{};
         if (single) {
-            for (u32 q = 0; q < rparts; ++q) QD_TRY(ensure_big(ctx->rpart[q], u64(gpp) * gbytes, u64(gpp) * gbytes));
-            for (u32 g = 0; g < ngroups; ++g) GB.g[g] = ctx->rpart[g / gpp].as<Entry>() + u64(g % gpp) * gr * capr;
+            for (u32 g = 0; g < ngroups; ++g) GB.g[g] = ctx->rpart[0].as<Entry>() + u64(g) * gr * capr;
             QD_HIP(hipMemsetAsync(errw, 0, sizeof(u32), s));
             prof_begin(ctx, QD_PHASE_GENERATE);
             hipLaunchKernelGGL(k_range_walkwrite<KIND>, dim3(gr), dim3(kGenThreads), 0, s,
@@ -1509,6 +1668,7 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
             if (capped && ns > 2 * max_obj) QD_TRY(trim_pool());
         }
     }
+    tl[5] = wall_ms();
     phase_mark(ctx, PH_TRUNC);
     if (capped && ns > max_obj) QD_TRY(trim_pool());
     st.num_unique = Utot;
@@ -1529,19 +1689,23 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
         return QD_OK;
     }
     // order: survivors in ascending representative order, through a bitmap
-    // over the candidates (allocated now that the passes' buffers are free)
+    // over the candidates and its popcount prefix, in the regions' memory
+    // (read by now) when it holds them, else in buffers of their own
     phase_mark(ctx, PH_PREP_FINAL);
-    QD_HIP(hipStreamSynchronize(s));
-    for (DBuf &b : ctx->rpart) b.release();
-    ctx->ebuf2.release();
-    ctx->tags.release();
-    QD_TRY(ctx->bitmap.ensure(W * sizeof(u64)));
-    QD_TRY(ctx->wprefix.ensure((W + 1) * sizeof(u64)));
-    u64 *bitmap = ctx->bitmap.as<u64>();
+    u64 *bitmap, *wprefix;
+    if (ctx->rpart[0].cap >= (2 * W + 1) * sizeof(u64)) {
+        bitmap = ctx->rpart[0].as<u64>();
+        wprefix = bitmap + W;
+    } else {
+        QD_TRY(ctx->bitmap.ensure(W * sizeof(u64)));
+        QD_TRY(ctx->wprefix.ensure((W + 1) * sizeof(u64)));
+        bitmap = ctx->bitmap.as<u64>();
+        wprefix = ctx->wprefix.as<u64>();
+    }
     prof_begin(ctx, QD_PHASE_ORDER);
     QD_HIP(hipMemsetAsync(bitmap, 0, W * sizeof(u64), s));
     hipLaunchKernelGGL(k_mark_reps, dim3(grid_for(K)), dim3(256), 0, s, (const u64 *)Pr, K, bitmap);
-    QD_TRY(run_scan(ctx, LoadPopc{bitmap}, W, ctx->wprefix.as<u64>()));
+    QD_TRY(run_scan(ctx, LoadPopc{bitmap}, W, wprefix));
     QD_TRY(out->begin.ensure((K + 1) * sizeof(u64)));
     QD_TRY(out->size.ensure(K * sizeof(u32)));
     QD_TRY(out->mag.ensure(K * sizeof(double2)));
@@ -1549,7 +1713,7 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
     QD_TRY(ctx->keys.ensure(K * sizeof(u64)));
     u64 *orep = ctx->keys.as<u64>();
     hipLaunchKernelGGL(k_place_list, dim3(grid_for(K)), dim3(256), 0, s, (const u64 *)Pr, (const double2 *)Pm, K,
-                       (const u64 *)bitmap, (const u64 *)ctx->wprefix.as<u64>(), orep, out->mag.as<double2>());
+                       (const u64 *)bitmap, (const u64 *)wprefix, orep, out->mag.as<double2>());
     QD_HIP(hipGetLastError());
     prof_end(ctx, QD_PHASE_ORDER);
     phase_mark(ctx, PH_FINAL);
@@ -1559,6 +1723,13 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
     prof_collect(ctx);
     st.total_proba = out->total_proba;
     st.reserved |= 4;  // hash-range passes
+    if (log_plan) {
+        tl[6] = wall_ms();
+        std::fprintf(stderr,
+                     "[quids_hip] hash-range step host ms: release %.1f, tags %.1f, count %.1f, pass buffers %.1f, "
+                     "passes %.1f, truncate+finalize %.1f\n",
+                     tl[1] - tl[0], tl[2] - tl[1], tl[3] - tl[2], tl[4] - tl[3], tl[5] - tl[4], tl[6] - tl[5]);
+    }
     return QD_OK;
 }
 
@@ -1795,29 +1966,44 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, u64 in_mem, co
     phase_mark(ctx, PH_PREP_INDEX);
 
     // ---- 1 prep
-    QD_TRY(ctx->nc.ensure(N * sizeof(u32)));
-    QD_TRY(ctx->cb.ensure((N + 1) * sizeof(u64)));
-    QD_TRY(ctx->pre.ensure(N * sizeof(u64)));
+    const double tp0 = wall_ms();
+    QD_TRY(ensure_relieved(ctx, ctx->nc, N * sizeof(u32)));
+    QD_TRY(ensure_relieved(ctx, ctx->cb, (N + 1) * sizeof(u64)));
+    QD_TRY(ensure_relieved(ctx, ctx->pre, N * sizeof(u64)));
     QD_TRY(ctx->counters.ensure(64 * sizeof(u64)));
-    if (KIND == QD_RULE_QCGD_SPLIT_MERGE) QD_TRY(ctx->tab.ensure((sm_tab_base(in_bytes) + kSmSlots * 64 + 16) * sizeof(u64)));
+    bool fast = KIND != QD_RULE_HADAMARD && !ctx->tune.generic_path;
+    // split_merge's name-root table.  After a step in hash-range passes (the
+    // context holds their slab) it waits until the step's path is known: a
+    // range step places it in the slab, a single pass in a buffer of its
+    // own, and both fill it with a second prep launch.
+    const u64 tab_bytes = (sm_tab_base(in_bytes) + kSmSlots * 64 + 16) * sizeof(u64);
+    bool defer_tab = KIND == QD_RULE_QCGD_SPLIT_MERGE && !BATCH && fast && ctx->slab.p != nullptr;
+    if (ctx->tab.view) ctx->tab.release();  // a view into the slab is this step's to place again
+    if (KIND == QD_RULE_QCGD_SPLIT_MERGE && !defer_tab) QD_TRY(ensure_relieved(ctx, ctx->tab, tab_bytes));
     u64 *counters = ctx->counters.as<u64>();
     // counters: [0] unique count, [1] select count A, [2] select count B,
     // [3] low: max child size, [4] low: error word, [5] oversized buckets
     QD_HIP(hipMemsetAsync(counters, 0, 64 * sizeof(u64), s));
     u32 *maxsz = reinterpret_cast<u32 *>(counters + 3);
     u32 *errw = reinterpret_cast<u32 *>(counters + 4);
-    bool fast = KIND != QD_RULE_HADAMARD && !ctx->tune.generic_path;
     if (BATCH && !fast) return fail(QD_ERR_UNSUPPORTED, "the batched step needs the QCGD fast path");
     u64 *hb = ctx->host;
     u64 nitems = 0;  // work items of the fast generate
     if (fast) {
-        QD_TRY(ctx->pdesc.ensure(N * sizeof(PDesc)));
-        QD_TRY(ctx->wi.ensure(N * sizeof(u32)));
-        QD_TRY(ctx->wb.ensure((N + 1) * sizeof(u64)));
+        QD_TRY(ensure_relieved(ctx, ctx->pdesc, N * sizeof(PDesc)));
+        QD_TRY(ensure_relieved(ctx, ctx->wi, N * sizeof(u32)));
+        QD_TRY(ensure_relieved(ctx, ctx->wb, (N + 1) * sizeof(u64)));
+        static const bool log_prep = std::getenv("QD_LOG_RANGE") != nullptr;
+        if (log_prep) {
+            size_t fb = 0, tb = 0;
+            (void)hipMemGetInfo(&fb, &tb);
+            std::fprintf(stderr, "[quids_hip] step prep buffers: %.1f host ms; free %.1f GB, context scratch %.1f GB\n",
+                         wall_ms() - tp0, fb / 1e9, ctx_scratch(ctx) / 1e9);
+        }
         prof_begin(ctx, QD_PHASE_PREP);
         hipLaunchKernelGGL(k_prep_fast<KIND>, dim3(grid_for(N, 16, 65536)), dim3(256), 0, s, P, keep,
-                           ctx->pdesc.as<PDesc>(), ctx->nc.as<u32>(), ctx->wi.as<u32>(), ctx->tab.as<u64>(), maxsz,
-                           errw);
+                           ctx->pdesc.as<PDesc>(), ctx->nc.as<u32>(), ctx->wi.as<u32>(),
+                           defer_tab ? (u64 *)nullptr : ctx->tab.as<u64>(), maxsz, errw);
         QD_HIP(hipGetLastError());
         prof_end(ctx, QD_PHASE_PREP);
         prof_begin(ctx, QD_PHASE_SCAN);
@@ -1832,7 +2018,7 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, u64 in_mem, co
         nitems = hb[3];
         if (nitems && !(u32(hb[2]) & ERR_WIDE)) {
             const u64 nchunk = (nitems + kGenThreads - 1) / kGenThreads;
-            QD_TRY(ctx->cpar.ensure(nchunk * sizeof(u64)));
+            QD_TRY(ensure_relieved(ctx, ctx->cpar, nchunk * sizeof(u64)));
             hipLaunchKernelGGL(k_chunk_parent, dim3(grid_for(nchunk)), dim3(256), 0, s, ctx->wb.as<u64>(), N, nchunk,
                                ctx->cpar.as<u64>());
             QD_HIP(hipGetLastError());
@@ -1842,6 +2028,11 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, u64 in_mem, co
         if (u32(hb[2]) & ERR_WIDE) {  // a graph wider than 64 nodes: generic kernels
             fast = false;
             QD_HIP(hipMemsetAsync(counters + 3, 0, 2 * sizeof(u64), s));
+            if (defer_tab) {  // the generic prep fills it
+                release_range_cache(ctx);
+                QD_TRY(ensure_relieved(ctx, ctx->tab, tab_bytes));
+                defer_tab = false;
+            }
         }
     }
     if (!fast) {
@@ -1879,8 +2070,18 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, u64 in_mem, co
                 const u64 per = kMemPerObject + max_child + prm.align_byte_length;
                 max_obj = std::max<u64>(1, target / per);
             }
-            return range_step<KIND>(ctx, P, in_bytes, rule, C, nitems, max_obj, prm, out, sy, st);
+            return range_step<KIND>(ctx, P, in_bytes, rule, C, nitems, max_obj, prm, out, sy, st, keep, defer_tab);
         }
+    }
+    release_range_cache(ctx);  // one pass: its buffers are sized from the free memory
+    if (defer_tab) {
+        QD_TRY(ensure_relieved(ctx, ctx->tab, tab_bytes));
+        prof_begin(ctx, QD_PHASE_PREP);
+        hipLaunchKernelGGL(k_prep_fast<KIND>, dim3(grid_for(N, 16, 65536)), dim3(256), 0, s, P, keep,
+                           ctx->pdesc.as<PDesc>(), ctx->nc.as<u32>(), ctx->wi.as<u32>(), ctx->tab.as<u64>(), maxsz,
+                           errw);
+        QD_HIP(hipGetLastError());
+        prof_end(ctx, QD_PHASE_PREP);
     }
 
     // ---- 3 generate: hash and magnitude at the candidate index, or (fused)
@@ -2050,13 +2251,18 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, u64 in_mem, co
 // more than a quarter of the device while less than half is free, it is
 // released, so that the coming step's buffers (sized for its own
 // candidates) can be mapped; otherwise it is kept, and chained steps of
-// similar size map nothing.
+// similar size map nothing.  The range passes' cache stays: the step
+// releases it once it knows it runs in one pass, or when an allocation
+// needs its room.
 static int relieve_memory(qd_ctx *ctx) {
+    if (ctx->slab.p) return QD_OK;  // hash-range steps: their buffers are sized and kept on purpose
     size_t freeb = 0, totalb = 0;
     QD_HIP(hipMemGetInfo(&freeb, &totalb));
     if (ctx_scratch(ctx) > totalb / 4 && freeb < totalb / 2) {
         QD_HIP(hipStreamSynchronize(ctx->stream));
-        for (DBuf *b : ctx->all()) b->release();
+        const std::vector<DBuf *> keep = range_cache(ctx);
+        for (DBuf *b : ctx->all())
+            if (std::find(keep.begin(), keep.end(), b) == keep.end()) b->release();
     }
     return QD_OK;
 }
@@ -2077,6 +2283,7 @@ static int adjoint_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, const qd
                         const double2 *gout, double2 *gin, double *grad, u64 max_obj) {
     hipStream_t s = ctx->stream;
     const u64 N = in->n, K = out->n;
+    release_range_cache(ctx);
     for (int t = 0; t < ndir; ++t) grad[t] = 0.0;
     if (N == 0) return QD_OK;
     if (max_obj < N)
@@ -2123,7 +2330,7 @@ static int adjoint_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, const qd
             QD_HIP(hipMemsetAsync(counters + 3, 0, 2 * sizeof(u64), s));
         } else if (nitems) {
             const u64 nchunk = (nitems + kGenThreads - 1) / kGenThreads;
-            QD_TRY(ctx->cpar.ensure(nchunk * sizeof(u64)));
+            QD_TRY(ensure_relieved(ctx, ctx->cpar, nchunk * sizeof(u64)));
             hipLaunchKernelGGL(k_chunk_parent, dim3(grid_for(nchunk)), dim3(256), 0, s, ctx->wb.as<u64>(), N, nchunk,
                                ctx->cpar.as<u64>());
             QD_HIP(hipGetLastError());
@@ -2427,6 +2634,7 @@ int qd_ctx_create(qd_ctx **out, int device) {
     qd_ctx *c = new (std::nothrow) qd_ctx;
     if (!c) return fail(QD_ERR_OOM, "host allocation failed");
     c->device = device;
+    c->set_names();
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -2477,7 +2685,8 @@ int qd_ctx_synchronize(qd_ctx *c) {
 int qd_ctx_scratch_bytes(qd_ctx *c, uint64_t *bytes) {
     if (!c || !bytes) return fail(QD_ERR_INVALID, "null argument");
     u64 t = 0;
-    for (DBuf *b : c->all()) t += b->cap;
+    for (DBuf *b : c->all())
+        if (!b->view) t += b->cap;
     *bytes = t;
     return QD_OK;
 }
@@ -2603,6 +2812,10 @@ int qd_state_create(qd_ctx *c, qd_state **out) {
     qd_state *s = new (std::nothrow) qd_state;
     if (!s) return fail(QD_ERR_OOM, "host allocation failed");
     s->ctx = c;
+    s->objects.name = "state objects";
+    s->begin.name = "state begin";
+    s->size.name = "state size";
+    s->mag.name = "state mag";
     QD_HIP(hipSetDevice(c->device));
     int rc = state_reserve(s, 0, 0);
     if (rc) {
