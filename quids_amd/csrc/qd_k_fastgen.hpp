@@ -391,6 +391,40 @@ __device__ inline void flip_run(W &w, const PDesc &pd, BT bits, u32 dlimit, cons
     }
 }
 
+// One child of a flip rule walked on its own, with all its decision bits
+// (bit d: decision d, in node order).  flip_run's three sequences are
+// independent: the magnitude takes one factor per decision node in node
+// order (chosen by the node's parent left bit and the decision), and the
+// left and right folds run over the nodes whose bits end up set (the
+// parent's masks with the taken decision nodes flipped), in node order.
+// Computed apart, each over the set bits of a mask, they are the same
+// products and folds in the same order, without flip_run's per-node
+// branches, which lanes holding different children would take both ways.
+template <typename KC>
+__device__ inline u64 flip_child(const PDesc &pd, u64 bits, const KC &k, cplx &mag) {
+    u64 dm = pd.dmask, flips = 0;
+    u32 d = 0;
+    while (dm) {
+        const u32 i = u32(__builtin_ctzll(dm));
+        dm &= dm - 1;
+        const bool take = (bits >> d++) & 1;
+        const bool l = (pd.lmask >> i) & 1;
+        const cplx c = take ? (l ? k.dc : k.d) : (l ? cneg(k.dnc) : k.dn);
+        mag = cmul(mag, c);
+        flips |= u64(take) << i;
+    }
+    u64 lm = pd.lmask ^ flips, rm = pd.rmask ^ flips, lh = 0, rh = 0;
+    while (lm) {
+        lh = small_step_pm(lh, u64(__builtin_ctzll(lm)));
+        lm &= lm - 1;
+    }
+    while (rm) {
+        rh = small_step_pm(rh, u64(__builtin_ctzll(rm)));
+        rm &= rm - 1;
+    }
+    return finish_pm(pd.nf, lh, rh);
+}
+
 // The children of a work item, depth first over its t = min(k, 3) trailing
 // decisions s, s+1, s+2 (bit j of the child's item index u is decision
 // s + j): each branch walks the nodes up to the next decision once and its
@@ -853,17 +887,19 @@ __device__ __attribute__((always_inline)) inline void range_walks(const PDesc *D
                 const double2 m0 = pmag[p];
                 Walk wk{};
                 wk.mag = cplx{m0.x, m0.y};
-                const u64 *ptab = tab + sm_tab_base(pbegin[p]);
                 // the child's own branch of walk_item's tree in one walk over
                 // all its decisions (the same folds): one loop over the nodes,
                 // so that lanes of different parents wait for each other
                 // once, not once per segment
                 const u64 bits = u64(j0) | (u64(u) << s);
-                if (KIND == QD_RULE_QCGD_SPLIT_MERGE) sm_run(wk, pd, ptab, bits, s + t, k);
-                else flip_run<KIND == QD_RULE_QCGD_COIN>(wk, pd, bits, s + t, k);
                 u64 h;
-                if (KIND == QD_RULE_QCGD_SPLIT_MERGE) h = sm_finish(wk, ptab);
-                else h = finish_pm(pd.nf, wk.lh, wk.rh);
+                if (KIND == QD_RULE_QCGD_SPLIT_MERGE) {
+                    const u64 *ptab = tab + sm_tab_base(pbegin[p]);
+                    sm_run(wk, pd, ptab, bits, s + t, k);
+                    h = sm_finish(wk, ptab);
+                } else {
+                    h = flip_child(pd, bits, k, wk.mag);
+                }
                 const u32 top = u32(h >> 52);
                 if (top >= rs.rlo && top < rs.rhi) leaf(rotl12(h), wk.mag, cb[p] + j0 + (u64(u) << s));
             }

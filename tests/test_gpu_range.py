@@ -71,6 +71,34 @@ def test_range_passes_match_single_pass(nodes, seed, n_iter, cap, chunk, min_c, 
     assert max(sx["num_candidates"] for _, sx in one) >= min_c
 
 
+def test_coin_in_range_passes_matches_single_pass():
+    """coin (flip where left != right) through hash-range passes."""
+    hs = HostState.from_objects([qcgd.seeded_graph(9, 11)])
+    rules = (qcgd.coin(0.8, 0.3), qcgd.split_merge(0.5, 0.2), qcgd.erase_create(0.9, 1.3))
+
+    def chain(chunk):
+        ctx = q.default_context()
+        ctx.set_tuning(chunk_log2=chunk)
+        try:
+            a, b = q.it_t(), q.it_t()
+            a.upload(hs)
+            out = []
+            for _ in range(3):
+                q.simulate(a, qcgd.step)
+                for r in rules:
+                    st = q.simulate(a, r, b, None, 2000)
+                    a, b = b, a
+                    out.append((a.download(), st))
+            return out
+        finally:
+            ctx.set_tuning(chunk_log2=0)
+
+    one, many = chain(0), chain(10)
+    assert any(sy["reserved"] & 4 for _, sy in many), "no step ran in hash-range passes"
+    for i, ((x, sx), (y, sy)) in enumerate(zip(one, many)):
+        _same(x, y, f"step {i}")
+
+
 def test_range_passes_match_oracle():
     hs = HostState.from_objects([qcgd.seeded_graph(6, 8)])
     ctx = q.default_context()
