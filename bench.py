@@ -231,7 +231,7 @@ def rocprof_summary(args):
         wv = _after_marker(wrows, _dispatches(wrows, prefixes, "WRITE_SIZE"))
         if not durs or not fv or not wv:
             continue
-        fams = sorted({n.split("<")[0] for n, _ in durs.values()})
+        fams = sorted({n.split("(")[0].split("<")[0] for n, _ in durs.values()})
         n = len(durs)
         avg_us = sum(v for _, v in durs.values()) / n / 1e3
         fk = sum(v for _, v in fv.values()) / len(fv)
