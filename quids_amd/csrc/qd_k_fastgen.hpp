@@ -391,6 +391,73 @@ __device__ inline void flip_run(W &w, const PDesc &pd, BT bits, u32 dlimit, cons
     }
 }
 
+// One split_merge child walked on its own, with all its decision bits (bit
+// d: decision d): sm_run and sm_finish with the per-node cases folded into
+// selects, so that lanes holding different children run one path: a
+// decision node multiplies the magnitude by the factor its case and its
+// decision pick; every node emits its first root (its own name tree, or,
+// taken, the split's left half or the merge), a taken split also its right
+// half, and a taken merge skips the next node.  The same products, roots and
+// folds in the same order as sm_run.
+template <typename KC>
+__device__ inline u64 sm_child(const PDesc &pd, const u64 *tab, u64 bits, const KC &k, cplx &mag) {
+    const u32 n = pd.n;
+    u64 nh = 0, lh = 0, rh = 0;
+    u32 c = 0, d = 0, i = 0, end = n;
+    bool overflow = false;
+    auto emit = [&](u64 root, bool l, bool r) {
+        nh = step_pm(nh, root);
+        if (l) lh = small_step_pm(lh, c);
+        if (r) rh = small_step_pm(rh, c);
+        ++c;
+    };
+    if (pd.flags & PD_LM) {  // the last node merges with the first (qcgd.hpp:647-668)
+        const bool take = bits & 1;
+        d = 1;
+        if (take) {
+            mag = cmul(mag, k.dc);
+            emit(tab[kSmSlots * (n - 1) + 1], true, true);
+            i = 1;
+            end = n - 1;
+        } else {
+            mag = cmul(mag, cneg(k.dnc));
+        }
+    } else if (pd.flags & PD_FS) {  // the first node's split (qcgd.hpp:629-633)
+        const bool take = bits & 1;
+        d = 1;
+        if (take) {
+            mag = cmul(mag, k.d);
+            if (pd.flags & PD_HMLZ) {
+                emit(tab[1], true, false);
+                emit(tab[2], false, true);
+            } else {
+                overflow = true;
+                emit(tab[2], false, true);
+            }
+        } else {
+            mag = cmul(mag, k.dn);
+            emit(tab[0], true, true);
+        }
+        i = 1;
+    }
+    while (i < end) {
+        const bool l = (pd.lmask >> i) & 1, r = (pd.rmask >> i) & 1;
+        const bool dec = (pd.dmask >> i) & 1;
+        const bool take = dec && ((bits >> d) & 1);
+        d += dec ? 1u : 0u;
+        const bool split = l && r;
+        if (dec) mag = cmul(mag, split ? (take ? k.d : k.dn) : (take ? k.dc : cneg(k.dnc)));
+        emit(tab[kSmSlots * i + (take ? 1u : 0u)], take || l, take ? !split : r);
+        if (take && split) emit(tab[kSmSlots * i + 2], false, true);
+        i += (take && !split) ? 2u : 1u;
+    }
+    if (overflow) {  // the left half of the first node goes last (qcgd.hpp:838-845)
+        nh = step_pm(nh, tab[1]);
+        lh = small_step_pm(lh, c);
+    }
+    return finish_pm(nh, lh, rh);
+}
+
 // One child of a flip rule walked on its own, with all its decision bits
 // (bit d: decision d, in node order).  flip_run's three sequences are
 // independent: the magnitude takes one factor per decision node in node
@@ -894,9 +961,7 @@ __device__ __attribute__((always_inline)) inline void range_walks(const PDesc *D
                 const u64 bits = u64(j0) | (u64(u) << s);
                 u64 h;
                 if (KIND == QD_RULE_QCGD_SPLIT_MERGE) {
-                    const u64 *ptab = tab + sm_tab_base(pbegin[p]);
-                    sm_run(wk, pd, ptab, bits, s + t, k);
-                    h = sm_finish(wk, ptab);
+                    h = sm_child(pd, tab + sm_tab_base(pbegin[p]), bits, k, wk.mag);
                 } else {
                     h = flip_child(pd, bits, k, wk.mag);
                 }
