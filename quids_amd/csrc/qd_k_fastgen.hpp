@@ -1131,22 +1131,76 @@ __global__ void k_excl_small(const u64 *total, u32 n, u64 *off) {
 
 // Each region's entries at its running places in the buckets: bucket d
 // starts at boff[d], region r's first place in it is bm[r][d] after the
-// column scan.
-__global__ __launch_bounds__(256) void k_region_place(const Entry *in, u64 capr, const u32 *rcount, u32 nreg,
-                                                      u32 shift, u32 nbin, const u32 *bm, const u64 *boff,
-                                                      Entry *out) {
+// column scan.  A chunk of kPlaceChunk entries is ordered by bucket in LDS
+// first, so that consecutive lanes write consecutive places of a bucket
+// (runs of about kPlaceChunk / nbin entries) rather than one entry to each
+// of up to 64 buckets per store.
+constexpr u32 kPlaceThreads = 256;
+constexpr u32 kPlacePer = 8;
+constexpr u32 kPlaceChunk = kPlaceThreads * kPlacePer;
+
+__global__ __launch_bounds__(kPlaceThreads) void k_region_place(const Entry *in, u64 capr, const u32 *rcount,
+                                                                u32 nreg, u32 shift, u32 nbin, const u32 *bm,
+                                                                const u64 *boff, Entry *out) {
+    __shared__ Entry stage[kPlaceChunk];
+    __shared__ u32 cnt[kRangeBins], lst[kRangeBins];
     __shared__ u64 cur[kRangeBins];
+    __shared__ u64 tmp[kPlaceThreads / kWave + 1];
+    constexpr u32 kBinsPer = kRangeBins / kPlaceThreads;
+    auto digit = [&](u64 key) { return nbin > 1 ? u32(key >> shift) & (nbin - 1) : 0u; };
     for (u32 r = blockIdx.x; r < nreg; r += gridDim.x) {
         __syncthreads();
-        for (u32 i = threadIdx.x; i < nbin; i += blockDim.x) cur[i] = boff[i] + bm[u64(r) * nbin + i];
-        __syncthreads();
+        for (u32 i = threadIdx.x; i < nbin; i += kPlaceThreads) cur[i] = boff[i] + bm[u64(r) * nbin + i];
         const u32 n = rcount[r];
         const Entry *src = in + u64(r) * capr;
-        for (u32 i = threadIdx.x; i < n; i += blockDim.x) {
-            const Entry e = src[i];
-            const u32 d = nbin > 1 ? u32(e.hash >> shift) & (nbin - 1) : 0u;
-            const u64 at = atomicAdd(reinterpret_cast<unsigned long long *>(&cur[d]), 1ull);
-            out[at] = e;
+        for (u32 c0 = 0; c0 < n; c0 += kPlaceChunk) {
+            const u32 m = min(kPlaceChunk, n - c0);
+            __syncthreads();  // the previous chunk is written and its counts added
+            for (u32 i = threadIdx.x; i < nbin; i += kPlaceThreads) cnt[i] = 0;
+            __syncthreads();
+            Entry e[kPlacePer];
+            u32 dr[kPlacePer];
+#pragma unroll
+            for (u32 q = 0; q < kPlacePer; ++q) {
+                const u32 j = threadIdx.x + kPlaceThreads * q;
+                if (j < m) {
+                    e[q] = src[c0 + j];
+                    const u32 d = digit(e[q].hash);
+                    dr[q] = (d << 16) | atomicAdd(&cnt[d], 1u);
+                }
+            }
+            __syncthreads();
+            // the buckets' starts within the chunk: an exclusive scan of cnt
+            u32 v[kBinsPer];
+            u64 sum = 0;
+#pragma unroll
+            for (u32 q = 0; q < kBinsPer; ++q) {
+                const u32 b = threadIdx.x * kBinsPer + q;
+                v[q] = b < nbin ? cnt[b] : 0u;
+                sum += v[q];
+            }
+            u64 tot;
+            u64 run = block_excl_scan(sum, tmp, tot);
+#pragma unroll
+            for (u32 q = 0; q < kBinsPer; ++q) {
+                const u32 b = threadIdx.x * kBinsPer + q;
+                if (b < nbin) lst[b] = u32(run);
+                run += v[q];
+            }
+            __syncthreads();
+#pragma unroll
+            for (u32 q = 0; q < kPlacePer; ++q) {
+                const u32 j = threadIdx.x + kPlaceThreads * q;
+                if (j < m) stage[lst[dr[q] >> 16] + (dr[q] & 0xFFFF)] = e[q];
+            }
+            __syncthreads();
+            for (u32 j = threadIdx.x; j < m; j += kPlaceThreads) {
+                const Entry x = stage[j];
+                const u32 d = digit(x.hash);
+                out[cur[d] + (j - lst[d])] = x;
+            }
+            __syncthreads();
+            for (u32 i = threadIdx.x; i < nbin; i += kPlaceThreads) cur[i] += cnt[i];
         }
     }
 }
