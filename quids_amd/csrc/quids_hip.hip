@@ -1391,8 +1391,10 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
     u32 tgt = ctx->tune.dedup_bucket_target, dcap = ctx->tune.dedup_cap;
     while ((nmax >> 22) > tgt && tgt < 1024) tgt *= 2;
     dcap = std::max(dcap, 2 * tgt);
+    // (a bucket mean up to 1.25 x the target: one radix bit fewer for
+    // passes just above a power of two times the target, half the buckets)
     u32 L = 0;
-    while ((nmax >> L) > tgt && L < 22) ++L;
+    while ((nmax >> L) > tgt + tgt / 4 && L < 22) ++L;
     const u32 b1 = std::min<u32>(L, 10), b2 = L - b1;
     const u64 nb1 = 1ull << b1;
     // the two-walk way's first-level buckets: all in the regions' memory
