@@ -647,13 +647,14 @@ static int state_reserve(qd_state *s, u64 n, u64 bytes) {
 }
 
 // Dedup layout for C candidates: total bucket bits L so that buckets
-// average at most dedup_bucket_target entries (at most half the LDS table
-// capacity, so a bucket overflows only under a pathological hash
-// distribution); the first pass takes up to first_pass_bits, the second
-// pass the rest (at most 12 bits).
+// average at most 1.25 x dedup_bucket_target entries (the target is at most
+// half the LDS table capacity, so a bucket overflows only under a
+// pathological hash distribution; fuller buckets cost the dedup less per
+// entry); the first pass takes up to first_pass_bits, the second pass the
+// rest (at most 12 bits).
 static void dedup_layout(u64 C, const qd_tuning &t, u32 &b1, u32 &b2) {
     u32 L = 0;
-    while ((C >> L) > t.dedup_bucket_target && L < 24) ++L;
+    while ((C >> L) > t.dedup_bucket_target + t.dedup_bucket_target / 4 && L < 24) ++L;
     b1 = std::min<u32>(L, t.first_pass_bits);
     if (L - b1 > 12) b1 = L - 12;
     b2 = L - b1;
