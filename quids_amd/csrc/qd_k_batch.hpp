@@ -121,11 +121,14 @@ __device__ inline bool prefix_eq(u64 hi, u64 lo, const SegSel &q) {
 
 // Groups the uniques of truncated instances by instance as key records
 // {hi, slot} (read once, in slot order): each block ranks its tile's slots
-// per instance in LDS and reserves one range per instance.
+// per instance in LDS and reserves one range per instance.  With UK (the
+// dedup's top 32 bits of each slot's |m|^2, plain keys) a record's hi holds
+// only those bits (lower half zero) and the key is read in full from the
+// slot once a digit reaches past them (SegLoop::part).
 constexpr int kSegTile = 4096;
 __global__ __launch_bounds__(256) void k_seg_scatter(const u16 *sinst, u64 nslots, const u64 *UH, const double2 *UM,
-                                                     const u32 *trunc, u32 n, u64 *cursor, SegItem *list,
-                                                     SelKey sk) {
+                                                     const u32 *UK, const u32 *trunc, u32 n, u64 *cursor,
+                                                     SegItem *list, SelKey sk) {
     __shared__ u32 cnt[kMaxInst];
     __shared__ u64 base[kMaxInst];
     constexpr int kPer = kSegTile / 256;
@@ -154,8 +157,12 @@ __global__ __launch_bounds__(256) void k_seg_scatter(const u16 *sinst, u64 nslot
             if (inst[q] != kNoInst) {
                 const u64 s = t0 + threadIdx.x + u64(q) * 256;
                 SegItem it;
-                u64 lo;
-                seg_key(UH, UM, s, inst[q], sk, it.hi, lo);
+                if (UK) {
+                    it.hi = u64(UK[s]) << 32;
+                } else {
+                    u64 lo;
+                    seg_key(UH, UM, s, inst[q], sk, it.hi, lo);
+                }
                 it.slot = s;
                 list[base[inst[q]] + rank[q]] = it;
             }
@@ -185,8 +192,20 @@ struct SegLoop {
     u64 *off, *cnt;    // an active instance's current list range
     u64 *cursor;       // its next list's cursor
     const u64 *UH;
+    const double2 *UM;
+    SelKey sk;
     u32 n, ntiles;
+    u32 part;  // the first list's records hold the top 32 key bits only (UK)
 };
+
+// A record's full high key word when the current digit (bits [pos, pos +
+// 12)) reaches past what it holds.
+__device__ inline u64 seg_hi_at(const SegLoop *L, const SegItem &it, u32 inst, u32 pos_end) {
+    if (!L->part || pos_end <= 32) return it.hi;
+    u64 hi, lo;
+    seg_key(L->UH, L->UM, it.slot, inst, L->sk, hi, lo);
+    return hi;
+}
 
 // One block of 1024 threads: the tiles of every active instance's list.
 __global__ __launch_bounds__(1024) void k_seg_tiles(SegLoop *L) {
@@ -228,8 +247,9 @@ __global__ __launch_bounds__(256) void k_seg_hist(const SegLoop *L, u32 parity) 
         __syncthreads();
         for (u64 k = tl.begin + threadIdx.x; k < tl.end; k += blockDim.x) {
             const SegItem it = list[k];
+            const u64 hi = seg_hi_at(L, it, tl.inst, q.pos + 12);
             const u64 lo = q.pos + 12 > 64 ? seg_lo(L->UH, it.slot, tl.inst) : 0;
-            atomicAdd(&h[key_digit(it.hi, lo, q.pos)], 1u);
+            atomicAdd(&h[key_digit(hi, lo, q.pos)], 1u);
         }
         __syncthreads();
         u32 *row = L->hist + u64(tl.inst) * kHistBins;
@@ -258,6 +278,7 @@ __global__ __launch_bounds__(256) void k_seg_collect(const SegLoop *L, u32 parit
             SegItem it{};
             if (k < tl.end) {
                 it = list[k];
+                it.hi = seg_hi_at(L, it, tl.inst, q.pos);  // (carried in full from here on)
                 take = prefix_eq(it.hi, q.pos > 64 ? seg_lo(L->UH, it.slot, tl.inst) : 0, q);
             }
             u64 total;
@@ -377,7 +398,8 @@ __global__ __launch_bounds__(1024) void k_seg_next(SegLoop *L) {
 // Marks the survivors: every unique of an instance that is not truncated,
 // and the keys >= prefix of a truncated one.  Lanes hold consecutive slots.
 __global__ __launch_bounds__(256) void k_seg_mark(const u16 *sinst, const u64 *UH, const u64 *UR, const double2 *UM,
-                                                  u64 nslots, const SegSel *sel, u64 *bitmap, u64 *sbm, SelKey sk) {
+                                                  const u32 *UK, u64 nslots, const SegSel *sel, u64 *bitmap, u64 *sbm,
+                                                  SelKey sk) {
     const u64 stride = u64(gridDim.x) * blockDim.x;
     for (u64 base = u64(blockIdx.x) * blockDim.x; base < nslots; base += stride) {
         const u64 s = base + threadIdx.x;
@@ -388,9 +410,15 @@ __global__ __launch_bounds__(256) void k_seg_mark(const u16 *sinst, const u64 *U
                 const SegSel q = sel[inst];
                 if (q.keepall) keep = true;
                 else {
-                    u64 hi, lo;
-                    seg_key(UH, UM, s, inst, sk, hi, lo);
-                    keep = hi > q.hi || (hi == q.hi && lo >= q.lo);
+                    // the top 32 key bits decide unless they tie (UK: plain keys)
+                    const u32 top = UK ? UK[s] : 0u, qtop = u32(q.hi >> 32);
+                    if (UK && top != qtop) {
+                        keep = top > qtop;
+                    } else {
+                        u64 hi, lo;
+                        seg_key(UH, UM, s, inst, sk, hi, lo);
+                        keep = hi > q.hi || (hi == q.hi && lo >= q.lo);
+                    }
                 }
             }
         }

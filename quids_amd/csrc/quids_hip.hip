@@ -1773,7 +1773,7 @@ static int batch_cseg(qd_ctx *ctx, Batch *bt) {
 // round histograms the current records and carries only the ones tied at
 // the chosen digit into the next round.
 static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const u64 *UR, const double2 *UM,
-                        u64 *bitmap, u64 *sbm) {
+                        const u32 *UK, u64 *bitmap, u64 *sbm) {
     hipStream_t s = ctx->stream;
     const u32 n = bt->n;
     QD_TRY(ctx->bcount.ensure(n * sizeof(u64)));
@@ -1816,7 +1816,7 @@ static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const
         QD_TRY(ctx->bhist.ensure(u64(n) * kHistBins * sizeof(u32)));
         SegItem *cur = ctx->ebuf1.as<SegItem>(), *nxt = ctx->ebuf2.as<SegItem>();
         const unsigned sg = unsigned(std::min<u64>((nslots + kSegTile - 1) / kSegTile, 16384));
-        hipLaunchKernelGGL(k_seg_scatter, dim3(std::max(sg, 1u)), dim3(256), 0, s, sinst, nslots, UH, UM,
+        hipLaunchKernelGGL(k_seg_scatter, dim3(std::max(sg, 1u)), dim3(256), 0, s, sinst, nslots, UH, UM, UK,
                            ctx->btrunc.as<u32>(), n, ctx->bcur.as<u64>(), cur, bt->sk);
         QD_HIP(hipMemsetAsync(ctx->bhist.p, 0, u64(n) * kHistBins * sizeof(u32), s));
         // the digit rounds, driven from the device (SegLoop)
@@ -1835,6 +1835,9 @@ static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const
         h.cnt = ctx->bscnt.as<u64>();
         h.cursor = ctx->bcur.as<u64>();
         h.UH = UH;
+        h.UM = UM;
+        h.sk = bt->sk;
+        h.part = UK ? 1u : 0u;
         h.n = n;
         static_assert(sizeof(SegLoop) <= 8 * (kHostSelLoop - kHostSegLoop), "staging area");
         std::memcpy(ctx->host + kHostSegLoop, &h, sizeof h);
@@ -1844,8 +1847,8 @@ static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const
         for (u32 i = 0; i < n; ++i)
             if (sel[i].err || sel[i].active) return fail(QD_ERR_HIP, "batched select did not converge");
     }
-    hipLaunchKernelGGL(k_seg_mark, dim3(grid_for(nslots, 256, 16384)), dim3(256), 0, s, sinst, UH, UR, UM, nslots,
-                       (const SegSel *)dsel, bitmap, sbm, bt->sk);
+    hipLaunchKernelGGL(k_seg_mark, dim3(grid_for(nslots, 256, 16384)), dim3(256), 0, s, sinst, UH, UR, UM, UK,
+                       nslots, (const SegSel *)dsel, bitmap, sbm, bt->sk);
     QD_HIP(hipGetLastError());
     return QD_OK;
 }
@@ -2105,7 +2108,7 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, u64 in_mem, co
         const u64 full = step_budget(ctx) * 32;
         QD_TRY(ensure_big(ctx->ebuf0, C * 32, full));
         QD_TRY(ensure_big(ctx->ebuf1, std::max(C, nbin1 * cap1) * 32, full + full / 32));
-        if (!BATCH && !sk.prob && (max_obj == 0 || C > max_obj)) {
+        if (!sk.prob && (BATCH || max_obj == 0 || C > max_obj)) {
             QD_TRY(ensure_big(ctx->ukey, C * sizeof(u32), full / 8));
             UK = ctx->ukey.as<u32>();
         }
@@ -2197,7 +2200,7 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, u64 in_mem, co
     prof_begin(ctx, QD_PHASE_SELECT);
     QD_HIP(hipMemsetAsync(bitmap, 0, W * sizeof(u64), s));
     if (BATCH) {
-        QD_TRY(batch_select(ctx, bt, C, UH, UR, UM, bitmap, sbm));
+        QD_TRY(batch_select(ctx, bt, C, UH, UR, UM, UK, bitmap, sbm));
     } else if (U <= max_obj) {
         if (U)
             hipLaunchKernelGGL(k_mark_all, dim3(grid_for(C, 256, 16384)), dim3(256), 0, s, UR, C, bitmap, sbm);
