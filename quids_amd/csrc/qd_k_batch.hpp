@@ -125,20 +125,24 @@ __device__ inline bool prefix_eq(u64 hi, u64 lo, const SegSel &q) {
 // dedup's top 32 bits of each slot's |m|^2, plain keys) a record's hi holds
 // only those bits (lower half zero) and the key is read in full from the
 // slot once a digit reaches past them (SegLoop::part).
-constexpr int kSegTile = 4096;
-__global__ __launch_bounds__(256) void k_seg_scatter(const u16 *sinst, u64 nslots, const u64 *UH, const double2 *UM,
+// A tile of 16384 slots per block of 1024 threads: one cursor reservation
+// per instance per tile (fewer same-address atomics on the instances'
+// cursors than tiles of 4096).
+constexpr int kSegTile = 16384;
+constexpr int kSegScatterThreads = 1024;
+__global__ __launch_bounds__(kSegScatterThreads) void k_seg_scatter(const u16 *sinst, u64 nslots, const u64 *UH, const double2 *UM,
                                                      const u32 *UK, const u32 *trunc, u32 n, u64 *cursor,
                                                      SegItem *list, SelKey sk) {
     __shared__ u32 cnt[kMaxInst];
     __shared__ u64 base[kMaxInst];
-    constexpr int kPer = kSegTile / 256;
+    constexpr int kPer = kSegTile / kSegScatterThreads;
     for (u64 t0 = u64(blockIdx.x) * kSegTile; t0 < nslots; t0 += u64(gridDim.x) * kSegTile) {
         for (u32 i = threadIdx.x; i < n; i += blockDim.x) cnt[i] = 0;
         __syncthreads();
         u32 inst[kPer], rank[kPer];
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
-            const u64 s = t0 + threadIdx.x + u64(q) * 256;
+            const u64 s = t0 + threadIdx.x + u64(q) * kSegScatterThreads;
             inst[q] = kNoInst;
             if (s < nslots) {
                 const u32 i = slot_inst16(sinst, s);
@@ -155,7 +159,7 @@ __global__ __launch_bounds__(256) void k_seg_scatter(const u16 *sinst, u64 nslot
 #pragma unroll
         for (int q = 0; q < kPer; ++q)
             if (inst[q] != kNoInst) {
-                const u64 s = t0 + threadIdx.x + u64(q) * 256;
+                const u64 s = t0 + threadIdx.x + u64(q) * kSegScatterThreads;
                 SegItem it;
                 if (UK) {
                     it.hi = u64(UK[s]) << 32;

@@ -1816,7 +1816,8 @@ static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const
         QD_TRY(ctx->bhist.ensure(u64(n) * kHistBins * sizeof(u32)));
         SegItem *cur = ctx->ebuf1.as<SegItem>(), *nxt = ctx->ebuf2.as<SegItem>();
         const unsigned sg = unsigned(std::min<u64>((nslots + kSegTile - 1) / kSegTile, 16384));
-        hipLaunchKernelGGL(k_seg_scatter, dim3(std::max(sg, 1u)), dim3(256), 0, s, sinst, nslots, UH, UM, UK,
+        hipLaunchKernelGGL(k_seg_scatter, dim3(std::max(sg, 1u)), dim3(kSegScatterThreads), 0, s, sinst, nslots, UH,
+                           UM, UK,
                            ctx->btrunc.as<u32>(), n, ctx->bcur.as<u64>(), cur, bt->sk);
         QD_HIP(hipMemsetAsync(ctx->bhist.p, 0, u64(n) * kHistBins * sizeof(u32), s));
         // the digit rounds, driven from the device (SegLoop)
