@@ -876,10 +876,11 @@ struct RangeSel {
 
 struct RangeLds {
     u64 swb[kRangeWin + 1];
-    u16 list[kRangeList];
+    u32 list[kRangeList];  // (parent offset in the window << 15) | (item in tile << 3) | child
     u64 tmp[kGenThreads / kWave + 1];
     u64 pfirst;
 };
+constexpr u32 kRangeNoWin = 0x1FFFFu;  // list entry's parent beyond the window: searched in wb
 
 // The walks' shared structure: leaf(key, magnitude, candidate) for every
 // child of the pass in the range, in a fixed order per block.
@@ -892,6 +893,9 @@ __device__ __attribute__((always_inline)) inline void range_walks(const PDesc *D
     for (u64 t0 = u64(blockIdx.x) * kRangeTile; t0 < W; t0 += u64(gridDim.x) * kRangeTile) {
         __syncthreads();
         if (threadIdx.x == 0) L.pfirst = cpar[t0 / kGenThreads];
+        __syncthreads();
+        const u64 pf = L.pfirst;
+        for (u32 q = threadIdx.x; q <= kRangeWin; q += blockDim.x) L.swb[q] = pf + q <= N ? wb[pf + q] : ~0ull;
         // this thread's items t0 + kRangePer * tid + [0, kRangePer): 64 child
         // bits per word of 8 items (bit 8i + u: item i, child u)
         u64 in[kRangePer / 8];
@@ -911,20 +915,33 @@ __device__ __attribute__((always_inline)) inline void range_walks(const PDesc *D
             cnt += u32(__popcll(in[q]));
         }
         u64 total;
-        const u32 first = u32(block_excl_scan(cnt, L.tmp, total));
-        const u64 pf = L.pfirst;
-        for (u32 q = threadIdx.x; q <= kRangeWin; q += blockDim.x) L.swb[q] = pf + q <= N ? wb[pf + q] : ~0ull;
+        const u32 first = u32(block_excl_scan(cnt, L.tmp, total));  // (its barriers: swb is in place)
+        // the parent of this thread's first item, in the window (one search;
+        // its other items' parents follow by stepping forward)
+        const u64 wt = t0 + u64(threadIdx.x) * kRangePer;
+        u32 lp0 = kRangeNoWin;
+        if (cnt && L.swb[kRangeWin] > wt + kRangePer - 1) {
+            u32 lo = 0, hi = kRangeWin;
+            while (lo + 1 < hi) {
+                const u32 mid = (lo + hi) >> 1;
+                if (L.swb[mid] <= wt) lo = mid;
+                else hi = mid;
+            }
+            lp0 = lo;
+        }
         for (u32 base = 0; base < u32(total); base += kRangeList) {
             if (base) __syncthreads();  // the previous round's list is walked
-            u32 o = first;
+            u32 o = first, lp = lp0;
 #pragma unroll
             for (u32 q = 0; q < kRangePer / 8; ++q) {
                 u64 m = in[q];
                 while (m) {
                     const u32 bit = u32(__builtin_ctzll(m));
                     m &= m - 1;
-                    if (o >= base && o < base + kRangeList)
-                        L.list[o - base] = u16((threadIdx.x * kRangePer + 8 * q) * 8 + bit);
+                    const u32 it = threadIdx.x * kRangePer + 8 * q + (bit >> 3);
+                    if (lp != kRangeNoWin)
+                        while (L.swb[lp + 1] <= t0 + it) ++lp;
+                    if (o >= base && o < base + kRangeList) L.list[o - base] = (lp << 15) | (it << 3) | (bit & 7u);
                     ++o;
                 }
             }
@@ -932,20 +949,10 @@ __device__ __attribute__((always_inline)) inline void range_walks(const PDesc *D
             const u32 nl = min(u32(total) - base, kRangeList);
             for (u32 i = threadIdx.x; i < nl; i += kGenThreads) {
                 const u32 e = L.list[i];
-                const u64 w = t0 + (e >> 3);
+                const u64 w = t0 + ((e & 0x7FFFu) >> 3);
                 const u32 u = e & 7u;
-                u64 p;
-                if (L.swb[kRangeWin] > w) {
-                    u32 lo = 0, hi = kRangeWin;
-                    while (lo + 1 < hi) {
-                        const u32 mid = (lo + hi) >> 1;
-                        if (L.swb[mid] <= w) lo = mid;
-                        else hi = mid;
-                    }
-                    p = pf + lo;
-                } else {
-                    p = find_parent(wb, N, w);
-                }
+                const u32 lp = e >> 15;
+                const u64 p = lp != kRangeNoWin ? pf + lp : find_parent(wb, N, w);
                 const PDesc pd = D[p];
                 const u32 t = min(u32(pd.k), kItemBits);
                 if (u >> t) continue;  // tagged as absent: never listed
@@ -1073,16 +1080,20 @@ __global__ __launch_bounds__(kGenThreads) void k_range_walkwrite(const PDesc *D,
     bool spill = false;
     range_walks<KIND>(D, pmag, cb, wb, N, cpar, tab, pbegin, k, W, rs, L, [&](u64 key, const cplx &m, u64 cand) {
         const u32 g = gbits ? u32(key >> (64 - gbits)) : 0u;
-        u32 pos = 0;
-        for (u32 q = 0; q < ng; ++q) {
-            const u64 mk = __ballot(g == q);
-            if (!mk) continue;
-            const u32 leader = u32(__ffsll((unsigned long long)mk) - 1);
-            u32 base = 0;
-            if (lane == leader) base = atomicAdd(&lcnt[q], u32(__popcll(mk)));
-            base = __shfl(base, int(leader), kWave);
-            if (g == q) pos = base + u32(__popcll(mk & ((1ull << lane) - 1)));
+        // the active lanes of this lane's group: three ballots on its bits;
+        // the group's lowest lane reserves the group's places
+        u64 same = __ballot(true);
+#pragma unroll
+        for (u32 b = 0; b < 3; ++b) {
+            if (b >= gbits) break;
+            const u64 mb = __ballot((g >> b) & 1u);
+            same &= ((g >> b) & 1u) ? mb : ~mb;
         }
+        const u32 leader = u32(__ffsll((unsigned long long)same) - 1);
+        u32 base = 0;
+        if (lane == leader) base = atomicAdd(&lcnt[g], u32(__popcll(same)));
+        base = __shfl(base, int(leader), kWave);
+        const u32 pos = base + u32(__popcll(same & ((1ull << lane) - 1)));
         if (pos < capr) {
             Entry e;
             e.hash = key;
