@@ -997,10 +997,10 @@ __global__ __launch_bounds__(kGenThreads) void k_range_bincount(const PDesc *D, 
     for (u32 i = threadIdx.x; i < nbin; i += blockDim.x) bmat[u64(blockIdx.x) * nbin + i] = bcnt[i];
 }
 
-// Column scan of bmat (rows = the walks' blocks): every block's first slot
-// in bucket d, counted from the bucket's start; total[d] = the bucket's
-// size.  One workgroup per bucket.
-__global__ __launch_bounds__(256) void k_range_colscan(u32 *bmat, u32 rows, u32 nbin, u64 *total) {
+// Column scan of bmat (rows = the walks' blocks, `stride` counters apart;
+// nbin columns): every block's first slot in bucket d, counted from the
+// bucket's start; total[d] = the bucket's size.  One workgroup per bucket.
+__global__ __launch_bounds__(256) void k_range_colscan(u32 *bmat, u32 rows, u32 nbin, u32 stride, u64 *total) {
     __shared__ u64 tmp[256 / kWave + 1];
     const u32 d = blockIdx.x;
     constexpr u32 kPer = kRangeGrid / 256;
@@ -1009,7 +1009,7 @@ __global__ __launch_bounds__(256) void k_range_colscan(u32 *bmat, u32 rows, u32 
 #pragma unroll
     for (u32 q = 0; q < kPer; ++q) {
         const u32 r = threadIdx.x * kPer + q;
-        v[q] = r < rows ? bmat[u64(r) * nbin + d] : 0u;
+        v[q] = r < rows ? bmat[u64(r) * stride + d] : 0u;
         sum += v[q];
     }
     u64 tot;
@@ -1017,7 +1017,7 @@ __global__ __launch_bounds__(256) void k_range_colscan(u32 *bmat, u32 rows, u32 
 #pragma unroll
     for (u32 q = 0; q < kPer; ++q) {
         const u32 r = threadIdx.x * kPer + q;
-        if (r < rows) bmat[u64(r) * nbin + d] = u32(run);
+        if (r < rows) bmat[u64(r) * stride + d] = u32(run);
         run += v[q];
     }
     if (threadIdx.x == 0) total[d] = tot;
@@ -1055,9 +1055,11 @@ __global__ __launch_bounds__(kGenThreads) void k_range_binwrite(const PDesc *D, 
 // rcount[q * gridDim.x + b] receives the region's count.  Places within a
 // region come from wavefront-aggregated LDS counters, one per group.  A
 // region that would pass capr raises ERR_SPILL1 (the caller then walks the
-// pass again the two-walk way).  Each group's regions are then merged into
-// its first-level buckets with exact offsets (k_region_hist,
-// k_range_colscan, k_excl_small, k_region_place).
+// pass again the two-walk way).  The block also counts its children per
+// first-level bucket (the top b1 key bits) as row blockIdx.x of bmat, so
+// that each group's regions merge into its first-level buckets at exact
+// offsets (k_range_colscan over the group's columns, k_excl_small,
+// k_region_place).
 // ---------------------------------------------------------------------------
 constexpr u32 kRangeGroupsMax = 8;
 
@@ -1070,10 +1072,14 @@ __global__ __launch_bounds__(kGenThreads) void k_range_walkwrite(const PDesc *D,
                                                                  const u64 *wb, u64 N, const u64 *cpar,
                                                                  const u64 *tab, const u64 *pbegin, qd_rule rule,
                                                                  u64 W, RangeSel rs, u32 gbits, GroupBases E,
-                                                                 u64 capr, u32 *rcount, u32 *err) {
+                                                                 u64 capr, u32 *rcount, u32 b1, u32 *bmat,
+                                                                 u32 *err) {
     __shared__ u32 lcnt[kRangeGroupsMax];
+    __shared__ u32 bcnt[kRangeBins];
     __shared__ RangeLds L;
+    const u32 nbin = 1u << b1;
     if (threadIdx.x < kRangeGroupsMax) lcnt[threadIdx.x] = 0;
+    for (u32 i = threadIdx.x; i < nbin; i += blockDim.x) bcnt[i] = 0;
     const QcgdCoef k = coef_of(rule);
     const u32 lane = threadIdx.x & (kWave - 1);
     const u32 ng = 1u << gbits;
@@ -1094,6 +1100,7 @@ __global__ __launch_bounds__(kGenThreads) void k_range_walkwrite(const PDesc *D,
         if (lane == leader) base = atomicAdd(&lcnt[g], u32(__popcll(same)));
         base = __shfl(base, int(leader), kWave);
         const u32 pos = base + u32(__popcll(same & ((1ull << lane) - 1)));
+        atomicAdd(&bcnt[b1 ? u32(key >> (64 - b1)) : 0u], 1u);
         if (pos < capr) {
             Entry e;
             e.hash = key;
@@ -1107,25 +1114,8 @@ __global__ __launch_bounds__(kGenThreads) void k_range_walkwrite(const PDesc *D,
     });
     __syncthreads();
     if (threadIdx.x < ng) rcount[threadIdx.x * gridDim.x + blockIdx.x] = min(lcnt[threadIdx.x], u32(capr));
+    for (u32 i = threadIdx.x; i < nbin; i += blockDim.x) bmat[u64(blockIdx.x) * nbin + i] = bcnt[i];
     if (spill) atomicOr(err, ERR_SPILL1);
-}
-
-// Digit counts of each region (nreg regions of capr entries, rcount[r]
-// used): row r of bm, nbin columns; the digit is (key >> shift) % nbin.
-__global__ __launch_bounds__(256) void k_region_hist(const Entry *in, u64 capr, const u32 *rcount, u32 nreg,
-                                                     u32 shift, u32 nbin, u32 *bm) {
-    __shared__ u32 h[kRangeBins];
-    for (u32 r = blockIdx.x; r < nreg; r += gridDim.x) {
-        __syncthreads();
-        for (u32 i = threadIdx.x; i < nbin; i += blockDim.x) h[i] = 0;
-        __syncthreads();
-        const u32 n = rcount[r];
-        const Entry *src = in + u64(r) * capr;
-        for (u32 i = threadIdx.x; i < n; i += blockDim.x)
-            atomicAdd(&h[nbin > 1 ? u32(src[i].hash >> shift) & (nbin - 1) : 0u], 1u);
-        __syncthreads();
-        for (u32 i = threadIdx.x; i < nbin; i += blockDim.x) bm[u64(r) * nbin + i] = h[i];
-    }
 }
 
 // off[i] = total[0] + ... + total[i - 1] for i <= n (n small).
@@ -1141,8 +1131,8 @@ __global__ void k_excl_small(const u64 *total, u32 n, u64 *off) {
 }
 
 // Each region's entries at its running places in the buckets: bucket d
-// starts at boff[d], region r's first place in it is bm[r][d] after the
-// column scan.  A chunk of kPlaceChunk entries is ordered by bucket in LDS
+// starts at boff[d], region r's first place in it is bm[r * bstride + d]
+// after the column scan.  A chunk of kPlaceChunk entries is ordered by bucket in LDS
 // first, so that consecutive lanes write consecutive places of a bucket
 // (runs of about kPlaceChunk / nbin entries) rather than one entry to each
 // of up to 64 buckets per store.
@@ -1152,7 +1142,7 @@ constexpr u32 kPlaceChunk = kPlaceThreads * kPlacePer;
 
 __global__ __launch_bounds__(kPlaceThreads) void k_region_place(const Entry *in, u64 capr, const u32 *rcount,
                                                                 u32 nreg, u32 shift, u32 nbin, const u32 *bm,
-                                                                const u64 *boff, Entry *out) {
+                                                                u32 bstride, const u64 *boff, Entry *out) {
     __shared__ Entry stage[kPlaceChunk];
     __shared__ u32 cnt[kRangeBins], lst[kRangeBins];
     __shared__ u64 cur[kRangeBins];
@@ -1161,7 +1151,7 @@ __global__ __launch_bounds__(kPlaceThreads) void k_region_place(const Entry *in,
     auto digit = [&](u64 key) { return nbin > 1 ? u32(key >> shift) & (nbin - 1) : 0u; };
     for (u32 r = blockIdx.x; r < nreg; r += gridDim.x) {
         __syncthreads();
-        for (u32 i = threadIdx.x; i < nbin; i += kPlaceThreads) cur[i] = boff[i] + bm[u64(r) * nbin + i];
+        for (u32 i = threadIdx.x; i < nbin; i += kPlaceThreads) cur[i] = boff[i] + bm[u64(r) * bstride + i];
         const u32 n = rcount[r];
         const Entry *src = in + u64(r) * capr;
         for (u32 c0 = 0; c0 < n; c0 += kPlaceChunk) {

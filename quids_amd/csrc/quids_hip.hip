@@ -1503,7 +1503,7 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
                                (const PDesc *)ctx->pdesc.as<PDesc>(), P.mag, (const u64 *)ctx->cb.as<u64>(),
                                (const u64 *)ctx->wb.as<u64>(), N, (const u64 *)ctx->cpar.as<u64>(),
                                (const u64 *)ctx->tab.as<u64>(), P.begin, rule, nitems, rs, gbits, GB, capr,
-                               ctx->rcnt.as<u32>(), errw);
+                               ctx->rcnt.as<u32>(), b1, ctx->bmat.as<u32>(), errw);
             QD_HIP(hipGetLastError());
             prof_end(ctx, QD_PHASE_GENERATE);
             u32 ew = 0;
@@ -1527,7 +1527,7 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
                                (const u64 *)ctx->wb.as<u64>(), N, (const u64 *)ctx->cpar.as<u64>(),
                                (const u64 *)ctx->tab.as<u64>(), P.begin, rule, nitems, rs, b1, ctx->bmat.as<u32>());
             hipLaunchKernelGGL(k_range_colscan, dim3(unsigned(nb1)), dim3(256), 0, s, ctx->bmat.as<u32>(), u32(gr),
-                               u32(nb1), ctx->cur1.as<u64>());
+                               u32(nb1), u32(nb1), ctx->cur1.as<u64>());
             QD_HIP(hipGetLastError());
             prof_end(ctx, QD_PHASE_GENERATE);
             QD_TRY(sync_read(ctx, ctx->cur1.p, nb1 * sizeof(u64), c1.data()));
@@ -1582,14 +1582,15 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
                     return fail(QD_ERR_CAPACITY, "hash-range pass: hash keys too skewed for the first radix pass");
                 const u32 *rcg = ctx->rcnt.as<u32>() + u64(g) * gr;
                 prof_begin(ctx, QD_PHASE_PARTITION);
-                hipLaunchKernelGGL(k_region_hist, dim3(gr), dim3(256), 0, s, (const Entry *)GB.g[g], capr, rcg, u32(gr),
-                                   64 - b1, u32(gbk), ctx->bmat.as<u32>());
-                hipLaunchKernelGGL(k_range_colscan, dim3(unsigned(gbk)), dim3(256), 0, s, ctx->bmat.as<u32>(), u32(gr),
-                                   u32(gbk), ctx->cur1.as<u64>());
+                // the walk counted each block's children per first-level
+                // bucket (row = block, nb1 columns): the group's columns
+                u32 *bmg = ctx->bmat.as<u32>() + u64(g) * gbk;
+                hipLaunchKernelGGL(k_range_colscan, dim3(unsigned(gbk)), dim3(256), 0, s, bmg, u32(gr), u32(gbk),
+                                   u32(nb1), ctx->cur1.as<u64>());
                 hipLaunchKernelGGL(k_excl_small, dim3(1), dim3(64), 0, s, (const u64 *)ctx->cur1.as<u64>(), u32(gbk),
                                    ctx->off1.as<u64>());
                 hipLaunchKernelGGL(k_region_place, dim3(gr), dim3(256), 0, s, (const Entry *)GB.g[g], capr, rcg, u32(gr),
-                                   64 - b1, u32(gbk), (const u32 *)ctx->bmat.as<u32>(), (const u64 *)ctx->off1.as<u64>(),
+                                   64 - b1, u32(gbk), (const u32 *)bmg, u32(nb1), (const u64 *)ctx->off1.as<u64>(),
                                    ctx->abuf.as<Entry>());
                 QD_HIP(hipGetLastError());
                 prof_end(ctx, QD_PHASE_PARTITION);
