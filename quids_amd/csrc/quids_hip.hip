@@ -1222,7 +1222,7 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
     u32 T = 4;
     auto tag_bits = [&](u64 b) {
         u32 t = 4;
-        while (t < kMaxTagBits && double((1u << t) - 1) * double(b) < 1.03 * double(C)) ++t;
+        while (t < kMaxTagBits && double((1u << t) - 1) * double(b) < 1.02 * double(C)) ++t;
         return t;
     };
     const u64 tab_bytes = fill_tab ? (sm_tab_base(in_bytes) + kSmSlots * 64 + 16) * sizeof(u64) : 0;
@@ -1253,7 +1253,7 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
         const double next = std::max(0.0, 1.5 * double(in_bytes) / double(std::max<u64>(N, 1)) * Kp + 60.0 * Kp -
                                               double(state_mem(out)));
         const double fixed = next + double(2ull << 30);
-        const double avail = 0.92 * (double(freeb) + double(ctx->slab.cap) + double(reuse)) - fixed;
+        const double avail = 0.95 * (double(freeb) + double(ctx->slab.cap) + double(reuse)) - fixed;
         // keep the slab while it is within 80-115 % of what would be made now
         // (the output state's buffers, counted in `next`, change size from
         // one step to the next while the states grow; `next` is an upper
