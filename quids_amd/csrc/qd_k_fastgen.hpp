@@ -1136,8 +1136,8 @@ __global__ void k_excl_small(const u64 *total, u32 n, u64 *off) {
 // first, so that consecutive lanes write consecutive places of a bucket
 // (runs of about kPlaceChunk / nbin entries) rather than one entry to each
 // of up to 64 buckets per store.
-constexpr u32 kPlaceThreads = 256;
-constexpr u32 kPlacePer = 8;
+constexpr u32 kPlaceThreads = 1024;
+constexpr u32 kPlacePer = 2;
 constexpr u32 kPlaceChunk = kPlaceThreads * kPlacePer;
 
 __global__ __launch_bounds__(kPlaceThreads) void k_region_place(const Entry *in, u64 capr, const u32 *rcount,

@@ -1589,7 +1589,8 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
                                    u32(nb1), ctx->cur1.as<u64>());
                 hipLaunchKernelGGL(k_excl_small, dim3(1), dim3(64), 0, s, (const u64 *)ctx->cur1.as<u64>(), u32(gbk),
                                    ctx->off1.as<u64>());
-                hipLaunchKernelGGL(k_region_place, dim3(gr), dim3(256), 0, s, (const Entry *)GB.g[g], capr, rcg, u32(gr),
+                hipLaunchKernelGGL(k_region_place, dim3(gr), dim3(kPlaceThreads), 0, s, (const Entry *)GB.g[g], capr, rcg,
+                                   u32(gr),
                                    64 - b1, u32(gbk), (const u32 *)bmg, u32(nb1), (const u64 *)ctx->off1.as<u64>(),
                                    ctx->abuf.as<Entry>());
                 QD_HIP(hipGetLastError());
