@@ -39,8 +39,10 @@ __global__ __launch_bounds__(256) void k_pool_append(const u64 *SK, const u64 *U
 #pragma unroll
         for (int k = 0; k < kPoolPer; ++k) {
             const u64 i = t0 + threadIdx.x + 256 * k;
-            r[k] = i < nslots ? UR[i] : kNoRep;
-            if (UK && r[k] != kNoRep && (UK[i] >> 8) < t24) r[k] = kNoRep;
+            // (with UK, the 4-byte key first: below t24, the slot, empty or
+            // not, is left out without reading its representative)
+            r[k] = kNoRep;
+            if (i < nslots && (!UK || (UK[i] >> 8) >= t24)) r[k] = UR[i];
             m[k] = __ballot(r[k] != kNoRep);
             if (lane == 0) roff[k * kWaves + wid] = __popcll(m[k]);
         }
