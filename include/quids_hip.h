@@ -127,6 +127,9 @@ const char *qd_last_error(void);
 int qd_ctx_create(qd_ctx **ctx, int device);
 int qd_ctx_destroy(qd_ctx *ctx);
 int qd_ctx_synchronize(qd_ctx *ctx);
+/* The context's HIP stream (a hipStream_t): work the caller enqueues on it
+ * runs in order with the library's (user rules' and observables' kernels). */
+int qd_ctx_stream(qd_ctx *ctx, void **stream);
 /* Device-side bytes currently held by the context's scratch pool. */
 int qd_ctx_scratch_bytes(qd_ctx *ctx, uint64_t *bytes);
 /* Release the context's cached scratch buffers. */
@@ -236,6 +239,61 @@ int qd_simulate(qd_ctx *ctx, qd_state *in, const qd_rule *rule, qd_state *out,
                 qd_symbolic *sy, uint64_t max_num_object, const qd_params *params,
                 qd_step_stats *stats);
 
+/* ---- user rules: the reference's plugin API, a subclass of quids::rule
+ *      with get_num_child / populate_child / populate_child_simple / hasher
+ *      (src/quids.hpp:105-146, README.md:112-180).  A rule the library has
+ *      no kernels for is compiled by the caller as a device functor
+ *      (quids_amd/user_rule.hpp turns a struct of __host__ __device__
+ *      methods into this table); its callbacks enqueue the caller's kernels
+ *      on `stream` (a hipStream_t) and return 0, or a nonzero code that the
+ *      step reports as QD_ERR_INVALID.  The library runs everything else of
+ *      the step: pre-truncation, the num_child scan, dedup, truncation,
+ *      order, normalisation.  All arrays are device memory. */
+typedef struct qd_objects_view {
+    const uint8_t *objects; /* object bytes */
+    const uint64_t *begin;  /* n + 1 padded offsets */
+    const uint32_t *size;   /* n sizes */
+    const double *mag;      /* n magnitudes (re, im) */
+    uint64_t n;
+} qd_objects_view;
+
+typedef struct qd_user_rule {
+    void *self; /* handed back to every callback */
+    /* get_num_child of every parent (src/quids.hpp:557-567): num_child[i],
+     * max_child_size[i] */
+    int (*num_child)(void *self, const qd_objects_view *parents, uint32_t *num_child,
+                     uint32_t *max_child_size, void *stream);
+    /* populate_child + hasher of every candidate (src/quids.hpp:705-719):
+     * candidate c is child c - child_begin[p] of the parent p with
+     * child_begin[p] <= c < child_begin[p + 1] (child_begin: n + 1 offsets);
+     * its magnitude starts as the parent's.  Writes hash[c], mag[2c],
+     * mag[2c + 1] and size[c].  scratch holds scratch_bytes bytes for the
+     * children's bytes (room for at least one child of max_child_size
+     * bytes per 256 candidates, in slices of max_child_size rounded up to
+     * 16 bytes). */
+    int (*generate)(void *self, const qd_objects_view *parents, const uint64_t *child_begin,
+                    uint64_t num_candidates, uint32_t max_child_size, uint8_t *scratch,
+                    uint64_t scratch_bytes, uint64_t *hash, double *mag, uint32_t *size,
+                    void *stream);
+    /* populate_child_simple of the survivors (src/quids.hpp:956-967):
+     * candidate cand[k] (as above) written at out_objects + out_begin[k],
+     * k < n */
+    int (*populate)(void *self, const qd_objects_view *parents, const uint64_t *child_begin,
+                    const uint64_t *cand, uint64_t n, const uint64_t *out_begin,
+                    uint8_t *out_objects, void *stream);
+    /* hasher of every object of a state: the key of probabilistic
+     * truncation (qd_params.simple_truncation == 0) for the parents */
+    int (*hash_objects)(void *self, const qd_objects_view *objects, uint64_t *hash, void *stream);
+} qd_user_rule;
+
+/* qd_simulate with a user rule, in one pass over its candidates (a step
+ * whose candidates exceed the scratch memory is rejected with
+ * QD_ERR_CAPACITY).  Bit-identical to qd_simulate for a user rule that
+ * restates a built-in one with the same hasher. */
+int qd_simulate_user(qd_ctx *ctx, qd_state *in, const qd_user_rule *rule, qd_state *out,
+                     qd_symbolic *sy, uint64_t max_num_object, const qd_params *params,
+                     qd_step_stats *stats);
+
 /* ---- batched step (no counterpart in the reference: many independent
  *      QCGD simulations advanced by one launch sequence).  `in` holds the
  *      instances back to back: instance i owns objects [in_seg[i],
@@ -269,6 +327,13 @@ int qd_normalize(qd_ctx *ctx, qd_state *state, double *total_proba);
  *      the adjoint starts from. */
 int qd_average_value(qd_ctx *ctx, qd_state *state, int observable, int arg, double *value,
                      double *cot);
+/* average_value for any observable (the reference's observable_t,
+ * src/quids.hpp:83, 208-234): obs (device, one double per object) holds the
+ * observable of every object, computed by the caller's own kernel
+ * (quids_amd/user_rule.hpp, device_observable); value = sum obs |m|^2 in the
+ * fixed summation order of qd_average_value; cot as there. */
+int qd_average_value_of(qd_ctx *ctx, qd_state *state, const double *obs, double *value,
+                        double *cot);
 /* The same without any device allocation (no counterpart in the reference:
  * for callers that cannot allocate, such as code under HIP graph capture or
  * tight serving and training loops).  Its scratch is a static device array

@@ -296,6 +296,21 @@ class sy_it_t:
             pass
 
 
+class user_rule:
+    """A rule compiled as a device functor by the caller (the reference's
+    quids::rule subclass, src/quids.hpp:105-146): ``table`` is the
+    qd_user_rule (QdUserRule) that the caller's compiled code filled with
+    the launch callbacks of its kernels (quids_amd/user_rule.hpp,
+    make_user_rule).  ``keepalive`` holds whatever owns the table's functor
+    (its library)."""
+
+    def __init__(self, table, keepalive=None):
+        if not isinstance(table, _abi.QdUserRule):
+            raise TypeError("user_rule needs a QdUserRule table")
+        self.table = table
+        self.keepalive = keepalive
+
+
 def simulate(it, rule, next_iteration=None, symbolic_iteration=None, max_num_object=0, mid_step_function=None):
     """quids::simulate.
 
@@ -313,12 +328,13 @@ def simulate(it, rule, next_iteration=None, symbolic_iteration=None, max_num_obj
         return None
     if next_iteration is None:
         raise TypeError("simulate(it, rule, next_iteration, ...) needs a next iteration")
-    if not hasattr(rule, "desc"):
+    user = isinstance(rule, user_rule)
+    if not user and not hasattr(rule, "desc"):
         raise QuidsError(_abi.QD_ERR_UNSUPPORTED, "simulate", "rule has no device implementation")
     if max_num_object is None or max_num_object < 0:
         max_num_object = UINT64_MAX
     it._sync_to_device()
-    r = rule.desc()
+    r = rule.table if user else rule.desc()
     prm = _params()
     st = QdStepStats()
     sy = symbolic_iteration.handle if symbolic_iteration is not None else None
@@ -327,8 +343,9 @@ def simulate(it, rule, next_iteration=None, symbolic_iteration=None, max_num_obj
         hook = _abi.PHASE_HOOK(lambda name, _user: mid_step_function(name.decode()))
         check(lib().qd_ctx_set_phase_hook(it.ctx.handle, hook, None), "qd_ctx_set_phase_hook")
     try:
-        check(lib().qd_simulate(it.ctx.handle, it.handle, ctypes.byref(r), next_iteration.handle, sy,
-                                int(max_num_object), ctypes.byref(prm), ctypes.byref(st)), "qd_simulate")
+        fn, name = (lib().qd_simulate_user, "qd_simulate_user") if user else (lib().qd_simulate, "qd_simulate")
+        check(fn(it.ctx.handle, it.handle, ctypes.byref(r), next_iteration.handle, sy, int(max_num_object),
+                 ctypes.byref(prm), ctypes.byref(st)), name)
     finally:
         if hook is not None:
             lib().qd_ctx_set_phase_hook(it.ctx.handle, _abi.PHASE_HOOK(), None)  # a null hook

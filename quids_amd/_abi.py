@@ -127,6 +127,18 @@ _pu64 = ctypes.POINTER(ctypes.c_uint64)
 _pd = ctypes.POINTER(ctypes.c_double)
 _pvp = ctypes.POINTER(ctypes.c_void_p)
 
+class QdObjectsView(ctypes.Structure):
+    _fields_ = [("objects", ctypes.c_void_p), ("begin", ctypes.c_void_p), ("size", ctypes.c_void_p),
+                ("mag", ctypes.c_void_p), ("n", ctypes.c_uint64)]
+
+
+class QdUserRule(ctypes.Structure):
+    """qd_user_rule: the launch callbacks of a rule compiled as a device
+    functor (filled by the caller's compiled code, quids_amd/user_rule.hpp)."""
+    _fields_ = [("self", ctypes.c_void_p), ("num_child", ctypes.c_void_p), ("generate", ctypes.c_void_p),
+                ("populate", ctypes.c_void_p), ("hash_objects", ctypes.c_void_p)]
+
+
 # name -> (argtypes); every entry point returns int except the three noted.
 SIGNATURES = {
     "qd_abi_version": [],
@@ -169,6 +181,10 @@ SIGNATURES = {
     "qd_simulate_adjoint": [_vp, _vp, ctypes.POINTER(QdRule), ctypes.POINTER(QdRule), ctypes.c_int, _vp, _vp, _vp,
                             _pd, _u64],
     "qd_modifier_adjoint": [_vp, _vp, ctypes.POINTER(QdModifier), _vp],
+    "qd_ctx_stream": [_vp, _pvp],
+    "qd_simulate_user": [_vp, _vp, ctypes.POINTER(QdUserRule), _vp, _vp, _u64, ctypes.POINTER(QdParams),
+                         ctypes.POINTER(QdStepStats)],
+    "qd_average_value_of": [_vp, _vp, _vp, _pd, _vp],
 }
 RESTYPES = {"qd_status_string": ctypes.c_char_p, "qd_last_error": ctypes.c_char_p}
 

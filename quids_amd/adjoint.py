@@ -76,6 +76,22 @@ def average_value(it, observable, arg=0, with_cot=False):
     return (v.value, cot[:n]) if with_cot else v.value
 
 
+def average_value_of(it, obs, with_cot=False):
+    """Device average_value of any observable (the reference's observable_t,
+    src/quids.hpp:208-234): obs is a float64 device tensor holding the
+    observable of every object, computed by the caller's own kernel
+    (quids_amd/user_rule.hpp, eval_observable); same summation order and
+    cotangent as average_value."""
+    it._sync_to_device()
+    n = it.num_object
+    cot = torch.empty((max(n, 1), 2), dtype=torch.float64, device=_device(it)) if with_cot else None
+    v = ctypes.c_double()
+    torch.cuda.synchronize(_device(it))
+    check(lib().qd_average_value_of(it.ctx.handle, it.handle, _ptr(obs), ctypes.byref(v), _ptr(cot)),
+          "qd_average_value_of")
+    return (v.value, cot[:n]) if with_cot else v.value
+
+
 def step_adjoint(it_in, rule, it_out, out_cot, ndir, max_num_object=UINT64_MAX):
     """Adjoint of simulate(it_in, rule, it_out): (cotangent of it_in's
     magnitudes as an (N, 2) device tensor, [dL/dparam_i for i < ndir])."""

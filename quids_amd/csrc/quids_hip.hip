@@ -55,6 +55,7 @@
 #include "qd_k_adjoint.hpp"
 #include "qd_k_batch.hpp"
 #include "qd_k_range.hpp"
+#include "qd_k_user.hpp"
 
 using namespace qd;
 
@@ -244,6 +245,7 @@ struct qd_ctx {
                                                 // children's slice tags, per-block bucket counts, group offsets,
     DBuf rcnt, abuf;                            // region counts, a group's merged first-level buckets
     DBuf slab;  // one allocation kept across hash-range steps, carved into their big buffers (views)
+    DBuf usize, uscr;  // user rules: every candidate's child size, the generate's child bytes
     // output buffers of batched chunks (objects, begin, size, mag per chunk),
     // kept between calls
     std::vector<DBuf> parts;
@@ -259,7 +261,7 @@ struct qd_ctx {
                 &biglist, &redolist, &bigsoff, &bigkey, &bigcand, &bigmag, &bigseg, &pdesc, &wi, &wb, &cpar, &opar, &ebuf2, &cnt2, &slist, &ukey, &selloop, &segloop, &bsoff, &bscnt,
                 &adjk, &adjv, &adjg, &adjd, &adjp, &adjw, &bseg, &boseg, &brules, &bsinst, &bcount, &bsel, &btrunc,
                 &blist, &bcur, &btile, &bhist, &bkseg, &bchunk, &bcsum, &btot, &pool0, &pool1, &pwpre, &blist2, &mw, &rpart[0], &rpart[1], &rpart[2],
-                &rpart[3], &pk0, &pk1, &tags, &bmat, &goff, &rcnt, &abuf, &slab};
+                &rpart[3], &pk0, &pk1, &tags, &bmat, &goff, &rcnt, &abuf, &slab, &usize, &uscr};
     }
     void set_names() {
         static const char *const names[] = {
@@ -269,10 +271,11 @@ struct qd_ctx {
             "slist", "ukey", "selloop", "segloop", "bsoff", "bscnt", "adjk", "adjv", "adjg", "adjd", "adjp", "adjw",
             "bseg", "boseg", "brules", "bsinst", "bcount", "bsel", "btrunc", "blist", "bcur", "btile", "bhist",
             "bkseg", "bchunk", "bcsum", "btot", "pool0", "pool1", "pwpre", "blist2", "mw", "rpart0", "rpart1",
-            "rpart2", "rpart3", "pk0", "pk1", "tags", "bmat", "goff", "rcnt", "abuf", "slab"};
+            "rpart2", "rpart3", "pk0", "pk1", "tags", "bmat", "goff", "rcnt", "abuf", "slab", "usize", "uscr"};
         std::vector<DBuf *> v = all_named();
-        static_assert(sizeof(names) / sizeof(names[0]) == 79, "one name per buffer");
-        for (size_t i = 0; i < v.size() && i < 79; ++i) v[i]->name = names[i];
+        constexpr size_t kNames = sizeof(names) / sizeof(names[0]);
+        static_assert(kNames == 81, "one name per buffer");
+        for (size_t i = 0; i < v.size() && i < kNames; ++i) v[i]->name = names[i];
     }
 };
 
@@ -447,10 +450,12 @@ static int normalize_state(qd_ctx *ctx, qd_state *s) {
 
 // Parent keys of the pre-truncation: (|m|^2 bits, or the random key of the
 // parent's hash, ~index).
-__global__ __launch_bounds__(256) void k_parent_items(Parents P, qd_rule rule, SelKey sk, SelItem *S) {
+__global__ __launch_bounds__(256) void k_parent_items(Parents P, qd_rule rule, SelKey sk, SelItem *S,
+                                                      const u64 *phash) {
     for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < P.n; i += u64(gridDim.x) * blockDim.x) {
         const double2 m = P.mag[i];
-        const u64 h = sk.prob ? rule_hash_object(rule, P.obj + P.begin[i], P.size[i]) : 0;
+        // (the object hashes from phash when given: a user rule's hasher)
+        const u64 h = sk.prob ? (phash ? phash[i] : rule_hash_object(rule, P.obj + P.begin[i], P.size[i])) : 0;
         S[i] = SelItem{sel_hi(m.x, m.y, h, sk), ~i, i};
     }
 }
@@ -1046,33 +1051,30 @@ static int mem_search(u64 n, u64 target, bool upsize, Mem &&mem, u64 &kept) {
 }
 
 // Parent side (src/quids.hpp:459-486): cap = the number of parents the
-// memory keeps (UINT64_MAX when all fit).
-template <int KIND>
-static int mem_parent_cap(qd_ctx *ctx, const Parents &P, const qd_rule &rule, u64 prev_mem, u64 next_mem,
-                          const qd_params &prm, const SelKey &sk, qd_step_stats &st, u64 &cap) {
+// memory keeps (UINT64_MAX when all fit), with ctx->nc holding every
+// parent's num_child; items(S) launches the fill of the parents' select
+// items (truncation keys) into S.
+template <class Items>
+static int mem_parent_cap_nc(qd_ctx *ctx, const Parents &P, u64 prev_mem, u64 next_mem, const qd_params &prm,
+                             qd_step_stats &st, u64 &cap, Items &&items) {
     hipStream_t s = ctx->stream;
     const u64 N = P.n;
     cap = UINT64_MAX;
-    QD_TRY(ctx->nc.ensure(N * sizeof(u32)));
     QD_TRY(ctx->mw.ensure(N * sizeof(u64)));
     QD_TRY(ctx->counters.ensure(64 * sizeof(u64)));
     u64 *counters = ctx->counters.as<u64>();
-    QD_HIP(hipMemsetAsync(counters, 0, 64 * sizeof(u64), s));
-    hipLaunchKernelGGL(k_num_child<KIND>, dim3(grid_for(N)), dim3(256), 0, s, P, rule, ctx->nc.as<u32>(),
-                       reinterpret_cast<u32 *>(counters + 4));
+    QD_HIP(hipMemsetAsync(counters + 8, 0, sizeof(u64), s));
     hipLaunchKernelGGL(k_parent_weight, dim3(grid_for(N)), dim3(256), 0, s, P, (const u32 *)ctx->nc.as<u32>(),
                        ctx->mw.as<u64>());
     hipLaunchKernelGGL(k_masked_sum, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, (const u64 *)ctx->mw.as<u64>(),
                        (const u64 *)nullptr, N, counters + 8);
     QD_HIP(hipGetLastError());
-    u64 hb[16];
-    QD_TRY(sync_read(ctx, counters, 16 * sizeof(u64), hb));
-    QD_TRY(check_err_word(u32(hb[4])));
+    u64 total = 0;
+    QD_TRY(sync_read(ctx, counters + 8, sizeof(u64), &total));
     u64 freeb = 0;
     QD_TRY(free_mem(ctx, freeb));
     const u64 target = target_memory(next_mem, prev_mem, ctx_scratch(ctx), 0, freeb, prm);
     st.mem_target_parents = target;
-    const u64 total = hb[8];
     if (total + total / 8 <= target) return QD_OK;
     // top-k parents by the pre-truncation key; their memory
     QD_TRY(ctx->sel0.ensure(N * sizeof(SelItem)));
@@ -1082,7 +1084,7 @@ static int mem_parent_cap(qd_ctx *ctx, const Parents &P, const qd_rule &rule, u6
     u64 *pbits = ctx->kept.as<u64>();
     auto mem = [&](u64 k, u64 &m) -> int {
         QD_HIP(hipMemsetAsync(pbits, 0, ((N + 63) / 64) * sizeof(u64), s));
-        hipLaunchKernelGGL(k_parent_items, dim3(grid_for(N)), dim3(256), 0, s, P, rule, sk, ctx->sel0.as<SelItem>());
+        items(ctx->sel0.as<SelItem>());
         QD_HIP(hipGetLastError());
         QD_TRY(select_loop(ctx, ctx->sel0.as<SelItem>(), ctx->sel1.as<SelItem>(), N, k, 0, pbits, nullptr, nullptr));
         QD_HIP(hipMemsetAsync(counters + 8, 0, sizeof(u64), s));
@@ -1097,14 +1099,36 @@ static int mem_parent_cap(qd_ctx *ctx, const Parents &P, const qd_rule &rule, u6
     return QD_OK;
 }
 
-// Survivor side (src/quids.hpp:510-536), after the dedup: max_obj = the
-// number of uniques the memory keeps (UINT64_MAX when all fit).  A probe
-// runs the step's own top-K select for k and sums the memory of the slots
-// it marks.
+// The same for a built-in rule: its num_child first.
 template <int KIND>
-static int mem_survivor_cap(qd_ctx *ctx, const Parents &P, const qd_rule &rule, u64 prev_mem, qd_state *out,
-                            const qd_params &prm, const SelKey &sk, const u64 *UH, const u64 *UR, const double2 *UM,
-                            const u32 *UK, u64 C, u64 U, qd_step_stats &st, u64 &max_obj) {
+static int mem_parent_cap(qd_ctx *ctx, const Parents &P, const qd_rule &rule, u64 prev_mem, u64 next_mem,
+                          const qd_params &prm, const SelKey &sk, qd_step_stats &st, u64 &cap) {
+    hipStream_t s = ctx->stream;
+    const u64 N = P.n;
+    QD_TRY(ctx->nc.ensure(N * sizeof(u32)));
+    QD_TRY(ctx->counters.ensure(64 * sizeof(u64)));
+    u64 *counters = ctx->counters.as<u64>();
+    QD_HIP(hipMemsetAsync(counters + 4, 0, sizeof(u64), s));
+    hipLaunchKernelGGL(k_num_child<KIND>, dim3(grid_for(N)), dim3(256), 0, s, P, rule, ctx->nc.as<u32>(),
+                       reinterpret_cast<u32 *>(counters + 4));
+    QD_HIP(hipGetLastError());
+    u64 ew = 0;
+    QD_TRY(sync_read(ctx, counters + 4, sizeof(u64), &ew));
+    QD_TRY(check_err_word(u32(ew)));
+    return mem_parent_cap_nc(ctx, P, prev_mem, next_mem, prm, st, cap, [&](SelItem *S) {
+        hipLaunchKernelGGL(k_parent_items, dim3(grid_for(N)), dim3(256), 0, s, P, rule, sk, S, (const u64 *)nullptr);
+    });
+}
+
+// Survivor side (src/quids.hpp:510-536), after the dedup: max_obj = the
+// number of uniques the memory keeps (UINT64_MAX when all fit).  weights()
+// launches the fill of ctx->mw (C words) with every dedup slot's modelled
+// memory.  A probe runs the step's own top-K select for k and sums the
+// memory of the slots it marks.
+template <class Weights>
+static int mem_survivor_cap_w(qd_ctx *ctx, u64 prev_mem, qd_state *out, const qd_params &prm, const SelKey &sk,
+                              const u64 *UH, const u64 *UR, const double2 *UM, const u32 *UK, u64 C, u64 U,
+                              qd_step_stats &st, u64 &max_obj, Weights &&weights) {
     hipStream_t s = ctx->stream;
     max_obj = UINT64_MAX;
     u64 freeb = 0;
@@ -1121,8 +1145,7 @@ static int mem_survivor_cap(qd_ctx *ctx, const Parents &P, const qd_rule &rule, 
         QD_HIP(hipGetLastError());
         return sync_read(ctx, sum, sizeof(u64), &m);
     };
-    hipLaunchKernelGGL(k_slot_weight<KIND>, dim3(grid_for(C)), dim3(256), 0, s, P, (const u64 *)ctx->cb.as<u64>(), rule,
-                       UR, C, prm.align_byte_length, ctx->mw.as<u64>());
+    weights();
     QD_HIP(hipGetLastError());
     u64 total = 0;
     QD_TRY(masked(nullptr, total));
@@ -1148,6 +1171,17 @@ static int mem_survivor_cap(qd_ctx *ctx, const Parents &P, const qd_rule &rule, 
     QD_TRY(mem_search(U, target, false, mem, kept));
     if (kept < U) max_obj = kept;
     return QD_OK;
+}
+
+// The same for a built-in rule: slot weights from its child sizes.
+template <int KIND>
+static int mem_survivor_cap(qd_ctx *ctx, const Parents &P, const qd_rule &rule, u64 prev_mem, qd_state *out,
+                            const qd_params &prm, const SelKey &sk, const u64 *UH, const u64 *UR, const double2 *UM,
+                            const u32 *UK, u64 C, u64 U, qd_step_stats &st, u64 &max_obj) {
+    return mem_survivor_cap_w(ctx, prev_mem, out, prm, sk, UH, UR, UM, UK, C, U, st, max_obj, [&]() {
+        hipLaunchKernelGGL(k_slot_weight<KIND>, dim3(grid_for(C)), dim3(256), 0, ctx->stream, P,
+                           (const u64 *)ctx->cb.as<u64>(), rule, UR, C, prm.align_byte_length, ctx->mw.as<u64>());
+    });
 }
 
 // ===========================================================================
@@ -1920,6 +1954,28 @@ static int batch_normalize(qd_ctx *ctx, Batch *bt, qd_state *out) {
 
 // One step over the parents P (of a state whose objects span in_bytes).  With
 // BATCH, the parents are the instances of bt, each with its own rule and cap.
+// 6 select of a single-instance step: marks the survivors among the C dedup
+// slots (U uniques) in bitmap (by candidate index) and sbm (by slot): all of
+// them when U <= max_obj, else the top max_obj by the truncation key.
+static int plain_select(qd_ctx *ctx, u64 C, u64 U, u64 max_obj, const SelKey &sk, const u64 *UH, const u64 *UR,
+                        const double2 *UM, const u32 *UK, u64 *bitmap, u64 *sbm) {
+    hipStream_t s = ctx->stream;
+    if (U <= max_obj) {
+        if (U) hipLaunchKernelGGL(k_mark_all, dim3(grid_for(C, 256, 16384)), dim3(256), 0, s, UR, C, bitmap, sbm);
+        QD_HIP(hipGetLastError());
+        return QD_OK;
+    }
+    std::vector<u64> h(kHistBins);
+    if (sk.prob) {  // the dedup's histogram is of |m|^2; the keys are random
+        QD_HIP(hipMemsetAsync(ctx->nhist.p, 0, kHistBins * sizeof(u64), s));
+        hipLaunchKernelGGL(k_key_hist0, dim3(grid_for(C, 256, 4096)), dim3(256), 0, s, UH, UR, UM, C,
+                           ctx->nhist.as<u64>(), sk);
+        QD_HIP(hipGetLastError());
+    }
+    QD_TRY(sync_read(ctx, ctx->nhist.p, kHistBins * sizeof(u64), h.data()));
+    return select_topk(ctx, UH, UR, UM, UK, C, max_obj, h, bitmap, sbm, sk);
+}
+
 template <int KIND, bool BATCH>
 static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, u64 in_mem, const qd_rule &rule, Batch *bt,
                      qd_state *out, qd_symbolic *sy, u64 max_obj, const qd_params &prm, qd_step_stats &st) {
@@ -1964,7 +2020,8 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, u64 in_mem, co
         QD_TRY(ctx->hist1.ensure(kHistBins * sizeof(u64)));
         u64 *pbits = ctx->kept.as<u64>();
         QD_HIP(hipMemsetAsync(pbits, 0, ((N + 63) / 64) * sizeof(u64), s));
-        hipLaunchKernelGGL(k_parent_items, dim3(grid_for(N)), dim3(256), 0, s, P, rule, sk, ctx->sel0.as<SelItem>());
+        hipLaunchKernelGGL(k_parent_items, dim3(grid_for(N)), dim3(256), 0, s, P, rule, sk, ctx->sel0.as<SelItem>(),
+                           (const u64 *)nullptr);
         QD_HIP(hipGetLastError());
         SelItem *S0 = ctx->sel0.as<SelItem>(), *S1 = ctx->sel1.as<SelItem>();
         QD_TRY(select_loop(ctx, S0, S1, N, pcap, 0, pbits, nullptr, nullptr));
@@ -2202,23 +2259,8 @@ static int step_impl(qd_ctx *ctx, const Parents &P, u64 in_bytes, u64 in_mem, co
     u64 *sbm = ctx->slist.as<u64>();
     prof_begin(ctx, QD_PHASE_SELECT);
     QD_HIP(hipMemsetAsync(bitmap, 0, W * sizeof(u64), s));
-    if (BATCH) {
-        QD_TRY(batch_select(ctx, bt, C, UH, UR, UM, UK, bitmap, sbm));
-    } else if (U <= max_obj) {
-        if (U)
-            hipLaunchKernelGGL(k_mark_all, dim3(grid_for(C, 256, 16384)), dim3(256), 0, s, UR, C, bitmap, sbm);
-        QD_HIP(hipGetLastError());
-    } else {
-        std::vector<u64> h(kHistBins);
-        if (sk.prob) {  // the dedup's histogram is of |m|^2; the keys are random
-            QD_HIP(hipMemsetAsync(ctx->nhist.p, 0, kHistBins * sizeof(u64), s));
-            hipLaunchKernelGGL(k_key_hist0, dim3(grid_for(C, 256, 4096)), dim3(256), 0, s, UH, UR, UM, C,
-                               ctx->nhist.as<u64>(), sk);
-            QD_HIP(hipGetLastError());
-        }
-        QD_TRY(sync_read(ctx, ctx->nhist.p, kHistBins * sizeof(u64), h.data()));
-        QD_TRY(select_topk(ctx, UH, UR, UM, UK, C, max_obj, h, bitmap, sbm, sk));
-    }
+    if (BATCH) QD_TRY(batch_select(ctx, bt, C, UH, UR, UM, UK, bitmap, sbm));
+    else QD_TRY(plain_select(ctx, C, U, max_obj, sk, UH, UR, UM, UK, bitmap, sbm));
     prof_end(ctx, QD_PHASE_SELECT);
     phase_mark(ctx, PH_PREP_FINAL);
     prof_begin(ctx, QD_PHASE_ORDER);
@@ -2282,6 +2324,242 @@ static int simulate_kind(qd_ctx *ctx, qd_state *in, const qd_rule &rule, qd_stat
     QD_TRY(relieve_memory(ctx));
     const Parents P{in->objects.as<u8>(), in->begin.as<u64>(), in->size.as<u32>(), in->mag.as<double2>(), in->n};
     return step_impl<KIND, false>(ctx, P, in->bytes, state_mem(in), rule, nullptr, out, sy, max_obj, prm, st);
+}
+
+// ===========================================================================
+// the step with a user rule (qd_user_rule, src/quids.hpp:105-146): the
+// rule's callbacks launch its num_child, generate (populate_child + hasher),
+// populate (populate_child_simple) and object-hash kernels; the rest is the
+// single-pass path of step_impl with the same kernels, so a user rule that
+// restates a built-in one gives bit-identical results
+// ===========================================================================
+static int user_cb(int rc, const char *what) {
+    if (rc != 0)
+        return fail(QD_ERR_INVALID, std::string("user rule: the ") + what + " callback returned " + std::to_string(rc));
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return fail(QD_ERR_HIP, std::string("user rule: the ") + what + " callback's launch: " + hipGetErrorString(e));
+    return QD_OK;
+}
+
+// The child bytes' scratch of a user generate: slices of max_child bytes
+// (rounded up to 16) for a grid of at most 2^18 threads, a multiple of 256,
+// and at most about 1 GB.
+static void user_scratch_shape(u64 C, u32 max_child, u64 &slice, u64 &threads) {
+    slice = (u64(std::max<u32>(max_child, 1)) + 15) / 16 * 16;
+    threads = std::min<u64>((C + 255) / 256 * 256, 1ull << 18);
+    const u64 by_mem = std::max<u64>(256, ((1ull << 30) / slice) / 256 * 256);
+    threads = std::min(threads, by_mem);
+}
+
+static int user_step(qd_ctx *ctx, qd_state *in, const qd_user_rule &ur, qd_state *out, qd_symbolic *sy, u64 max_obj,
+                     const qd_params &prm, qd_step_stats &st) {
+    QD_TRY(relieve_memory(ctx));
+    hipStream_t s = ctx->stream;
+    const Parents P{in->objects.as<u8>(), in->begin.as<u64>(), in->size.as<u32>(), in->mag.as<double2>(), in->n};
+    const u64 N = P.n;
+    const u64 in_mem = state_mem(in);
+    st.num_parent = N;
+    const SelKey sk{prm.simple_truncation ? 0u : 1u, 0u, prm.seed};
+    const qd_objects_view pv{P.obj, P.begin, P.size, reinterpret_cast<const double *>(P.mag), N};
+    auto finish_empty = [&]() -> int {
+        QD_TRY(state_reserve(out, 0, 0));
+        QD_HIP(hipMemsetAsync(out->begin.p, 0, sizeof(u64), s));
+        out->n = 0;
+        out->bytes = 0;
+        out->total_proba = 0;
+        st.total_proba = 0;
+        if (sy) {
+            sy->num_object = st.num_candidates;
+            sy->num_object_after_interferences = st.num_unique;
+        }
+        QD_HIP(hipStreamSynchronize(s));
+        return QD_OK;
+    };
+    if (N == 0) return finish_empty();
+
+    // ---- num_child of every parent (max child sizes in wi), the parents'
+    // hashes when the truncation keys need them
+    phase_mark(ctx, PH_NUM_CHILD);
+    QD_TRY(ensure_relieved(ctx, ctx->nc, N * sizeof(u32)));
+    QD_TRY(ensure_relieved(ctx, ctx->wi, N * sizeof(u32)));
+    QD_TRY(ensure_relieved(ctx, ctx->cb, (N + 1) * sizeof(u64)));
+    QD_TRY(ctx->counters.ensure(64 * sizeof(u64)));
+    prof_begin(ctx, QD_PHASE_PREP);
+    QD_TRY(user_cb(ur.num_child(ur.self, &pv, ctx->nc.as<u32>(), ctx->wi.as<u32>(), (void *)s), "num_child"));
+    const u64 *phash = nullptr;
+    if (sk.prob) {
+        QD_TRY(ensure_relieved(ctx, ctx->pre, N * sizeof(u64)));
+        QD_TRY(user_cb(ur.hash_objects(ur.self, &pv, ctx->pre.as<u64>(), (void *)s), "hash_objects"));
+        phash = ctx->pre.as<u64>();
+    }
+    prof_end(ctx, QD_PHASE_PREP);
+    auto items = [&](SelItem *S) {
+        hipLaunchKernelGGL(k_parent_items, dim3(grid_for(N)), dim3(256), 0, s, P, qd_rule{}, sk, S, phash);
+    };
+
+    // ---- pre-truncation of parents (quids.hpp:613-642), from the memory
+    // model when max_num_object == 0 (src/quids.hpp:459-486)
+    u64 pcap = max_obj;
+    if (max_obj == 0) QD_TRY(mem_parent_cap_nc(ctx, P, in_mem, state_mem(out), prm, st, pcap, items));
+    phase_mark(ctx, PH_TRUNC_SYM);
+    const u64 *keep = nullptr;
+    u64 nkept = N;
+    if (pcap != 0 && pcap < N) {
+        QD_TRY(ctx->sel0.ensure(N * sizeof(SelItem)));
+        QD_TRY(ctx->sel1.ensure(N * sizeof(SelItem)));
+        QD_TRY(ctx->kept.ensure(((N + 63) / 64) * sizeof(u64)));
+        QD_TRY(ctx->hist1.ensure(kHistBins * sizeof(u64)));
+        u64 *pbits = ctx->kept.as<u64>();
+        QD_HIP(hipMemsetAsync(pbits, 0, ((N + 63) / 64) * sizeof(u64), s));
+        items(ctx->sel0.as<SelItem>());
+        QD_HIP(hipGetLastError());
+        QD_TRY(select_loop(ctx, ctx->sel0.as<SelItem>(), ctx->sel1.as<SelItem>(), N, pcap, 0, pbits, nullptr,
+                           nullptr));
+        keep = pbits;
+        nkept = pcap;
+    }
+    st.num_parent_kept = nkept;
+    phase_mark(ctx, PH_PREP_INDEX);
+
+    // ---- child offsets: the kept parents' num_child scanned
+    u64 *counters = ctx->counters.as<u64>();
+    QD_HIP(hipMemsetAsync(counters, 0, 64 * sizeof(u64), s));
+    u32 *maxsz = reinterpret_cast<u32 *>(counters + 3);
+    prof_begin(ctx, QD_PHASE_SCAN);
+    hipLaunchKernelGGL(k_user_keep, dim3(grid_for(N)), dim3(256), 0, s, ctx->nc.as<u32>(),
+                       (const u32 *)ctx->wi.as<u32>(), keep, N, maxsz);
+    QD_HIP(hipGetLastError());
+    QD_TRY(run_scan(ctx, LoadU32{ctx->nc.as<u32>()}, N, ctx->cb.as<u64>()));
+    prof_end(ctx, QD_PHASE_SCAN);
+    u64 *hb = ctx->host;
+    QD_HIP(hipMemcpyAsync(hb, ctx->cb.as<u64>() + N, sizeof(u64), hipMemcpyDeviceToHost, s));
+    QD_HIP(hipMemcpyAsync(hb + 1, counters + 3, sizeof(u64), hipMemcpyDeviceToHost, s));
+    QD_HIP(hipStreamSynchronize(s));
+    prof_collect(ctx);
+    const u64 C = hb[0];
+    const u32 max_child = u32(hb[1]);
+    st.num_candidates = C;
+    if (C == 0) return finish_empty();
+    const u64 budget = step_budget(ctx);
+    if (C > budget)
+        return fail(QD_ERR_CAPACITY, "user rule: " + std::to_string(C) + " candidates exceed the one-pass budget of " +
+                                         std::to_string(budget) + " (user rules run in one pass)");
+    release_range_cache(ctx);
+
+    // ---- generate: the rule's populate_child + hasher per candidate
+    u32 b1, b2;
+    dedup_layout(C, ctx->tune, b1, b2);
+    st.dedup_levels = (b1 ? 1 : 0) + (b2 ? 1 : 0);
+    u32 *UK = nullptr;
+    {
+        const u64 full = budget * 32;
+        QD_TRY(ensure_big(ctx->ebuf0, C * 32, full));
+        QD_TRY(ensure_big(ctx->ebuf1, C * 32, full + full / 32));
+        if (!sk.prob && (max_obj == 0 || C > max_obj)) {
+            QD_TRY(ensure_big(ctx->ukey, C * sizeof(u32), full / 8));
+            UK = ctx->ukey.as<u32>();
+        }
+    }
+    QD_TRY(ensure_relieved(ctx, ctx->usize, C * sizeof(u32)));
+    u64 slice = 0, threads = 0;
+    user_scratch_shape(C, max_child, slice, threads);
+    QD_TRY(ensure_relieved(ctx, ctx->uscr, slice * threads));
+    QD_TRY(ctx->hist1.ensure(kHistBins * sizeof(u64)));
+    QD_TRY(ctx->off1.ensure((kHistBins + 1) * sizeof(u64)));
+    QD_TRY(ctx->cur1.ensure(kHistBins * sizeof(u64)));
+    QD_TRY(ctx->nhist.ensure(kHistBins * sizeof(u64)));
+    u64 *H0 = ctx->ebuf0.as<u64>();
+    double2 *M0 = reinterpret_cast<double2 *>(H0 + C);
+    u32 *csize = ctx->usize.as<u32>();
+    QD_HIP(hipMemsetAsync(ctx->hist1.p, 0, kHistBins * sizeof(u64), s));
+    QD_HIP(hipMemsetAsync(ctx->nhist.p, 0, kHistBins * sizeof(u64), s));
+    phase_mark(ctx, PH_SYMBOLIC);
+    prof_begin(ctx, QD_PHASE_GENERATE);
+    QD_TRY(user_cb(ur.generate(ur.self, &pv, ctx->cb.as<u64>(), C, max_child, ctx->uscr.as<u8>(), slice * threads, H0,
+                               reinterpret_cast<double *>(M0), csize, (void *)s),
+                   "generate"));
+    if (b1)
+        hipLaunchKernelGGL(k_key_hist_top, dim3(grid_for(C, 256, 4096)), dim3(256), 0, s, (const u64 *)H0, C, b1,
+                           ctx->hist1.as<u64>());
+    QD_HIP(hipGetLastError());
+    prof_end(ctx, QD_PHASE_GENERATE);
+
+    // ---- dedup, memory-driven survivor cap, select, order
+    u64 *UH, *UR;
+    double2 *UM;
+    u64 U;
+    phase_mark(ctx, PH_COLL_PREP);
+    QD_TRY(partition_dedup(ctx, H0, M0, nullptr, C, b1, b2, prm.tolerance, st, UH, UR, UM, U, 0, UK));
+    phase_mark(ctx, PH_COLL_FINAL);
+    st.num_unique = U;
+    if (max_obj == 0)
+        QD_TRY(mem_survivor_cap_w(ctx, in_mem, out, prm, sk, UH, UR, UM, UK, C, U, st, max_obj, [&]() {
+            hipLaunchKernelGGL(k_user_slot_weight, dim3(grid_for(C)), dim3(256), 0, s, (const u64 *)UR, C,
+                               (const u32 *)csize, prm.align_byte_length, kMemPerObject, ctx->mw.as<u64>());
+        }));
+    phase_mark(ctx, PH_TRUNC);
+    const u64 W = (C + 63) / 64;
+    QD_TRY(ctx->bitmap.ensure(W * sizeof(u64)));
+    QD_TRY(ctx->wprefix.ensure((W + 1) * sizeof(u64)));
+    QD_TRY(ctx->slist.ensure(W * sizeof(u64)));
+    u64 *bitmap = ctx->bitmap.as<u64>(), *sbm = ctx->slist.as<u64>();
+    prof_begin(ctx, QD_PHASE_SELECT);
+    QD_HIP(hipMemsetAsync(bitmap, 0, W * sizeof(u64), s));
+    QD_TRY(plain_select(ctx, C, U, max_obj, sk, UH, UR, UM, UK, bitmap, sbm));
+    prof_end(ctx, QD_PHASE_SELECT);
+    phase_mark(ctx, PH_PREP_FINAL);
+    prof_begin(ctx, QD_PHASE_ORDER);
+    QD_TRY(run_scan(ctx, LoadPopc{bitmap}, W, ctx->wprefix.as<u64>()));
+    prof_end(ctx, QD_PHASE_ORDER);
+    QD_TRY(sync_read(ctx, ctx->wprefix.as<u64>() + W, sizeof(u64), hb));
+    const u64 K = hb[0];
+    st.num_survivors = K;
+    if (K == 0) return finish_empty();
+    QD_TRY(out->begin.ensure((K + 1) * sizeof(u64)));
+    QD_TRY(out->size.ensure(K * sizeof(u32)));
+    QD_TRY(out->mag.ensure(K * sizeof(double2)));
+    QD_TRY(ctx->padded.ensure(K * sizeof(u64)));
+    QD_TRY(ctx->keys.ensure(K * sizeof(u64)));
+    u64 *orep = ctx->keys.as<u64>();
+    prof_begin(ctx, QD_PHASE_ORDER);
+    hipLaunchKernelGGL(k_place, dim3(grid_for((C + 63) / 64, 256, 4096)), dim3(256), 0, s, UR, UM, (const u64 *)sbm, C,
+                       bitmap, ctx->wprefix.as<u64>(), orep, out->mag.as<double2>());
+    QD_HIP(hipGetLastError());
+    prof_end(ctx, QD_PHASE_ORDER);
+
+    // ---- final: sizes from the generate, bytes from the rule's populate
+    // (the padding is zero)
+    phase_mark(ctx, PH_FINAL);
+    prof_begin(ctx, QD_PHASE_FINALIZE);
+    hipLaunchKernelGGL(k_user_out_size, dim3(grid_for(K)), dim3(256), 0, s, (const u64 *)orep, K, (const u32 *)csize,
+                       prm.align_byte_length, out->size.as<u32>(), ctx->padded.as<u64>());
+    QD_HIP(hipGetLastError());
+    QD_TRY(run_scan(ctx, LoadU64{ctx->padded.as<u64>()}, K, out->begin.as<u64>()));
+    prof_end(ctx, QD_PHASE_FINALIZE);
+    QD_TRY(sync_read(ctx, out->begin.as<u64>() + K, sizeof(u64), hb));
+    const u64 bytes = hb[0];
+    QD_TRY(ensure_relieved(ctx, out->objects, std::max<u64>(bytes, 1)));
+    prof_begin(ctx, QD_PHASE_FINALIZE);
+    QD_HIP(hipMemsetAsync(out->objects.p, 0, std::max<u64>(bytes, 1), s));
+    QD_TRY(user_cb(ur.populate(ur.self, &pv, ctx->cb.as<u64>(), orep, K, out->begin.as<u64>(), out->objects.as<u8>(),
+                               (void *)s),
+                   "populate"));
+    prof_end(ctx, QD_PHASE_FINALIZE);
+    out->n = K;
+    out->bytes = bytes;
+    st.output_bytes = bytes;
+
+    // ---- normalize
+    phase_mark(ctx, PH_NORMALIZE);
+    QD_TRY(normalize_state(ctx, out));
+    prof_collect(ctx);
+    st.total_proba = out->total_proba;
+    if (sy) {
+        sy->num_object = C;
+        sy->num_object_after_interferences = U;
+    }
+    return QD_OK;
 }
 
 // ===========================================================================
@@ -2691,6 +2969,12 @@ int qd_ctx_synchronize(qd_ctx *c) {
     return QD_OK;
 }
 
+int qd_ctx_stream(qd_ctx *c, void **stream) {
+    if (!c || !stream) return fail(QD_ERR_INVALID, "null argument");
+    *stream = (void *)c->stream;
+    return QD_OK;
+}
+
 int qd_ctx_scratch_bytes(qd_ctx *c, uint64_t *bytes) {
     if (!c || !bytes) return fail(QD_ERR_INVALID, "null argument");
     u64 t = 0;
@@ -2980,6 +3264,25 @@ int qd_simulate(qd_ctx *c, qd_state *in, const qd_rule *rule, qd_state *out, qd_
     return rc;
 }
 
+int qd_simulate_user(qd_ctx *c, qd_state *in, const qd_user_rule *rule, qd_state *out, qd_symbolic *sy,
+                     uint64_t max_num_object, const qd_params *params, qd_step_stats *stats) {
+    if (!c || !in || !rule || !out) return fail(QD_ERR_INVALID, "null argument");
+    if (!rule->num_child || !rule->generate || !rule->populate || !rule->hash_objects)
+        return fail(QD_ERR_INVALID, "user rule: a callback is missing");
+    if (in == out) return fail(QD_ERR_INVALID, "input and output states must differ");
+    if (in->ctx != c || out->ctx != c) return fail(QD_ERR_INVALID, "state belongs to another context");
+    qd_params prm{1e-30, 8, 1, 0, 0, 0.2, 0.25};
+    if (params) prm = *params;
+    qd_step_stats st;
+    std::memset(&st, 0, sizeof st);
+    QD_HIP(hipSetDevice(c->device));
+    c->next_phase = 0;
+    const int rc = user_step(c, in, *rule, out, sy, max_num_object, prm, st);
+    if (rc == QD_OK) phase_mark(c, PH_END);
+    if (stats) *stats = st;
+    return rc;
+}
+
 int qd_ctx_set_phase_hook(qd_ctx *c, qd_phase_hook hook, void *user) {
     if (!c) return fail(QD_ERR_INVALID, "null ctx");
     c->hook = hook;
@@ -3109,6 +3412,19 @@ int qd_average_value(qd_ctx *c, qd_state *s, int observable, int arg, double *va
     QD_TRY(c->adjd.ensure(s->n * sizeof(double)));
     hipLaunchKernelGGL(k_observable, dim3(grid_for(s->n)), dim3(256), 0, c->stream, s->objects.as<u8>(),
                        s->begin.as<u64>(), s->size.as<u32>(), s->mag.as<double2>(), s->n, observable, arg,
+                       c->adjd.as<double>(), reinterpret_cast<double2 *>(cot));
+    QD_HIP(hipGetLastError());
+    return canonical_sum(c, nullptr, c->adjd.as<double>(), s->n, value);
+}
+
+int qd_average_value_of(qd_ctx *c, qd_state *s, const double *obs, double *value, double *cot) {
+    if (!c || !s || !value || (!obs && s->n)) return fail(QD_ERR_INVALID, "null argument");
+    if (s->ctx != c) return fail(QD_ERR_INVALID, "state belongs to another context");
+    QD_HIP(hipSetDevice(c->device));
+    *value = 0.0;
+    if (s->n == 0) return QD_OK;
+    QD_TRY(c->adjd.ensure(s->n * sizeof(double)));
+    hipLaunchKernelGGL(k_observable_of, dim3(grid_for(s->n)), dim3(256), 0, c->stream, obs, s->mag.as<double2>(), s->n,
                        c->adjd.as<double>(), reinterpret_cast<double2 *>(cot));
     QD_HIP(hipGetLastError());
     return canonical_sum(c, nullptr, c->adjd.as<double>(), s->n, value);

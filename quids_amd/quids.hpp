@@ -116,6 +116,17 @@ public:
     }
     /// Fills the device descriptor of this rule; false when the rule has none.
     virtual bool device_rule(qd_rule &) const { return false; }
+    /// Fills the callback table of a rule compiled as a device functor
+    /// (quids_amd/user_rule.hpp, device_rule<R>); false when the rule has none.
+    virtual bool user_rule(qd_user_rule &) const { return false; }
+};
+
+/// An observable evaluated on the device (quids_amd/user_rule.hpp builds one
+/// from a functor, device_observable<F>): the sum of obs |m|^2 over a state.
+class device_observable_base {
+public:
+    virtual ~device_observable_base() {}
+    virtual double average(qd_ctx *ctx, qd_state *state) const = 0;
 };
 
 /// A rule whose four methods run the library's functors (the same code the
@@ -228,6 +239,11 @@ public:
             avg += observable(objects_.data() + begin_[i], objects_.data() + begin_[i] + size_[i]) * std::norm(mag_[i]);
         return avg;
     }
+    /// iteration::average_value for an observable evaluated on the device
+    double average_value(const device_observable_base &observable) const {
+        push();
+        return observable.average(detail::context(), state_);
+    }
     /// iteration::get_object (read-only), src/quids.hpp:254-258
     void get_object(size_t const object_id, char const *&object_begin_, uint &object_size_, mag_t &mag) const {
         pull();
@@ -256,9 +272,9 @@ private:
     mutable std::vector<uint32_t> size_;
     mutable std::vector<mag_t> mag_;
     mutable bool host_valid_ = true;
-    bool device_valid_ = true;
+    mutable bool device_valid_ = true;
 
-    void push() {
+    void push() const {
         if (device_valid_) return;
         detail::check(qd_state_upload(state_, num_object, objects_.data(), begin_.data(), size_.data(),
                                       reinterpret_cast<const double *>(mag_.data()), total_proba),
@@ -321,14 +337,21 @@ inline void simulate(it_t &iteration, modifier_t const rule) {
 inline void simulate(it_t &iteration, rule_t const *rule, it_t &next_iteration, sy_it_t &symbolic_iteration,
                      size_t max_num_object = 0, debug_t mid_step_function = [](const char *) {}) {
     qd_rule d;
-    if (!rule || !rule->device_rule(d)) throw std::runtime_error("simulate: rule has no device implementation");
+    qd_user_rule u;
+    const bool builtin = rule && rule->device_rule(d);
+    if (!builtin && !(rule && rule->user_rule(u)))
+        throw std::runtime_error("simulate: rule has no device implementation");
     iteration.push();
     qd_params p = detail::params();
     // the library calls the hook between its phases, with the reference's names
     auto trampoline = [](const char *phase, void *user) { (*static_cast<debug_t *>(user))(phase); };
     detail::check(qd_ctx_set_phase_hook(detail::context(), trampoline, &mid_step_function), "qd_ctx_set_phase_hook");
-    const int rc = qd_simulate(detail::context(), iteration.state_, &d, next_iteration.state_, symbolic_iteration.sy_,
-                               uint64_t(max_num_object), &p, &symbolic_iteration.stats);
+    const int rc = builtin ? qd_simulate(detail::context(), iteration.state_, &d, next_iteration.state_,
+                                         symbolic_iteration.sy_, uint64_t(max_num_object), &p,
+                                         &symbolic_iteration.stats)
+                           : qd_simulate_user(detail::context(), iteration.state_, &u, next_iteration.state_,
+                                              symbolic_iteration.sy_, uint64_t(max_num_object), &p,
+                                              &symbolic_iteration.stats);
     qd_ctx_set_phase_hook(detail::context(), nullptr, nullptr);
     detail::check(rc, "qd_simulate");
     symbolic_iteration.num_object = symbolic_iteration.stats.num_candidates;
