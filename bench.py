@@ -189,6 +189,31 @@ def _after_marker(rows, values):
     return {d: v for d, v in values.items() if d > m}
 
 
+def counter_summary(folder):
+    """Per kernel family (name without namespace or template arguments),
+    the per-dispatch average of every counter a rocprofv3 --pmc run of this
+    script collected, over the timed dispatches (after the marker copy).
+    Prints a markdown table, kernels by SQ_INSTS_VALU."""
+    rows = _csv(os.path.join(folder, "**", "*counter_collection.csv"))
+    after = _after_marker(rows, {int(r["Dispatch_Id"]): r for r in rows})
+    sums, disp, names = {}, {}, []
+    for r in rows:
+        if int(r["Dispatch_Id"]) not in after:
+            continue
+        fam = r["Kernel_Name"].replace("void ", "").replace("qd::", "").split("(")[0].split("<")[0]
+        c = r["Counter_Name"]
+        if c not in names:
+            names.append(c)
+        sums.setdefault(fam, {}).setdefault(c, 0.0)
+        sums[fam][c] += float(r["Counter_Value"])
+        disp.setdefault(fam, set()).add(int(r["Dispatch_Id"]))
+    print("| kernel | dispatches | " + " | ".join(names) + " |")
+    print("|---|---:|" + "---:|" * len(names))
+    for fam, c in sorted(sums.items(), key=lambda x: -x[1].get("SQ_INSTS_VALU", 0)):
+        n = len(disp[fam])
+        print(f"| `{fam}` | {n} | " + " | ".join(f"{c.get(k, 0) / n:.4g}" for k in names) + " |")
+
+
 def rocprof_summary(args):
     """Formats the rocprofv3 output of this script's own runs of one command:
     kernel-trace stats, then per phase kernel family its timed dispatches
@@ -429,6 +454,9 @@ def main():
                     help="only run the library's plain copy of this many GiB (counter calibration) and exit")
     ap.add_argument("--rocprof-summary", nargs=5, metavar=("KT", "FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE"),
                     help="summarize rocprofv3 output folders of this script's runs and exit")
+    ap.add_argument("--counter-summary", metavar="PMC",
+                    help="per-dispatch averages per kernel of a rocprofv3 --pmc output folder of this script's run, "
+                         "then exit")
     ap.add_argument("--calib-json", help="the --copy-probe JSON line of the calibration run")
     ap.add_argument("--bench-json", help="the JSON line of the profiled bench run (--rocprof-summary)")
     ap.add_argument("--traffic-out", default=TRAFFIC_FILE)
@@ -447,6 +475,9 @@ def main():
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="set one qd_tuning field before the run (repeatable), e.g. --tune split_generate=1")
     args = ap.parse_args()
+    if args.counter_summary:
+        counter_summary(args.counter_summary)
+        return
     if args.rocprof_summary:
         args.workload_key = workload_key(args)
         rocprof_summary(args)
