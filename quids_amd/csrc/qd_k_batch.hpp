@@ -212,9 +212,14 @@ __device__ inline u64 seg_hi_at(const SegLoop *L, const SegItem &it, u32 inst, u
 }
 
 // One block of 1024 threads: the tiles of every active instance's list.
+// Per chunk of 1024 instances, the tile counts' scan in LDS; then each
+// thread writes every 1024th tile of the chunk (its instance found by a
+// search of the scan), so that an instance of many tiles is not written by
+// one thread.
 __global__ __launch_bounds__(1024) void k_seg_tiles(SegLoop *L) {
     __shared__ u64 tmp[1024 / kWave + 1];
     __shared__ u64 carry;
+    __shared__ u64 sfirst[1024], soff[1024], scnt[1024];
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
     for (u32 i0 = 0; i0 < L->n; i0 += blockDim.x) {
@@ -226,9 +231,25 @@ __global__ __launch_bounds__(1024) void k_seg_tiles(SegLoop *L) {
             nt = (cnt + kSegTileItems - 1) / kSegTileItems;
         }
         u64 total;
-        const u64 first = carry + block_excl_scan(nt, tmp, total);
-        for (u64 b = 0; b < nt; ++b)
-            L->tiles[first + b] = SegTile{off + b * kSegTileItems, off + min(cnt, (b + 1) * kSegTileItems), i, 0};
+        sfirst[threadIdx.x] = block_excl_scan(nt, tmp, total);
+        soff[threadIdx.x] = off;
+        scnt[threadIdx.x] = cnt;
+        __syncthreads();
+        const u64 c0 = carry;
+        for (u64 t = threadIdx.x; t < total; t += blockDim.x) {
+            // the last thread j of the chunk with sfirst[j] <= t: its
+            // instance has tiles (a thread with none shares its first with
+            // the next thread)
+            u32 lo = 0, hi = blockDim.x - 1;
+            while (lo < hi) {
+                const u32 mid = (lo + hi + 1) >> 1;
+                if (sfirst[mid] <= t) lo = mid;
+                else hi = mid - 1;
+            }
+            const u64 b = t - sfirst[lo];
+            L->tiles[c0 + t] =
+                SegTile{soff[lo] + b * kSegTileItems, soff[lo] + min(scnt[lo], (b + 1) * kSegTileItems), i0 + lo, 0};
+        }
         __syncthreads();
         if (threadIdx.x == 0) carry += total;
         __syncthreads();
