@@ -1143,7 +1143,11 @@ constexpr u32 kPlaceChunk = kPlaceThreads * kPlacePer;
 __global__ __launch_bounds__(kPlaceThreads) void k_region_place(const Entry *in, u64 capr, const u32 *rcount,
                                                                 u32 nreg, u32 shift, u32 nbin, const u32 *bm,
                                                                 u32 bstride, const u64 *boff, Entry *out) {
-    __shared__ Entry stage[kPlaceChunk];
+    // the chunk's entries by field (8-byte words: an entry's place is random,
+    // and a 32-byte store per lane to random places conflicts on the LDS
+    // banks far more than four 8-byte ones)
+    __shared__ u64 sh[kPlaceChunk], sc[kPlaceChunk];
+    __shared__ double sre[kPlaceChunk], sim[kPlaceChunk];
     __shared__ u32 cnt[kRangeBins], lst[kRangeBins];
     __shared__ u64 cur[kRangeBins];
     __shared__ u64 tmp[kPlaceThreads / kWave + 1];
@@ -1192,11 +1196,21 @@ __global__ __launch_bounds__(kPlaceThreads) void k_region_place(const Entry *in,
 #pragma unroll
             for (u32 q = 0; q < kPlacePer; ++q) {
                 const u32 j = threadIdx.x + kPlaceThreads * q;
-                if (j < m) stage[lst[dr[q] >> 16] + (dr[q] & 0xFFFF)] = e[q];
+                if (j < m) {
+                    const u32 at = lst[dr[q] >> 16] + (dr[q] & 0xFFFF);
+                    sh[at] = e[q].hash;
+                    sc[at] = e[q].cand;
+                    sre[at] = e[q].re;
+                    sim[at] = e[q].im;
+                }
             }
             __syncthreads();
             for (u32 j = threadIdx.x; j < m; j += kPlaceThreads) {
-                const Entry x = stage[j];
+                Entry x;
+                x.hash = sh[j];
+                x.cand = sc[j];
+                x.re = sre[j];
+                x.im = sim[j];
                 const u32 d = digit(x.hash);
                 out[cur[d] + (j - lst[d])] = x;
             }
