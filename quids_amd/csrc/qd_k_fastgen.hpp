@@ -1068,7 +1068,7 @@ struct GroupBases {
 };
 
 template <int KIND>
-__global__ __launch_bounds__(kGenThreads) void k_range_walkwrite(const PDesc *D, const double2 *pmag, const u64 *cb,
+__global__ __launch_bounds__(kGenThreads, 6) void k_range_walkwrite(const PDesc *D, const double2 *pmag, const u64 *cb,
                                                                  const u64 *wb, u64 N, const u64 *cpar,
                                                                  const u64 *tab, const u64 *pbegin, qd_rule rule,
                                                                  u64 W, RangeSel rs, u32 gbits, GroupBases E,

@@ -1446,7 +1446,12 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
     // spill takes the exact path)
     const u64 cap2 = (u64(mean2 + 16.0 * std::sqrt(mean2)) + 256 + 63) / 64 * 64;
     const u64 pcap = std::max<u64>(capped ? 2 * max_obj + gcap : C, 1);  // trimmed to max_obj beyond 2 max_obj
-    const unsigned gr = unsigned(std::max<u64>(std::min<u64>((nitems + kRangeTile - 1) / kRangeTile, kRangeGrid), 1));
+    // (the walks' blocks: as many as are resident on the device at once,
+    // 6 per CU, so that no block starts late)
+    int ncu_w = 256;
+    (void)hipDeviceGetAttribute(&ncu_w, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    const u64 gmax = std::min<u64>(kRangeGrid, u64(std::max(ncu_w, 1)) * 6);
+    const unsigned gr = unsigned(std::max<u64>(std::min<u64>((nitems + kRangeTile - 1) / kRangeTile, gmax), 1));
     QD_TRY(ctx->bmat.ensure(u64(gr) * nb1 * sizeof(u32)));
     QD_TRY(ctx->off1.ensure((nb1 + 1) * sizeof(u64)));
     QD_TRY(ctx->cnt2.ensure(nb2g * sizeof(u64)));
