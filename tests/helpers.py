@@ -77,7 +77,7 @@ def kth_norm(hs, k):
 # workloads
 # ---------------------------------------------------------------------------
 def qcgd_iteration_rules(theta_sm=PI / 4, phi_sm=0.0, theta_ec=PI / 3, phi_ec=PI / 5):
-    """One QCGD iteration of the benchmark configs: step, split_merge, erase_create."""
+    """One QCGD iteration of the benchmark workloads: step, split_merge, erase_create."""
     return [qcgd.step, qcgd.split_merge(theta_sm, phi_sm), qcgd.erase_create(theta_ec, phi_ec)]
 
 

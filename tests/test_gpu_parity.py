@@ -45,7 +45,7 @@ def test_hadamard_single_object(ctx):
 
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_circuit_chain_16_qubits(ctx, seed):
-    """Config-1 style circuit (16 one-byte qubits, 24 steps), GPU and oracle
+    """A seeded circuit of 24 steps on 16 one-byte qubits, GPU and oracle
     chained independently, compared after every step."""
     hs_g = HostState.from_objects([bytes(16)])
     hs_o = hs_g
