@@ -399,8 +399,20 @@ __device__ inline void flip_run(W &w, const PDesc &pd, BT bits, u32 dlimit, cons
 // taken, the split's left half or the merge), a taken split also its right
 // half, and a taken merge skips the next node.  The same products, roots and
 // folds in the same order as sm_run.
-template <typename KC>
-__device__ inline u64 sm_child(const PDesc &pd, const u64 *tab, u64 bits, const KC &k, cplx &mag) {
+// The four factors of a decision in a table (LDS) indexed by
+// (taken << 1) | x: x = 0 -> do_not / do_, x = 1 -> -do_not_conj / do_conj.
+// A flip-rule decision node has x = its left bit; a split_merge operation
+// x = 1 for a merge, 0 for a split (qcgd.hpp:466-471, 541-546, 614-619).
+// One 16-byte LDS read per decision in place of a select among four
+// uniform complex constants (about 30 instructions per decision).
+__device__ inline void fill_factor_table(const QcgdCoef &k, cplx *ct) {
+    ct[0] = k.dn;
+    ct[1] = cneg(k.dnc);
+    ct[2] = k.d;
+    ct[3] = k.dc;
+}
+
+__device__ inline u64 sm_child(const PDesc &pd, const u64 *tab, u64 bits, const cplx *ct, cplx &mag) {
     const u32 n = pd.n;
     u64 nh = 0, lh = 0, rh = 0;
     u32 c = 0, d = 0, i = 0, end = n;
@@ -414,19 +426,17 @@ __device__ inline u64 sm_child(const PDesc &pd, const u64 *tab, u64 bits, const 
     if (pd.flags & PD_LM) {  // the last node merges with the first (qcgd.hpp:647-668)
         const bool take = bits & 1;
         d = 1;
+        mag = cmul(mag, ct[take ? 3u : 1u]);  // a merge: do_conj / -do_not_conj
         if (take) {
-            mag = cmul(mag, k.dc);
             emit(tab[kSmSlots * (n - 1) + 1], true, true);
             i = 1;
             end = n - 1;
-        } else {
-            mag = cmul(mag, cneg(k.dnc));
         }
     } else if (pd.flags & PD_FS) {  // the first node's split (qcgd.hpp:629-633)
         const bool take = bits & 1;
         d = 1;
+        mag = cmul(mag, ct[take ? 2u : 0u]);  // a split: do_ / do_not
         if (take) {
-            mag = cmul(mag, k.d);
             if (pd.flags & PD_HMLZ) {
                 emit(tab[1], true, false);
                 emit(tab[2], false, true);
@@ -435,7 +445,6 @@ __device__ inline u64 sm_child(const PDesc &pd, const u64 *tab, u64 bits, const 
                 emit(tab[2], false, true);
             }
         } else {
-            mag = cmul(mag, k.dn);
             emit(tab[0], true, true);
         }
         i = 1;
@@ -446,7 +455,7 @@ __device__ inline u64 sm_child(const PDesc &pd, const u64 *tab, u64 bits, const 
         const bool take = dec && ((bits >> d) & 1);
         d += dec ? 1u : 0u;
         const bool split = l && r;
-        if (dec) mag = cmul(mag, split ? (take ? k.d : k.dn) : (take ? k.dc : cneg(k.dnc)));
+        if (dec) mag = cmul(mag, ct[(take ? 2u : 0u) | (split ? 0u : 1u)]);
         emit(tab[kSmSlots * i + (take ? 1u : 0u)], take || l, take ? !split : r);
         if (take && split) emit(tab[kSmSlots * i + 2], false, true);
         i += (take && !split) ? 2u : 1u;
@@ -467,17 +476,15 @@ __device__ inline u64 sm_child(const PDesc &pd, const u64 *tab, u64 bits, const 
 // Computed apart, each over the set bits of a mask, they are the same
 // products and folds in the same order, without flip_run's per-node
 // branches, which lanes holding different children would take both ways.
-template <typename KC>
-__device__ inline u64 flip_child(const PDesc &pd, u64 bits, const KC &k, cplx &mag) {
+__device__ inline u64 flip_child(const PDesc &pd, u64 bits, const cplx *ct, cplx &mag) {
     u64 dm = pd.dmask, flips = 0;
     u32 d = 0;
     while (dm) {
         const u32 i = u32(__builtin_ctzll(dm));
         dm &= dm - 1;
         const bool take = (bits >> d++) & 1;
-        const bool l = (pd.lmask >> i) & 1;
-        const cplx c = take ? (l ? k.dc : k.d) : (l ? cneg(k.dnc) : k.dn);
-        mag = cmul(mag, c);
+        const u32 l = u32(pd.lmask >> i) & 1u;
+        mag = cmul(mag, ct[(take ? 2u : 0u) | l]);
         flips |= u64(take) << i;
     }
     u64 lm = pd.lmask ^ flips, rm = pd.rmask ^ flips, lh = 0, rh = 0;
@@ -875,6 +882,7 @@ struct RangeSel {
 };
 
 struct RangeLds {
+    cplx ct[4];  // the decision factors (fill_factor_table)
     u64 swb[kRangeWin + 1];
     u32 list[kRangeList];  // (parent offset in the window << 15) | (item in tile << 3) | child
     u64 tmp[kGenThreads / kWave + 1];
@@ -890,6 +898,7 @@ __device__ __attribute__((always_inline)) inline void range_walks(const PDesc *D
                                                                   const u64 *tab, const u64 *pbegin,
                                                                   const QcgdCoef &k, u64 W, const RangeSel &rs,
                                                                   RangeLds &L, Leaf &&leaf) {
+    if (threadIdx.x == 0) fill_factor_table(k, L.ct);  // (published by the tile loop's first barrier)
     for (u64 t0 = u64(blockIdx.x) * kRangeTile; t0 < W; t0 += u64(gridDim.x) * kRangeTile) {
         __syncthreads();
         if (threadIdx.x == 0) L.pfirst = cpar[t0 / kGenThreads];
@@ -968,9 +977,9 @@ __device__ __attribute__((always_inline)) inline void range_walks(const PDesc *D
                 const u64 bits = u64(j0) | (u64(u) << s);
                 u64 h;
                 if (KIND == QD_RULE_QCGD_SPLIT_MERGE) {
-                    h = sm_child(pd, tab + sm_tab_base(pbegin[p]), bits, k, wk.mag);
+                    h = sm_child(pd, tab + sm_tab_base(pbegin[p]), bits, L.ct, wk.mag);
                 } else {
-                    h = flip_child(pd, bits, k, wk.mag);
+                    h = flip_child(pd, bits, L.ct, wk.mag);
                 }
                 const u32 top = u32(h >> 52);
                 if (top >= rs.rlo && top < rs.rhi) leaf(rotl12(h), wk.mag, cb[p] + j0 + (u64(u) << s));
