@@ -412,8 +412,10 @@ __device__ inline void fill_factor_table(const QcgdCoef &k, cplx *ct) {
     ct[3] = k.dc;
 }
 
-__device__ inline u64 sm_child(const PDesc &pd, const u64 *tab, u64 bits, const cplx *ct, cplx &mag) {
+template <typename M>
+__device__ inline u64 sm_child_m(const PDesc &pd, const u64 *tab, u64 bits, const cplx *ct, cplx &mag) {
     const u32 n = pd.n;
+    const M lmask = M(pd.lmask), rmask = M(pd.rmask), dmask = M(pd.dmask);
     u64 nh = 0, lh = 0, rh = 0;
     u32 c = 0, d = 0, i = 0, end = n;
     bool overflow = false;
@@ -450,8 +452,8 @@ __device__ inline u64 sm_child(const PDesc &pd, const u64 *tab, u64 bits, const 
         i = 1;
     }
     while (i < end) {
-        const bool l = (pd.lmask >> i) & 1, r = (pd.rmask >> i) & 1;
-        const bool dec = (pd.dmask >> i) & 1;
+        const bool l = (lmask >> i) & 1, r = (rmask >> i) & 1;
+        const bool dec = (dmask >> i) & 1;
         const bool take = dec && ((bits >> d) & 1);
         d += dec ? 1u : 0u;
         const bool split = l && r;
@@ -467,6 +469,11 @@ __device__ inline u64 sm_child(const PDesc &pd, const u64 *tab, u64 bits, const 
     return finish_pm(nh, lh, rh);
 }
 
+// (32-bit masks when every node index fits)
+__device__ inline u64 sm_child(const PDesc &pd, const u64 *tab, u64 bits, const cplx *ct, cplx &mag) {
+    return pd.n <= 32 ? sm_child_m<u32>(pd, tab, bits, ct, mag) : sm_child_m<u64>(pd, tab, bits, ct, mag);
+}
+
 // One child of a flip rule walked on its own, with all its decision bits
 // (bit d: decision d, in node order).  flip_run's three sequences are
 // independent: the magnitude takes one factor per decision node in node
@@ -476,27 +483,39 @@ __device__ inline u64 sm_child(const PDesc &pd, const u64 *tab, u64 bits, const 
 // Computed apart, each over the set bits of a mask, they are the same
 // products and folds in the same order, without flip_run's per-node
 // branches, which lanes holding different children would take both ways.
-__device__ inline u64 flip_child(const PDesc &pd, u64 bits, const cplx *ct, cplx &mag) {
-    u64 dm = pd.dmask, flips = 0;
+__device__ inline u32 ctz_mask(u32 m) { return u32(__builtin_ctz(m)); }
+__device__ inline u32 ctz_mask(u64 m) { return u32(__builtin_ctzll(m)); }
+
+// (M: the masks' word, 32 bits when every node index fits, which halves the
+// bit scans' instructions)
+template <typename M>
+__device__ inline u64 flip_child_m(const PDesc &pd, u64 bits, const cplx *ct, cplx &mag) {
+    M dm = M(pd.dmask), flips = 0;
+    const M lmask = M(pd.lmask);
     u32 d = 0;
     while (dm) {
-        const u32 i = u32(__builtin_ctzll(dm));
+        const u32 i = ctz_mask(dm);
         dm &= dm - 1;
         const bool take = (bits >> d++) & 1;
-        const u32 l = u32(pd.lmask >> i) & 1u;
+        const u32 l = u32(lmask >> i) & 1u;
         mag = cmul(mag, ct[(take ? 2u : 0u) | l]);
-        flips |= u64(take) << i;
+        flips |= M(take) << i;
     }
-    u64 lm = pd.lmask ^ flips, rm = pd.rmask ^ flips, lh = 0, rh = 0;
+    M lm = lmask ^ flips, rm = M(pd.rmask) ^ flips;
+    u64 lh = 0, rh = 0;
     while (lm) {
-        lh = small_step_pm(lh, u64(__builtin_ctzll(lm)));
+        lh = small_step_pm(lh, u64(ctz_mask(lm)));
         lm &= lm - 1;
     }
     while (rm) {
-        rh = small_step_pm(rh, u64(__builtin_ctzll(rm)));
+        rh = small_step_pm(rh, u64(ctz_mask(rm)));
         rm &= rm - 1;
     }
     return finish_pm(pd.nf, lh, rh);
+}
+
+__device__ inline u64 flip_child(const PDesc &pd, u64 bits, const cplx *ct, cplx &mag) {
+    return pd.n <= 32 ? flip_child_m<u32>(pd, bits, ct, mag) : flip_child_m<u64>(pd, bits, ct, mag);
 }
 
 // The children of a work item, depth first over its t = min(k, 3) trailing

@@ -50,6 +50,7 @@ def _chain(hs, n_iter, cap, chunk_log2, two_walk=0):
     (7, 5, 3, q.UINT64_MAX, 14, 0),
     (8, 42, 3, 700, 10, 0),
     (9, 2, 3, q.UINT64_MAX, 12, 255 * 4096 + 1),
+    (40, 7, 2, 24, 16, 0),  # graphs of more than 32 nodes: the walks' 64-bit masks
 ])
 @pytest.mark.parametrize("two_walk", [0, 1])
 def test_range_passes_match_single_pass(nodes, seed, n_iter, cap, chunk, min_c, two_walk):
