@@ -887,7 +887,7 @@ struct EParts {
 constexpr u32 kRangeBins = 1024;  // first-pass buckets of a range pass, at most
 constexpr u32 kRangeTile = 4096;  // work items per tile
 constexpr u32 kRangePer = kRangeTile / kGenThreads;  // consecutive items a thread lists (16)
-constexpr u32 kRangeList = 4096;  // listed children walked per round of a tile
+constexpr u32 kRangeList = 3584;  // listed children walked per round of a tile
 constexpr u32 kRangeWin = 512;    // parents of the LDS window of wb (beyond: a global search)
 constexpr u32 kRangeGrid = 2048;  // blocks of the two walks (rows of bmat)
 static_assert(kRangePer % 8 == 0 && kRangeTile % kGenThreads == 0, "tiles of whole chunks, 8-item plane words");
@@ -1095,8 +1095,13 @@ struct GroupBases {
     Entry *g[kRangeGroupsMax];
 };
 
+// Wavefronts per SIMD of the one-walk pass: a flip-rule walk fits 7 (72
+// registers; its LDS, 22.7 KB a block, fits 7 blocks), a split_merge walk 6.
 template <int KIND>
-__global__ __launch_bounds__(kGenThreads, 6) void k_range_walkwrite(const PDesc *D, const double2 *pmag, const u64 *cb,
+constexpr u32 kRangeWalkWaves = KIND == QD_RULE_QCGD_SPLIT_MERGE ? 6u : 7u;
+
+template <int KIND>
+__global__ __launch_bounds__(kGenThreads, kRangeWalkWaves<KIND>) void k_range_walkwrite(const PDesc *D, const double2 *pmag, const u64 *cb,
                                                                  const u64 *wb, u64 N, const u64 *cpar,
                                                                  const u64 *tab, const u64 *pbegin, qd_rule rule,
                                                                  u64 W, RangeSel rs, u32 gbits, GroupBases E,

@@ -1447,10 +1447,11 @@ static int range_step(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
     const u64 cap2 = (u64(mean2 + 16.0 * std::sqrt(mean2)) + 256 + 63) / 64 * 64;
     const u64 pcap = std::max<u64>(capped ? 2 * max_obj + gcap : C, 1);  // trimmed to max_obj beyond 2 max_obj
     // (the walks' blocks: as many as are resident on the device at once,
-    // 6 per CU, so that no block starts late)
+    // one per wavefront slot of a SIMD on every CU, so that no block starts
+    // late)
     int ncu_w = 256;
     (void)hipDeviceGetAttribute(&ncu_w, hipDeviceAttributeMultiprocessorCount, ctx->device);
-    const u64 gmax = std::min<u64>(kRangeGrid, u64(std::max(ncu_w, 1)) * 6);
+    const u64 gmax = std::min<u64>(kRangeGrid, u64(std::max(ncu_w, 1)) * kRangeWalkWaves<KIND>);
     const unsigned gr = unsigned(std::max<u64>(std::min<u64>((nitems + kRangeTile - 1) / kRangeTile, gmax), 1));
     QD_TRY(ctx->bmat.ensure(u64(gr) * nb1 * sizeof(u32)));
     QD_TRY(ctx->off1.ensure((nb1 + 1) * sizeof(u64)));
