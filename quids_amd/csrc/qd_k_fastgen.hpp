@@ -391,14 +391,6 @@ __device__ inline void flip_run(W &w, const PDesc &pd, BT bits, u32 dlimit, cons
     }
 }
 
-// One split_merge child walked on its own, with all its decision bits (bit
-// d: decision d): sm_run and sm_finish with the per-node cases folded into
-// selects, so that lanes holding different children run one path: a
-// decision node multiplies the magnitude by the factor its case and its
-// decision pick; every node emits its first root (its own name tree, or,
-// taken, the split's left half or the merge), a taken split also its right
-// half, and a taken merge skips the next node.  The same products, roots and
-// folds in the same order as sm_run.
 // The four factors of a decision in a table (LDS) indexed by
 // (taken << 1) | x: x = 0 -> do_not / do_, x = 1 -> -do_not_conj / do_conj.
 // A flip-rule decision node has x = its left bit; a split_merge operation
@@ -412,6 +404,14 @@ __device__ inline void fill_factor_table(const QcgdCoef &k, cplx *ct) {
     ct[3] = k.dc;
 }
 
+// One split_merge child walked on its own, with all its decision bits (bit
+// d: decision d): sm_run and sm_finish with the per-node cases folded into
+// selects, so that lanes holding different children run one path: a
+// decision node multiplies the magnitude by the factor its case and its
+// decision pick; every node emits its first root (its own name tree, or,
+// taken, the split's left half or the merge), a taken split also its right
+// half, and a taken merge skips the next node.  The same products, roots and
+// folds in the same order as sm_run.
 template <typename M>
 __device__ inline u64 sm_child_m(const PDesc &pd, const u64 *tab, u64 bits, const cplx *ct, cplx &mag) {
     const u32 n = pd.n;
