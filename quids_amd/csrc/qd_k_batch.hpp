@@ -21,7 +21,8 @@ constexpr u32 kMaxInst = 4096;                // instances per batched call (LDS
 constexpr u32 kNoInst = ~0u;
 
 // largest i < n with seg[i] <= x (seg holds n + 1 ascending offsets)
-__device__ inline u32 find_seg(const u64 *seg, u32 n, u64 x) {
+template <class T>
+__device__ inline u32 find_seg(const T *seg, u32 n, u64 x) {
     u32 lo = 0, hi = n - 1;
     while (lo < hi) {
         const u32 mid = (lo + hi + 1) >> 1;
@@ -174,6 +175,55 @@ __global__ __launch_bounds__(kSegScatterThreads) void k_seg_scatter(const u16 *s
     }
 }
 
+// With plain keys (UK) and fewer than 2^32 candidates: the per-instance
+// unique counts, the slots' instances and the first key list in one pass
+// over the slots.  Every unique's record goes into its instance's range of
+// the list, which starts at the instance's first candidate (relative to
+// instance 0's; an instance has no more uniques than candidates), as 8
+// bytes: the top 32 key bits over the slot.  cursor[i] starts at that range
+// and ends one past its last record, so instance i holds cursor[i] minus
+// its start uniques.  Records of instances that keep every unique are
+// written too and never read.
+__global__ __launch_bounds__(kSegScatterThreads) void k_seg_scatter_first(const u64 *UR, const u32 *UK, u64 nslots,
+                                                                          const u64 *cseg, u32 n, u64 *cursor,
+                                                                          u16 *sinst, u64 *list8) {
+    __shared__ u32 cnt[kMaxInst];
+    __shared__ u32 base[kMaxInst];
+    __shared__ u32 sseg[kMaxInst + 1];
+    const u64 c0 = cseg[0];
+    for (u32 i = threadIdx.x; i <= n; i += blockDim.x) sseg[i] = u32(cseg[i] - c0);
+    constexpr int kPer = kSegTile / kSegScatterThreads;
+    for (u64 t0 = u64(blockIdx.x) * kSegTile; t0 < nslots; t0 += u64(gridDim.x) * kSegTile) {
+        for (u32 i = threadIdx.x; i < n; i += blockDim.x) cnt[i] = 0;
+        __syncthreads();
+        u32 inst[kPer], rank[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const u64 s = t0 + threadIdx.x + u64(q) * kSegScatterThreads;
+            inst[q] = kNoInst;
+            if (s < nslots) {
+                const u64 r = UR[s];
+                if (r != kNoRep) {
+                    inst[q] = find_seg(sseg, n, r - c0);
+                    rank[q] = atomicAdd(&cnt[inst[q]], 1u);
+                }
+                sinst[s] = inst[q] == kNoInst ? kNoSlotInst : u16(inst[q]);
+            }
+        }
+        __syncthreads();
+        for (u32 i = threadIdx.x; i < n; i += blockDim.x)
+            if (cnt[i]) base[i] = u32(atomicAdd(reinterpret_cast<unsigned long long *>(cursor + i), (unsigned long long)cnt[i]));
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kPer; ++q)
+            if (inst[q] != kNoInst) {
+                const u64 s = t0 + threadIdx.x + u64(q) * kSegScatterThreads;
+                list8[base[inst[q]] + rank[q]] = (u64(UK[s]) << 32) | (s & 0xFFFFFFFFull);
+            }
+        __syncthreads();
+    }
+}
+
 // Tile of a key-record list: records [begin, end) of instance inst.
 struct SegTile {
     u64 begin, end;
@@ -199,8 +249,19 @@ struct SegLoop {
     const double2 *UM;
     SelKey sk;
     u32 n, ntiles;
-    u32 part;  // the first list's records hold the top 32 key bits only (UK)
+    u32 part;    // the first list's records hold the top 32 key bits only (UK)
+    u32 narrow;  // the first list holds k_seg_scatter_first's 8-byte records
 };
+
+// Record k of a round's list; the first round reads the 8-byte records when
+// SegLoop::narrow is set.
+__device__ inline SegItem seg_item(const SegItem *list, u64 k, bool narrow) {
+    if (narrow) {
+        const u64 v = reinterpret_cast<const u64 *>(list)[k];
+        return SegItem{v & ~0xFFFFFFFFull, v & 0xFFFFFFFFull};
+    }
+    return list[k];
+}
 
 // A record's full high key word when the current digit (bits [pos, pos +
 // 12)) reaches past what it holds.
@@ -260,9 +321,10 @@ __global__ __launch_bounds__(1024) void k_seg_tiles(SegLoop *L) {
 // One workgroup per tile of an active instance: LDS histogram of the 12
 // key bits at the instance's position, added into its global row.  Every
 // record of the list carries the instance's current prefix.
-__global__ __launch_bounds__(256) void k_seg_hist(const SegLoop *L, u32 parity) {
+__global__ __launch_bounds__(256) void k_seg_hist(const SegLoop *L, u32 parity, u32 first) {
     __shared__ u32 h[kHistBins];
     const SegItem *list = L->list[parity];
+    const bool narrow = first && L->narrow;
     const u32 ntiles = L->ntiles;
     for (u32 t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const SegTile tl = L->tiles[t];
@@ -271,7 +333,7 @@ __global__ __launch_bounds__(256) void k_seg_hist(const SegLoop *L, u32 parity) 
         for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) h[i] = 0;
         __syncthreads();
         for (u64 k = tl.begin + threadIdx.x; k < tl.end; k += blockDim.x) {
-            const SegItem it = list[k];
+            const SegItem it = seg_item(list, k, narrow);
             const u64 hi = seg_hi_at(L, it, tl.inst, q.pos + 12);
             const u64 lo = q.pos + 12 > 64 ? seg_lo(L->UH, it.slot, tl.inst) : 0;
             atomicAdd(&h[key_digit(hi, lo, q.pos)], 1u);
@@ -287,10 +349,11 @@ __global__ __launch_bounds__(256) void k_seg_hist(const SegLoop *L, u32 parity) 
 // After a pick: carries the records that match the instance's new prefix
 // (the tied ones) into the next list, at the instance's cursor; one
 // reservation per tile.
-__global__ __launch_bounds__(256) void k_seg_collect(const SegLoop *L, u32 parity) {
+__global__ __launch_bounds__(256) void k_seg_collect(const SegLoop *L, u32 parity, u32 first) {
     __shared__ u64 tmp[256 / kWave + 1];
     __shared__ u64 base;
     const SegItem *list = L->list[parity];
+    const bool narrow = first && L->narrow;
     SegItem *out = L->list[parity ^ 1u];
     const u32 ntiles = L->ntiles;
     for (u32 t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -302,7 +365,7 @@ __global__ __launch_bounds__(256) void k_seg_collect(const SegLoop *L, u32 parit
             bool take = false;
             SegItem it{};
             if (k < tl.end) {
-                it = list[k];
+                it = seg_item(list, k, narrow);
                 it.hi = seg_hi_at(L, it, tl.inst, q.pos);  // (carried in full from here on)
                 take = prefix_eq(it.hi, q.pos > 64 ? seg_lo(L->UH, it.slot, tl.inst) : 0, q);
             }

@@ -495,10 +495,11 @@ static int run_rounds(qd_ctx *ctx, qd_ctx::SelGraph &g, const void *L, Launch &&
 static int launch_seg_rounds(hipStream_t s, SegLoop *L) {
     for (u32 k = 0; k < 12; ++k) {
         hipLaunchKernelGGL(k_seg_tiles, dim3(1), dim3(1024), 0, s, L);
-        hipLaunchKernelGGL(k_seg_hist, dim3(kSegLoopGrid), dim3(256), 0, s, (const SegLoop *)L, k & 1u);
+        const u32 first = k == 0 ? 1u : 0u;
+        hipLaunchKernelGGL(k_seg_hist, dim3(kSegLoopGrid), dim3(256), 0, s, (const SegLoop *)L, k & 1u, first);
         hipLaunchKernelGGL(k_seg_pick, dim3(256), dim3(256), 0, s, L);
         hipLaunchKernelGGL(k_seg_next, dim3(1), dim3(1024), 0, s, L);
-        hipLaunchKernelGGL(k_seg_collect, dim3(kSegLoopGrid), dim3(256), 0, s, (const SegLoop *)L, k & 1u);
+        hipLaunchKernelGGL(k_seg_collect, dim3(kSegLoopGrid), dim3(256), 0, s, (const SegLoop *)L, k & 1u, first);
     }
     QD_HIP(hipGetLastError());
     return QD_OK;
@@ -1818,16 +1819,31 @@ static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const
                         const u32 *UK, u64 *bitmap, u64 *sbm) {
     hipStream_t s = ctx->stream;
     const u32 n = bt->n;
-    QD_TRY(ctx->bcount.ensure(n * sizeof(u64)));
     const u64 *cseg = ctx->bseg.as<u64>();
-    QD_HIP(hipMemsetAsync(ctx->bcount.p, 0, n * sizeof(u64), s));
     QD_TRY(ctx->bsinst.ensure(std::max<u64>(nslots, 1) * sizeof(u16)));
     const u16 *sinst = ctx->bsinst.as<u16>();
-    hipLaunchKernelGGL(k_seg_count, dim3(grid_for(nslots, 256, 8192)), dim3(256), 0, s, UR, nslots, cseg, n,
-                       ctx->bcount.as<u64>(), ctx->bsinst.as<u16>());
-    QD_HIP(hipGetLastError());
     bt->U.resize(n);
-    QD_TRY(sync_read(ctx, ctx->bcount.p, n * sizeof(u64), bt->U.data()));
+    // plain keys: counts, slot instances and the first list (8-byte records,
+    // in the first pass's buffer, at least 32 bytes per slot) in one pass
+    const bool narrow = UK && nslots < (1ull << 32);
+    if (narrow) {
+        QD_TRY(ctx->bcur.ensure(n * sizeof(u64)));
+        hipLaunchKernelGGL(k_add_offset, dim3(grid_for(n)), dim3(256), 0, s, cseg, u64(n), u64(0) - bt->cseg[0],
+                           ctx->bcur.as<u64>());
+        const unsigned sg = unsigned(std::min<u64>((nslots + kSegTile - 1) / kSegTile, 16384));
+        hipLaunchKernelGGL(k_seg_scatter_first, dim3(std::max(sg, 1u)), dim3(kSegScatterThreads), 0, s, UR, UK,
+                           nslots, cseg, n, ctx->bcur.as<u64>(), ctx->bsinst.as<u16>(), ctx->ebuf1.as<u64>());
+        QD_HIP(hipGetLastError());
+        QD_TRY(sync_read(ctx, ctx->bcur.p, n * sizeof(u64), bt->U.data()));
+        for (u32 i = 0; i < n; ++i) bt->U[i] -= bt->cseg[i] - bt->cseg[0];
+    } else {
+        QD_TRY(ctx->bcount.ensure(n * sizeof(u64)));
+        QD_HIP(hipMemsetAsync(ctx->bcount.p, 0, n * sizeof(u64), s));
+        hipLaunchKernelGGL(k_seg_count, dim3(grid_for(nslots, 256, 8192)), dim3(256), 0, s, UR, nslots, cseg, n,
+                           ctx->bcount.as<u64>(), ctx->bsinst.as<u16>());
+        QD_HIP(hipGetLastError());
+        QD_TRY(sync_read(ctx, ctx->bcount.p, n * sizeof(u64), bt->U.data()));
+    }
     std::vector<SegSel> sel(n);
     std::vector<u32> trunc(n, 0);
     std::vector<u64> off(n, 0), cnt(n, 0);
@@ -1838,7 +1854,7 @@ static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const
             q.r = bt->caps[i];
             q.active = 1;
             trunc[i] = 1;
-            off[i] = ntrunc;
+            off[i] = narrow ? bt->cseg[i] - bt->cseg[0] : ntrunc;
             cnt[i] = bt->U[i];
             ntrunc += bt->U[i];
         } else {
@@ -1849,18 +1865,19 @@ static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const
     QD_TRY(upload(ctx, ctx->bsel, sel.data(), n * sizeof(SegSel)));
     SegSel *dsel = ctx->bsel.as<SegSel>();
     if (ntrunc) {
-        QD_TRY(upload(ctx, ctx->btrunc, trunc.data(), n * sizeof(u32)));
-        QD_TRY(upload(ctx, ctx->bcur, off.data(), n * sizeof(u64)));
         // the key records go where the partition's entries were (dead by now):
         // the first list in the first pass's buffer (32 bytes per slot), the
         // next rounds' in the second pass's
         QD_TRY(ctx->ebuf2.ensure(ntrunc * sizeof(SegItem)));
         QD_TRY(ctx->bhist.ensure(u64(n) * kHistBins * sizeof(u32)));
         SegItem *cur = ctx->ebuf1.as<SegItem>(), *nxt = ctx->ebuf2.as<SegItem>();
-        const unsigned sg = unsigned(std::min<u64>((nslots + kSegTile - 1) / kSegTile, 16384));
-        hipLaunchKernelGGL(k_seg_scatter, dim3(std::max(sg, 1u)), dim3(kSegScatterThreads), 0, s, sinst, nslots, UH,
-                           UM, UK,
-                           ctx->btrunc.as<u32>(), n, ctx->bcur.as<u64>(), cur, bt->sk);
+        if (!narrow) {
+            QD_TRY(upload(ctx, ctx->btrunc, trunc.data(), n * sizeof(u32)));
+            QD_TRY(upload(ctx, ctx->bcur, off.data(), n * sizeof(u64)));
+            const unsigned sg = unsigned(std::min<u64>((nslots + kSegTile - 1) / kSegTile, 16384));
+            hipLaunchKernelGGL(k_seg_scatter, dim3(std::max(sg, 1u)), dim3(kSegScatterThreads), 0, s, sinst, nslots,
+                               UH, UM, UK, ctx->btrunc.as<u32>(), n, ctx->bcur.as<u64>(), cur, bt->sk);
+        }
         QD_HIP(hipMemsetAsync(ctx->bhist.p, 0, u64(n) * kHistBins * sizeof(u32), s));
         // the digit rounds, driven from the device (SegLoop)
         QD_TRY(upload(ctx, ctx->bsoff, off.data(), n * sizeof(u64)));
@@ -1881,6 +1898,7 @@ static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const
         h.UM = UM;
         h.sk = bt->sk;
         h.part = UK ? 1u : 0u;
+        h.narrow = narrow ? 1u : 0u;
         h.n = n;
         static_assert(sizeof(SegLoop) <= 8 * (kHostSelLoop - kHostSegLoop), "staging area");
         std::memcpy(ctx->host + kHostSegLoop, &h, sizeof h);
