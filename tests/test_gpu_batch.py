@@ -22,9 +22,10 @@ def _same(a, b, what):
 
 
 def _run(specs, n_iter, caps):
-    """specs: (nodes, seed, theta1, phi1, theta2, phi2) per instance."""
+    """specs: (nodes, seed, theta1, phi1, theta2, phi2) per instance; an
+    instance of 0 nodes starts with no object at all."""
     print(f"batch instances={len(specs)} iters={n_iter} caps={caps}", flush=True)
-    states = [HostState.from_objects([qcgd.seeded_graph(n, s)]) for n, s, *_ in specs]
+    states = [HostState.from_objects([qcgd.seeded_graph(n, s)] if n else []) for n, s, *_ in specs]
     a, b = batch_t.from_states(states), batch_t()
     plain = []
     for hs in states:
@@ -69,6 +70,17 @@ def test_batch_matches_plain_truncated_uneven():
                       float(rng.uniform(0, math.pi))))
     caps = [int(c) for c in rng.choice([7, 50, 300, 2000, 10 ** 6], size=len(specs))]
     _run(specs, 4, caps)
+
+
+def test_empty_instances():
+    """Instances with no object first, between truncated ones and last, then
+    a batch of empty instances only: every instance as its own plain step
+    (an empty state steps to an empty state of total probability 0)."""
+    specs = [(0, 0, 0.7, 0.3, 1.1, 0.4), (5, 1, 0.7, 0.3, 1.1, 0.4), (0, 0, 0.2, 1.3, 0.5, 0.9),
+             (6, 2, math.pi / 4, 0.0, math.pi / 3, math.pi / 5), (4, 5, 1.2, 0.5, 0.3, 2.5),
+             (0, 0, 0.2, 1.3, 0.5, 0.9)]
+    _run(specs, 3, [10, 20, 10, 30, 10 ** 6, 10])
+    _run(specs[:1] + specs[2:3], 1, 10)
 
 
 def test_equal_objects_of_different_instances_do_not_merge():
