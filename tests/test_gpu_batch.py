@@ -96,6 +96,17 @@ def test_many_small_instances():
     _run(specs, 2, 256)
 
 
+def test_most_instances():
+    """4096 instances, the most one call takes (the select's per-instance
+    counters in LDS and its instance scans over chunks of 1024 all full),
+    most of them truncated."""
+    rng = np.random.default_rng(13)
+    specs = [(int(rng.integers(2, 6)), int(rng.integers(0, 10 ** 6)), float(rng.uniform(0, 1.5)),
+              float(rng.uniform(0, 3)), float(rng.uniform(0, 1.5)), float(rng.uniform(0, 3))) for _ in range(4096)]
+    caps = [int(c) for c in rng.choice([2, 3, 5, 10 ** 6], size=len(specs))]
+    _run(specs, 1, caps)
+
+
 def test_batch_split_into_chunks():
     """A candidate budget of 2^10 per launch sequence splits the batch into
     many chunks (concatenated on the device): results are unchanged."""
