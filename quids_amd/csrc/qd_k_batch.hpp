@@ -184,7 +184,10 @@ __global__ __launch_bounds__(kSegScatterThreads) void k_seg_scatter(const u16 *s
 // and ends one past its last record, so instance i holds cursor[i] minus
 // its start uniques.  Records of instances that keep every unique are
 // written too and never read.
-__global__ __launch_bounds__(kSegScatterThreads) void k_seg_scatter_first(const u64 *UR, const u32 *UK, u64 nslots,
+constexpr int kSegFirstThreads = 512;
+constexpr int kSegFirstTile = 8192;
+static_assert(kSegFirstTile <= 65536 && kMaxInst < 65536, "instance and rank share a 32-bit word");
+__global__ __launch_bounds__(kSegFirstThreads) void k_seg_scatter_first(const u64 *UR, const u32 *UK, u64 nslots,
                                                                           const u64 *cseg, u32 n, u64 *cursor,
                                                                           u16 *sinst, u64 *list8) {
     __shared__ u32 cnt[kMaxInst];
@@ -192,22 +195,24 @@ __global__ __launch_bounds__(kSegScatterThreads) void k_seg_scatter_first(const 
     __shared__ u32 sseg[kMaxInst + 1];
     const u64 c0 = cseg[0];
     for (u32 i = threadIdx.x; i <= n; i += blockDim.x) sseg[i] = u32(cseg[i] - c0);
-    constexpr int kPer = kSegTile / kSegScatterThreads;
-    for (u64 t0 = u64(blockIdx.x) * kSegTile; t0 < nslots; t0 += u64(gridDim.x) * kSegTile) {
+    constexpr int kPer = kSegFirstTile / kSegFirstThreads;
+    for (u64 t0 = u64(blockIdx.x) * kSegFirstTile; t0 < nslots; t0 += u64(gridDim.x) * kSegFirstTile) {
         for (u32 i = threadIdx.x; i < n; i += blockDim.x) cnt[i] = 0;
         __syncthreads();
-        u32 inst[kPer], rank[kPer];
+        u32 ir[kPer];  // instance << 16 | rank in the tile; kNoInst for an empty slot
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
-            const u64 s = t0 + threadIdx.x + u64(q) * kSegScatterThreads;
-            inst[q] = kNoInst;
+            const u64 s = t0 + threadIdx.x + u64(q) * kSegFirstThreads;
+            ir[q] = kNoInst;
             if (s < nslots) {
                 const u64 r = UR[s];
+                u16 si = kNoSlotInst;
                 if (r != kNoRep) {
-                    inst[q] = find_seg(sseg, n, r - c0);
-                    rank[q] = atomicAdd(&cnt[inst[q]], 1u);
+                    const u32 inst = find_seg(sseg, n, r - c0);
+                    ir[q] = (inst << 16) | atomicAdd(&cnt[inst], 1u);
+                    si = u16(inst);
                 }
-                sinst[s] = inst[q] == kNoInst ? kNoSlotInst : u16(inst[q]);
+                sinst[s] = si;
             }
         }
         __syncthreads();
@@ -216,9 +221,9 @@ __global__ __launch_bounds__(kSegScatterThreads) void k_seg_scatter_first(const 
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < kPer; ++q)
-            if (inst[q] != kNoInst) {
-                const u64 s = t0 + threadIdx.x + u64(q) * kSegScatterThreads;
-                list8[base[inst[q]] + rank[q]] = (u64(UK[s]) << 32) | (s & 0xFFFFFFFFull);
+            if (ir[q] != kNoInst) {
+                const u64 s = t0 + threadIdx.x + u64(q) * kSegFirstThreads;
+                list8[base[ir[q] >> 16] + (ir[q] & 0xFFFFu)] = (u64(UK[s]) << 32) | (s & 0xFFFFFFFFull);
             }
         __syncthreads();
     }
@@ -347,35 +352,64 @@ __global__ __launch_bounds__(256) void k_seg_hist(const SegLoop *L, u32 parity, 
 }
 
 // After a pick: carries the records that match the instance's new prefix
-// (the tied ones) into the next list, at the instance's cursor; one
-// reservation per tile.
+// (the tied ones) into the next list, at the instance's cursor.  Each
+// thread holds kCollectPer records of a chunk of the tile; each store
+// instruction's lanes write side by side (wavefront ballots), and the
+// block makes one reservation per chunk.
+constexpr int kCollectPer = 16;
 __global__ __launch_bounds__(256) void k_seg_collect(const SegLoop *L, u32 parity, u32 first) {
-    __shared__ u64 tmp[256 / kWave + 1];
+    constexpr int kWaves = 256 / kWave;
+    __shared__ u64 wsum[kWaves];
     __shared__ u64 base;
     const SegItem *list = L->list[parity];
-    const bool narrow = first && L->narrow;
     SegItem *out = L->list[parity ^ 1u];
+    const bool narrow = first && L->narrow;
     const u32 ntiles = L->ntiles;
+    const u32 lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const u64 below = (1ull << lane) - 1;
     for (u32 t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const SegTile tl = L->tiles[t];
         const SegSel q = L->sel[tl.inst];
         if (!q.active) continue;  // uniform over the block
-        for (u64 k0 = tl.begin; k0 < tl.end; k0 += blockDim.x) {
-            const u64 k = k0 + threadIdx.x;
-            bool take = false;
-            SegItem it{};
-            if (k < tl.end) {
-                it = seg_item(list, k, narrow);
-                it.hi = seg_hi_at(L, it, tl.inst, q.pos);  // (carried in full from here on)
-                take = prefix_eq(it.hi, q.pos > 64 ? seg_lo(L->UH, it.slot, tl.inst) : 0, q);
+        for (u64 k0 = tl.begin; k0 < tl.end; k0 += 256 * kCollectPer) {
+            SegItem it[kCollectPer];
+            u64 m[kCollectPer];
+            u64 wtot = 0;
+#pragma unroll
+            for (int j = 0; j < kCollectPer; ++j) {
+                const u64 k = k0 + threadIdx.x + 256 * j;
+                bool take = false;
+                it[j] = SegItem{0, 0};
+                if (k < tl.end) {
+                    it[j] = seg_item(list, k, narrow);
+                    it[j].hi = seg_hi_at(L, it[j], tl.inst, q.pos);  // (carried in full from here on)
+                    take = prefix_eq(it[j].hi, q.pos > 64 ? seg_lo(L->UH, it[j].slot, tl.inst) : 0, q);
+                }
+                m[j] = __ballot(take);
+                wtot += u64(__popcll(m[j]));
             }
-            u64 total;
-            const u64 off = block_excl_scan(take ? 1 : 0, tmp, total);
-            if (threadIdx.x == 0 && total)
-                base = atomicAdd(reinterpret_cast<unsigned long long *>(L->cursor + tl.inst),
-                                 (unsigned long long)total);
+            if (lane == 0) wsum[wid] = wtot;
             __syncthreads();
-            if (take) out[base + off] = it;
+            if (threadIdx.x == 0) {
+                u64 acc = 0;
+                for (int w = 0; w < kWaves; ++w) {
+                    const u64 c = wsum[w];
+                    wsum[w] = acc;
+                    acc += c;
+                }
+                if (acc)
+                    base = atomicAdd(reinterpret_cast<unsigned long long *>(L->cursor + tl.inst),
+                                     (unsigned long long)acc);
+            }
+            __syncthreads();
+            if (wtot) {  // uniform over the wavefront
+                u64 pos = base + wsum[wid];
+#pragma unroll
+                for (int j = 0; j < kCollectPer; ++j) {
+                    if ((m[j] >> lane) & 1) out[pos + u64(__popcll(m[j] & below))] = it[j];
+                    pos += u64(__popcll(m[j]));
+                }
+            }
             __syncthreads();
         }
     }

@@ -1830,8 +1830,8 @@ static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const
         QD_TRY(ctx->bcur.ensure(n * sizeof(u64)));
         hipLaunchKernelGGL(k_add_offset, dim3(grid_for(n)), dim3(256), 0, s, cseg, u64(n), u64(0) - bt->cseg[0],
                            ctx->bcur.as<u64>());
-        const unsigned sg = unsigned(std::min<u64>((nslots + kSegTile - 1) / kSegTile, 16384));
-        hipLaunchKernelGGL(k_seg_scatter_first, dim3(std::max(sg, 1u)), dim3(kSegScatterThreads), 0, s, UR, UK,
+        const unsigned sg = unsigned(std::min<u64>((nslots + kSegFirstTile - 1) / kSegFirstTile, 16384));
+        hipLaunchKernelGGL(k_seg_scatter_first, dim3(std::max(sg, 1u)), dim3(kSegFirstThreads), 0, s, UR, UK,
                            nslots, cseg, n, ctx->bcur.as<u64>(), ctx->bsinst.as<u16>(), ctx->ebuf1.as<u64>());
         QD_HIP(hipGetLastError());
         QD_TRY(sync_read(ctx, ctx->bcur.p, n * sizeof(u64), bt->U.data()));
