@@ -519,9 +519,20 @@ __global__ __launch_bounds__(1024) void k_seg_next(SegLoop *L) {
 
 // Marks the survivors: every unique of an instance that is not truncated,
 // and the keys >= prefix of a truncated one.  Lanes hold consecutive slots.
+// With UK each instance's top 32 threshold bits sit in LDS (dynamic, n
+// words; 0 for an instance that keeps everything: every key passes a zero
+// threshold), so that a lane reads its instance's threshold from the block
+// and the full key and threshold only when the top 32 bits tie.
 __global__ __launch_bounds__(256) void k_seg_mark(const u16 *sinst, const u64 *UH, const u64 *UR, const double2 *UM,
-                                                  const u32 *UK, u64 nslots, const SegSel *sel, u64 *bitmap, u64 *sbm,
-                                                  SelKey sk) {
+                                                  const u32 *UK, u64 nslots, const SegSel *sel, u32 n, u64 *bitmap,
+                                                  u64 *sbm, SelKey sk) {
+    extern __shared__ u32 qtop[];
+    if (UK)
+        for (u32 i = threadIdx.x; i < n; i += blockDim.x) {
+            const SegSel q = sel[i];
+            qtop[i] = q.keepall ? 0u : u32(q.hi >> 32);
+        }
+    __syncthreads();
     const u64 stride = u64(gridDim.x) * blockDim.x;
     for (u64 base = u64(blockIdx.x) * blockDim.x; base < nslots; base += stride) {
         const u64 s = base + threadIdx.x;
@@ -529,14 +540,24 @@ __global__ __launch_bounds__(256) void k_seg_mark(const u16 *sinst, const u64 *U
         if (s < nslots) {
             const u32 inst = slot_inst16(sinst, s);
             if (inst != kNoInst) {
-                const SegSel q = sel[inst];
-                if (q.keepall) keep = true;
-                else {
-                    // the top 32 key bits decide unless they tie (UK: plain keys)
-                    const u32 top = UK ? UK[s] : 0u, qtop = u32(q.hi >> 32);
-                    if (UK && top != qtop) {
-                        keep = top > qtop;
+                if (UK) {
+                    // the top 32 key bits decide unless they tie (plain keys)
+                    const u32 top = UK[s], qt = qtop[inst];
+                    if (top != qt) {
+                        keep = top > qt;
                     } else {
+                        const SegSel q = sel[inst];
+                        if (q.keepall) keep = true;
+                        else {
+                            u64 hi, lo;
+                            seg_key(UH, UM, s, inst, sk, hi, lo);
+                            keep = hi > q.hi || (hi == q.hi && lo >= q.lo);
+                        }
+                    }
+                } else {
+                    const SegSel q = sel[inst];
+                    if (q.keepall) keep = true;
+                    else {
                         u64 hi, lo;
                         seg_key(UH, UM, s, inst, sk, hi, lo);
                         keep = hi > q.hi || (hi == q.hi && lo >= q.lo);

@@ -1908,8 +1908,8 @@ static int batch_select(qd_ctx *ctx, Batch *bt, u64 nslots, const u64 *UH, const
         for (u32 i = 0; i < n; ++i)
             if (sel[i].err || sel[i].active) return fail(QD_ERR_HIP, "batched select did not converge");
     }
-    hipLaunchKernelGGL(k_seg_mark, dim3(grid_for(nslots, 256, 16384)), dim3(256), 0, s, sinst, UH, UR, UM, UK,
-                       nslots, (const SegSel *)dsel, bitmap, sbm, bt->sk);
+    hipLaunchKernelGGL(k_seg_mark, dim3(grid_for(nslots, 256, 16384)), dim3(256), UK ? n * sizeof(u32) : 0, s, sinst,
+                       UH, UR, UM, UK, nslots, (const SegSel *)dsel, n, bitmap, sbm, bt->sk);
     QD_HIP(hipGetLastError());
     return QD_OK;
 }
