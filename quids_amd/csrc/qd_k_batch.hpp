@@ -183,7 +183,10 @@ __global__ __launch_bounds__(kSegScatterThreads) void k_seg_scatter(const u16 *s
 // bytes: the top 32 key bits over the slot.  cursor[i] starts at that range
 // and ends one past its last record, so instance i holds cursor[i] minus
 // its start uniques.  Records of instances that keep every unique are
-// written too and never read.
+// written too and never read.  Tiles of 8192 slots per 512-thread block
+// (one cursor reservation per instance per tile): several blocks share a
+// CU, so that one block's barriers and reservations overlap another's
+// loads; each slot's instance and rank share one register.
 constexpr int kSegFirstThreads = 512;
 constexpr int kSegFirstTile = 8192;
 static_assert(kSegFirstTile <= 65536 && kMaxInst < 65536, "instance and rank share a 32-bit word");
