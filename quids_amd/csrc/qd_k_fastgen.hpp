@@ -40,7 +40,8 @@ constexpr u32 ERR_WIDE = 8u;
 // node: SEG = 64, or 32 / 16 when two / four parents share a wavefront; the
 // segment's lanes are active together).  have: the segment
 // has a parent.  Ballots and shuffles stay within the segment.
-template <int KIND, int SEG>
+// NC_ONLY: the child count alone (nc; D, wi and tab untouched).
+template <int KIND, int SEG, bool NC_ONLY = false>
 __device__ inline void prep_parent(const Parents &P, u64 p, bool have, const u64 *keep, PDesc *D, u32 *nc, u32 *wi,
                                    u64 *tab, u32 &local_max, u32 *err) {
     const u32 lane = threadIdx.x & (kWave - 1);
@@ -89,8 +90,10 @@ __device__ inline void prep_parent(const Parents &P, u64 p, bool have, const u64
     if (!ok) {
         if (sl == 0) {
             nc[p] = 0;
-            wi[p] = 0;
-            D[p] = d;
+            if (!NC_ONLY) {
+                wi[p] = 0;
+                D[p] = d;
+            }
         }
         return;
     }
@@ -111,10 +114,12 @@ __device__ inline void prep_parent(const Parents &P, u64 p, bool have, const u64
         const bool fs = (lm & 1) && (rm & 1);
         const bool lmg = !fs && n > 1 && (rm & 1) && ((lm >> (n - 1)) & 1) && !((rm >> (n - 1)) & 1);
         d.flags = u8((fs ? PD_FS : 0) | (lmg ? PD_LM : 0));
-        if (sl == 0 && sm_first_hmlz(g)) d.flags |= PD_HMLZ;
-        d.flags = u8(__shfl(int(d.flags), int(sbase), kWave));
+        if (!NC_ONLY) {
+            if (sl == 0 && sm_first_hmlz(g)) d.flags |= PD_HMLZ;
+            d.flags = u8(__shfl(int(d.flags), int(sbase), kWave));
+        }
         k = u32(__popcll(d.dmask)) + (lmg ? 1 : 0);
-        if (in && tab) {  // the slots of sm_fill_table (tab null: filled by a later launch)
+        if (!NC_ONLY && in && tab) {  // the slots of sm_fill_table (tab null: filled by a later launch)
             u64 *t = tab + sm_tab_base(P.begin[p]) + kSmSlots * sl;
             const u32 b = g.name_begin(sl);
             t[0] = g.sub_hash(b);
@@ -126,19 +131,23 @@ __device__ inline void prep_parent(const Parents &P, u64 p, bool have, const u64
         const bool coin = KIND == QD_RULE_QCGD_COIN;
         d.dmask = sballot(in && (coin ? (Li != Ri) : (Li == Ri)));
         k = u32(__popcll(d.dmask));
-        const u64 hr = in ? g.sub_hash(g.name_begin(sl)) : 0;
-        u64 nf = 0;
-        for (u32 i = 0; i < n; ++i) nf = hash_combine(nf, __shfl(hr, int(sbase + i), kWave));
-        d.nf = nf * kMul;  // pre-multiplied, as the walks' folds carry it
+        if (!NC_ONLY) {
+            const u64 hr = in ? g.sub_hash(g.name_begin(sl)) : 0;
+            u64 nf = 0;
+            for (u32 i = 0; i < n; ++i) nf = hash_combine(nf, __shfl(hr, int(sbase + i), kWave));
+            d.nf = nf * kMul;  // pre-multiplied, as the walks' folds carry it
+        }
         local_max = max(local_max, sz);
     }
     d.k = u8(k);
     d.nc = k >= 32 ? 0u : (1u << k);
     if (sl == 0) {
-        const u32 t = min(k, kItemBits);
         nc[p] = d.nc;
-        wi[p] = d.nc >> t;
-        D[p] = d;
+        if (!NC_ONLY) {
+            const u32 t = min(k, kItemBits);
+            wi[p] = d.nc >> t;
+            D[p] = d;
+        }
     }
 }
 
@@ -169,6 +178,34 @@ __global__ __launch_bounds__(256) void k_prep_fast(Parents P, const u64 *keep, P
     }
     for (int off = kWave / 2; off > 0; off >>= 1) local_max = max(local_max, (u32)__shfl_xor(int(local_max), off, kWave));
     if (lane == 0) raise_max(maxsz, local_max);
+}
+
+// The child counts alone, in k_prep_fast's segments (the batched step's
+// planning pass): the same checks and counts as the step's prep.  A graph
+// of more than 64 nodes raises ERR_WIDE and counts 0; the caller then
+// counts with k_num_child.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_num_child_fast(Parents P, u32 *nc, u32 *err) {
+    const u32 lane = threadIdx.x & (kWave - 1);
+    const u64 nwaves = u64(gridDim.x) * (blockDim.x / kWave);
+    u32 local_max = 0;
+    for (u64 p0 = 4 * (u64(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave); p0 < P.n; p0 += 4 * nwaves) {
+        u32 nq = 0;
+        if (lane < 4 && p0 + lane < P.n && P.size[p0 + lane] >= 4) nq = ld16(P.obj + P.begin[p0 + lane]);
+        if (__ballot(nq > 16) == 0) {
+            const u64 p = p0 + lane / 16;
+            prep_parent<KIND, 16, true>(P, p, p < P.n, nullptr, nullptr, nc, nullptr, nullptr, local_max, err);
+        } else if (__ballot(nq > 32) == 0) {
+            for (u32 h = 0; h < 2; ++h) {
+                const u64 p = p0 + 2 * h + lane / 32;
+                prep_parent<KIND, 32, true>(P, p, p < P.n, nullptr, nullptr, nc, nullptr, nullptr, local_max, err);
+            }
+        } else {
+            for (u32 q = 0; q < 4; ++q)
+                prep_parent<KIND, kWave, true>(P, p0 + q, p0 + q < P.n, nullptr, nullptr, nc, nullptr, nullptr,
+                                               local_max, err);
+        }
+    }
 }
 
 // QCGD step / reversed_step (qcgd.hpp:443-457) on a segment of SEG lanes,

@@ -2735,14 +2735,22 @@ static int batch_plan(qd_ctx *ctx, const Parents &P, u64 in_bytes, const qd_rule
     u64 *counters = ctx->counters.as<u64>();
     QD_HIP(hipMemsetAsync(counters, 0, 64 * sizeof(u64), s));
     QD_HIP(hipMemsetAsync(ctx->btrunc.p, 0, bt->n * sizeof(u32), s));
-    hipLaunchKernelGGL(k_num_child<KIND>, dim3(grid_for(N)), dim3(256), 0, s, P, rule, ctx->nc.as<u32>(),
-                       reinterpret_cast<u32 *>(counters + 4));
+    u32 *errw = reinterpret_cast<u32 *>(counters + 4);
+    hipLaunchKernelGGL(k_num_child_fast<KIND>, dim3(grid_for(N, 16, 65536)), dim3(256), 0, s, P, ctx->nc.as<u32>(),
+                       errw);
     hipLaunchKernelGGL(k_wide_flags, dim3(grid_for(N)), dim3(256), 0, s, P.obj, P.begin, P.size, N, bt->d_oseg, bt->n,
                        ctx->btrunc.as<u32>());
     QD_HIP(hipGetLastError());
     QD_TRY(run_scan(ctx, LoadU32{ctx->nc.as<u32>()}, N, ctx->cb.as<u64>()));
     u64 hb[2];
     QD_TRY(sync_read(ctx, counters + 3, 2 * sizeof(u64), hb));
+    if (u32(hb[1]) & ERR_WIDE) {  // a graph beyond 64 nodes: every count one thread per parent
+        QD_HIP(hipMemsetAsync(errw, 0, sizeof(u32), s));
+        hipLaunchKernelGGL(k_num_child<KIND>, dim3(grid_for(N)), dim3(256), 0, s, P, rule, ctx->nc.as<u32>(), errw);
+        QD_HIP(hipGetLastError());
+        QD_TRY(run_scan(ctx, LoadU32{ctx->nc.as<u32>()}, N, ctx->cb.as<u64>()));
+        QD_TRY(sync_read(ctx, counters + 3, 2 * sizeof(u64), hb));
+    }
     QD_TRY(check_err_word(u32(hb[1])));
     QD_TRY(sync_read(ctx, ctx->btrunc.p, bt->n * sizeof(u32), wide.data()));
     return batch_cseg(ctx, bt);

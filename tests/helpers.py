@@ -113,3 +113,15 @@ def random_graph(n, rng):
     L = [rng.random() < 0.5 for _ in range(n)]
     R = [rng.random() < 0.5 for _ in range(n)]
     return qcgd.with_particles(qcgd.make_graph(n), L, R)
+
+
+def sparse_graph(n, splits, merges):
+    """An n-node graph with particles on both sides of the nodes in splits
+    and a merge pair (left on i, right on i + 1) for each i in merges, so
+    that split_merge and coin have few decisions even when n is large."""
+    left, right = [0] * n, [0] * n
+    for i in splits:
+        left[i] = right[i] = 1
+    for i in merges:  # node i merges with i + 1 (left[i + 1] stays 0)
+        left[i] = right[i + 1] = 1
+    return qcgd.with_particles(qcgd.make_graph(n), left, right)

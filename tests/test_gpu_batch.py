@@ -10,6 +10,8 @@ from quids_amd.batch import apply_modifier, batch_t, simulate_batched
 from quids_amd.rules import qcgd
 from quids_amd.state import HostState
 
+from helpers import sparse_graph
+
 pytestmark = pytest.mark.gpu
 
 
@@ -105,6 +107,38 @@ def test_most_instances():
               float(rng.uniform(0, 3)), float(rng.uniform(0, 1.5)), float(rng.uniform(0, 3))) for _ in range(4096)]
     caps = [int(c) for c in rng.choice([2, 3, 5, 10 ** 6], size=len(specs))]
     _run(specs, 1, caps)
+
+
+def test_wide_instance_steps_alone():
+    """Instances holding graphs of more than 64 nodes (beyond the batched
+    kernels) between small ones: they step alone through the plain path and
+    the small ones batched, the outputs concatenated; every instance as its
+    own plain step.  (split_merge and coin only: erase_create would give the
+    sparse graphs 2^n children.)"""
+    print("batch with wide instances", flush=True)
+    states = [HostState.from_objects([g]) for g in (
+        qcgd.seeded_graph(5, 1), sparse_graph(70, (3, 11, 20, 33, 47, 58, 66), (7, 25, 40)),
+        qcgd.seeded_graph(6, 2), qcgd.seeded_graph(4, 3), sparse_graph(66, (2, 9, 30, 44, 51), (14, 36)))]
+    a, b = batch_t.from_states(states), batch_t()
+    plain = []
+    for hs in states:
+        it = q.it_t()
+        it.upload(hs)
+        plain.append(it)
+    caps = [40, 600, 30, 10 ** 6, 600]
+    for k, rule in enumerate([qcgd.split_merge(0.6, 0.3), qcgd.coin(0.9, 0.4), qcgd.split_merge(1.1, 0.2)]):
+        stats = simulate_batched(a, [rule] * len(states), b, caps)
+        a, b = b, a
+        got = a.instances()
+        for i, it in enumerate(plain):
+            nxt = q.it_t()
+            st = q.simulate(it, rule, nxt, None, caps[i])
+            plain[i] = nxt
+            what = f"step {k} instance {i}"
+            _same(got[i], nxt.download(), what)
+            for key in ("num_candidates", "num_unique", "num_survivors", "output_bytes"):
+                assert stats[i][key] == st[key], (what, key, stats[i][key], st[key])
+            assert stats[i]["total_proba"] == st["total_proba"], what
 
 
 def test_batch_split_into_chunks():

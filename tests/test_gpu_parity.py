@@ -10,7 +10,7 @@ from quids_amd.rules import qcgd
 from quids_amd.rules import quantum_computer as qc
 from quids_amd.state import HostState
 
-from helpers import assert_states_identical, qcgd_initial, qcgd_iteration_rules, random_circuit
+from helpers import assert_states_identical, qcgd_initial, qcgd_iteration_rules, random_circuit, sparse_graph
 
 pytestmark = pytest.mark.gpu
 
@@ -121,15 +121,6 @@ def test_truncation_key_ranges(ctx, scale):
         assert_states_identical(g, o, f"scale {scale} cap {cap}")
 
 
-def _sparse_graph(n, splits, merges):
-    left, right = [0] * n, [0] * n
-    for i in splits:
-        left[i] = right[i] = 1
-    for i in merges:  # node i merges with i + 1 (left[i + 1] stays 0)
-        left[i] = right[i + 1] = 1
-    return qcgd.with_particles(qcgd.make_graph(n), left, right)
-
-
 def test_populate_large_objects(ctx):
     """Objects beyond the populate kernels' LDS buffers and node lists
     (parents of 1.2-1.6 KB, children of up to 74 nodes, graphs of more than
@@ -138,8 +129,8 @@ def test_populate_large_objects(ctx):
     split_merge and coin steps identical to the oracle.  (The sparse graphs
     have few split / merge / coin operations; erase_create would give them
     2^n children.)"""
-    objs = [_sparse_graph(70, (3, 11, 20, 33, 47, 58, 66), (7, 25, 40)),
-            _sparse_graph(60, (2, 9, 30, 44, 51), (14, 36)),
+    objs = [sparse_graph(70, (3, 11, 20, 33, 47, 58, 66), (7, 25, 40)),
+            sparse_graph(60, (2, 9, 30, 44, 51), (14, 36)),
             qcgd.seeded_graph(12, 8), qcgd.seeded_graph(9, 2)]
     rng = np.random.default_rng(17)
     mags = rng.normal(size=len(objs)) + 1j * rng.normal(size=len(objs))
