@@ -306,7 +306,11 @@ int qd_simulate_user(qd_ctx *ctx, qd_state *in, const qd_user_rule *rule, qd_sta
  *      into `out`; out_seg (host, num_instances + 1) receives the instances'
  *      object offsets and stats (host, optional, num_instances) their
  *      counters.  Equal objects of different instances never merge: the
- *      instance index is folded into the dedup key. */
+ *      instance index is folded into the dedup key.  num_instances is 1 to
+ *      4096; an instance may hold no object (it steps to an empty one); an
+ *      instance holding a graph of more than 64 nodes, or more candidates
+ *      than one pass takes, steps on its own inside the call, with the
+ *      same results. */
 int qd_simulate_batched(qd_ctx *ctx, qd_state *in, const uint64_t *in_seg, uint32_t num_instances,
                         const qd_rule *rules, qd_state *out, uint64_t *out_seg,
                         const uint64_t *max_num_object, const qd_params *params,
