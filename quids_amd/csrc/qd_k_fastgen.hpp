@@ -712,6 +712,10 @@ __global__ __launch_bounds__(kGenThreads, 3) void k_generate_scatter(const PDesc
     __shared__ u64 bidx[kFusedBins];   // its entry index in E1: bucket * cap1 + first slot
     __shared__ u64 swb[kGenThreads + 1];
     __shared__ u64 pfirst;
+    // batched: the children's magnitudes wait in LDS (each thread its own
+    // slots) rather than registers, which the instance search and offsets
+    // need
+    __shared__ double2 sms[BATCH ? kKids * kGenThreads : 1];
     const u32 nbin = 1u << b1;
     QcgdCoef k = coef_of(rule);
     bool spill = false;
@@ -771,7 +775,8 @@ __global__ __launch_bounds__(kGenThreads, 3) void k_generate_scatter(const PDesc
                 if (BATCH) h += hadd;
                 const u32 d = b1 ? u32(h >> (64 - b1)) : 0u;
                 hs[u] = h;
-                ms[u] = make_double2(m.re, m.im);
+                if (BATCH) sms[u * kGenThreads + threadIdx.x] = make_double2(m.re, m.im);
+                else ms[u] = make_double2(m.re, m.im);
                 br[u] = (d << 16) | atomicAdd(&bcnt[d], 1u);
             });
         }
@@ -788,11 +793,12 @@ __global__ __launch_bounds__(kGenThreads, 3) void k_generate_scatter(const PDesc
             if (u < nk) {
                 const u32 d = br[u] >> 16, r = br[u] & 0xFFFF;
                 if (bbase[d] + r < cap1) {
+                    const double2 mu = BATCH ? sms[u * kGenThreads + threadIdx.x] : ms[u];
                     Entry e;
                     e.hash = hs[u];
                     e.cand = cbase + (u64(u) << s);
-                    e.re = ms[u].x;
-                    e.im = ms[u].y;
+                    e.re = mu.x;
+                    e.im = mu.y;
                     E1[bidx[d] + r] = e;
                 } else {
                     spill = true;
